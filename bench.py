@@ -1,0 +1,219 @@
+"""Throughput of the batched mass-spring step on one MI355X.
+
+Workload (default): 512 voxel robots on a 9x9x9 lattice (8^3 voxels, 729
+masses, 7448 springs each including air springs), materials drawn per voxel
+from a seed with p(empty) = 0.2, land environment (gravity, ground, static /
+kinetic friction), FP32, dt = 1e-4, one "step" = one 1 s evaluation of the
+whole population (the reference's float time loop: 10000 substeps), launches
+captured in a hipGraph.  Metric: spring-updates/s = sum over robots of all
+springs (air included, as optimize/benchmark.cpp:104-109 counts them) x
+substeps / wall seconds of the simulate calls.
+
+    python bench.py [--steps K] [--warmup W] [--solver force|xpbd]
+
+Prints one JSON line.  --gpus N with N > 1 exits: this build measures one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+import mercury_amd as mh  # noqa: E402
+
+LAND = dict(gravity=9.81, ground_k=1.0e5, mu_s=1.0, mu_k=0.8, drag=0.0, damping=0.99)
+# LDS bytes per spring-update of the LDS-resident kernel, per robot size:
+# 57 B per spring (two endpoint positions 24 B, spring record 9 B, result
+# write 12 B and gather read 12 B) + 48 B per mass (pos/vel read + write).
+LDS_BYTES_PER_SPRING = 57.0
+LDS_BYTES_PER_MASS = 48.0
+# MI355X: 256 CUs x 128 B/clk x 2.4 GHz LDS bandwidth
+LDS_PEAK_GBS = 256 * 128 * 2.4
+
+
+def float_loop_steps(duration, dt):
+    r, n = np.float32(duration), 0
+    while r > np.float32(0):
+        r = np.float32(r - np.float32(dt))
+        n += 1
+    return n
+
+
+def cpu_threads():
+    n = os.environ.get("OMP_NUM_THREADS")
+    if n and n.isdigit():
+        return max(1, int(n))
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(els, solver, dt, steps, threads):
+    """The oracle's FP32 mirror (plain C, -O3, no fast-math) on host cores."""
+    import oracle
+    from mercury_amd.simulator import pack
+    a = pack(els)
+    if solver == mh.MH_SOLVER_FORCE:
+        base = oracle.make_base(mh.materials.WORKLOAD_ROWS)
+    else:
+        base = oracle.make_base(mh.materials.WORKLOAD_ROWS)
+    by_id, _, _ = oracle.composite_tables(base)
+    mt = oracle.MaterialTable(by_id["k"], by_id["dL0"], by_id["omega"], by_id["phi"])
+    pop = oracle.Population(a["mass_offset"], a["spring_offset"], a["mass"], a["pairs"],
+                            a["mean_length"], a["spring_material"])
+    env = oracle.make_env(dt, **LAND)
+
+    def run(n, th):
+        t = time.perf_counter()
+        if solver == mh.MH_SOLVER_FORCE:
+            oracle.force_f32(pop, mt, env, 0.0, n, a["pos"], a["vel"], nthreads=th)
+        else:
+            oracle.xpbd_f32(pop, mt, dt, 0.0, n, a["pos"], a["vel"], nthreads=th)
+        return time.perf_counter() - t
+
+    run(max(1, steps // 20), threads)  # warm-up
+    secs = run(steps, threads)
+    springs = len(a["mean_length"])
+    return springs * steps / secs, secs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed 1 s evaluations")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--robots", type=int, default=512)
+    ap.add_argument("--voxels", type=int, default=8)
+    ap.add_argument("--duration", type=float, default=1.0)
+    ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--solver", choices=["force", "xpbd"], default="force")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--substeps", type=int, default=0, help="substeps per launch (0 = tuned)")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-xpbd", action="store_true", help="skip the XPBD side measurement")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(robots are independent: a multi-GPU run shards the population by robot).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    solver = mh.MH_SOLVER_FORCE if args.solver == "force" else mh.MH_SOLVER_XPBD_REF
+    ms = mh.workload_materials()
+    t_build = time.perf_counter()
+    els = mh.voxel_population(ms, args.robots, voxels_per_axis=args.voxels, p_empty=0.2,
+                              seed=args.seed, mass=0.001)
+    t_build = time.perf_counter() - t_build
+    n_springs = sum(e.n_springs for e in els)
+    n_masses = sum(e.n_masses for e in els)
+    n_active = int(sum((e.spring_material != 0).sum() for e in els))
+    n_sub = float_loop_steps(args.duration, args.dt)
+
+    def make_sim(slv):
+        cfg = mh.SimConfig(solver=slv, time_step=args.dt, substeps_per_launch=args.substeps,
+                           use_graph=1, **LAND)
+        sim = mh.Simulator(ms, cfg)
+        t = time.perf_counter()
+        sim.SetElements(els)
+        t_up = time.perf_counter() - t
+        if not args.no_autotune:
+            sim.autotune(200)
+        return sim, t_up
+
+    def timed(sim, k, warm):
+        for _ in range(warm):
+            sim.Reset()
+            sim.Simulate(args.duration)
+        sim.synchronize()
+        sim.set_timing(True)
+        t = time.perf_counter()
+        for _ in range(k):
+            sim.Reset()
+            sim.Simulate(args.duration)
+        sim.synchronize()
+        secs = time.perf_counter() - t
+        kms, launches = sim.last_kernel_ms()
+        sim.set_timing(False)
+        return secs, kms, launches
+
+    sim, t_upload = make_sim(solver)
+    secs, kernel_ms, launches = timed(sim, args.steps, args.warmup)
+    info = sim.info()
+    updates = n_springs * n_sub * args.steps
+    value = updates / secs
+
+    # roofline of the dominant (step) kernel from its average launch time
+    steps_per_launch = n_sub / max(launches, 1)
+    lds_bytes = (LDS_BYTES_PER_SPRING * n_springs + LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
+    achieved = lds_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
+
+    out = {
+        "metric": "spring-updates/s (512x8^3 voxel robots)",
+        "value": value,
+        "unit": "spring-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": secs / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded voxel materials)",
+        "config": {
+            "workload": f"{args.robots} voxel robots, {args.voxels}^3 voxels "
+                        f"({els[0].n_masses} masses, {els[0].n_springs} springs each), "
+                        f"{args.solver} solver, land, dt={args.dt}, {args.duration} s evaluation",
+            "robots": args.robots, "springs_total": n_springs, "active_springs": n_active,
+            "masses_total": n_masses, "substeps_per_eval": n_sub, "solver": args.solver,
+            "kernel": {1: "lds", 2: "stream"}.get(info.kernel, "?"), "block": info.block_size,
+            "blocks": info.n_blocks, "lds_bytes_per_block": info.lds_bytes,
+            "substeps_per_launch": info.substeps_per_launch, "hipgraph": True,
+        },
+        "active_spring_updates_per_s": n_active * n_sub * args.steps / secs,
+        "roofline": {
+            "bound": "lds",
+            "achieved": achieved,
+            "peak": LDS_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (achieved / LDS_PEAK_GBS) if achieved else None,
+            "traffic": None,
+            "kernel_ms_per_launch": kernel_ms,
+            "launches_per_step": launches / args.steps if launches else None,
+            "algorithmic_bytes_per_spring_update":
+                LDS_BYTES_PER_SPRING + LDS_BYTES_PER_MASS * n_masses / n_springs,
+        },
+        "host_seconds": {"topology_build": t_build, "upload": t_upload},
+    }
+    if not args.no_xpbd and solver == mh.MH_SOLVER_FORCE:
+        sx, _ = make_sim(mh.MH_SOLVER_XPBD_REF)
+        xs, xkms, _ = timed(sx, 1, 1)
+        out["xpbd_ref_spring_updates_per_s"] = n_springs * n_sub / xs
+        sx.close()
+    if not args.no_cpu:
+        th = cpu_threads()
+        v_all, s_all = cpu_baseline(els, solver, args.dt, args.cpu_steps, th)
+        sub = els[:32]
+        v_one, s_one = cpu_baseline(sub, solver, args.dt, max(50, args.cpu_steps // 8), 1)
+        out["cpu_baseline"] = {
+            "value": v_all, "unit": "spring-updates/s", "cores": th, "kind": "port",
+            "sample": f"oracle FP32 mirror, same {args.robots} robots, {args.cpu_steps} substeps "
+                      f"({s_all:.1f} s wall on {th} threads)",
+            "single_core_value": v_one,
+            "single_core_sample": f"32 robots, {max(50, args.cpu_steps // 8)} substeps, 1 thread",
+        }
+        out["gpu_vs_cpu"] = value / v_all
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

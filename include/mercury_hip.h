@@ -1,0 +1,185 @@
+/*
+ * mercury_hip.h -- C-ABI of the MI355X-native batched mass-spring step.
+ *
+ * Drop-in boundary for the reference's host simulator API
+ * (common/simulator/Simulator.h:19-149 of DawsonCohen/Mercury).  Plain C
+ * types only: no torch, no Eigen, no STL.  Every call returns MH_OK (0) or a
+ * negative error code; mh_last_error() gives the calling thread's message
+ * for its last failing call.  Handles are independent (no process-global
+ * simulation state) and each handle is used from one host thread at a time.
+ *
+ * Reference interface each group replaces (paths in the reference repo):
+ *   mh_materials_*     materials::{matLookup, get_composite_id, avg,
+ *                      id_lookup, decode, encodedCompositeIdx}
+ *                      common/simulator/material.h:109-322
+ *   mh_voxel_build     VoxelRobot::Build / Strip / BuildSpringsRecurse /
+ *                      ShiftX / ShiftY  common/evolvables/VoxelRobot.cpp:30-331
+ *   mh_rest_lengths    SoftBody::Reset spring mean lengths
+ *                      common/evolvables/SoftBody.cpp:68-80
+ *   mh_baseline        SoftBody::updateBaseline/updateCOM/updateLength
+ *                      common/evolvables/SoftBody.h:78-82, SoftBody.cpp:98-122
+ *   mh_sim_create      Simulator::Initialize  Simulator.cu:103-198
+ *   mh_sim_set_population  Simulator::SetElements / AllocateElement
+ *                      Simulator.cu:206-367, 460-529
+ *   mh_sim_simulate    Simulator::Simulate  Simulator.cu:369-458 with
+ *                      integrateBodies  sim_kernel.cu:271-325
+ *   mh_sim_collect     Simulator::Collect  Simulator.cu:537-595
+ *   mh_sim_fitness     SoftBody::updateFitness  SoftBody.cpp:124-137
+ *   mh_sim_reset / mh_sim_total_time / mh_sim_set_time_step
+ *                      Simulator::{Reset, getTotalTime, setTimeStep,
+ *                      getDeltaT}  Simulator.h:42-48
+ */
+#ifndef MERCURY_HIP_H
+#define MERCURY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MH_OK 0
+#define MH_ERR_ARG (-1)      /* invalid argument */
+#define MH_ERR_HIP (-2)      /* HIP runtime failure (no device, launch) */
+#define MH_ERR_RANGE (-3)    /* size beyond what a kernel supports */
+#define MH_ERR_STATE (-4)    /* call out of order (e.g. no population) */
+#define MH_ERR_NOMEM (-5)    /* host or device allocation failed */
+#define MH_ERR_INTERNAL (-6)
+
+/* Message of the calling thread's last failing call ("" if none). */
+const char* mh_last_error(void);
+int mh_version(void);
+
+/* ------------------------------------------------------------ materials */
+typedef struct mh_material_set mh_material_set;
+typedef struct {
+  float k, dL0, omega, phi;
+} mh_base_material;
+
+/* The reference's 22 base materials (+ air), material.h:111-133. */
+int mh_materials_reference(mh_material_set** out);
+/* n_base non-air base materials; base i (0-based) gets id i+1 and encoding
+ * bit i+1; bit 0 is air.  Composites follow the reference rules. */
+int mh_materials_create(int n_base, const mh_base_material* base,
+                        mh_material_set** out);
+int mh_materials_destroy(mh_material_set* set);
+/* Composite count = n_base*(n_base+1)/2 + 1 (254 for the reference set). */
+int mh_materials_count(const mh_material_set* set);
+/* id_lookup(id): (k, dL0, omega, phi) and encoding. */
+int mh_materials_by_id(const mh_material_set* set, int id, float kdwp[4],
+                       uint32_t* encoding);
+/* decode(encoding): composite id, and encodedCompositeIdx(encoding). */
+int mh_materials_decode(const mh_material_set* set, uint32_t encoding,
+                        uint32_t* id, int32_t* encoded_index);
+
+/* ---------------------------------------------------------- voxel build */
+/* Upper bound on the spring count of an nx*ny*nz lattice. */
+int mh_voxel_max_springs(int nx, int ny, int nz);
+/* voxel_encoding[nx*ny*nz]: material encoding of each voxel (decoded with
+ * the set's decode table).  Writes nx*ny*nz masses and the springs in the
+ * reference's DFS emission order; *n_springs receives the count.
+ * stripped_encoding (may be NULL) receives the voxel encodings after Strip. */
+int mh_voxel_build(const mh_material_set* set, int nx, int ny, int nz,
+                   float resolution, const uint32_t* voxel_encoding,
+                   float* mass_pos, uint8_t* mass_material_id,
+                   uint16_t* spring_pairs, float* spring_length,
+                   uint8_t* spring_material_id, uint32_t* spring_encoding,
+                   int max_springs, int* n_springs,
+                   uint32_t* stripped_encoding);
+/* Mean lengths |p[m0] - p[m1]| as SoftBody::Reset computes them. */
+int mh_rest_lengths(const float* pos, const uint16_t* pairs, int n_springs,
+                    float* out);
+/* Baseline COM (3 floats) and length of one robot (non-air masses). */
+int mh_baseline(const float* pos, const uint8_t* mass_material_id,
+                int n_masses, float com[3], float* length);
+
+/* ------------------------------------------------------------ simulator */
+enum { MH_SOLVER_XPBD_REF = 0, MH_SOLVER_FORCE = 1 };
+enum { MH_KERNEL_AUTO = 0, MH_KERNEL_LDS = 1, MH_KERNEL_STREAM = 2 };
+enum { MH_FITNESS_REFERENCE = 0, MH_FITNESS_DISPLACEMENT = 1 };
+
+typedef struct {
+  int solver;          /* MH_SOLVER_* */
+  float dt;            /* time step, s */
+  float gravity;       /* force solver: m/s^2 along -y */
+  float ground_k;      /* force solver: ground stiffness N/m, 0 = no ground */
+  float mu_s, mu_k;    /* force solver: static / kinetic friction */
+  float drag;          /* force solver: linear drag coefficient N s/m */
+  float damping;       /* force solver: velocity factor per step */
+  int device;          /* HIP device ordinal */
+  int kernel;          /* MH_KERNEL_* */
+  int block_size;      /* 0 = autotune; else 256, 512 or 1024 */
+  int robots_per_block;/* 0 = as many as fit; else an upper bound */
+  int substeps_per_launch; /* 0 = autotune */
+  int use_graph;       /* capture each simulate's launches in a hipGraph */
+} mh_config;
+
+/* Reference defaults: XPBD, dt = 1e-4 (Simulator.h:55), the Land
+ * environment constants of environment.h:17-23 for the force solver
+ * except ground_k (1e5, stable at dt = 1e-4 with 1 g masses). */
+void mh_config_default(mh_config* cfg);
+
+typedef struct {
+  int32_t n_robots;
+  const uint32_t* mass_offset;        /* n_robots+1 prefix offsets */
+  const uint32_t* spring_offset;      /* n_robots+1 prefix offsets */
+  const float* pos;                   /* 3*M, xyz per mass */
+  const float* vel;                   /* 3*M or NULL (zero) */
+  const float* mass;                  /* M or NULL (1.0) */
+  const uint8_t* mass_material_id;    /* M composite id; 0 = air (fitness) */
+  const uint16_t* spring_pairs;       /* 2*S, robot-local mass indices */
+  const float* spring_mean_length;    /* S */
+  const uint8_t* spring_material_id;  /* S composite id; 0 = air (skipped) */
+} mh_population;
+
+typedef struct mh_sim mh_sim;
+
+int mh_sim_create(const mh_config* cfg, const mh_material_set* set,
+                  mh_sim** out);
+int mh_sim_destroy(mh_sim* sim);
+/* Copies the population to the device (uneven robots allowed), builds the
+ * per-block packing, CSR gather lists and material tables, zeroes stresses
+ * and the simulated time. */
+int mh_sim_set_population(mh_sim* sim, const mh_population* pop);
+/* Runs the reference's float time loop: while (remaining > 0) { step(t);
+ * t += dt; remaining -= dt; }.  Asynchronous: returns once launched. */
+int mh_sim_simulate(mh_sim* sim, float duration, int track_stresses);
+/* Runs exactly n_steps steps (same time recurrence). */
+int mh_sim_step(mh_sim* sim, int n_steps, int track_stresses);
+int mh_sim_synchronize(mh_sim* sim);
+/* Caller-allocated outputs, each may be NULL: pos/vel 3*M, stress S
+ * (original spring order; air springs stay 0). */
+int mh_sim_collect(mh_sim* sim, float* pos, float* vel, float* stress);
+/* Per-robot fitness on the device: MH_FITNESS_REFERENCE gives
+ * max((COMx - COMx0)/L, 0) with the baseline taken at set_population;
+ * MH_FITNESS_DISPLACEMENT gives COMx - COMx0. */
+int mh_sim_fitness(mh_sim* sim, int mode, float* out);
+/* Per-robot COM of non-air masses (3 floats each), device-computed. */
+int mh_sim_com(mh_sim* sim, float* out);
+int mh_sim_reset(mh_sim* sim);               /* simulated time := 0 */
+float mh_sim_total_time(const mh_sim* sim);
+float mh_sim_get_time_step(const mh_sim* sim);
+int mh_sim_set_time_step(mh_sim* sim, float dt);
+/* Population counts and the launch plan chosen for it. */
+typedef struct {
+  int n_robots, n_masses, n_springs, n_active_springs;
+  int kernel, block_size, n_blocks, substeps_per_launch, lds_bytes;
+  int max_masses_per_block, max_springs_per_block;
+} mh_sim_info;
+int mh_sim_get_info(const mh_sim* sim, mh_sim_info* info);
+/* Times the candidate launch plans on the current population for n_steps
+ * each and keeps the fastest (results do not depend on the choice). */
+int mh_sim_autotune(mh_sim* sim, int n_steps);
+/* Device pointers of the state (float4 per mass: xyz + mass / xyz + 0). */
+int mh_sim_device_state(mh_sim* sim, void** pos4, void** vel4,
+                        void** stream);
+/* Average duration (ms) of the step kernel launches of the last simulate,
+ * measured with HIP events on the simulator's stream; 0 if not timed. */
+int mh_sim_set_timing(mh_sim* sim, int enable);
+int mh_sim_last_kernel_ms(const mh_sim* sim, float* avg_ms, int* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MERCURY_HIP_H */

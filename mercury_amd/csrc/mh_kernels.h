@@ -1,0 +1,74 @@
+// mh_kernels.h -- launch interface between the host simulator (sim.cpp) and
+// the HIP step kernels (step_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <cstdint>
+
+namespace mh {
+
+// One workgroup's share of the population: a contiguous range of robots.
+// All offsets are global element offsets into the device arrays.
+struct PackDesc {
+  uint32_t mass_begin;    // first mass (float4 pos/vel arrays)
+  uint32_t n_mass;
+  uint32_t spring_begin;  // first active spring
+  uint32_t n_spring;
+  uint32_t csr_begin;     // first gather entry (2 per active spring)
+  uint32_t row_begin;     // first row pointer (n_mass + 1 entries)
+  uint32_t pad0, pad1;
+};
+
+struct StepParams {
+  const PackDesc* packs;
+  float4* pos;               // xyz + mass
+  float4* vel;               // xyz + 0
+  const uint32_t* spair;     // pack-local a | b << 16
+  const float* slbar;
+  const uint8_t* smat;       // compact material index
+  float* sstress;            // per active spring (may be null)
+  const uint32_t* csr;       // (pack-local spring << 1) | (mass is endpoint b)
+  const uint32_t* row;       // pack-local gather offsets, n_mass + 1 per pack
+  float* scratch;            // streaming kernel: 3 floats per active spring
+  const float* mat_k;        // spring constant
+  const float* mat_kx;       // xpbd: 2 + ((1/k)/dt)/dt
+  const float* mat_dL0;
+  const float* mat_omega;
+  const float* mat_phi;
+  int n_mat;
+  int mass_cap;              // LDS capacity in masses (layout stride)
+  int spring_cap;            // LDS capacity in springs (layout stride)
+  int n_steps;
+  float dt;
+  float t0;
+  int track_stress;
+  float gravity, ground_k, mu_s, mu_k, drag, damping;
+};
+
+// A compiled step-kernel variant.
+struct KernelVariant {
+  int solver;     // MH_SOLVER_*
+  int kind;       // MH_KERNEL_LDS / MH_KERNEL_STREAM
+  int block;      // threads per workgroup
+  int nspt;       // springs per thread held in registers (LDS kind)
+  int mpt;        // masses per thread
+  const void* fn; // kernel symbol
+};
+
+// All compiled variants (table lives in step_kernels.hip).
+const KernelVariant* kernel_variants(int* count);
+
+// LDS bytes a pack of (masses, springs, materials) needs in the LDS kernel;
+// the streaming kernel needs only masses and the material table.
+size_t lds_bytes_lds(int mass_cap, int spring_cap, int n_mat);
+size_t lds_bytes_stream(int mass_cap, int n_mat);
+
+hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes,
+                       hipStream_t stream, const StepParams& p);
+
+// Per-robot COM of non-air masses (SoftBody::updateCOM order).
+hipError_t launch_com(const float4* pos, const uint8_t* mass_id,
+                      const uint32_t* robot_mass_begin, int n_robots,
+                      float* com_out, hipStream_t stream);
+
+}  // namespace mh

@@ -1,0 +1,806 @@
+// sim.cpp -- the simulator handle: population packing, launch planning,
+// step loop (optionally captured in a hipGraph), collect and fitness.
+//
+// Reference: common/simulator/Simulator.cu (SetElements 206-367,
+// Simulate 369-458, AllocateElement 460-529, Collect 537-595).  Differences
+// by design: robots may have different mass and spring counts (per-robot
+// offsets instead of a uniform stride), air springs are dropped from the
+// device arrays (the reference skips them in the kernel, sim_kernel.cu:211),
+// all state stays on the device between calls, errors are return codes.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <numeric>
+
+#include "mh_internal.h"
+#include "mh_kernels.h"
+
+namespace mh {
+namespace {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw Error(MH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+    n = count;
+  }
+  void upload(const T* src, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count)
+      hip_check(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s), "H2D copy");
+  }
+};
+
+struct Plan {
+  const KernelVariant* v = nullptr;
+  std::vector<PackDesc> packs;      // launch order (largest work first)
+  int mass_cap = 0, spring_cap = 0;
+  size_t lds = 0;
+};
+
+constexpr int kMaxLdsBytes = 160 * 1024;
+
+}  // namespace
+}  // namespace mh
+
+struct mh_sim {
+  mh_config cfg{};
+  std::vector<mh::Material> mats_by_id;
+  int composite_count = 0;
+  hipStream_t stream = nullptr;
+  int lds_limit = mh::kMaxLdsBytes;
+
+  // population (host copies used for packing and collect)
+  int n_robots = 0;
+  std::vector<uint32_t> mass_off, spring_off, active_off;  // per robot prefix offsets
+  std::vector<uint16_t> act_pairs;   // active springs, robot-local
+  std::vector<uint32_t> active_src;  // active spring -> original spring index
+  std::vector<uint8_t> act_mat;      // compact material index
+  std::vector<uint8_t> compact_to_id;
+  std::vector<float> base_com, base_len;
+  size_t n_masses = 0, n_springs = 0, n_active = 0;
+  bool has_population = false;
+
+  // device state
+  mh::DevBuf<float4> pos, vel;
+  mh::DevBuf<float> lbar, stress, scratch, com;
+  mh::DevBuf<uint8_t> smat, mass_id;
+  mh::DevBuf<uint32_t> spair, csr, row, robot_begin;
+  mh::DevBuf<mh::PackDesc> packs;
+  mh::DevBuf<float> mat_k, mat_kx, mat_dL0, mat_omega, mat_phi;
+
+  mh::Plan plan;
+  int substeps = 0;
+  float total_time = 0.f;
+
+  // graph cache
+  hipGraphExec_t graph = nullptr;
+  int g_steps = -1, g_track = -1, g_sub = -1;
+  uint32_t g_t0 = 0;
+  const void* g_fn = nullptr;
+
+  // timing
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int last_launches = 0;
+  bool have_timing = false;
+
+  ~mh_sim() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace mh {
+namespace {
+
+void drop_graph(mh_sim* s) {
+  if (s->graph) (void)hipGraphExecDestroy(s->graph);
+  s->graph = nullptr;
+  s->g_steps = -1;
+}
+
+// Material tables for the compact indices (xpbd stiffness term depends on dt).
+void upload_materials(mh_sim* s) {
+  const int n = static_cast<int>(s->compact_to_id.size());
+  std::vector<float> k(n), kx(n), a(n), w(n), ph(n);
+  const float dt = s->cfg.dt;
+  for (int i = 0; i < n; i++) {
+    const Material& m = s->mats_by_id[s->compact_to_id[i]];
+    k[i] = m.k;
+    float alpha = 1.0f / m.k;  // sim_kernel.cu:220, ((1/k)/dt)/dt
+    alpha = alpha / dt;
+    alpha = alpha / dt;
+    kx[i] = 2.0f + alpha;      // sim_kernel.cu:225
+    a[i] = m.dL0;
+    w[i] = m.omega;
+    ph[i] = m.phi;
+  }
+  s->mat_k.upload(k.data(), n, s->stream);
+  s->mat_kx.upload(kx.data(), n, s->stream);
+  s->mat_dL0.upload(a.data(), n, s->stream);
+  s->mat_omega.upload(w.data(), n, s->stream);
+  s->mat_phi.upload(ph.data(), n, s->stream);
+}
+
+int robot_masses(const mh_sim* s, int r) { return static_cast<int>(s->mass_off[r + 1] - s->mass_off[r]); }
+int robot_active(const mh_sim* s, int r) { return static_cast<int>(s->active_off[r + 1] - s->active_off[r]); }
+
+// Greedy packing of consecutive robots into workgroups for variant v.
+// Returns false if some robot does not fit the variant.
+bool make_plan(const mh_sim* s, const KernelVariant& v, Plan& plan) {
+  plan = Plan{};
+  plan.v = &v;
+  const int n_mat = static_cast<int>(s->compact_to_id.size());
+  const int rpb = s->cfg.robots_per_block > 0 ? s->cfg.robots_per_block : std::numeric_limits<int>::max();
+  if (v.kind == MH_KERNEL_STREAM) {
+    for (int r = 0; r < s->n_robots; r++) {
+      const int m = robot_masses(s, r);
+      if (lds_bytes_stream(m, n_mat) > static_cast<size_t>(s->lds_limit)) return false;
+      plan.mass_cap = std::max(plan.mass_cap, m);
+      plan.spring_cap = std::max(plan.spring_cap, robot_active(s, r));
+      PackDesc d{};
+      d.mass_begin = s->mass_off[r];
+      d.n_mass = static_cast<uint32_t>(m);
+      d.spring_begin = s->active_off[r];
+      d.n_spring = static_cast<uint32_t>(robot_active(s, r));
+      plan.packs.push_back(d);
+    }
+    plan.lds = lds_bytes_stream(std::max(plan.mass_cap, 1), n_mat);
+  } else {
+    const int mcap = v.block * v.mpt, scap = v.block * v.nspt;
+    int r = 0;
+    while (r < s->n_robots) {
+      PackDesc d{};
+      d.mass_begin = s->mass_off[r];
+      d.spring_begin = s->active_off[r];
+      int count = 0;
+      while (r < s->n_robots && count < rpb) {
+        const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
+        const int sp = static_cast<int>(d.n_spring) + robot_active(s, r);
+        const int new_mcap = std::max(plan.mass_cap, m), new_scap = std::max(plan.spring_cap, sp);
+        const bool fits = m <= mcap && sp <= scap && 2 * sp <= 65535 && m <= 65535 &&
+                          lds_bytes_lds(new_mcap, new_scap, n_mat) <= static_cast<size_t>(s->lds_limit);
+        if (!fits) break;
+        d.n_mass = static_cast<uint32_t>(m);
+        d.n_spring = static_cast<uint32_t>(sp);
+        r++;
+        count++;
+      }
+      if (count == 0) return false;
+      plan.mass_cap = std::max(plan.mass_cap, static_cast<int>(d.n_mass));
+      plan.spring_cap = std::max(plan.spring_cap, static_cast<int>(d.n_spring));
+      plan.packs.push_back(d);
+    }
+    plan.lds = lds_bytes_lds(std::max(plan.mass_cap, 1), std::max(plan.spring_cap, 1), n_mat);
+  }
+  return true;
+}
+
+// Device arrays that depend on the packing: pack-local spring pairs, the
+// gather lists (ascending spring index per mass) and row pointers.
+void upload_plan(mh_sim* s, Plan& plan) {
+  const size_t na = s->n_active;
+  std::vector<uint32_t> spair(na), csr(2 * na), row;
+  row.reserve(s->n_masses + plan.packs.size());
+  std::vector<uint32_t> count;
+  for (PackDesc& d : plan.packs) {
+    d.csr_begin = 2 * d.spring_begin;
+    d.row_begin = static_cast<uint32_t>(row.size());
+    const uint32_t nm = d.n_mass;
+    count.assign(nm + 1, 0);
+    // pack-local pairs: robot-local index + robot's offset inside the pack
+    int r = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
+                             s->mass_off.begin()) - 1;
+    for (uint32_t k = 0; k < d.n_spring; k++) {
+      const uint32_t g = d.spring_begin + k;
+      while (g >= s->active_off[r + 1]) r++;
+      const uint32_t off = s->mass_off[r] - d.mass_begin;
+      const uint32_t a = s->act_pairs[2 * g] + off, b = s->act_pairs[2 * g + 1] + off;
+      spair[g] = a | (b << 16);
+      count[a + 1]++;
+      count[b + 1]++;
+    }
+    for (uint32_t i = 0; i < nm; i++) count[i + 1] += count[i];
+    for (uint32_t i = 0; i <= nm; i++) row.push_back(count[i]);
+    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
+    for (uint32_t k = 0; k < d.n_spring; k++) {
+      const uint32_t pr = spair[d.spring_begin + k];
+      const uint32_t a = pr & 0xffffu, b = pr >> 16;
+      csr[d.csr_begin + fill[a]++] = k << 1;
+      csr[d.csr_begin + fill[b]++] = (k << 1) | 1u;
+    }
+  }
+  // launch order: most work first, so uneven packs balance across CUs
+  std::stable_sort(plan.packs.begin(), plan.packs.end(), [](const PackDesc& x, const PackDesc& y) {
+    return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
+  });
+  s->spair.upload(spair.data(), na, s->stream);
+  s->csr.upload(csr.data(), 2 * na, s->stream);
+  s->row.upload(row.data(), row.size(), s->stream);
+  s->packs.upload(plan.packs.data(), plan.packs.size(), s->stream);
+  if (plan.v->kind == MH_KERNEL_STREAM) s->scratch.alloc(3 * std::max<size_t>(na, 1));
+  if (plan.lds > 64 * 1024)
+    (void)hipFuncSetAttribute(plan.v->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(plan.lds));
+  hip_check(hipStreamSynchronize(s->stream), "plan upload");
+  s->plan = std::move(plan);
+  drop_graph(s);
+}
+
+std::vector<const KernelVariant*> candidates(const mh_sim* s) {
+  int nv = 0;
+  const KernelVariant* all = kernel_variants(&nv);
+  std::vector<const KernelVariant*> out;
+  for (int i = 0; i < nv; i++) {
+    const KernelVariant& v = all[i];
+    if (v.solver != s->cfg.solver) continue;
+    if (s->cfg.kernel != MH_KERNEL_AUTO && v.kind != s->cfg.kernel) continue;
+    if (s->cfg.block_size && v.block != s->cfg.block_size) continue;
+    out.push_back(&v);
+  }
+  return out;
+}
+
+// Default plan: LDS variants in table order (largest blocks first), the
+// first one every robot fits; otherwise the streaming kernel.
+void choose_default_plan(mh_sim* s) {
+  auto cand = candidates(s);
+  Plan p;
+  for (int pass = 0; pass < 2; pass++) {
+    for (const KernelVariant* v : cand) {
+      if ((pass == 0) != (v->kind == MH_KERNEL_LDS)) continue;
+      if (make_plan(s, *v, p)) {
+        upload_plan(s, p);
+        return;
+      }
+    }
+  }
+  throw Error(MH_ERR_RANGE, "no step kernel fits this population (robot too large for LDS)");
+}
+
+StepParams make_params(mh_sim* s, float t0, int n_steps, int track) {
+  StepParams p{};
+  p.packs = s->packs.p;
+  p.pos = s->pos.p;
+  p.vel = s->vel.p;
+  p.spair = s->spair.p;
+  p.slbar = s->lbar.p;
+  p.smat = s->smat.p;
+  p.sstress = s->stress.p;
+  p.csr = s->csr.p;
+  p.row = s->row.p;
+  p.scratch = s->scratch.p;
+  p.mat_k = s->mat_k.p;
+  p.mat_kx = s->mat_kx.p;
+  p.mat_dL0 = s->mat_dL0.p;
+  p.mat_omega = s->mat_omega.p;
+  p.mat_phi = s->mat_phi.p;
+  p.n_mat = static_cast<int>(s->compact_to_id.size());
+  p.mass_cap = s->plan.mass_cap;
+  p.spring_cap = s->plan.spring_cap;
+  p.n_steps = n_steps;
+  p.dt = s->cfg.dt;
+  p.t0 = t0;
+  p.track_stress = track;
+  p.gravity = s->cfg.gravity;
+  p.ground_k = s->cfg.ground_k;
+  p.mu_s = s->cfg.mu_s;
+  p.mu_k = s->cfg.mu_k;
+  p.drag = s->cfg.drag;
+  p.damping = s->cfg.damping;
+  return p;
+}
+
+// Issue the launches of n_steps steps starting at simulated time t0.
+int issue(mh_sim* s, float t0, int n_steps, int track) {
+  const int K = s->substeps > 0 ? s->substeps : n_steps;
+  int launches = 0;
+  float t = t0;
+  for (int done = 0; done < n_steps;) {
+    const int n = std::min(K, n_steps - done);
+    StepParams p = make_params(s, t, n, track);
+    hip_check(launch_step(*s->plan.v, static_cast<int>(s->plan.packs.size()), s->plan.lds, s->stream, p),
+              "step launch");
+    for (int k = 0; k < n; k++) t += s->cfg.dt;  // the reference's float time recurrence
+    done += n;
+    launches++;
+  }
+  return launches;
+}
+
+void run_steps(mh_sim* s, int n_steps, int track) {
+  if (n_steps <= 0 || s->n_robots == 0 || s->n_active + s->n_masses == 0) {
+    for (int k = 0; k < n_steps; k++) s->total_time += s->cfg.dt;
+    return;
+  }
+  if (track && !s->stress.p) throw Error(MH_ERR_STATE, "no stress buffer");
+  const float t0 = s->total_time;
+  if (s->timing) hip_check(hipEventRecord(s->ev0, s->stream), "event");
+  int launches = 0;
+  if (s->cfg.use_graph) {
+    uint32_t t0_bits;
+    std::memcpy(&t0_bits, &t0, 4);
+    const bool hit = s->graph && s->g_steps == n_steps && s->g_track == track &&
+                     s->g_sub == s->substeps && s->g_t0 == t0_bits && s->g_fn == s->plan.v->fn;
+    if (!hit) {
+      drop_graph(s);
+      hipGraph_t g = nullptr;
+      hip_check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+      try {
+        issue(s, t0, n_steps, track);
+      } catch (...) {
+        (void)hipStreamEndCapture(s->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(s->stream, &g), "end capture");
+      hipError_t e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      hip_check(e, "graph instantiate");
+      s->g_steps = n_steps;
+      s->g_track = track;
+      s->g_sub = s->substeps;
+      s->g_t0 = t0_bits;
+      s->g_fn = s->plan.v->fn;
+    }
+    hip_check(hipGraphLaunch(s->graph, s->stream), "graph launch");
+    const int K = s->substeps > 0 ? s->substeps : n_steps;
+    launches = (n_steps + K - 1) / K;
+  } else {
+    launches = issue(s, t0, n_steps, track);
+  }
+  if (s->timing) {
+    hip_check(hipEventRecord(s->ev1, s->stream), "event");
+    s->last_launches = launches;
+    s->have_timing = true;
+  }
+  for (int k = 0; k < n_steps; k++) s->total_time += s->cfg.dt;
+}
+
+int step_count(float duration, float dt) {
+  // Simulator.cu:428-449: while (remaining > 0) { ...; remaining -= dt; }
+  float remaining = duration;
+  long n = 0;
+  while (remaining > 0.0f) {
+    remaining -= dt;
+    if (++n > 1000000000L) throw Error(MH_ERR_RANGE, "step count overflow (dt too small)");
+  }
+  return static_cast<int>(n);
+}
+
+}  // namespace
+}  // namespace mh
+
+#define MH_TRY try {
+#define MH_CATCH                                   \
+  }                                                \
+  catch (const mh::Error& e) {                     \
+    mh::set_error(e.what());                       \
+    return e.code;                                 \
+  }                                                \
+  catch (const std::bad_alloc&) {                  \
+    mh::set_error("host allocation failed");       \
+    return MH_ERR_NOMEM;                           \
+  }                                                \
+  catch (const std::exception& e) {                \
+    mh::set_error(e.what());                       \
+    return MH_ERR_INTERNAL;                        \
+  }
+
+extern "C" {
+
+void mh_config_default(mh_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->solver = MH_SOLVER_XPBD_REF;
+  c->dt = 0.0001f;      // Simulator.h:55
+  c->gravity = 9.81f;   // environment.h:19 (EnvironmentLand)
+  c->ground_k = 1.0e5f;
+  c->mu_s = 1.0f;
+  c->mu_k = 0.8f;       // environment.h:20
+  c->drag = 0.0f;
+  c->damping = 0.99f;   // environment.h:13
+  c->device = 0;
+  c->kernel = MH_KERNEL_AUTO;
+  c->block_size = 0;
+  c->robots_per_block = 0;
+  c->substeps_per_launch = 0;
+  c->use_graph = 1;
+}
+
+int mh_sim_create(const mh_config* cfg, const mh_material_set* set, mh_sim** out) {
+  MH_TRY
+  if (!cfg || !set || !out) throw mh::Error(MH_ERR_ARG, "null argument");
+  if (cfg->solver != MH_SOLVER_XPBD_REF && cfg->solver != MH_SOLVER_FORCE)
+    throw mh::Error(MH_ERR_ARG, "unknown solver");
+  if (!(cfg->dt > 0.f)) throw mh::Error(MH_ERR_ARG, "dt must be > 0");
+  if (cfg->block_size && cfg->block_size != 256 && cfg->block_size != 512 && cfg->block_size != 1024)
+    throw mh::Error(MH_ERR_ARG, "block_size must be 0, 256, 512 or 1024");
+  if (cfg->kernel < MH_KERNEL_AUTO || cfg->kernel > MH_KERNEL_STREAM)
+    throw mh::Error(MH_ERR_ARG, "unknown kernel kind");
+  int ndev = 0;
+  mh::hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+  if (cfg->device < 0 || cfg->device >= ndev) throw mh::Error(MH_ERR_HIP, "no such HIP device");
+  mh::hip_check(hipSetDevice(cfg->device), "hipSetDevice");
+  auto s = std::make_unique<mh_sim>();
+  s->cfg = *cfg;
+  s->mats_by_id = set->by_id;
+  s->composite_count = set->composite_count;
+  s->substeps = cfg->substeps_per_launch > 0 ? cfg->substeps_per_launch : 0;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, cfg->device) == hipSuccess &&
+      lds > 0)
+    s->lds_limit = std::min(lds, mh::kMaxLdsBytes);
+  mh::hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream");
+  mh::hip_check(hipEventCreate(&s->ev0), "event");
+  mh::hip_check(hipEventCreate(&s->ev1), "event");
+  *out = s.release();
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_destroy(mh_sim* sim) {
+  if (sim) {
+    (void)hipSetDevice(sim->cfg.device);
+    if (sim->stream) (void)hipStreamSynchronize(sim->stream);
+    delete sim;
+  }
+  return MH_OK;
+}
+
+int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
+  MH_TRY
+  if (!s || !pop) throw mh::Error(MH_ERR_ARG, "null argument");
+  if (pop->n_robots < 0) throw mh::Error(MH_ERR_ARG, "n_robots < 0");
+  const int R = pop->n_robots;
+  if (R > 0 && (!pop->mass_offset || !pop->spring_offset))
+    throw mh::Error(MH_ERR_ARG, "offsets are null");
+  mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  std::vector<uint32_t> mo(R + 1, 0), so(R + 1, 0);
+  if (R > 0) {
+    std::copy(pop->mass_offset, pop->mass_offset + R + 1, mo.begin());
+    std::copy(pop->spring_offset, pop->spring_offset + R + 1, so.begin());
+  }
+  if (mo[0] != 0 || so[0] != 0) throw mh::Error(MH_ERR_ARG, "offsets must start at 0");
+  for (int r = 0; r < R; r++) {
+    if (mo[r + 1] < mo[r] || so[r + 1] < so[r]) throw mh::Error(MH_ERR_ARG, "offsets must be non-decreasing");
+    if (mo[r + 1] - mo[r] > 65536) throw mh::Error(MH_ERR_RANGE, "robot has more than 65536 masses");
+  }
+  const size_t M = mo[R], S = so[R];
+  if ((M && (!pop->pos || !pop->mass_material_id)) ||
+      (S && (!pop->spring_pairs || !pop->spring_mean_length || !pop->spring_material_id)))
+    throw mh::Error(MH_ERR_ARG, "population arrays are null");
+
+  // validate and compact springs (air springs are no-ops, sim_kernel.cu:211)
+  std::vector<int> id_to_compact(256, -1);
+  std::vector<uint8_t> compact_to_id;
+  std::vector<uint32_t> ao(R + 1, 0), src;
+  std::vector<uint16_t> ap;
+  std::vector<float> alb;
+  std::vector<uint8_t> amat;
+  src.reserve(S);
+  for (int r = 0; r < R; r++) {
+    const uint32_t nm = mo[r + 1] - mo[r];
+    for (uint32_t k = so[r]; k < so[r + 1]; k++) {
+      const uint16_t a = pop->spring_pairs[2 * k], b = pop->spring_pairs[2 * k + 1];
+      const uint8_t id = pop->spring_material_id[k];
+      if (a >= nm || b >= nm) throw mh::Error(MH_ERR_ARG, "spring endpoint outside its robot");
+      if (id >= s->composite_count) throw mh::Error(MH_ERR_ARG, "spring material id outside the material set");
+      if (id == 0) continue;
+      if (id_to_compact[id] < 0) {
+        id_to_compact[id] = static_cast<int>(compact_to_id.size());
+        compact_to_id.push_back(id);
+      }
+      src.push_back(k);
+      ap.push_back(a);
+      ap.push_back(b);
+      alb.push_back(pop->spring_mean_length[k]);
+      amat.push_back(static_cast<uint8_t>(id_to_compact[id]));
+    }
+    ao[r + 1] = static_cast<uint32_t>(src.size());
+  }
+  if (compact_to_id.empty()) compact_to_id.push_back(0);  // keeps tables non-empty
+
+  // masses: float4 (xyz, mass) and (v, 0)
+  std::vector<float4> p4(M), v4(M);
+  for (size_t i = 0; i < M; i++) {
+    const float m = pop->mass ? pop->mass[i] : 1.0f;
+    if (s->cfg.solver == MH_SOLVER_FORCE && !(m > 0.f)) throw mh::Error(MH_ERR_ARG, "mass must be > 0");
+    p4[i] = make_float4(pop->pos[3 * i], pop->pos[3 * i + 1], pop->pos[3 * i + 2], m);
+    v4[i] = pop->vel ? make_float4(pop->vel[3 * i], pop->vel[3 * i + 1], pop->vel[3 * i + 2], 0.f)
+                     : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  // baseline COM and length per robot (SoftBody::updateBaseline at Reset)
+  std::vector<float> bc(3 * R), bl(R);
+  for (int r = 0; r < R; r++)
+    mh::baseline(pop->pos + 3 * mo[r], pop->mass_material_id + mo[r], static_cast<int>(mo[r + 1] - mo[r]),
+                 &bc[3 * r], &bl[r]);
+
+  mh::drop_graph(s);
+  s->has_population = false;
+  s->n_robots = R;
+  s->mass_off = mo;
+  s->spring_off = so;
+  s->active_off = ao;
+  s->act_pairs = std::move(ap);
+  s->active_src = std::move(src);
+  s->act_mat = std::move(amat);
+  s->compact_to_id = std::move(compact_to_id);
+  s->base_com = std::move(bc);
+  s->base_len = std::move(bl);
+  s->n_masses = M;
+  s->n_springs = S;
+  s->n_active = s->active_src.size();
+  s->total_time = 0.f;
+
+  s->pos.upload(p4.data(), M, s->stream);
+  s->vel.upload(v4.data(), M, s->stream);
+  s->lbar.upload(alb.data(), s->n_active, s->stream);
+  s->smat.upload(s->act_mat.data(), s->n_active, s->stream);
+  s->mass_id.upload(pop->mass_material_id, M, s->stream);
+  s->robot_begin.upload(mo.data(), R + 1, s->stream);
+  s->stress.alloc(std::max<size_t>(s->n_active, 1));
+  mh::hip_check(hipMemsetAsync(s->stress.p, 0, sizeof(float) * s->stress.n, s->stream), "memset");
+  s->com.alloc(3 * std::max(R, 1));
+  mh::upload_materials(s);
+  if (R > 0) mh::choose_default_plan(s);
+  mh::hip_check(hipStreamSynchronize(s->stream), "upload");
+  s->has_population = true;
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_simulate(mh_sim* s, float duration, int track_stresses) {
+  MH_TRY
+  if (!s) throw mh::Error(MH_ERR_ARG, "null sim");
+  if (!s->has_population) throw mh::Error(MH_ERR_STATE, "no population set");
+  mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  mh::run_steps(s, mh::step_count(duration, s->cfg.dt), track_stresses ? 1 : 0);
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_step(mh_sim* s, int n_steps, int track_stresses) {
+  MH_TRY
+  if (!s) throw mh::Error(MH_ERR_ARG, "null sim");
+  if (!s->has_population) throw mh::Error(MH_ERR_STATE, "no population set");
+  if (n_steps < 0) throw mh::Error(MH_ERR_ARG, "n_steps < 0");
+  mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  mh::run_steps(s, n_steps, track_stresses ? 1 : 0);
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_synchronize(mh_sim* s) {
+  MH_TRY
+  if (!s) throw mh::Error(MH_ERR_ARG, "null sim");
+  mh::hip_check(hipStreamSynchronize(s->stream), "synchronize");
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_collect(mh_sim* s, float* pos, float* vel, float* stress) {
+  MH_TRY
+  if (!s) throw mh::Error(MH_ERR_ARG, "null sim");
+  if (!s->has_population) throw mh::Error(MH_ERR_STATE, "no population set");
+  const size_t M = s->n_masses;
+  std::vector<float4> tmp(M);
+  if (pos && M) {
+    mh::hip_check(hipMemcpyAsync(tmp.data(), s->pos.p, M * sizeof(float4), hipMemcpyDeviceToHost, s->stream), "D2H");
+    mh::hip_check(hipStreamSynchronize(s->stream), "D2H");
+    for (size_t i = 0; i < M; i++) {
+      pos[3 * i] = tmp[i].x;
+      pos[3 * i + 1] = tmp[i].y;
+      pos[3 * i + 2] = tmp[i].z;
+    }
+  }
+  if (vel && M) {
+    mh::hip_check(hipMemcpyAsync(tmp.data(), s->vel.p, M * sizeof(float4), hipMemcpyDeviceToHost, s->stream), "D2H");
+    mh::hip_check(hipStreamSynchronize(s->stream), "D2H");
+    for (size_t i = 0; i < M; i++) {
+      vel[3 * i] = tmp[i].x;
+      vel[3 * i + 1] = tmp[i].y;
+      vel[3 * i + 2] = tmp[i].z;
+    }
+  }
+  if (stress) {
+    std::fill(stress, stress + s->n_springs, 0.f);
+    if (s->n_active) {
+      std::vector<float> st(s->n_active);
+      mh::hip_check(hipMemcpyAsync(st.data(), s->stress.p, s->n_active * sizeof(float), hipMemcpyDeviceToHost,
+                                   s->stream), "D2H");
+      mh::hip_check(hipStreamSynchronize(s->stream), "D2H");
+      for (size_t k = 0; k < s->n_active; k++) stress[s->active_src[k]] = st[k];
+    }
+  }
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_com(mh_sim* s, float* out) {
+  MH_TRY
+  if (!s || !out) throw mh::Error(MH_ERR_ARG, "null argument");
+  if (!s->has_population) throw mh::Error(MH_ERR_STATE, "no population set");
+  if (s->n_robots == 0) return MH_OK;
+  mh::hip_check(mh::launch_com(s->pos.p, s->mass_id.p, s->robot_begin.p, s->n_robots, s->com.p, s->stream),
+                "com launch");
+  mh::hip_check(hipMemcpyAsync(out, s->com.p, 3 * s->n_robots * sizeof(float), hipMemcpyDeviceToHost, s->stream),
+                "D2H");
+  mh::hip_check(hipStreamSynchronize(s->stream), "D2H");
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_fitness(mh_sim* s, int mode, float* out) {
+  MH_TRY
+  if (!s || !out) throw mh::Error(MH_ERR_ARG, "null argument");
+  if (mode != MH_FITNESS_REFERENCE && mode != MH_FITNESS_DISPLACEMENT) throw mh::Error(MH_ERR_ARG, "bad mode");
+  std::vector<float> c(3 * std::max(s->n_robots, 1));
+  int rc = mh_sim_com(s, c.data());
+  if (rc != MH_OK) return rc;
+  for (int r = 0; r < s->n_robots; r++) {
+    const float dx = c[3 * r] - s->base_com[3 * r];
+    if (mode == MH_FITNESS_DISPLACEMENT) {
+      out[r] = dx;
+    } else {
+      const float f = dx / s->base_len[r];  // SoftBody.cpp:132
+      out[r] = f > 0.0f ? f : 0.0f;
+    }
+  }
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_reset(mh_sim* s) {
+  if (!s) return MH_ERR_ARG;
+  s->total_time = 0.f;
+  return MH_OK;
+}
+
+float mh_sim_total_time(const mh_sim* s) { return s ? s->total_time : 0.f; }
+float mh_sim_get_time_step(const mh_sim* s) { return s ? s->cfg.dt : 0.f; }
+
+int mh_sim_set_time_step(mh_sim* s, float dt) {
+  MH_TRY
+  if (!s || !(dt > 0.f)) throw mh::Error(MH_ERR_ARG, "dt must be > 0");
+  s->cfg.dt = dt;
+  if (s->has_population) {
+    mh::upload_materials(s);
+    mh::hip_check(hipStreamSynchronize(s->stream), "upload");
+  }
+  mh::drop_graph(s);
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
+  MH_TRY
+  if (!s || !info) throw mh::Error(MH_ERR_ARG, "null argument");
+  std::memset(info, 0, sizeof(*info));
+  info->n_robots = s->n_robots;
+  info->n_masses = static_cast<int>(s->n_masses);
+  info->n_springs = static_cast<int>(s->n_springs);
+  info->n_active_springs = static_cast<int>(s->n_active);
+  if (s->plan.v) {
+    info->kernel = s->plan.v->kind;
+    info->block_size = s->plan.v->block;
+    info->n_blocks = static_cast<int>(s->plan.packs.size());
+    info->lds_bytes = static_cast<int>(s->plan.lds);
+    info->max_masses_per_block = s->plan.mass_cap;
+    info->max_springs_per_block = s->plan.spring_cap;
+  }
+  info->substeps_per_launch = s->substeps;
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_autotune(mh_sim* s, int n_steps) {
+  MH_TRY
+  if (!s) throw mh::Error(MH_ERR_ARG, "null sim");
+  if (!s->has_population) throw mh::Error(MH_ERR_STATE, "no population set");
+  if (s->n_robots == 0) return MH_OK;
+  if (n_steps < 1) n_steps = 1;
+  mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  // candidates run on a copy of the state; the population is restored after
+  std::vector<float4> p0(s->n_masses), v0(s->n_masses);
+  std::vector<float> st0(s->stress.n);
+  mh::hip_check(hipMemcpy(p0.data(), s->pos.p, p0.size() * sizeof(float4), hipMemcpyDeviceToHost), "D2H");
+  mh::hip_check(hipMemcpy(v0.data(), s->vel.p, v0.size() * sizeof(float4), hipMemcpyDeviceToHost), "D2H");
+  mh::hip_check(hipMemcpy(st0.data(), s->stress.p, st0.size() * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+  const float t_save = s->total_time;
+  const int use_graph = s->cfg.use_graph;
+  s->cfg.use_graph = 0;
+  const int saved_sub = s->substeps;
+  s->substeps = 0;
+  double best = std::numeric_limits<double>::infinity();
+  const mh::KernelVariant* best_v = nullptr;
+  for (const mh::KernelVariant* v : mh::candidates(s)) {
+    mh::Plan p;
+    if (!mh::make_plan(s, *v, p)) continue;
+    if (s->cfg.robots_per_block == 0 && v->kind == MH_KERNEL_LDS && p.lds > static_cast<size_t>(s->lds_limit))
+      continue;
+    mh::upload_plan(s, p);
+    mh::run_steps(s, std::max(1, n_steps / 8), 0);  // warm-up
+    mh::hip_check(hipStreamSynchronize(s->stream), "sync");
+    auto t0 = std::chrono::steady_clock::now();
+    mh::run_steps(s, n_steps, 0);
+    mh::hip_check(hipStreamSynchronize(s->stream), "sync");
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (dt < best) {
+      best = dt;
+      best_v = v;
+    }
+  }
+  if (!best_v) throw mh::Error(MH_ERR_RANGE, "no step kernel fits this population");
+  mh::Plan p;
+  mh::make_plan(s, *best_v, p);
+  mh::upload_plan(s, p);
+  // substeps per launch: the longest launch is cheapest; keep launches
+  // bounded to ~1k steps so a caller's synchronize stays responsive
+  s->substeps = saved_sub > 0 ? saved_sub : 1000;
+  s->cfg.use_graph = use_graph;
+  mh::hip_check(hipMemcpy(s->pos.p, p0.data(), p0.size() * sizeof(float4), hipMemcpyHostToDevice), "H2D");
+  mh::hip_check(hipMemcpy(s->vel.p, v0.data(), v0.size() * sizeof(float4), hipMemcpyHostToDevice), "H2D");
+  mh::hip_check(hipMemcpy(s->stress.p, st0.data(), st0.size() * sizeof(float), hipMemcpyHostToDevice), "H2D");
+  s->total_time = t_save;
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_device_state(mh_sim* s, void** pos4, void** vel4, void** stream) {
+  if (!s) return MH_ERR_ARG;
+  if (pos4) *pos4 = s->pos.p;
+  if (vel4) *vel4 = s->vel.p;
+  if (stream) *stream = s->stream;
+  return MH_OK;
+}
+
+int mh_sim_set_timing(mh_sim* s, int enable) {
+  if (!s) return MH_ERR_ARG;
+  s->timing = enable != 0;
+  s->have_timing = false;
+  return MH_OK;
+}
+
+int mh_sim_last_kernel_ms(const mh_sim* s, float* avg_ms, int* launches) {
+  MH_TRY
+  if (!s || !avg_ms) throw mh::Error(MH_ERR_ARG, "null argument");
+  *avg_ms = 0.f;
+  if (launches) *launches = 0;
+  if (!s->have_timing || s->last_launches == 0) return MH_OK;
+  mh::hip_check(hipEventSynchronize(s->ev1), "event sync");
+  float ms = 0.f;
+  mh::hip_check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
+  *avg_ms = ms / s->last_launches;
+  if (launches) *launches = s->last_launches;
+  return MH_OK;
+  MH_CATCH
+}
+
+}  // extern "C"

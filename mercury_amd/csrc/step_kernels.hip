@@ -1,0 +1,407 @@
+// step_kernels.hip -- fused persistent step kernels for gfx950.
+//
+// One workgroup owns one pack (a contiguous range of robots) for n_steps
+// substeps.  Every substep is
+//   phase 1 (one thread per spring): spring term from the two endpoint
+//     positions in LDS, written to a per-spring buffer;
+//   barrier;
+//   phase 2 (one thread per mass): gather of the mass's spring terms in
+//     ascending spring index (CSR list, sign bit = mass is the spring's
+//     second endpoint), integration, new position written to LDS; the next
+//     substep's actuation table is refreshed;
+//   barrier.
+// No atomics: the per-mass summation order is fixed, so results are
+// bit-reproducible and equal to the CPU mirror of the same step.
+//
+// Step semantics (reference sim_kernel.cu:141-269 for the XPBD solver;
+// force solver per the project spec written in the header of oracle/step.c):
+//   XPBD  x~ = x + v dt;  lambda = -(d - L)/(2 + ((1/k)/dt)/dt);
+//         dp = lambda * (x~a - x~b)/(d + 1e-12);  x' = x~ + sum(+-dp);
+//         v' = (float)(0.99 * (double)(x' - x) / (double)dt)
+//   force c = k (L - d) (xa - xb)/(d + 1e-12);  F = sum(+-c) - m g y^ - c_d v
+//         + ground normal/friction;  u = v + (F/m) dt;  x' = x + u dt;
+//         v' = damping * u
+//   L = fma(Lbar, A sin(fma(omega, t, phi)), Lbar)
+//
+// LDS kernel: masses (float4), spring buffer (3 x float), CSR (u16) and row
+// pointers (u16) resident in LDS; spring records (pair, Lbar, material,
+// stiffness) resident in VGPRs for the whole launch.  Streaming kernel:
+// only masses and the actuation table in LDS; spring records, CSR and the
+// spring buffer streamed from HBM/L2 every substep (robots too large for
+// the LDS kernel).
+#include <hip/hip_runtime.h>
+
+#include "mercury_hip.h"
+#include "mh_device.h"
+#include "mh_kernels.h"
+
+namespace mh {
+
+namespace {
+
+constexpr float kEps = 1e-12f;
+
+__device__ __forceinline__ float flip(float v, uint32_t sign_bit) {
+  return __uint_as_float(__float_as_uint(v) ^ sign_bit);
+}
+
+struct MassState {
+  float x, y, z;     // position
+  float vx, vy, vz;  // velocity
+  float tx, ty, tz;  // xpbd: predicted position x~
+  float m, inv_m, mg;
+};
+
+// Phase-2 update of one mass from its gathered sum (ax, ay, az).
+template <int SOLVER>
+__device__ __forceinline__ float4 integrate(MassState& s, float ax, float ay, float az,
+                                            const StepParams& p) {
+  if constexpr (SOLVER == MH_SOLVER_XPBD_REF) {
+    const float nx = s.tx + ax, ny = s.ty + ay, nz = s.tz + az;
+    const double ddt = static_cast<double>(p.dt);
+    s.vx = static_cast<float>((0.99 * static_cast<double>(nx - s.x)) / ddt);
+    s.vy = static_cast<float>((0.99 * static_cast<double>(ny - s.y)) / ddt);
+    s.vz = static_cast<float>((0.99 * static_cast<double>(nz - s.z)) / ddt);
+    s.x = nx;
+    s.y = ny;
+    s.z = nz;
+    s.tx = s.x + s.vx * p.dt;
+    s.ty = s.y + s.vy * p.dt;
+    s.tz = s.z + s.vz * p.dt;
+    return make_float4(s.tx, s.ty, s.tz, 0.f);
+  } else {
+    float fx = ax, fy = ay, fz = az;
+    float vx = s.vx, vy = s.vy, vz = s.vz;
+    fy = fy - s.mg;
+    if (p.drag != 0.0f) {
+      fx = fx - p.drag * vx;
+      fy = fy - p.drag * vy;
+      fz = fz - p.drag * vz;
+    }
+    if (p.ground_k != 0.0f && s.y < 0.0f) {
+      const float fn = p.ground_k * (-s.y);
+      fy = fy + fn;
+      const float fh = sqrtf(fx * fx + fz * fz);
+      if (fh <= p.mu_s * fn) {
+        fx = 0.0f;
+        fz = 0.0f;
+        vx = 0.0f;
+        vz = 0.0f;
+      } else {
+        const float sc = 1.0f - (p.mu_k * fn) / fh;
+        fx = fx * sc;
+        fz = fz * sc;
+      }
+    }
+    const float ux = vx + (fx * s.inv_m) * p.dt;
+    const float uy = vy + (fy * s.inv_m) * p.dt;
+    const float uz = vz + (fz * s.inv_m) * p.dt;
+    s.x = s.x + ux * p.dt;
+    s.y = s.y + uy * p.dt;
+    s.z = s.z + uz * p.dt;
+    s.vx = p.damping * ux;
+    s.vy = p.damping * uy;
+    s.vz = p.damping * uz;
+    return make_float4(s.x, s.y, s.z, 0.f);
+  }
+}
+
+__device__ __forceinline__ void load_mass(MassState& s, const float4* pos, const float4* vel,
+                                          uint32_t gi, const StepParams& p) {
+  const float4 q = pos[gi];
+  const float4 w = vel[gi];
+  s.x = q.x; s.y = q.y; s.z = q.z; s.m = q.w;
+  s.vx = w.x; s.vy = w.y; s.vz = w.z;
+  s.inv_m = 1.0f / s.m;
+  s.mg = s.m * p.gravity;
+  s.tx = s.x + s.vx * p.dt;
+  s.ty = s.y + s.vy * p.dt;
+  s.tz = s.z + s.vz * p.dt;
+}
+
+template <int SOLVER>
+__device__ __forceinline__ float4 lds_position(const MassState& s) {
+  if constexpr (SOLVER == MH_SOLVER_XPBD_REF) return make_float4(s.tx, s.ty, s.tz, 0.f);
+  return make_float4(s.x, s.y, s.z, 0.f);
+}
+
+// Spring term of one spring: returns (cx, cy, cz); xpbd also yields lambda.
+template <int SOLVER>
+__device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, float R,
+                                            float kk, float& cx, float& cy, float& cz,
+                                            float& lambda) {
+  const float L = fmaf(lbar, R, lbar);
+  const float dx = pa.x - pb.x;
+  const float dy = pa.y - pb.y;
+  const float dz = pa.z - pb.z;
+  const float d = norm3(dx, dy, dz);
+  const float den = d + kEps;
+  const float nx = dx / den, ny = dy / den, nz = dz / den;
+  if constexpr (SOLVER == MH_SOLVER_XPBD_REF) {
+    const float C = d - L;
+    lambda = -C / kk;
+    cx = lambda * nx;
+    cy = lambda * ny;
+    cz = lambda * nz;
+  } else {
+    const float f = kk * (L - d);
+    lambda = 0.f;
+    cx = f * nx;
+    cy = f * ny;
+    cz = f * nz;
+  }
+}
+
+// ----------------------------------------------------------------- LDS kernel
+template <int SOLVER, int BLOCK, int NSPT, int MPT>
+__global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PackDesc d = p.packs[blockIdx.x];
+  const int nm = static_cast<int>(d.n_mass);
+  const int ns = static_cast<int>(d.n_spring);
+  float4* s_pos = reinterpret_cast<float4*>(smem);
+  float* s_cx = reinterpret_cast<float*>(s_pos + p.mass_cap);
+  float* s_cy = s_cx + p.spring_cap;
+  float* s_cz = s_cy + p.spring_cap;
+  uint16_t* s_csr = reinterpret_cast<uint16_t*>(s_cz + p.spring_cap);
+  uint16_t* s_row = s_csr + 2 * p.spring_cap;
+  float* s_R = reinterpret_cast<float*>(s_row + ((p.mass_cap + 2) & ~1));
+  const int tid = threadIdx.x;
+
+  for (int i = tid; i < 2 * ns; i += BLOCK) s_csr[i] = static_cast<uint16_t>(p.csr[d.csr_begin + i]);
+  for (int i = tid; i <= nm; i += BLOCK) s_row[i] = static_cast<uint16_t>(p.row[d.row_begin + i]);
+
+  // spring records -> registers
+  uint32_t sp[NSPT];
+  float lb[NSPT], kk[NSPT], st[NSPT];
+  int mi[NSPT];
+#pragma unroll
+  for (int k = 0; k < NSPT; ++k) {
+    const int s = tid + k * BLOCK;
+    sp[k] = 0; lb[k] = 0.f; kk[k] = 1.f; st[k] = 0.f; mi[k] = 0;
+    if (s < ns) {
+      const uint32_t g = d.spring_begin + s;
+      sp[k] = p.spair[g];
+      lb[k] = p.slbar[g];
+      mi[k] = p.smat[g];
+      kk[k] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[mi[k]] : p.mat_k[mi[k]];
+      if (p.track_stress) st[k] = p.sstress[g];
+    }
+  }
+  // masses -> registers, first positions -> LDS
+  MassState ms[MPT];
+#pragma unroll
+  for (int j = 0; j < MPT; ++j) {
+    const int i = tid + j * BLOCK;
+    if (i < nm) {
+      load_mass(ms[j], p.pos, p.vel, d.mass_begin + i, p);
+      s_pos[i] = lds_position<SOLVER>(ms[j]);
+    }
+  }
+  float t = p.t0;
+  if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], t);
+  __syncthreads();
+
+  for (int step = 0; step < p.n_steps; ++step) {
+#pragma unroll
+    for (int k = 0; k < NSPT; ++k) {
+      const int s = tid + k * BLOCK;
+      if (s < ns) {
+        const float4 pa = s_pos[sp[k] & 0xffffu];
+        const float4 pb = s_pos[sp[k] >> 16];
+        float cx, cy, cz, lambda;
+        spring_term<SOLVER>(pa, pb, lb[k], s_R[mi[k]], kk[k], cx, cy, cz, lambda);
+        if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) st[k] = st[k] + lambda / lb[k];
+        s_cx[s] = cx;
+        s_cy[s] = cy;
+        s_cz[s] = cz;
+      }
+    }
+    __syncthreads();
+    const float tn = t + p.dt;
+#pragma unroll
+    for (int j = 0; j < MPT; ++j) {
+      const int i = tid + j * BLOCK;
+      if (i < nm) {
+        const int e1 = s_row[i + 1];
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        for (int e = s_row[i]; e < e1; ++e) {
+          const uint32_t c = s_csr[e];
+          const uint32_t s = c >> 1;
+          const uint32_t sgn = (c & 1u) << 31;
+          ax = ax + flip(s_cx[s], sgn);
+          ay = ay + flip(s_cy[s], sgn);
+          az = az + flip(s_cz[s], sgn);
+        }
+        s_pos[i] = integrate<SOLVER>(ms[j], ax, ay, az, p);
+      }
+    }
+    if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
+    t = tn;
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int j = 0; j < MPT; ++j) {
+    const int i = tid + j * BLOCK;
+    if (i < nm) {
+      const uint32_t gi = d.mass_begin + i;
+      p.pos[gi] = make_float4(ms[j].x, ms[j].y, ms[j].z, ms[j].m);
+      p.vel[gi] = make_float4(ms[j].vx, ms[j].vy, ms[j].vz, 0.f);
+    }
+  }
+  if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
+#pragma unroll
+    for (int k = 0; k < NSPT; ++k) {
+      const int s = tid + k * BLOCK;
+      if (s < ns) p.sstress[d.spring_begin + s] = st[k];
+    }
+  }
+}
+
+// ----------------------------------------------------------- streaming kernel
+template <int SOLVER, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PackDesc d = p.packs[blockIdx.x];
+  const int nm = static_cast<int>(d.n_mass);
+  const int ns = static_cast<int>(d.n_spring);
+  float4* s_pos = reinterpret_cast<float4*>(smem);
+  float* s_R = reinterpret_cast<float*>(s_pos + p.mass_cap);
+  const int tid = threadIdx.x;
+  const uint32_t* spair = p.spair + d.spring_begin;
+  const float* slbar = p.slbar + d.spring_begin;
+  const uint8_t* smat = p.smat + d.spring_begin;
+  float* sc = p.scratch + 3u * d.spring_begin;  // SoA: x | y | z, ns each
+  const uint32_t* csr = p.csr + d.csr_begin;
+  const uint32_t* row = p.row + d.row_begin;
+
+  // masses stay in the registers of their owner across substeps; the
+  // owner loop is strided so any mass count up to the LDS capacity works
+  for (int i = tid; i < nm; i += BLOCK) {
+    MassState s;
+    load_mass(s, p.pos, p.vel, d.mass_begin + i, p);
+    s_pos[i] = lds_position<SOLVER>(s);
+  }
+  float t = p.t0;
+  if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], t);
+  __syncthreads();
+
+  for (int step = 0; step < p.n_steps; ++step) {
+    for (int s = tid; s < ns; s += BLOCK) {
+      const uint32_t pr = spair[s];
+      const int m = smat[s];
+      const float kk = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[m] : p.mat_k[m];
+      float cx, cy, cz, lambda;
+      spring_term<SOLVER>(s_pos[pr & 0xffffu], s_pos[pr >> 16], slbar[s], s_R[m], kk, cx, cy,
+                          cz, lambda);
+      if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
+        float* stp = p.sstress + d.spring_begin + s;
+        *stp = *stp + lambda / slbar[s];
+      }
+      sc[s] = cx;
+      sc[ns + s] = cy;
+      sc[2 * ns + s] = cz;
+    }
+    __syncthreads();
+    const float tn = t + p.dt;
+    for (int i = tid; i < nm; i += BLOCK) {
+      MassState s;
+      // state lives in global memory between substeps for this kernel
+      const float4 q = p.pos[d.mass_begin + i];
+      const float4 w = p.vel[d.mass_begin + i];
+      s.x = q.x; s.y = q.y; s.z = q.z; s.m = q.w;
+      s.vx = w.x; s.vy = w.y; s.vz = w.z;
+      s.inv_m = 1.0f / s.m;
+      s.mg = s.m * p.gravity;
+      const float4 lp = s_pos[i];
+      s.tx = lp.x; s.ty = lp.y; s.tz = lp.z;
+      const int e1 = static_cast<int>(row[i + 1]);
+      float ax = 0.0f, ay = 0.0f, az = 0.0f;
+      for (int e = static_cast<int>(row[i]); e < e1; ++e) {
+        const uint32_t c = csr[e];
+        const uint32_t sidx = c >> 1;
+        const uint32_t sgn = (c & 1u) << 31;
+        ax = ax + flip(sc[sidx], sgn);
+        ay = ay + flip(sc[ns + sidx], sgn);
+        az = az + flip(sc[2 * ns + sidx], sgn);
+      }
+      s_pos[i] = integrate<SOLVER>(s, ax, ay, az, p);
+      p.pos[d.mass_begin + i] = make_float4(s.x, s.y, s.z, s.m);
+      p.vel[d.mass_begin + i] = make_float4(s.vx, s.vy, s.vz, 0.f);
+    }
+    if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
+    t = tn;
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ COM
+__global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint32_t* begin,
+                          int n_robots, float* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_robots) return;
+  float cx = 0.f, cy = 0.f, cz = 0.f, count = 0.f;
+  for (uint32_t i = begin[r]; i < begin[r + 1]; ++i) {
+    if (mass_id[i] == 0) continue;
+    const float4 q = pos[i];
+    const float den = count + 1.f;
+    cx = cx + (q.x - cx) / den;
+    cy = cy + (q.y - cy) / den;
+    cz = cz + (q.z - cz) / den;
+    count = den;
+  }
+  out[3 * r] = cx;
+  out[3 * r + 1] = cy;
+  out[3 * r + 2] = cz;
+}
+
+#define MH_LDS(SOLVER, B, S, M) \
+  {SOLVER, MH_KERNEL_LDS, B, S, M, reinterpret_cast<const void*>(&step_lds<SOLVER, B, S, M>)}
+#define MH_STREAM(SOLVER, B) \
+  {SOLVER, MH_KERNEL_STREAM, B, 0, 0, reinterpret_cast<const void*>(&step_stream<SOLVER, B>)}
+#define MH_LDS_SET(SOLVER)                                                         \
+  MH_LDS(SOLVER, 1024, 8, 1), MH_LDS(SOLVER, 1024, 8, 2), MH_LDS(SOLVER, 1024, 4, 1), \
+      MH_LDS(SOLVER, 512, 8, 2), MH_LDS(SOLVER, 512, 16, 2), \
+      MH_LDS(SOLVER, 512, 16, 4), MH_LDS(SOLVER, 256, 16, 4), MH_LDS(SOLVER, 256, 8, 2), \
+      MH_STREAM(SOLVER, 1024), MH_STREAM(SOLVER, 512), MH_STREAM(SOLVER, 256)
+
+const KernelVariant kVariants[] = {MH_LDS_SET(MH_SOLVER_XPBD_REF), MH_LDS_SET(MH_SOLVER_FORCE)};
+
+}  // namespace
+
+const KernelVariant* kernel_variants(int* count) {
+  *count = static_cast<int>(sizeof(kVariants) / sizeof(kVariants[0]));
+  return kVariants;
+}
+
+size_t lds_bytes_lds(int mass_cap, int spring_cap, int n_mat) {
+  size_t b = 16u * static_cast<size_t>(mass_cap);   // float4 positions
+  b += 12u * static_cast<size_t>(spring_cap);       // spring buffer x|y|z
+  b += 4u * static_cast<size_t>(spring_cap);        // CSR, 2 x u16 per spring
+  b += 2u * static_cast<size_t>((mass_cap + 2) & ~1);  // row pointers
+  b += 4u * static_cast<size_t>(n_mat);             // actuation table
+  return b;
+}
+
+size_t lds_bytes_stream(int mass_cap, int n_mat) {
+  return 16u * static_cast<size_t>(mass_cap) + 4u * static_cast<size_t>(n_mat);
+}
+
+hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes, hipStream_t stream,
+                       const StepParams& p) {
+  StepParams args = p;
+  void* kargs[] = {&args};
+  return hipLaunchKernel(v.fn, dim3(n_packs), dim3(v.block), kargs, lds_bytes, stream);
+}
+
+hipError_t launch_com(const float4* pos, const uint8_t* mass_id, const uint32_t* robot_mass_begin,
+                      int n_robots, float* com_out, hipStream_t stream) {
+  const int threads = 256;
+  const int blocks = (n_robots + threads - 1) / threads;
+  hipLaunchKernelGGL(robot_com, dim3(blocks), dim3(threads), 0, stream, pos, mass_id,
+                     robot_mass_begin, n_robots, com_out);
+  return hipGetLastError();
+}
+
+}  // namespace mh

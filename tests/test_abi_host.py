@@ -1,0 +1,114 @@
+"""C-ABI host functions (no GPU): exports, materials, voxel topology, baseline.
+
+The product library is checked against the golden vectors of the compiled
+reference (tests/golden) and against the oracle.
+"""
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN, REPO
+
+mh = pytest.importorskip("mercury_amd")
+
+
+def bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def header_functions():
+    names = []
+    for h in glob.glob(os.path.join(REPO, "include", "*.h")):
+        txt = open(h).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**(mh_[a-z0-9_]+)\s*\(",
+                            txt, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    lib = mh.load()
+    names = header_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_reference_material_tables():
+    g = np.load(os.path.join(GOLDEN, "materials_reference.npz"))
+    ms = mh.MaterialSet.reference()
+    assert ms.count() == 254
+    gid = g["by_id"]
+    for i in range(254):
+        kdwp, enc = ms.by_id(i)
+        assert np.array_equal(bits(kdwp), bits([gid["k"][i], gid["dL0"][i], gid["omega"][i],
+                                                gid["phi"][i]])), i
+        assert enc == gid["enc"][i]
+    for e, idx, mid in zip(g["dec_input"], g["dec_idx"], g["dec_id"]):
+        got_id, got_idx = ms.decode(int(e))
+        assert (got_id, got_idx) == (mid, idx), hex(int(e))
+
+
+def test_custom_material_set_matches_oracle():
+    rows = mh.materials.WORKLOAD_ROWS
+    ms = mh.workload_materials()
+    assert ms.count() == 11
+    by_id, by_enc, _ = oracle.composite_tables(oracle.make_base(rows))
+    for i in range(11):
+        kdwp, enc = ms.by_id(i)
+        want = [by_id[f][i] for f in ("k", "dL0", "omega", "phi")]
+        assert np.array_equal(bits(kdwp), bits(want)), i
+        assert enc == by_id["encoding"][i]
+    for e in range(1 << 5):
+        assert ms.decode(e)[1] == oracle.encoded_idx(e, 11)
+
+
+VOXEL_FILES = sorted(glob.glob(os.path.join(GOLDEN, "voxel_*.npz")))
+
+
+@pytest.mark.parametrize("path", VOXEL_FILES, ids=[os.path.basename(p) for p in VOXEL_FILES])
+def test_voxel_build_matches_reference(path):
+    g = np.load(path)
+    ms = mh.MaterialSet.reference()
+    e = mh.voxel_build(ms, int(g["n"]), float(g["res"]), g["voxel_enc"])
+    assert np.array_equal(bits(e.pos), bits(g["mass_pos"]))
+    assert np.array_equal(e.mass_material, g["mass_id"])
+    assert np.array_equal(e.pairs, g["pairs"])
+    assert np.array_equal(bits(e.mean_length), bits(g["mean_len"]))
+    assert np.array_equal(e.spring_material, g["spring_id"])
+    assert np.array_equal(e.spring_encoding, g["spring_enc"])
+    reset = mh.robots.rest_lengths(e.pos, e.pairs)
+    assert np.array_equal(bits(reset), bits(g["reset_mean_len"]))
+    com, ln = mh.robots.baseline(e.pos, e.mass_material)
+    assert np.array_equal(bits(com), bits(g["base_com"]))
+    assert bits(ln) == bits(g["length"])
+
+
+def test_voxel_build_workload_set_matches_oracle():
+    ms = mh.workload_materials()
+    _, _, by_enc_struct = oracle.composite_tables(oracle.make_base(mh.materials.WORKLOAD_ROWS))
+    rng = np.random.default_rng(7)
+    for trial in range(6):
+        n = int(rng.integers(3, 10))
+        enc = mh.robots.random_voxel_encodings(rng, n - 1, 4, p_empty=float(rng.uniform(0, 0.7)))
+        e = mh.voxel_build(ms, n, 10.0, enc)
+        o = oracle.voxel_build(n, 10.0, enc, by_enc_struct)
+        assert np.array_equal(bits(e.pos), bits(o["mass_pos"]))
+        assert np.array_equal(e.mass_material, o["mass_id"])
+        assert np.array_equal(e.pairs, o["pairs"])
+        assert np.array_equal(bits(e.mean_length), bits(o["length"]))
+        assert np.array_equal(e.spring_material, o["spring_id"])
+        assert np.array_equal(e.stripped_encoding, o["stripped_enc"])
+
+
+def test_voxel_build_rejects_bad_input():
+    ms = mh.workload_materials()
+    with pytest.raises(mh.MhError):
+        mh.voxel_build(ms, 3, 10.0, np.full(27, 1 << 9, np.uint32))  # bit outside the set
+    with pytest.raises(mh.MhError):
+        mh.voxel_build(ms, 3, 0.0, np.full(27, 2, np.uint32))       # resolution 0
