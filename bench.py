@@ -32,8 +32,8 @@ LAND = dict(gravity=9.81, ground_k=1.0e5, mu_s=1.0, mu_k=0.8, drag=0.0, damping=
 # write 12 B and gather read 12 B) + 48 B per mass (pos/vel read + write).
 LDS_BYTES_PER_SPRING = 57.0
 LDS_BYTES_PER_MASS = 48.0
-# MI355X: 256 CUs x 128 B/clk x 2.4 GHz LDS bandwidth
-LDS_PEAK_GBS = 256 * 128 * 2.4
+# MI355X nominal LDS bandwidth: 256 CUs x 128 B/clk x 2.4 GHz
+LDS_NOMINAL_GBS = 256 * 128 * 2.4
 
 
 def float_loop_steps(duration, dt):
@@ -99,7 +99,17 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-xpbd", action="store_true", help="skip the XPBD side measurement")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the known-size copy kernel (counter calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import ctypes
+        ms = ctypes.c_float()
+        mh._lib.check(mh.load().mh_calibrate_copy(0, args.calibrate_copy, 5, ctypes.byref(ms)))
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "avg_ms": ms.value,
+                          "GB/s": 2 * args.calibrate_copy / (ms.value * 1e-3) / 1e9}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
@@ -151,7 +161,12 @@ def main():
     updates = n_springs * n_sub * args.steps
     value = updates / secs
 
-    # roofline of the dominant (step) kernel from its average launch time
+    # roofline of the dominant (step) kernel from its average launch time;
+    # launches counts the launches of one simulate call (one step)
+    import ctypes
+    lds_peak = ctypes.c_float()
+    mh._lib.check(mh.load().mh_calibrate_lds(0, ctypes.byref(lds_peak)))
+    lds_peak = float(lds_peak.value)
     steps_per_launch = n_sub / max(launches, 1)
     lds_bytes = (LDS_BYTES_PER_SPRING * n_springs + LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
     achieved = lds_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
@@ -183,12 +198,14 @@ def main():
         "roofline": {
             "bound": "lds",
             "achieved": achieved,
-            "peak": LDS_PEAK_GBS,
+            "peak": lds_peak,
             "unit": "GB/s",
-            "frac": (achieved / LDS_PEAK_GBS) if achieved else None,
+            "frac": (achieved / lds_peak) if achieved else None,
             "traffic": None,
+            "peak_kind": "measured ds_read_b128 bandwidth (mh_calibrate_lds)",
+            "nominal_peak": LDS_NOMINAL_GBS,
             "kernel_ms_per_launch": kernel_ms,
-            "launches_per_step": launches / args.steps if launches else None,
+            "launches_per_step": launches,
             "algorithmic_bytes_per_spring_update":
                 LDS_BYTES_PER_SPRING + LDS_BYTES_PER_MASS * n_masses / n_springs,
         },

@@ -179,6 +179,14 @@ int mh_sim_device_state(mh_sim* sim, void** pos4, void** vel4,
 int mh_sim_set_timing(mh_sim* sim, int enable);
 int mh_sim_last_kernel_ms(const mh_sim* sim, float* avg_ms, int* launches);
 
+/* ------------------------------------------------ measurement utilities */
+/* No reference counterpart: figures for the roofline of the step kernels.
+ * LDS read bandwidth of the device (GB/s), from a ds_read_b128 kernel. */
+int mh_calibrate_lds(int device, float* gbytes_per_s);
+/* Average time (ms) of a plain float4 copy of `bytes` bytes, the known-size
+ * reference for scaling hardware byte counters. */
+int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,6 +89,8 @@ _SIGS = {
                                     ctypes.POINTER(vp)]),
     "mh_sim_set_timing": (c_int, [vp, c_int]),
     "mh_sim_last_kernel_ms": (c_int, [vp, f32p, ctypes.POINTER(c_int)]),
+    "mh_calibrate_lds": (c_int, [c_int, f32p]),
+    "mh_calibrate_copy": (c_int, [c_int, ctypes.c_size_t, c_int, f32p]),
 }
 
 _lib = None

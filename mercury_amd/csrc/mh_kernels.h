@@ -14,7 +14,7 @@ struct PackDesc {
   uint32_t n_mass;
   uint32_t spring_begin;  // first active spring
   uint32_t n_spring;
-  uint32_t csr_begin;     // first gather entry (2 per active spring)
+  uint32_t csr_begin;     // first gather entry
   uint32_t row_begin;     // first row pointer (n_mass + 1 entries)
   uint32_t pad0, pad1;
 };
@@ -27,9 +27,11 @@ struct StepParams {
   const float* slbar;
   const uint8_t* smat;       // compact material index
   float* sstress;            // per active spring (may be null)
-  const uint32_t* csr;       // (pack-local spring << 1) | (mass is endpoint b)
+  const uint32_t* csr;       // (pack-local spring << 1) | (mass is endpoint b);
+                             // LDS plans pad rows to 4 with the zero slot
   const uint32_t* row;       // pack-local gather offsets, n_mass + 1 per pack
-  float* scratch;            // streaming kernel: 3 floats per active spring
+  const uint16_t* mperm;     // per pack: phase-2 visiting order of its masses
+  float* scratch;            // streaming kernel: float4 per active spring
   const float* mat_k;        // spring constant
   const float* mat_kx;       // xpbd: 2 + ((1/k)/dt)/dt
   const float* mat_dL0;
@@ -38,6 +40,7 @@ struct StepParams {
   int n_mat;
   int mass_cap;              // LDS capacity in masses (layout stride)
   int spring_cap;            // LDS capacity in springs (layout stride)
+  int csr_cap;               // LDS capacity in gather entries (multiple of 4)
   int n_steps;
   float dt;
   float t0;
@@ -60,7 +63,7 @@ const KernelVariant* kernel_variants(int* count);
 
 // LDS bytes a pack of (masses, springs, materials) needs in the LDS kernel;
 // the streaming kernel needs only masses and the material table.
-size_t lds_bytes_lds(int mass_cap, int spring_cap, int n_mat);
+size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat);
 size_t lds_bytes_stream(int mass_cap, int n_mat);
 
 hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes,

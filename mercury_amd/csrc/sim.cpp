@@ -57,7 +57,7 @@ struct DevBuf {
 struct Plan {
   const KernelVariant* v = nullptr;
   std::vector<PackDesc> packs;      // launch order (largest work first)
-  int mass_cap = 0, spring_cap = 0;
+  int mass_cap = 0, spring_cap = 0, csr_cap = 0;
   size_t lds = 0;
 };
 
@@ -76,6 +76,7 @@ struct mh_sim {
   // population (host copies used for packing and collect)
   int n_robots = 0;
   std::vector<uint32_t> mass_off, spring_off, active_off;  // per robot prefix offsets
+  std::vector<uint32_t> gather_pad;  // per robot: sum over masses of deg rounded up to 4
   std::vector<uint16_t> act_pairs;   // active springs, robot-local
   std::vector<uint32_t> active_src;  // active spring -> original spring index
   std::vector<uint8_t> act_mat;      // compact material index
@@ -88,6 +89,7 @@ struct mh_sim {
   mh::DevBuf<float4> pos, vel;
   mh::DevBuf<float> lbar, stress, scratch, com;
   mh::DevBuf<uint8_t> smat, mass_id;
+  mh::DevBuf<uint16_t> mperm;
   mh::DevBuf<uint32_t> spair, csr, row, robot_begin;
   mh::DevBuf<mh::PackDesc> packs;
   mh::DevBuf<float> mat_k, mat_kx, mat_dL0, mat_omega, mat_phi;
@@ -179,25 +181,32 @@ bool make_plan(const mh_sim* s, const KernelVariant& v, Plan& plan) {
       PackDesc d{};
       d.mass_begin = s->mass_off[r];
       d.spring_begin = s->active_off[r];
-      int count = 0;
+      int count = 0, gathers = 0;
       while (r < s->n_robots && count < rpb) {
         const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
         const int sp = static_cast<int>(d.n_spring) + robot_active(s, r);
+        const int g = gathers + static_cast<int>(s->gather_pad[r]);
         const int new_mcap = std::max(plan.mass_cap, m), new_scap = std::max(plan.spring_cap, sp);
-        const bool fits = m <= mcap && sp <= scap && 2 * sp <= 65535 && m <= 65535 &&
-                          lds_bytes_lds(new_mcap, new_scap, n_mat) <= static_cast<size_t>(s->lds_limit);
+        const int new_ccap = std::max(plan.csr_cap, g);
+        // u16 LDS indices: spring slots (incl. sink and zero) < 32768, entries < 65536
+        const bool fits = m <= mcap && sp <= scap && sp + 2 <= 32767 && g <= 65535 && m <= 65535 &&
+                          lds_bytes_lds(new_mcap, new_scap, new_ccap, n_mat) <=
+                              static_cast<size_t>(s->lds_limit);
         if (!fits) break;
         d.n_mass = static_cast<uint32_t>(m);
         d.n_spring = static_cast<uint32_t>(sp);
+        gathers = g;
         r++;
         count++;
       }
       if (count == 0) return false;
       plan.mass_cap = std::max(plan.mass_cap, static_cast<int>(d.n_mass));
       plan.spring_cap = std::max(plan.spring_cap, static_cast<int>(d.n_spring));
+      plan.csr_cap = std::max(plan.csr_cap, gathers);
       plan.packs.push_back(d);
     }
-    plan.lds = lds_bytes_lds(std::max(plan.mass_cap, 1), std::max(plan.spring_cap, 1), n_mat);
+    plan.lds = lds_bytes_lds(std::max(plan.mass_cap, 1), std::max(plan.spring_cap, 1),
+                             plan.csr_cap, n_mat);
   }
   return true;
 }
@@ -206,11 +215,16 @@ bool make_plan(const mh_sim* s, const KernelVariant& v, Plan& plan) {
 // gather lists (ascending spring index per mass) and row pointers.
 void upload_plan(mh_sim* s, Plan& plan) {
   const size_t na = s->n_active;
-  std::vector<uint32_t> spair(na), csr(2 * na), row;
+  const bool padded = plan.v->kind == MH_KERNEL_LDS;
+  const uint32_t zero_entry = static_cast<uint32_t>(plan.spring_cap + 1) << 1;
+  std::vector<uint32_t> spair(na), csr, row;
+  csr.reserve(2 * na + 3 * s->n_masses);
+  std::vector<uint16_t> perm(s->n_masses);
+  std::vector<uint32_t> order;
   row.reserve(s->n_masses + plan.packs.size());
   std::vector<uint32_t> count;
   for (PackDesc& d : plan.packs) {
-    d.csr_begin = 2 * d.spring_begin;
+    d.csr_begin = static_cast<uint32_t>(csr.size());
     d.row_begin = static_cast<uint32_t>(row.size());
     const uint32_t nm = d.n_mass;
     count.assign(nm + 1, 0);
@@ -226,8 +240,20 @@ void upload_plan(mh_sim* s, Plan& plan) {
       count[a + 1]++;
       count[b + 1]++;
     }
-    for (uint32_t i = 0; i < nm; i++) count[i + 1] += count[i];
+    // phase-2 visiting order: longest gather lists first, so the lanes of
+    // a wave have similar trip counts (ties keep mass order)
+    order.resize(nm);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return count[x + 1] > count[y + 1]; });
+    for (uint32_t i = 0; i < nm; i++) perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
+    // row starts; LDS plans round every row up to a multiple of 4 entries
+    for (uint32_t i = 0; i < nm; i++) {
+      const uint32_t deg = count[i + 1];
+      count[i + 1] = count[i] + (padded ? (deg + 3u) & ~3u : deg);
+    }
     for (uint32_t i = 0; i <= nm; i++) row.push_back(count[i]);
+    csr.resize(d.csr_begin + count[nm], zero_entry);
     std::vector<uint32_t> fill(count.begin(), count.end() - 1);
     for (uint32_t k = 0; k < d.n_spring; k++) {
       const uint32_t pr = spair[d.spring_begin + k];
@@ -241,10 +267,11 @@ void upload_plan(mh_sim* s, Plan& plan) {
     return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
   });
   s->spair.upload(spair.data(), na, s->stream);
-  s->csr.upload(csr.data(), 2 * na, s->stream);
+  s->csr.upload(csr.data(), csr.size(), s->stream);
   s->row.upload(row.data(), row.size(), s->stream);
   s->packs.upload(plan.packs.data(), plan.packs.size(), s->stream);
-  if (plan.v->kind == MH_KERNEL_STREAM) s->scratch.alloc(3 * std::max<size_t>(na, 1));
+  s->mperm.upload(perm.data(), perm.size(), s->stream);
+  if (plan.v->kind == MH_KERNEL_STREAM) s->scratch.alloc(4 * std::max<size_t>(na, 1));
   if (plan.lds > 64 * 1024)
     (void)hipFuncSetAttribute(plan.v->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(plan.lds));
@@ -295,6 +322,7 @@ StepParams make_params(mh_sim* s, float t0, int n_steps, int track) {
   p.sstress = s->stress.p;
   p.csr = s->csr.p;
   p.row = s->row.p;
+  p.mperm = s->mperm.p;
   p.scratch = s->scratch.p;
   p.mat_k = s->mat_k.p;
   p.mat_kx = s->mat_kx.p;
@@ -304,6 +332,7 @@ StepParams make_params(mh_sim* s, float t0, int n_steps, int track) {
   p.n_mat = static_cast<int>(s->compact_to_id.size());
   p.mass_cap = s->plan.mass_cap;
   p.spring_cap = s->plan.spring_cap;
+  p.csr_cap = s->plan.csr_cap;
   p.n_steps = n_steps;
   p.dt = s->cfg.dt;
   p.t0 = t0;
@@ -527,6 +556,22 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   }
   if (compact_to_id.empty()) compact_to_id.push_back(0);  // keeps tables non-empty
 
+  // per robot: gather entries with every row rounded up to 4
+  std::vector<uint32_t> gpad(R, 0);
+  {
+    std::vector<uint32_t> deg;
+    for (int r = 0; r < R; r++) {
+      deg.assign(mo[r + 1] - mo[r], 0);
+      for (uint32_t k = ao[r]; k < ao[r + 1]; k++) {
+        deg[ap[2 * k]]++;
+        deg[ap[2 * k + 1]]++;
+      }
+      uint32_t g = 0;
+      for (uint32_t x : deg) g += (x + 3u) & ~3u;
+      gpad[r] = g;
+    }
+  }
+
   // masses: float4 (xyz, mass) and (v, 0)
   std::vector<float4> p4(M), v4(M);
   for (size_t i = 0; i < M; i++) {
@@ -548,6 +593,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   s->mass_off = mo;
   s->spring_off = so;
   s->active_off = ao;
+  s->gather_pad = std::move(gpad);
   s->act_pairs = std::move(ap);
   s->active_src = std::move(src);
   s->act_mat = std::move(amat);

@@ -41,10 +41,6 @@ namespace {
 
 constexpr float kEps = 1e-12f;
 
-__device__ __forceinline__ float flip(float v, uint32_t sign_bit) {
-  return __uint_as_float(__float_as_uint(v) ^ sign_bit);
-}
-
 struct MassState {
   float x, y, z;     // position
   float vx, vy, vz;  // velocity
@@ -153,49 +149,72 @@ __device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, fl
 }
 
 // ----------------------------------------------------------------- LDS kernel
+// LDS layout: positions float4[mass_cap] | spring terms xy float2[spring_cap
+// + 2] | gather entries u16[csr_cap] | spring terms z float[spring_cap + 2] |
+// row starts u16[mass_cap + 1, even] | actuation float[n_mat].  Slot
+// spring_cap is the sink of lanes without a spring, slot spring_cap + 1
+// holds zeros and pads every gather row to a multiple of 4 entries (adding
+// +0 leaves a sum that started at +0 unchanged).  Springs are dealt to
+// waves in contiguous runs of whole 64-lane iterations, so every wave runs
+// ceil(springs / BLOCK) iterations.  Phase-2 lanes visit masses in the
+// order of p.mperm (longest gather rows first).
 template <int SOLVER, int BLOCK, int NSPT, int MPT>
 __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
+  constexpr int kWaves = BLOCK / 64;
   extern __shared__ __align__(16) unsigned char smem[];
   const PackDesc d = p.packs[blockIdx.x];
   const int nm = static_cast<int>(d.n_mass);
   const int ns = static_cast<int>(d.n_spring);
+  const int sink = p.spring_cap, zero = p.spring_cap + 1;
   float4* s_pos = reinterpret_cast<float4*>(smem);
-  float* s_cx = reinterpret_cast<float*>(s_pos + p.mass_cap);
-  float* s_cy = s_cx + p.spring_cap;
-  float* s_cz = s_cy + p.spring_cap;
-  uint16_t* s_csr = reinterpret_cast<uint16_t*>(s_cz + p.spring_cap);
-  uint16_t* s_row = s_csr + 2 * p.spring_cap;
+  float2* s_cxy = reinterpret_cast<float2*>(s_pos + p.mass_cap);
+  uint16_t* s_csr = reinterpret_cast<uint16_t*>(s_cxy + p.spring_cap + 2);
+  float* s_cz = reinterpret_cast<float*>(s_csr + p.csr_cap);
+  uint16_t* s_row = reinterpret_cast<uint16_t*>(s_cz + p.spring_cap + 2);
   float* s_R = reinterpret_cast<float*>(s_row + ((p.mass_cap + 2) & ~1));
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int iters = (ns + 64 * kWaves - 1) / (64 * kWaves);  // <= NSPT
+  const int wave_first = wave * iters * 64;
 
-  for (int i = tid; i < 2 * ns; i += BLOCK) s_csr[i] = static_cast<uint16_t>(p.csr[d.csr_begin + i]);
+  const int n_csr = static_cast<int>(p.row[d.row_begin + nm]);
+  for (int i = tid; i < n_csr; i += BLOCK) s_csr[i] = static_cast<uint16_t>(p.csr[d.csr_begin + i]);
   for (int i = tid; i <= nm; i += BLOCK) s_row[i] = static_cast<uint16_t>(p.row[d.row_begin + i]);
+  if (tid == 0) {
+    s_cxy[zero] = make_float2(0.f, 0.f);
+    s_cz[zero] = 0.f;
+  }
 
-  // spring records -> registers
+  // spring records -> registers; lanes past the pack's springs compute a
+  // dummy spring (0,0) whose term goes to the sink slot
   uint32_t sp[NSPT];
   float lb[NSPT], kk[NSPT], st[NSPT];
-  int mi[NSPT];
+  int mi[NSPT], slot[NSPT];
 #pragma unroll
   for (int k = 0; k < NSPT; ++k) {
-    const int s = tid + k * BLOCK;
-    sp[k] = 0; lb[k] = 0.f; kk[k] = 1.f; st[k] = 0.f; mi[k] = 0;
-    if (s < ns) {
+    const int s = wave_first + k * 64 + lane;
+    sp[k] = 0; lb[k] = 1.f; kk[k] = 1.f; st[k] = 0.f; mi[k] = 0; slot[k] = sink;
+    if (k < iters && s < ns) {
       const uint32_t g = d.spring_begin + s;
       sp[k] = p.spair[g];
       lb[k] = p.slbar[g];
       mi[k] = p.smat[g];
       kk[k] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[mi[k]] : p.mat_k[mi[k]];
       if (p.track_stress) st[k] = p.sstress[g];
+      slot[k] = s;
     }
   }
   // masses -> registers, first positions -> LDS
   MassState ms[MPT];
+  int mid[MPT];
 #pragma unroll
   for (int j = 0; j < MPT; ++j) {
     const int i = tid + j * BLOCK;
+    mid[j] = -1;
     if (i < nm) {
-      load_mass(ms[j], p.pos, p.vel, d.mass_begin + i, p);
-      s_pos[i] = lds_position<SOLVER>(ms[j]);
+      mid[j] = p.mperm[d.mass_begin + i];
+      load_mass(ms[j], p.pos, p.vel, d.mass_begin + mid[j], p);
+      s_pos[mid[j]] = lds_position<SOLVER>(ms[j]);
     }
   }
   float t = p.t0;
@@ -205,33 +224,40 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
   for (int step = 0; step < p.n_steps; ++step) {
 #pragma unroll
     for (int k = 0; k < NSPT; ++k) {
-      const int s = tid + k * BLOCK;
-      if (s < ns) {
+      if (k < iters) {
         const float4 pa = s_pos[sp[k] & 0xffffu];
         const float4 pb = s_pos[sp[k] >> 16];
         float cx, cy, cz, lambda;
         spring_term<SOLVER>(pa, pb, lb[k], s_R[mi[k]], kk[k], cx, cy, cz, lambda);
         if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) st[k] = st[k] + lambda / lb[k];
-        s_cx[s] = cx;
-        s_cy[s] = cy;
-        s_cz[s] = cz;
+        s_cxy[slot[k]] = make_float2(cx, cy);
+        s_cz[slot[k]] = cz;
       }
     }
     __syncthreads();
     const float tn = t + p.dt;
 #pragma unroll
     for (int j = 0; j < MPT; ++j) {
-      const int i = tid + j * BLOCK;
-      if (i < nm) {
+      if (mid[j] >= 0) {
+        const int i = mid[j];
         const int e1 = s_row[i + 1];
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        for (int e = s_row[i]; e < e1; ++e) {
-          const uint32_t c = s_csr[e];
-          const uint32_t s = c >> 1;
-          const uint32_t sgn = (c & 1u) << 31;
-          ax = ax + flip(s_cx[s], sgn);
-          ay = ay + flip(s_cy[s], sgn);
-          az = az + flip(s_cz[s], sgn);
+        // ascending spring order, four gathers in flight; the sign is an
+        // exact multiply inside the fma, so each step is one rounding of
+        // acc +- term
+        for (int e = s_row[i]; e < e1; e += 4) {
+          const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
+          const uint32_t c0 = w.x & 0xffffu, c1 = w.x >> 16, c2 = w.y & 0xffffu, c3 = w.y >> 16;
+          const float2 q0 = s_cxy[c0 >> 1], q1 = s_cxy[c1 >> 1], q2 = s_cxy[c2 >> 1],
+                       q3 = s_cxy[c3 >> 1];
+          const float z0 = s_cz[c0 >> 1], z1 = s_cz[c1 >> 1], z2 = s_cz[c2 >> 1],
+                      z3 = s_cz[c3 >> 1];
+          const float g0 = (c0 & 1u) ? -1.0f : 1.0f, g1 = (c1 & 1u) ? -1.0f : 1.0f,
+                      g2 = (c2 & 1u) ? -1.0f : 1.0f, g3 = (c3 & 1u) ? -1.0f : 1.0f;
+          ax = fmaf(g0, q0.x, ax); ay = fmaf(g0, q0.y, ay); az = fmaf(g0, z0, az);
+          ax = fmaf(g1, q1.x, ax); ay = fmaf(g1, q1.y, ay); az = fmaf(g1, z1, az);
+          ax = fmaf(g2, q2.x, ax); ay = fmaf(g2, q2.y, ay); az = fmaf(g2, z2, az);
+          ax = fmaf(g3, q3.x, ax); ay = fmaf(g3, q3.y, ay); az = fmaf(g3, z3, az);
         }
         s_pos[i] = integrate<SOLVER>(ms[j], ax, ay, az, p);
       }
@@ -243,9 +269,8 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
 
 #pragma unroll
   for (int j = 0; j < MPT; ++j) {
-    const int i = tid + j * BLOCK;
-    if (i < nm) {
-      const uint32_t gi = d.mass_begin + i;
+    if (mid[j] >= 0) {
+      const uint32_t gi = d.mass_begin + mid[j];
       p.pos[gi] = make_float4(ms[j].x, ms[j].y, ms[j].z, ms[j].m);
       p.vel[gi] = make_float4(ms[j].vx, ms[j].vy, ms[j].vz, 0.f);
     }
@@ -253,8 +278,8 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
   if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
 #pragma unroll
     for (int k = 0; k < NSPT; ++k) {
-      const int s = tid + k * BLOCK;
-      if (s < ns) p.sstress[d.spring_begin + s] = st[k];
+      const int s = wave_first + k * 64 + lane;
+      if (k < iters && s < ns) p.sstress[d.spring_begin + s] = st[k];
     }
   }
 }
@@ -272,7 +297,7 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
   const uint32_t* spair = p.spair + d.spring_begin;
   const float* slbar = p.slbar + d.spring_begin;
   const uint8_t* smat = p.smat + d.spring_begin;
-  float* sc = p.scratch + 3u * d.spring_begin;  // SoA: x | y | z, ns each
+  float4* sc = reinterpret_cast<float4*>(p.scratch) + d.spring_begin;
   const uint32_t* csr = p.csr + d.csr_begin;
   const uint32_t* row = p.row + d.row_begin;
 
@@ -299,9 +324,7 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
         float* stp = p.sstress + d.spring_begin + s;
         *stp = *stp + lambda / slbar[s];
       }
-      sc[s] = cx;
-      sc[ns + s] = cy;
-      sc[2 * ns + s] = cz;
+      sc[s] = make_float4(cx, cy, cz, 0.f);
     }
     __syncthreads();
     const float tn = t + p.dt;
@@ -320,11 +343,11 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
       float ax = 0.0f, ay = 0.0f, az = 0.0f;
       for (int e = static_cast<int>(row[i]); e < e1; ++e) {
         const uint32_t c = csr[e];
-        const uint32_t sidx = c >> 1;
-        const uint32_t sgn = (c & 1u) << 31;
-        ax = ax + flip(sc[sidx], sgn);
-        ay = ay + flip(sc[ns + sidx], sgn);
-        az = az + flip(sc[2 * ns + sidx], sgn);
+        const float4 q = sc[c >> 1];
+        const float sg = (c & 1u) ? -1.0f : 1.0f;
+        ax = fmaf(sg, q.x, ax);
+        ay = fmaf(sg, q.y, ay);
+        az = fmaf(sg, q.z, az);
       }
       s_pos[i] = integrate<SOLVER>(s, ax, ay, az, p);
       p.pos[d.mass_begin + i] = make_float4(s.x, s.y, s.z, s.m);
@@ -375,12 +398,12 @@ const KernelVariant* kernel_variants(int* count) {
   return kVariants;
 }
 
-size_t lds_bytes_lds(int mass_cap, int spring_cap, int n_mat) {
-  size_t b = 16u * static_cast<size_t>(mass_cap);   // float4 positions
-  b += 12u * static_cast<size_t>(spring_cap);       // spring buffer x|y|z
-  b += 4u * static_cast<size_t>(spring_cap);        // CSR, 2 x u16 per spring
-  b += 2u * static_cast<size_t>((mass_cap + 2) & ~1);  // row pointers
-  b += 4u * static_cast<size_t>(n_mat);             // actuation table
+size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat) {
+  size_t b = 16u * static_cast<size_t>(mass_cap);       // float4 positions
+  b += 12u * static_cast<size_t>(spring_cap + 2);       // spring terms xy | z, sink, zero
+  b += 2u * static_cast<size_t>(csr_cap);               // gather entries (u16)
+  b += 2u * static_cast<size_t>((mass_cap + 2) & ~1);   // row starts
+  b += 4u * static_cast<size_t>(n_mat);                 // actuation table
   return b;
 }
 

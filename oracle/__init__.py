@@ -250,3 +250,61 @@ def knn(pos, K):
     rc = lib().orc_knn(_p(pos, f32p), n, K, _p(nbr, u16p), _p(dist, f32p))
     assert rc == 0
     return nbr.reshape(n, K), dist.reshape(n, K)
+
+
+# ------------------------------------------------------------------ surrogate
+def _dpp(arrs):
+    t = (f64p * len(arrs))(*[a.ctypes.data_as(f64p) for a in arrs])
+    return t
+
+
+def _mlp_sig(L):
+    if getattr(L, "_mlp_sig", False):
+        return
+    dpp = ctypes.POINTER(f64p)
+    L.orc_mlp_forward.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), dpp, dpp,
+                                  ctypes.c_int, f64p, f64p]
+    L.orc_mlp_loss_grad.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), dpp, dpp,
+                                    ctypes.c_int, f64p, f64p, dpp, dpp]
+    L.orc_mlp_loss_grad.restype = ctypes.c_double
+    L.orc_adam.argtypes = [ctypes.c_int, f64p, f64p, f64p, f64p, ctypes.c_double,
+                           ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L._mlp_sig = True
+
+
+def mlp_forward(W, b, x):
+    """W: list of [out x in] arrays, b: list of [out]; GELU hidden layers."""
+    L = lib()
+    _mlp_sig(L)
+    W = [np.ascontiguousarray(w, np.float64) for w in W]
+    b = [np.ascontiguousarray(v, np.float64) for v in b]
+    sizes = (ctypes.c_int * (len(W) + 1))(*([W[0].shape[1]] + [w.shape[0] for w in W]))
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.zeros((x.shape[0], W[-1].shape[0]), np.float64)
+    L.orc_mlp_forward(len(W), sizes, _dpp(W), _dpp(b), x.shape[0], x.ctypes.data_as(f64p),
+                      y.ctypes.data_as(f64p))
+    return y
+
+
+def mlp_loss_grad(W, b, x, target):
+    L = lib()
+    _mlp_sig(L)
+    W = [np.ascontiguousarray(w, np.float64) for w in W]
+    b = [np.ascontiguousarray(v, np.float64) for v in b]
+    sizes = (ctypes.c_int * (len(W) + 1))(*([W[0].shape[1]] + [w.shape[0] for w in W]))
+    x = np.ascontiguousarray(x, np.float64)
+    t = np.ascontiguousarray(target, np.float64).reshape(x.shape[0], -1)
+    gW = [np.zeros_like(w) for w in W]
+    gb = [np.zeros_like(v) for v in b]
+    loss = L.orc_mlp_loss_grad(len(W), sizes, _dpp(W), _dpp(b), x.shape[0],
+                               x.ctypes.data_as(f64p), t.ctypes.data_as(f64p), _dpp(gW), _dpp(gb))
+    return loss, gW, gb
+
+
+def adam(p, g, m, v, lr, beta1, beta2, eps, t):
+    L = lib()
+    _mlp_sig(L)
+    for a in (p, g, m, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    L.orc_adam(len(p), p.ctypes.data_as(f64p), g.ctypes.data_as(f64p), m.ctypes.data_as(f64p),
+               v.ctypes.data_as(f64p), lr, beta1, beta2, eps, t)
