@@ -27,6 +27,11 @@ sys.path.insert(0, REPO)
 import mercury_amd as mh  # noqa: E402
 
 LAND = dict(gravity=9.81, ground_k=1.0e5, mu_s=1.0, mu_k=0.8, drag=0.0, damping=0.99)
+WATER = dict(gravity=0.0, ground_k=0.0, mu_s=0.0, mu_k=0.0, drag=1.0, damping=0.99)
+# streaming kernel, HBM bytes per spring-update: record 9 B + term write
+# 16 B + two gathers of (entry 4 B + term 16 B)
+STREAM_BYTES_PER_SPRING = 65.0
+HBM_PEAK_GBS = 8000.0
 # LDS bytes per spring-update of the LDS-resident kernel, per robot size:
 # 57 B per spring (two endpoint positions 24 B, spring record 9 B, result
 # write 12 B and gather read 12 B) + 48 B per mass (pos/vel read + write).
@@ -54,7 +59,7 @@ def cpu_threads():
         return max(1, min(16, os.cpu_count() or 1))
 
 
-def cpu_baseline(els, solver, dt, steps, threads):
+def cpu_baseline(els, solver, dt, steps, threads, env_kw):
     """The oracle's FP32 mirror (plain C, -O3, no fast-math) on host cores."""
     import oracle
     from mercury_amd.simulator import pack
@@ -67,7 +72,7 @@ def cpu_baseline(els, solver, dt, steps, threads):
     mt = oracle.MaterialTable(by_id["k"], by_id["dL0"], by_id["omega"], by_id["phi"])
     pop = oracle.Population(a["mass_offset"], a["spring_offset"], a["mass"], a["pairs"],
                             a["mean_length"], a["spring_material"])
-    env = oracle.make_env(dt, **LAND)
+    env = oracle.make_env(dt, **env_kw)
 
     def run(n, th):
         t = time.perf_counter()
@@ -88,7 +93,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3, help="timed 1 s evaluations")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--robots", type=int, default=512)
+    ap.add_argument("--workload", choices=["voxel", "nn"], default="voxel",
+                    help="voxel: 512 8^3 voxel robots on land; nn: 1024 uneven KNN robots in water")
+    ap.add_argument("--robots", type=int, default=0, help="0 = the workload's count")
     ap.add_argument("--voxels", type=int, default=8)
     ap.add_argument("--duration", type=float, default=1.0)
     ap.add_argument("--dt", type=float, default=1e-4)
@@ -120,8 +127,24 @@ def main():
     solver = mh.MH_SOLVER_FORCE if args.solver == "force" else mh.MH_SOLVER_XPBD_REF
     ms = mh.workload_materials()
     t_build = time.perf_counter()
-    els = mh.voxel_population(ms, args.robots, voxels_per_axis=args.voxels, p_empty=0.2,
-                              seed=args.seed, mass=0.001)
+    if args.workload == "voxel":
+        args.robots = args.robots or 512
+        env_kw = LAND
+        els = mh.voxel_population(ms, args.robots, voxels_per_axis=args.voxels, p_empty=0.2,
+                                  seed=args.seed, mass=0.001)
+        metric = f"spring-updates/s ({args.robots}x{args.voxels}^3 voxel robots)"
+        workload = (f"{args.robots} voxel robots, {args.voxels}^3 voxels ({els[0].n_masses} masses, "
+                    f"{els[0].n_springs} springs each), {args.solver} solver, land, dt={args.dt}, "
+                    f"{args.duration} s evaluation")
+    else:
+        args.robots = args.robots or 1024
+        env_kw = WATER
+        els = mh.robots.nn_population(ms, args.robots, mass_range=(200, 2000), K=25,
+                                      seed=args.seed, mass=0.01)
+        metric = f"spring-updates/s ({args.robots} uneven NN robots, water)"
+        workload = (f"{args.robots} NN robots, masses U[200,2000], K=25 nearest-neighbour springs, "
+                    f"MLP 3-25-25-4 materials, {args.solver} solver, water drag 1.0, dt={args.dt}, "
+                    f"{args.duration} s evaluation")
     t_build = time.perf_counter() - t_build
     n_springs = sum(e.n_springs for e in els)
     n_masses = sum(e.n_masses for e in els)
@@ -130,7 +153,7 @@ def main():
 
     def make_sim(slv):
         cfg = mh.SimConfig(solver=slv, time_step=args.dt, substeps_per_launch=args.substeps,
-                           use_graph=1, **LAND)
+                           use_graph=1, **env_kw)
         sim = mh.Simulator(ms, cfg)
         t = time.perf_counter()
         sim.SetElements(els)
@@ -168,11 +191,18 @@ def main():
     mh._lib.check(mh.load().mh_calibrate_lds(0, ctypes.byref(lds_peak)))
     lds_peak = float(lds_peak.value)
     steps_per_launch = n_sub / max(launches, 1)
-    lds_bytes = (LDS_BYTES_PER_SPRING * n_springs + LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
-    achieved = lds_bytes / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
+    if info.kernel == 1:  # LDS kernel only
+        bound, peak, unit = "lds", lds_peak, "GB/s"
+        per_launch = (LDS_BYTES_PER_SPRING * n_springs + LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
+        per_unit = LDS_BYTES_PER_SPRING + LDS_BYTES_PER_MASS * n_masses / n_springs
+    else:  # the streaming kernel carries the robots over the LDS budget
+        bound, peak, unit = "hbm", HBM_PEAK_GBS, "GB/s"
+        per_launch = STREAM_BYTES_PER_SPRING * info.n_stream_springs * steps_per_launch
+        per_unit = STREAM_BYTES_PER_SPRING
+    achieved = per_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
 
     out = {
-        "metric": "spring-updates/s (512x8^3 voxel robots)",
+        "metric": metric,
         "value": value,
         "unit": "spring-updates/s",
         "n_gpus": 1,
@@ -185,42 +215,41 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded voxel materials)",
         "config": {
-            "workload": f"{args.robots} voxel robots, {args.voxels}^3 voxels "
-                        f"({els[0].n_masses} masses, {els[0].n_springs} springs each), "
-                        f"{args.solver} solver, land, dt={args.dt}, {args.duration} s evaluation",
+            "workload": workload,
             "robots": args.robots, "springs_total": n_springs, "active_springs": n_active,
             "masses_total": n_masses, "substeps_per_eval": n_sub, "solver": args.solver,
-            "kernel": {1: "lds", 2: "stream"}.get(info.kernel, "?"), "block": info.block_size,
+            "kernel": {1: "lds", 2: "stream", 3: "lds+stream"}.get(info.kernel, "?"), "block": info.block_size,
             "blocks": info.n_blocks, "lds_bytes_per_block": info.lds_bytes,
             "substeps_per_launch": info.substeps_per_launch, "hipgraph": True,
+            "lds_robots": info.n_lds_robots, "stream_robots": info.n_stream_robots,
+            "robots_per_block": info.robots_per_block,
         },
         "active_spring_updates_per_s": n_active * n_sub * args.steps / secs,
         "roofline": {
-            "bound": "lds",
+            "bound": bound,
             "achieved": achieved,
-            "peak": lds_peak,
-            "unit": "GB/s",
-            "frac": (achieved / lds_peak) if achieved else None,
+            "peak": peak,
+            "unit": unit,
+            "frac": (achieved / peak) if achieved else None,
             "traffic": None,
-            "peak_kind": "measured ds_read_b128 bandwidth (mh_calibrate_lds)",
-            "nominal_peak": LDS_NOMINAL_GBS,
+            "peak_kind": ("measured ds_read_b128 bandwidth (mh_calibrate_lds); nominal "
+                          f"{LDS_NOMINAL_GBS:.0f}") if bound == "lds" else "MI355X HBM3E",
             "kernel_ms_per_launch": kernel_ms,
             "launches_per_step": launches,
-            "algorithmic_bytes_per_spring_update":
-                LDS_BYTES_PER_SPRING + LDS_BYTES_PER_MASS * n_masses / n_springs,
+            "algorithmic_bytes_per_spring_update": per_unit,
         },
         "host_seconds": {"topology_build": t_build, "upload": t_upload},
     }
-    if not args.no_xpbd and solver == mh.MH_SOLVER_FORCE:
+    if not args.no_xpbd and solver == mh.MH_SOLVER_FORCE and args.workload == "voxel":
         sx, _ = make_sim(mh.MH_SOLVER_XPBD_REF)
         xs, xkms, _ = timed(sx, 1, 1)
         out["xpbd_ref_spring_updates_per_s"] = n_springs * n_sub / xs
         sx.close()
     if not args.no_cpu:
         th = cpu_threads()
-        v_all, s_all = cpu_baseline(els, solver, args.dt, args.cpu_steps, th)
+        v_all, s_all = cpu_baseline(els, solver, args.dt, args.cpu_steps, th, env_kw)
         sub = els[:32]
-        v_one, s_one = cpu_baseline(sub, solver, args.dt, max(50, args.cpu_steps // 8), 1)
+        v_one, s_one = cpu_baseline(sub, solver, args.dt, max(50, args.cpu_steps // 8), 1, env_kw)
         out["cpu_baseline"] = {
             "value": v_all, "unit": "spring-updates/s", "cores": th, "kind": "port",
             "sample": f"oracle FP32 mirror, same {args.robots} robots, {args.cpu_steps} substeps "

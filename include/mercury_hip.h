@@ -161,15 +161,21 @@ int mh_sim_reset(mh_sim* sim);               /* simulated time := 0 */
 float mh_sim_total_time(const mh_sim* sim);
 float mh_sim_get_time_step(const mh_sim* sim);
 int mh_sim_set_time_step(mh_sim* sim, float dt);
-/* Population counts and the launch plan chosen for it. */
+/* Population counts and the launch plan chosen for it.  kernel is a bit
+ * set of MH_KERNEL_LDS / MH_KERNEL_STREAM (robots over the LDS budget run
+ * in the streaming kernel, concurrently with the LDS kernel's robots). */
 typedef struct {
   int n_robots, n_masses, n_springs, n_active_springs;
   int kernel, block_size, n_blocks, substeps_per_launch, lds_bytes;
   int max_masses_per_block, max_springs_per_block;
+  int n_lds_robots, n_stream_robots, stream_block_size, robots_per_block;
+  int n_stream_springs;  /* active springs handled by the streaming kernel */
 } mh_sim_info;
 int mh_sim_get_info(const mh_sim* sim, mh_sim_info* info);
-/* Times the candidate launch plans on the current population for n_steps
- * each and keeps the fastest (results do not depend on the choice). */
+/* Times the candidate launch plans (LDS variant x robots per block, then
+ * streaming variant) on the current population for n_steps each and keeps
+ * the fastest; results do not depend on the choice.  The choice is cached
+ * per (solver, device, histogram of robot sizes) for the process. */
 int mh_sim_autotune(mh_sim* sim, int n_steps);
 /* Device pointers of the state (float4 per mass: xyz + mass / xyz + 0). */
 int mh_sim_device_state(mh_sim* sim, void** pos4, void** vel4,
@@ -178,6 +184,57 @@ int mh_sim_device_state(mh_sim* sim, void** pos4, void** vel4,
  * measured with HIP events on the simulator's stream; 0 if not timed. */
 int mh_sim_set_timing(mh_sim* sim, int enable);
 int mh_sim_last_kernel_ms(const mh_sim* sim, float* avg_ms, int* launches);
+
+/* ------------------------------------------------------------ NN robots */
+/* K nearest neighbours of every mass of every robot, on the device
+ * (Triangulation::KNN / KNN_CPU, knn.cu:62-190): nbr/dist are
+ * [total masses x K], ranked by (distance, index); a zero distance (the mass
+ * itself or a coincident mass) counts as +inf.  1 <= K <= 32, every
+ * non-empty robot needs more than K masses. */
+int mh_knn_batch(int device, int n_robots, const uint32_t* mass_offset,
+                 const float* pos, int K, uint16_t* nbr, float* dist);
+/* Springs of one robot from its KNN rows (NNRobot::Build,
+ * NNRobot.cpp:282-301): edge (i, nbr[i][r]) for every rank r, skipped when
+ * shorter than min_length or non-finite; material decode(enc_i | enc_j). */
+int mh_knn_springs(const mh_material_set* set, int n, int K,
+                   const uint16_t* nbr, const float* dist,
+                   const uint32_t* mass_encoding, float min_length,
+                   uint16_t* pairs, float* length, uint8_t* material_id,
+                   int max_springs, int* n_springs);
+/* Material index per mass from a bias-free MLP on its position
+ * (NNRobot::forward, NNRobot.h:41-92): ReLU hidden layers, index of the
+ * largest output (first maximum).  weights: row-major [out x in] per layer,
+ * concatenated; sizes[0] must be 3. */
+int mh_nn_materials(int n_layers, const int* sizes, const float* weights,
+                    const float* pos, int n, uint8_t* out_index);
+
+/* ------------------------------------------------------- surrogate MLP */
+/* No reference counterpart: the network that pre-screens offspring before
+ * the optimizer's evaluate call (Evaluator_impl.h:39-107) simulates them.
+ * Fully connected sizes[0] -> ... -> sizes[n_layers]; GELU (tanh form) on
+ * hidden layers, linear output.  Weights are bf16 on the device with an
+ * fp32 master copy; products accumulate in fp32; He-uniform init from seed. */
+typedef struct mh_mlp mh_mlp;
+int mh_mlp_create(int device, int n_layers, const int* sizes, uint64_t seed,
+                  mh_mlp** out);
+int mh_mlp_destroy(mh_mlp* net);
+int mh_mlp_param_count(const mh_mlp* net);
+/* fp32 master parameters, layer by layer: W_l (row-major [out x in]) then
+ * b_l.  set_params also resets the Adam moments. */
+int mh_mlp_get_params(const mh_mlp* net, float* out);
+int mh_mlp_set_params(mh_mlp* net, const float* in);
+/* y[batch x out] = net(x[batch x in]); host buffers. */
+int mh_mlp_forward(mh_mlp* net, const float* x, int batch, float* y);
+/* Loss = mean 0.5 (y - t)^2 and its gradient w.r.t. the parameters (same
+ * layout as get_params), evaluated with the bf16 weights. */
+int mh_mlp_loss_grad(mh_mlp* net, const float* x, const float* target,
+                     int batch, float* loss, float* grad);
+/* Adam (Kingma & Ba 2015; beta 0.9/0.999, eps 1e-8) over n_epochs of
+ * shuffled minibatches of the host data set; returns the last epoch's
+ * mean loss. */
+int mh_mlp_train(mh_mlp* net, const float* x, const float* target, int n,
+                 int batch, int n_epochs, float lr, uint64_t seed,
+                 float* final_loss);
 
 /* ------------------------------------------------ measurement utilities */
 /* No reference counterpart: figures for the roofline of the step kernels.

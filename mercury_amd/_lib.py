@@ -51,7 +51,8 @@ class SimInfo(ctypes.Structure):
     _fields_ = [(n, c_int) for n in (
         "n_robots", "n_masses", "n_springs", "n_active_springs", "kernel", "block_size",
         "n_blocks", "substeps_per_launch", "lds_bytes", "max_masses_per_block",
-        "max_springs_per_block")]
+        "max_springs_per_block", "n_lds_robots", "n_stream_robots", "stream_block_size",
+        "robots_per_block", "n_stream_springs")]
 
 
 # name -> (restype, argtypes)
@@ -89,6 +90,18 @@ _SIGS = {
                                     ctypes.POINTER(vp)]),
     "mh_sim_set_timing": (c_int, [vp, c_int]),
     "mh_sim_last_kernel_ms": (c_int, [vp, f32p, ctypes.POINTER(c_int)]),
+    "mh_knn_batch": (c_int, [c_int, c_int, u32p, f32p, c_int, u16p, f32p]),
+    "mh_knn_springs": (c_int, [vp, c_int, c_int, u16p, f32p, u32p, c_float, u16p, f32p, u8p,
+                               c_int, ctypes.POINTER(c_int)]),
+    "mh_nn_materials": (c_int, [c_int, i32p, f32p, f32p, c_int, u8p]),
+    "mh_mlp_create": (c_int, [c_int, c_int, i32p, c_uint64, ctypes.POINTER(vp)]),
+    "mh_mlp_destroy": (c_int, [vp]),
+    "mh_mlp_param_count": (c_int, [vp]),
+    "mh_mlp_get_params": (c_int, [vp, f32p]),
+    "mh_mlp_set_params": (c_int, [vp, f32p]),
+    "mh_mlp_forward": (c_int, [vp, f32p, c_int, f32p]),
+    "mh_mlp_loss_grad": (c_int, [vp, f32p, f32p, c_int, f32p, f32p]),
+    "mh_mlp_train": (c_int, [vp, f32p, f32p, c_int, c_int, c_int, c_float, c_uint64, f32p]),
     "mh_calibrate_lds": (c_int, [c_int, f32p]),
     "mh_calibrate_copy": (c_int, [c_int, ctypes.c_size_t, c_int, f32p]),
 }

@@ -1,0 +1,487 @@
+// mlp.hip -- the surrogate network: fully connected layers with bf16
+// weights (fp32 master copy for Adam), fp32 activations and fp32
+// accumulation, trained on the device by hand-written forward, backward and
+// Adam kernels.
+//
+// Layer l: Z_l = A_{l-1} W_l^T + b_l, A_l = gelu(Z_l) on hidden layers,
+// A_L = Z_L.  gelu(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
+// Loss = mean over the batch of 0.5 (A_L - target)^2.  Adam follows Kingma
+// & Ba (2015), Algorithm 1 (bias-corrected moments).
+//
+// GEMMs: 64x64 output tiles, 256 threads, 4x4 outputs per thread, K in
+// steps of 16 staged through LDS.  Column sums (bias gradients) run one
+// lane per column over the rows, so every reduction has a fixed order and
+// training is deterministic.
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "mercury_hip.h"
+#include "mh_internal.h"
+
+namespace {
+
+constexpr int kTile = 64;
+constexpr int kTk = 16;
+constexpr float kGeluC = 0.7978845608028654f;
+
+__device__ __forceinline__ float gelu_f(float x) {
+  const float u = kGeluC * (x + 0.044715f * x * x * x);
+  return 0.5f * x * (1.0f + tanhf(u));
+}
+
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float u = kGeluC * (x + 0.044715f * x * x * x);
+  const float th = tanhf(u);
+  return 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * kGeluC * (1.0f + 3.0f * 0.044715f * x * x);
+}
+
+template <typename T>
+__device__ __forceinline__ float ld(const T* p, size_t i);
+template <>
+__device__ __forceinline__ float ld<float>(const float* p, size_t i) { return p[i]; }
+template <>
+__device__ __forceinline__ float ld<__hip_bfloat16>(const __hip_bfloat16* p, size_t i) {
+  return __bfloat162float(p[i]);
+}
+
+// C[M x N] = op(A) op(B) with op(A)[m][k] = TA ? A[k*lda + m] : A[m*lda + k]
+// and op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n].  Epilogue EPI:
+// 0 = store; 1 = + bias[n], store Z and gelu(Z) (hidden layer forward);
+// 2 = + bias[n], store (output layer forward); 3 = multiply by
+// gelu'(Zprev[m][n]) (hidden layer backward).
+template <typename TA, typename TB, bool TRA, bool TRB, int EPI>
+__global__ __launch_bounds__(256) void gemm(const TA* __restrict__ A, int lda, const TB* __restrict__ B,
+                                            int ldb, float* __restrict__ C, int ldc, int M, int N, int K,
+                                            const float* __restrict__ bias, float* __restrict__ Z,
+                                            const float* __restrict__ Zprev) {
+  __shared__ float sa[kTk][kTile + 1];
+  __shared__ float sb[kTk][kTile + 1];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  float acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += kTk) {
+    for (int e = threadIdx.x; e < kTile * kTk; e += 256) {
+      const int mm = e % kTile, kk = e / kTile;
+      const int m = m0 + mm, k = k0 + kk;
+      float va = 0.f;
+      if (m < M && k < K) va = TRA ? ld(A, static_cast<size_t>(k) * lda + m) : ld(A, static_cast<size_t>(m) * lda + k);
+      sa[kk][mm] = va;
+      const int n = n0 + mm;
+      float vb = 0.f;
+      if (n < N && k < K) vb = TRB ? ld(B, static_cast<size_t>(n) * ldb + k) : ld(B, static_cast<size_t>(k) * ldb + n);
+      sb[kk][mm] = vb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kTk; ++kk) {
+      float a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = sa[kk][ty * 4 + i];
+        b[i] = sb[kk][tx * 4 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int m = m0 + ty * 4 + i, n = n0 + tx * 4 + j;
+      if (m >= M || n >= N) continue;
+      const size_t o = static_cast<size_t>(m) * ldc + n;
+      float v = acc[i][j];
+      if (EPI == 1) {
+        v += bias[n];
+        Z[o] = v;
+        v = gelu_f(v);
+      } else if (EPI == 2) {
+        v += bias[n];
+      } else if (EPI == 3) {
+        v *= gelu_grad(Zprev[o]);
+      }
+      C[o] = v;
+    }
+}
+
+// dY = (Y - T) / B, per-row loss terms 0.5 (Y - T)^2
+__global__ void loss_grad(const float* Y, const float* T, int n, float inv_b, float* dY, float* terms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float e = Y[i] - T[i];
+  dY[i] = e * inv_b;
+  terms[i] = 0.5f * e * e;
+}
+
+// out[n] = sum over rows of X[m][n] (row order fixed)
+__global__ void col_sum(const float* X, int M, int N, float* out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int m = 0; m < M; ++m) s += X[static_cast<size_t>(m) * N + n];
+  out[n] = s;
+}
+
+__global__ void adam_step(float* p, const float* g, float* m, float* v, __hip_bfloat16* w16, int n,
+                          float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float pi = p[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+  p[i] = pi;
+  if (w16) w16[i] = __float2bfloat16(pi);
+}
+
+__global__ void to_bf16(const float* src, __hip_bfloat16* dst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = __float2bfloat16(src[i]);
+}
+
+__global__ void gather_rows(const float* src, const int* idx, int n_rows, int width, float* dst) {
+  const int r = blockIdx.x, c = threadIdx.x;
+  if (r >= n_rows) return;
+  for (int j = c; j < width; j += blockDim.x)
+    dst[static_cast<size_t>(r) * width + j] = src[static_cast<size_t>(idx[r]) * width + j];
+}
+
+void hipcheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw mh::Error(MH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+struct Dev {
+  T* p = nullptr;
+  size_t n = 0;
+  ~Dev() {
+    if (p) (void)hipFree(p);
+  }
+  void ensure(size_t count) {
+    if (count <= n) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    hipcheck(hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+    n = count;
+  }
+};
+
+}  // namespace
+
+struct mh_mlp {
+  int device = 0;
+  std::vector<int> sizes;     // n_layers + 1
+  std::vector<size_t> w_off;  // offset of W_l in the parameter vector; b_l follows W_l
+  size_t n_params = 0;
+  Dev<float> params, grads, m, v;
+  Dev<__hip_bfloat16> w16;    // bf16 copy of the whole parameter vector (weights used)
+  std::vector<Dev<float>> Z, A, dA;  // per layer activations / gradients
+  Dev<float> x, t, dY, terms, data_x, data_t;
+  Dev<int> idx;
+  int batch_cap = 0;
+  int adam_t = 0;
+  hipStream_t stream = nullptr;
+  ~mh_mlp() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int L() const { return static_cast<int>(sizes.size()) - 1; }
+  size_t b_off(int l) const { return w_off[l] + static_cast<size_t>(sizes[l + 1]) * sizes[l]; }
+};
+
+namespace {
+
+void ensure_batch(mh_mlp* net, int B) {
+  if (B <= net->batch_cap) return;
+  const int L = net->L();
+  net->Z.resize(L);
+  net->A.resize(L);
+  net->dA.resize(L + 1);
+  for (int l = 0; l < L; l++) {
+    net->Z[l].ensure(static_cast<size_t>(B) * net->sizes[l + 1]);
+    net->A[l].ensure(static_cast<size_t>(B) * net->sizes[l + 1]);
+  }
+  for (int l = 0; l <= L; l++) net->dA[l].ensure(static_cast<size_t>(B) * net->sizes[l]);
+  net->x.ensure(static_cast<size_t>(B) * net->sizes[0]);
+  net->t.ensure(static_cast<size_t>(B) * net->sizes[L]);
+  net->dY.ensure(static_cast<size_t>(B) * net->sizes[L]);
+  net->terms.ensure(static_cast<size_t>(B) * net->sizes[L]);
+  net->batch_cap = B;
+}
+
+dim3 grid_for(int M, int N) { return dim3((N + kTile - 1) / kTile, (M + kTile - 1) / kTile); }
+
+// Forward of a batch already in net->x (or `in`): activations in A[l].
+void forward(mh_mlp* net, const float* in, int B) {
+  const int L = net->L();
+  const float* prev = in;
+  for (int l = 0; l < L; l++) {
+    const int ni = net->sizes[l], no = net->sizes[l + 1];
+    const __hip_bfloat16* W = net->w16.p + net->w_off[l];
+    const float* b = net->params.p + net->b_off(l);
+    if (l < L - 1)
+      hipLaunchKernelGGL((gemm<float, __hip_bfloat16, false, true, 1>), grid_for(B, no), dim3(256), 0,
+                         net->stream, prev, ni, W, ni, net->A[l].p, no, B, no, ni, b, net->Z[l].p,
+                         static_cast<const float*>(nullptr));
+    else
+      hipLaunchKernelGGL((gemm<float, __hip_bfloat16, false, true, 2>), grid_for(B, no), dim3(256), 0,
+                         net->stream, prev, ni, W, ni, net->A[l].p, no, B, no, ni, b,
+                         static_cast<float*>(nullptr), static_cast<const float*>(nullptr));
+    hipcheck(hipGetLastError(), "mlp forward");
+    prev = net->A[l].p;
+  }
+}
+
+// Backward from dY (already (Y - T)/B): gradients into net->grads.
+void backward(mh_mlp* net, const float* in, int B) {
+  const int L = net->L();
+  const float* delta = net->dY.p;  // dLoss/dZ_l for the current layer
+  for (int l = L - 1; l >= 0; l--) {
+    const int ni = net->sizes[l], no = net->sizes[l + 1];
+    const float* a_prev = l == 0 ? in : net->A[l - 1].p;
+    float* gW = net->grads.p + net->w_off[l];
+    float* gb = net->grads.p + net->b_off(l);
+    // gW[no x ni] = delta^T[no x B] a_prev[B x ni]
+    hipLaunchKernelGGL((gemm<float, float, true, false, 0>), grid_for(no, ni), dim3(256), 0, net->stream,
+                       delta, no, a_prev, ni, gW, ni, no, ni, B, static_cast<const float*>(nullptr),
+                       static_cast<float*>(nullptr), static_cast<const float*>(nullptr));
+    hipLaunchKernelGGL(col_sum, dim3((no + 255) / 256), dim3(256), 0, net->stream, delta, B, no, gb);
+    if (l > 0) {
+      // delta_prev[B x ni] = (delta[B x no] W[no x ni]) * gelu'(Z_{l-1})
+      const __hip_bfloat16* W = net->w16.p + net->w_off[l];
+      hipLaunchKernelGGL((gemm<float, __hip_bfloat16, false, false, 3>), grid_for(B, ni), dim3(256), 0,
+                         net->stream, delta, no, W, ni, net->dA[l].p, ni, B, ni, no,
+                         static_cast<const float*>(nullptr), static_cast<float*>(nullptr),
+                         static_cast<const float*>(net->Z[l - 1].p));
+      delta = net->dA[l].p;
+    }
+    hipcheck(hipGetLastError(), "mlp backward");
+  }
+}
+
+float loss_and_grad(mh_mlp* net, const float* in, const float* target, int B) {
+  const int L = net->L();
+  forward(net, in, B);
+  const int n = B * net->sizes[L];
+  hipLaunchKernelGGL(loss_grad, dim3((n + 255) / 256), dim3(256), 0, net->stream, net->A[L - 1].p, target, n,
+                     1.0f / B, net->dY.p, net->terms.p);
+  backward(net, in, B);
+  std::vector<float> terms(n);
+  hipcheck(hipMemcpyAsync(terms.data(), net->terms.p, n * sizeof(float), hipMemcpyDeviceToHost, net->stream),
+           "D2H");
+  hipcheck(hipStreamSynchronize(net->stream), "sync");
+  double s = 0.0;
+  for (float v : terms) s += v;
+  return static_cast<float>(s / B);
+}
+
+int fail(const mh::Error& e) {
+  mh::set_error(e.what());
+  return e.code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_mlp_create(int device, int n_layers, const int* sizes, uint64_t seed, mh_mlp** out) {
+  try {
+    if (!out || !sizes || n_layers < 1) throw mh::Error(MH_ERR_ARG, "invalid layer list");
+    auto net = std::make_unique<mh_mlp>();
+    net->device = device;
+    hipcheck(hipSetDevice(device), "hipSetDevice");
+    hipcheck(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking), "stream");
+    net->sizes.assign(sizes, sizes + n_layers + 1);
+    for (int l = 0; l <= n_layers; l++)
+      if (sizes[l] < 1) throw mh::Error(MH_ERR_ARG, "layer sizes must be >= 1");
+    size_t off = 0;
+    for (int l = 0; l < n_layers; l++) {
+      net->w_off.push_back(off);
+      off += static_cast<size_t>(sizes[l + 1]) * sizes[l] + sizes[l + 1];
+    }
+    net->n_params = off;
+    // He-uniform weights (bound sqrt(6 / fan_in)), zero biases
+    std::mt19937_64 rng(seed);
+    std::vector<float> p(off, 0.f);
+    for (int l = 0; l < n_layers; l++) {
+      const float bound = std::sqrt(6.0f / sizes[l]);
+      std::uniform_real_distribution<float> u(-bound, bound);
+      const size_t nw = static_cast<size_t>(sizes[l + 1]) * sizes[l];
+      for (size_t i = 0; i < nw; i++) p[net->w_off[l] + i] = u(rng);
+    }
+    net->params.ensure(off);
+    net->grads.ensure(off);
+    net->m.ensure(off);
+    net->v.ensure(off);
+    net->w16.ensure(off);
+    std::vector<float> zeros(off, 0.f);
+    hipcheck(hipMemcpy(net->params.p, p.data(), off * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    hipcheck(hipMemcpy(net->m.p, zeros.data(), off * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    hipcheck(hipMemcpy(net->v.p, zeros.data(), off * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    hipLaunchKernelGGL(to_bf16, dim3((off + 255) / 256), dim3(256), 0, net->stream, net->params.p, net->w16.p,
+                       static_cast<int>(off));
+    hipcheck(hipStreamSynchronize(net->stream), "init");
+    *out = net.release();
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_mlp_destroy(mh_mlp* net) {
+  delete net;
+  return MH_OK;
+}
+
+int mh_mlp_param_count(const mh_mlp* net) { return net ? static_cast<int>(net->n_params) : MH_ERR_ARG; }
+
+int mh_mlp_get_params(const mh_mlp* net, float* out) {
+  try {
+    if (!net || !out) throw mh::Error(MH_ERR_ARG, "null argument");
+    hipcheck(hipMemcpy(out, net->params.p, net->n_params * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_mlp_set_params(mh_mlp* net, const float* in) {
+  try {
+    if (!net || !in) throw mh::Error(MH_ERR_ARG, "null argument");
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    hipcheck(hipMemcpy(net->params.p, in, net->n_params * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    hipLaunchKernelGGL(to_bf16, dim3((net->n_params + 255) / 256), dim3(256), 0, net->stream, net->params.p,
+                       net->w16.p, static_cast<int>(net->n_params));
+    hipcheck(hipStreamSynchronize(net->stream), "set params");
+    net->adam_t = 0;
+    std::vector<float> zeros(net->n_params, 0.f);
+    hipcheck(hipMemcpy(net->m.p, zeros.data(), zeros.size() * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    hipcheck(hipMemcpy(net->v.p, zeros.data(), zeros.size() * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_mlp_forward(mh_mlp* net, const float* x, int batch, float* y) {
+  try {
+    if (!net || (batch > 0 && (!x || !y)) || batch < 0) throw mh::Error(MH_ERR_ARG, "null argument");
+    if (batch == 0) return MH_OK;
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    const int L = net->L();
+    const int chunk = 8192;
+    ensure_batch(net, std::min(batch, chunk));
+    for (int b0 = 0; b0 < batch; b0 += chunk) {
+      const int B = std::min(chunk, batch - b0);
+      hipcheck(hipMemcpyAsync(net->x.p, x + static_cast<size_t>(b0) * net->sizes[0],
+                              static_cast<size_t>(B) * net->sizes[0] * sizeof(float), hipMemcpyHostToDevice,
+                              net->stream), "H2D");
+      forward(net, net->x.p, B);
+      hipcheck(hipMemcpyAsync(y + static_cast<size_t>(b0) * net->sizes[L], net->A[L - 1].p,
+                              static_cast<size_t>(B) * net->sizes[L] * sizeof(float), hipMemcpyDeviceToHost,
+                              net->stream), "D2H");
+    }
+    hipcheck(hipStreamSynchronize(net->stream), "sync");
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_mlp_loss_grad(mh_mlp* net, const float* x, const float* target, int batch, float* loss, float* grad) {
+  try {
+    if (!net || !x || !target || batch < 1) throw mh::Error(MH_ERR_ARG, "null argument");
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    ensure_batch(net, batch);
+    const int L = net->L();
+    hipcheck(hipMemcpyAsync(net->x.p, x, static_cast<size_t>(batch) * net->sizes[0] * sizeof(float),
+                            hipMemcpyHostToDevice, net->stream), "H2D");
+    hipcheck(hipMemcpyAsync(net->t.p, target, static_cast<size_t>(batch) * net->sizes[L] * sizeof(float),
+                            hipMemcpyHostToDevice, net->stream), "H2D");
+    const float l = loss_and_grad(net, net->x.p, net->t.p, batch);
+    if (loss) *loss = l;
+    if (grad)
+      hipcheck(hipMemcpy(grad, net->grads.p, net->n_params * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_mlp_train(mh_mlp* net, const float* x, const float* target, int n, int batch, int n_epochs, float lr,
+                 uint64_t seed, float* final_loss) {
+  try {
+    if (!net || !x || !target || n < 1 || batch < 1 || n_epochs < 0) throw mh::Error(MH_ERR_ARG, "bad arguments");
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    const int L = net->L();
+    const int ni = net->sizes[0], no = net->sizes[L];
+    batch = std::min(batch, n);
+    ensure_batch(net, batch);
+    net->data_x.ensure(static_cast<size_t>(n) * ni);
+    net->data_t.ensure(static_cast<size_t>(n) * no);
+    net->idx.ensure(n);
+    hipcheck(hipMemcpyAsync(net->data_x.p, x, static_cast<size_t>(n) * ni * sizeof(float), hipMemcpyHostToDevice,
+                            net->stream), "H2D");
+    hipcheck(hipMemcpyAsync(net->data_t.p, target, static_cast<size_t>(n) * no * sizeof(float),
+                            hipMemcpyHostToDevice, net->stream), "H2D");
+    std::mt19937_64 rng(seed);
+    std::vector<int> perm(n);
+    double epoch_loss = 0.0;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < n_epochs; ep++) {
+      for (int i = 0; i < n; i++) perm[i] = i;
+      std::shuffle(perm.begin(), perm.end(), rng);
+      hipcheck(hipMemcpyAsync(net->idx.p, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, net->stream), "H2D");
+      epoch_loss = 0.0;
+      int batches = 0;
+      for (int b0 = 0; b0 + batch <= n; b0 += batch) {
+        hipLaunchKernelGGL(gather_rows, dim3(batch), dim3(64), 0, net->stream, net->data_x.p, net->idx.p + b0,
+                           batch, ni, net->x.p);
+        hipLaunchKernelGGL(gather_rows, dim3(batch), dim3(64), 0, net->stream, net->data_t.p, net->idx.p + b0,
+                           batch, no, net->t.p);
+        forward(net, net->x.p, batch);
+        const int nt = batch * no;
+        hipLaunchKernelGGL(loss_grad, dim3((nt + 255) / 256), dim3(256), 0, net->stream, net->A[L - 1].p,
+                           net->t.p, nt, 1.0f / batch, net->dY.p, net->terms.p);
+        backward(net, net->x.p, batch);
+        net->adam_t++;
+        const float bc1 = 1.f - std::pow(b1, static_cast<float>(net->adam_t));
+        const float bc2 = 1.f - std::pow(b2, static_cast<float>(net->adam_t));
+        hipLaunchKernelGGL(adam_step, dim3((net->n_params + 255) / 256), dim3(256), 0, net->stream, net->params.p,
+                           net->grads.p, net->m.p, net->v.p, net->w16.p, static_cast<int>(net->n_params), lr, b1,
+                           b2, eps, bc1, bc2);
+        hipcheck(hipGetLastError(), "train step");
+        if (ep == n_epochs - 1) {
+          std::vector<float> terms(nt);
+          hipcheck(hipMemcpyAsync(terms.data(), net->terms.p, nt * sizeof(float), hipMemcpyDeviceToHost,
+                                  net->stream), "D2H");
+          hipcheck(hipStreamSynchronize(net->stream), "sync");
+          double s = 0.0;
+          for (float v : terms) s += v;
+          epoch_loss += s / batch;
+          batches++;
+        }
+      }
+      if (batches) epoch_loss /= batches;
+    }
+    hipcheck(hipStreamSynchronize(net->stream), "sync");
+    if (final_loss) *final_loss = static_cast<float>(epoch_loss);
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+}  // extern "C"

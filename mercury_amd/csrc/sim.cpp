@@ -14,8 +14,11 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
+#include <tuple>
 
 #include "mh_internal.h"
 #include "mh_kernels.h"
@@ -54,11 +57,21 @@ struct DevBuf {
   }
 };
 
-struct Plan {
+// One kernel's share of the population.
+struct Part {
   const KernelVariant* v = nullptr;
   std::vector<PackDesc> packs;      // launch order (largest work first)
   int mass_cap = 0, spring_cap = 0, csr_cap = 0;
   size_t lds = 0;
+  uint32_t pack_offset = 0;         // first descriptor in the device array
+  int n_robots = 0;
+};
+
+// Robots that fit the LDS kernel go to the LDS part, the rest to the
+// streaming part; each part is one launch per group of substeps.
+struct Plan {
+  std::vector<Part> parts;
+  int rpb = 0;  // robots-per-block cap of the LDS part (0 = as many as fit)
 };
 
 constexpr int kMaxLdsBytes = 160 * 1024;
@@ -71,6 +84,8 @@ struct mh_sim {
   std::vector<mh::Material> mats_by_id;
   int composite_count = 0;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;         // second part of a mixed plan
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int lds_limit = mh::kMaxLdsBytes;
 
   // population (host copies used for packing and collect)
@@ -102,7 +117,7 @@ struct mh_sim {
   hipGraphExec_t graph = nullptr;
   int g_steps = -1, g_track = -1, g_sub = -1;
   uint32_t g_t0 = 0;
-  const void* g_fn = nullptr;
+  const void* g_key = nullptr;
 
   // timing
   bool timing = false;
@@ -114,6 +129,9 @@ struct mh_sim {
     if (graph) (void)hipGraphExecDestroy(graph);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (aux) (void)hipStreamDestroy(aux);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -153,167 +171,192 @@ void upload_materials(mh_sim* s) {
 int robot_masses(const mh_sim* s, int r) { return static_cast<int>(s->mass_off[r + 1] - s->mass_off[r]); }
 int robot_active(const mh_sim* s, int r) { return static_cast<int>(s->active_off[r + 1] - s->active_off[r]); }
 
-// Greedy packing of consecutive robots into workgroups for variant v.
-// Returns false if some robot does not fit the variant.
-bool make_plan(const mh_sim* s, const KernelVariant& v, Plan& plan) {
+bool fits_lds_alone(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
+  const int m = robot_masses(s, r), sp = robot_active(s, r), g = static_cast<int>(s->gather_pad[r]);
+  return m <= v.block * v.mpt && sp <= v.block * v.nspt && sp + 2 <= 32767 && g <= 65535 &&
+         lds_bytes_lds(std::max(m, 1), std::max(sp, 1), g, n_mat) <= static_cast<size_t>(s->lds_limit);
+}
+
+// Greedy packing of consecutive LDS-eligible robots into workgroups of the
+// LDS variant (at most rpb robots each); every other robot gets a
+// workgroup of the streaming variant.  Returns false if a robot fits
+// neither kernel.
+bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv, int rpb, Plan& plan) {
   plan = Plan{};
-  plan.v = &v;
+  plan.rpb = rpb;
   const int n_mat = static_cast<int>(s->compact_to_id.size());
-  const int rpb = s->cfg.robots_per_block > 0 ? s->cfg.robots_per_block : std::numeric_limits<int>::max();
-  if (v.kind == MH_KERNEL_STREAM) {
-    for (int r = 0; r < s->n_robots; r++) {
+  const int cap = rpb > 0 ? rpb : std::numeric_limits<int>::max();
+  Part lp, stp;
+  lp.v = lv;
+  stp.v = sv;
+  int r = 0;
+  while (r < s->n_robots) {
+    if (!lv || !fits_lds_alone(s, *lv, r, n_mat)) {
       const int m = robot_masses(s, r);
-      if (lds_bytes_stream(m, n_mat) > static_cast<size_t>(s->lds_limit)) return false;
-      plan.mass_cap = std::max(plan.mass_cap, m);
-      plan.spring_cap = std::max(plan.spring_cap, robot_active(s, r));
+      if (!sv || lds_bytes_stream(std::max(m, 1), n_mat) > static_cast<size_t>(s->lds_limit)) return false;
       PackDesc d{};
       d.mass_begin = s->mass_off[r];
       d.n_mass = static_cast<uint32_t>(m);
       d.spring_begin = s->active_off[r];
       d.n_spring = static_cast<uint32_t>(robot_active(s, r));
-      plan.packs.push_back(d);
+      stp.mass_cap = std::max(stp.mass_cap, m);
+      stp.spring_cap = std::max(stp.spring_cap, static_cast<int>(d.n_spring));
+      stp.packs.push_back(d);
+      stp.n_robots++;
+      r++;
+      continue;
     }
-    plan.lds = lds_bytes_stream(std::max(plan.mass_cap, 1), n_mat);
-  } else {
-    const int mcap = v.block * v.mpt, scap = v.block * v.nspt;
-    int r = 0;
-    while (r < s->n_robots) {
-      PackDesc d{};
-      d.mass_begin = s->mass_off[r];
-      d.spring_begin = s->active_off[r];
-      int count = 0, gathers = 0;
-      while (r < s->n_robots && count < rpb) {
-        const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
-        const int sp = static_cast<int>(d.n_spring) + robot_active(s, r);
-        const int g = gathers + static_cast<int>(s->gather_pad[r]);
-        const int new_mcap = std::max(plan.mass_cap, m), new_scap = std::max(plan.spring_cap, sp);
-        const int new_ccap = std::max(plan.csr_cap, g);
-        // u16 LDS indices: spring slots (incl. sink and zero) < 32768, entries < 65536
-        const bool fits = m <= mcap && sp <= scap && sp + 2 <= 32767 && g <= 65535 && m <= 65535 &&
-                          lds_bytes_lds(new_mcap, new_scap, new_ccap, n_mat) <=
-                              static_cast<size_t>(s->lds_limit);
-        if (!fits) break;
-        d.n_mass = static_cast<uint32_t>(m);
-        d.n_spring = static_cast<uint32_t>(sp);
-        gathers = g;
-        r++;
-        count++;
-      }
-      if (count == 0) return false;
-      plan.mass_cap = std::max(plan.mass_cap, static_cast<int>(d.n_mass));
-      plan.spring_cap = std::max(plan.spring_cap, static_cast<int>(d.n_spring));
-      plan.csr_cap = std::max(plan.csr_cap, gathers);
-      plan.packs.push_back(d);
+    PackDesc d{};
+    d.mass_begin = s->mass_off[r];
+    d.spring_begin = s->active_off[r];
+    int count = 0, gathers = 0;
+    const int mcap = lv->block * lv->mpt, scap = lv->block * lv->nspt;
+    while (r < s->n_robots && count < cap && fits_lds_alone(s, *lv, r, n_mat)) {
+      const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
+      const int sp = static_cast<int>(d.n_spring) + robot_active(s, r);
+      const int g = gathers + static_cast<int>(s->gather_pad[r]);
+      const bool fits = m <= mcap && sp <= scap && sp + 2 <= 32767 && g <= 65535 &&
+                        lds_bytes_lds(std::max(lp.mass_cap, m), std::max(lp.spring_cap, sp),
+                                      std::max(lp.csr_cap, g), n_mat) <= static_cast<size_t>(s->lds_limit);
+      if (!fits) break;
+      d.n_mass = static_cast<uint32_t>(m);
+      d.n_spring = static_cast<uint32_t>(sp);
+      gathers = g;
+      r++;
+      count++;
     }
-    plan.lds = lds_bytes_lds(std::max(plan.mass_cap, 1), std::max(plan.spring_cap, 1),
-                             plan.csr_cap, n_mat);
+    lp.mass_cap = std::max(lp.mass_cap, static_cast<int>(d.n_mass));
+    lp.spring_cap = std::max(lp.spring_cap, static_cast<int>(d.n_spring));
+    lp.csr_cap = std::max(lp.csr_cap, gathers);
+    lp.packs.push_back(d);
+    lp.n_robots += count;
+  }
+  if (!lp.packs.empty()) {
+    lp.lds = lds_bytes_lds(std::max(lp.mass_cap, 1), std::max(lp.spring_cap, 1), lp.csr_cap, n_mat);
+    plan.parts.push_back(std::move(lp));
+  }
+  if (!stp.packs.empty()) {
+    stp.lds = lds_bytes_stream(std::max(stp.mass_cap, 1), n_mat);
+    plan.parts.push_back(std::move(stp));
   }
   return true;
 }
 
 // Device arrays that depend on the packing: pack-local spring pairs, the
-// gather lists (ascending spring index per mass) and row pointers.
+// gather lists (ascending spring index per mass; LDS parts pad every row to
+// a multiple of 4 with the zero slot), row starts and phase-2 order.
 void upload_plan(mh_sim* s, Plan& plan) {
   const size_t na = s->n_active;
-  const bool padded = plan.v->kind == MH_KERNEL_LDS;
-  const uint32_t zero_entry = static_cast<uint32_t>(plan.spring_cap + 1) << 1;
   std::vector<uint32_t> spair(na), csr, row;
   csr.reserve(2 * na + 3 * s->n_masses);
   std::vector<uint16_t> perm(s->n_masses);
-  std::vector<uint32_t> order;
-  row.reserve(s->n_masses + plan.packs.size());
-  std::vector<uint32_t> count;
-  for (PackDesc& d : plan.packs) {
-    d.csr_begin = static_cast<uint32_t>(csr.size());
-    d.row_begin = static_cast<uint32_t>(row.size());
-    const uint32_t nm = d.n_mass;
-    count.assign(nm + 1, 0);
-    // pack-local pairs: robot-local index + robot's offset inside the pack
-    int r = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
-                             s->mass_off.begin()) - 1;
-    for (uint32_t k = 0; k < d.n_spring; k++) {
-      const uint32_t g = d.spring_begin + k;
-      while (g >= s->active_off[r + 1]) r++;
-      const uint32_t off = s->mass_off[r] - d.mass_begin;
-      const uint32_t a = s->act_pairs[2 * g] + off, b = s->act_pairs[2 * g + 1] + off;
-      spair[g] = a | (b << 16);
-      count[a + 1]++;
-      count[b + 1]++;
+  std::vector<uint32_t> order, count;
+  std::vector<PackDesc> all;
+  row.reserve(s->n_masses + 64);
+  bool need_scratch = false;
+  for (Part& part : plan.parts) {
+    const bool padded = part.v->kind == MH_KERNEL_LDS;
+    need_scratch |= part.v->kind == MH_KERNEL_STREAM;
+    const uint32_t zero_entry = static_cast<uint32_t>(part.spring_cap + 1) << 1;
+    for (PackDesc& d : part.packs) {
+      d.csr_begin = static_cast<uint32_t>(csr.size());
+      d.row_begin = static_cast<uint32_t>(row.size());
+      const uint32_t nm = d.n_mass;
+      count.assign(nm + 1, 0);
+      // pack-local pairs: robot-local index + robot's offset inside the pack
+      int r = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
+                               s->mass_off.begin()) - 1;
+      for (uint32_t k = 0; k < d.n_spring; k++) {
+        const uint32_t g = d.spring_begin + k;
+        while (g >= s->active_off[r + 1]) r++;
+        const uint32_t off = s->mass_off[r] - d.mass_begin;
+        const uint32_t a = s->act_pairs[2 * g] + off, b = s->act_pairs[2 * g + 1] + off;
+        spair[g] = a | (b << 16);
+        count[a + 1]++;
+        count[b + 1]++;
+      }
+      // phase-2 visiting order: longest gather lists first (ties keep order)
+      order.resize(nm);
+      std::iota(order.begin(), order.end(), 0u);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](uint32_t x, uint32_t y) { return count[x + 1] > count[y + 1]; });
+      for (uint32_t i = 0; i < nm; i++) perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
+      for (uint32_t i = 0; i < nm; i++) {
+        const uint32_t deg = count[i + 1];
+        count[i + 1] = count[i] + (padded ? (deg + 3u) & ~3u : deg);
+      }
+      for (uint32_t i = 0; i <= nm; i++) row.push_back(count[i]);
+      csr.resize(d.csr_begin + count[nm], zero_entry);
+      std::vector<uint32_t> fill(count.begin(), count.end() - 1);
+      for (uint32_t k = 0; k < d.n_spring; k++) {
+        const uint32_t pr = spair[d.spring_begin + k];
+        const uint32_t a = pr & 0xffffu, b = pr >> 16;
+        csr[d.csr_begin + fill[a]++] = k << 1;
+        csr[d.csr_begin + fill[b]++] = (k << 1) | 1u;
+      }
     }
-    // phase-2 visiting order: longest gather lists first, so the lanes of
-    // a wave have similar trip counts (ties keep mass order)
-    order.resize(nm);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t x, uint32_t y) { return count[x + 1] > count[y + 1]; });
-    for (uint32_t i = 0; i < nm; i++) perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
-    // row starts; LDS plans round every row up to a multiple of 4 entries
-    for (uint32_t i = 0; i < nm; i++) {
-      const uint32_t deg = count[i + 1];
-      count[i + 1] = count[i] + (padded ? (deg + 3u) & ~3u : deg);
-    }
-    for (uint32_t i = 0; i <= nm; i++) row.push_back(count[i]);
-    csr.resize(d.csr_begin + count[nm], zero_entry);
-    std::vector<uint32_t> fill(count.begin(), count.end() - 1);
-    for (uint32_t k = 0; k < d.n_spring; k++) {
-      const uint32_t pr = spair[d.spring_begin + k];
-      const uint32_t a = pr & 0xffffu, b = pr >> 16;
-      csr[d.csr_begin + fill[a]++] = k << 1;
-      csr[d.csr_begin + fill[b]++] = (k << 1) | 1u;
-    }
+    // launch order: most work first, so uneven packs balance across CUs
+    std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
+      return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
+    });
+    part.pack_offset = static_cast<uint32_t>(all.size());
+    all.insert(all.end(), part.packs.begin(), part.packs.end());
+    if (part.lds > 64 * 1024)
+      (void)hipFuncSetAttribute(part.v->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(part.lds));
   }
-  // launch order: most work first, so uneven packs balance across CUs
-  std::stable_sort(plan.packs.begin(), plan.packs.end(), [](const PackDesc& x, const PackDesc& y) {
-    return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
-  });
   s->spair.upload(spair.data(), na, s->stream);
   s->csr.upload(csr.data(), csr.size(), s->stream);
   s->row.upload(row.data(), row.size(), s->stream);
-  s->packs.upload(plan.packs.data(), plan.packs.size(), s->stream);
+  s->packs.upload(all.data(), all.size(), s->stream);
   s->mperm.upload(perm.data(), perm.size(), s->stream);
-  if (plan.v->kind == MH_KERNEL_STREAM) s->scratch.alloc(4 * std::max<size_t>(na, 1));
-  if (plan.lds > 64 * 1024)
-    (void)hipFuncSetAttribute(plan.v->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              static_cast<int>(plan.lds));
+  if (need_scratch) s->scratch.alloc(4 * std::max<size_t>(na, 1));
   hip_check(hipStreamSynchronize(s->stream), "plan upload");
   s->plan = std::move(plan);
   drop_graph(s);
 }
 
-std::vector<const KernelVariant*> candidates(const mh_sim* s) {
+std::vector<const KernelVariant*> candidates(const mh_sim* s, int kind) {
   int nv = 0;
   const KernelVariant* all = kernel_variants(&nv);
   std::vector<const KernelVariant*> out;
   for (int i = 0; i < nv; i++) {
     const KernelVariant& v = all[i];
-    if (v.solver != s->cfg.solver) continue;
-    if (s->cfg.kernel != MH_KERNEL_AUTO && v.kind != s->cfg.kernel) continue;
+    if (v.solver != s->cfg.solver || v.kind != kind) continue;
     if (s->cfg.block_size && v.block != s->cfg.block_size) continue;
     out.push_back(&v);
   }
   return out;
 }
 
-// Default plan: LDS variants in table order (largest blocks first), the
-// first one every robot fits; otherwise the streaming kernel.
+// Default plan: the first LDS variant (table order, largest blocks first)
+// that takes every robot it can, the first streaming variant for the rest.
 void choose_default_plan(mh_sim* s) {
-  auto cand = candidates(s);
-  Plan p;
-  for (int pass = 0; pass < 2; pass++) {
-    for (const KernelVariant* v : cand) {
-      if ((pass == 0) != (v->kind == MH_KERNEL_LDS)) continue;
-      if (make_plan(s, *v, p)) {
-        upload_plan(s, p);
-        return;
-      }
+  const bool use_lds = s->cfg.kernel != MH_KERNEL_STREAM;
+  const bool use_stream = s->cfg.kernel != MH_KERNEL_LDS;
+  auto lc = use_lds ? candidates(s, MH_KERNEL_LDS) : std::vector<const KernelVariant*>{};
+  auto sc = use_stream ? candidates(s, MH_KERNEL_STREAM) : std::vector<const KernelVariant*>{};
+  const KernelVariant* sv = sc.empty() ? nullptr : sc.front();
+  const int n_mat = static_cast<int>(s->compact_to_id.size());
+  const KernelVariant* best = nullptr;
+  int best_fit = -1;
+  for (const KernelVariant* v : lc) {
+    int fit = 0;
+    for (int r = 0; r < s->n_robots; r++) fit += fits_lds_alone(s, *v, r, n_mat) ? 1 : 0;
+    if (fit > best_fit) {
+      best_fit = fit;
+      best = v;
     }
   }
-  throw Error(MH_ERR_RANGE, "no step kernel fits this population (robot too large for LDS)");
+  Plan p;
+  if (!make_plan(s, best, sv, s->cfg.robots_per_block, p))
+    throw Error(MH_ERR_RANGE, "no step kernel fits this population (robot too large for LDS)");
+  upload_plan(s, p);
 }
 
-StepParams make_params(mh_sim* s, float t0, int n_steps, int track) {
+StepParams make_params(mh_sim* s, const Part& part, float t0, int n_steps, int track) {
   StepParams p{};
-  p.packs = s->packs.p;
+  p.packs = s->packs.p + part.pack_offset;
   p.pos = s->pos.p;
   p.vel = s->vel.p;
   p.spair = s->spair.p;
@@ -330,9 +373,9 @@ StepParams make_params(mh_sim* s, float t0, int n_steps, int track) {
   p.mat_omega = s->mat_omega.p;
   p.mat_phi = s->mat_phi.p;
   p.n_mat = static_cast<int>(s->compact_to_id.size());
-  p.mass_cap = s->plan.mass_cap;
-  p.spring_cap = s->plan.spring_cap;
-  p.csr_cap = s->plan.csr_cap;
+  p.mass_cap = part.mass_cap;
+  p.spring_cap = part.spring_cap;
+  p.csr_cap = part.csr_cap;
   p.n_steps = n_steps;
   p.dt = s->cfg.dt;
   p.t0 = t0;
@@ -346,25 +389,40 @@ StepParams make_params(mh_sim* s, float t0, int n_steps, int track) {
   return p;
 }
 
-// Issue the launches of n_steps steps starting at simulated time t0.
+// Issue the launches of n_steps steps starting at simulated time t0.  The
+// parts own disjoint robots: the second one runs on the aux stream, forked
+// from and joined back into the main stream.  Returns launches per part.
 int issue(mh_sim* s, float t0, int n_steps, int track) {
   const int K = s->substeps > 0 ? s->substeps : n_steps;
+  const size_t np = s->plan.parts.size();
+  if (np > 1) {
+    hip_check(hipEventRecord(s->ev_fork, s->stream), "fork");
+    hip_check(hipStreamWaitEvent(s->aux, s->ev_fork, 0), "fork");
+  }
   int launches = 0;
-  float t = t0;
-  for (int done = 0; done < n_steps;) {
-    const int n = std::min(K, n_steps - done);
-    StepParams p = make_params(s, t, n, track);
-    hip_check(launch_step(*s->plan.v, static_cast<int>(s->plan.packs.size()), s->plan.lds, s->stream, p),
-              "step launch");
-    for (int k = 0; k < n; k++) t += s->cfg.dt;  // the reference's float time recurrence
-    done += n;
-    launches++;
+  for (size_t i = 0; i < np; i++) {
+    const Part& part = s->plan.parts[i];
+    hipStream_t st = i == 0 ? s->stream : s->aux;
+    float t = t0;
+    launches = 0;
+    for (int done = 0; done < n_steps;) {
+      const int n = std::min(K, n_steps - done);
+      StepParams p = make_params(s, part, t, n, track);
+      hip_check(launch_step(*part.v, static_cast<int>(part.packs.size()), part.lds, st, p), "step launch");
+      for (int k = 0; k < n; k++) t += s->cfg.dt;  // the reference's float time recurrence
+      done += n;
+      launches++;
+    }
+  }
+  if (np > 1) {
+    hip_check(hipEventRecord(s->ev_join, s->aux), "join");
+    hip_check(hipStreamWaitEvent(s->stream, s->ev_join, 0), "join");
   }
   return launches;
 }
 
 void run_steps(mh_sim* s, int n_steps, int track) {
-  if (n_steps <= 0 || s->n_robots == 0 || s->n_active + s->n_masses == 0) {
+  if (n_steps <= 0 || s->n_robots == 0 || s->plan.parts.empty()) {
     for (int k = 0; k < n_steps; k++) s->total_time += s->cfg.dt;
     return;
   }
@@ -376,7 +434,7 @@ void run_steps(mh_sim* s, int n_steps, int track) {
     uint32_t t0_bits;
     std::memcpy(&t0_bits, &t0, 4);
     const bool hit = s->graph && s->g_steps == n_steps && s->g_track == track &&
-                     s->g_sub == s->substeps && s->g_t0 == t0_bits && s->g_fn == s->plan.v->fn;
+                     s->g_sub == s->substeps && s->g_t0 == t0_bits && s->g_key == &s->plan;
     if (!hit) {
       drop_graph(s);
       hipGraph_t g = nullptr;
@@ -396,7 +454,7 @@ void run_steps(mh_sim* s, int n_steps, int track) {
       s->g_track = track;
       s->g_sub = s->substeps;
       s->g_t0 = t0_bits;
-      s->g_fn = s->plan.v->fn;
+      s->g_key = &s->plan;
     }
     hip_check(hipGraphLaunch(s->graph, s->stream), "graph launch");
     const int K = s->substeps > 0 ? s->substeps : n_steps;
@@ -487,6 +545,9 @@ int mh_sim_create(const mh_config* cfg, const mh_material_set* set, mh_sim** out
       lds > 0)
     s->lds_limit = std::min(lds, mh::kMaxLdsBytes);
   mh::hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream");
+  mh::hip_check(hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking), "stream");
+  mh::hip_check(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming), "event");
+  mh::hip_check(hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming), "event");
   mh::hip_check(hipEventCreate(&s->ev0), "event");
   mh::hip_check(hipEventCreate(&s->ev1), "event");
   *out = s.release();
@@ -753,18 +814,84 @@ int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
   info->n_masses = static_cast<int>(s->n_masses);
   info->n_springs = static_cast<int>(s->n_springs);
   info->n_active_springs = static_cast<int>(s->n_active);
-  if (s->plan.v) {
-    info->kernel = s->plan.v->kind;
-    info->block_size = s->plan.v->block;
-    info->n_blocks = static_cast<int>(s->plan.packs.size());
-    info->lds_bytes = static_cast<int>(s->plan.lds);
-    info->max_masses_per_block = s->plan.mass_cap;
-    info->max_springs_per_block = s->plan.spring_cap;
+  for (const mh::Part& part : s->plan.parts) {
+    info->kernel |= part.v->kind;
+    info->n_blocks += static_cast<int>(part.packs.size());
+    if (part.v->kind == MH_KERNEL_LDS) {
+      info->block_size = part.v->block;
+      info->lds_bytes = static_cast<int>(part.lds);
+      info->max_masses_per_block = part.mass_cap;
+      info->max_springs_per_block = part.spring_cap;
+      info->n_lds_robots = part.n_robots;
+    } else {
+      info->stream_block_size = part.v->block;
+      info->n_stream_robots = part.n_robots;
+      for (const mh::PackDesc& d : part.packs) info->n_stream_springs += static_cast<int>(d.n_spring);
+    }
   }
+  info->robots_per_block = s->plan.rpb;
   info->substeps_per_launch = s->substeps;
   return MH_OK;
   MH_CATCH
 }
+
+namespace mh {
+namespace {
+
+// Tuning results by (solver, device, population shape); the chosen plan
+// never changes results, only speed.
+struct TuneKey {
+  int solver, device;
+  std::vector<uint64_t> shape;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(solver, device, shape) < std::tie(o.solver, o.device, o.shape);
+  }
+};
+struct TuneChoice {
+  const KernelVariant* lds;
+  const KernelVariant* stream;
+  int rpb;
+};
+std::map<TuneKey, TuneChoice>& tune_cache() {
+  static std::map<TuneKey, TuneChoice> c;
+  return c;
+}
+std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+TuneKey tune_key(const mh_sim* s) {
+  // histogram of (masses, active springs) rounded to 1/16 of a power of two
+  auto bucket = [](uint32_t v) -> uint64_t {
+    if (v < 16) return v;
+    int e = 31 - __builtin_clz(v);
+    return (static_cast<uint64_t>(e) << 4) | ((v >> (e - 4)) & 15u);
+  };
+  std::map<uint64_t, uint32_t> h;
+  for (int r = 0; r < s->n_robots; r++)
+    h[(bucket(static_cast<uint32_t>(robot_masses(s, r))) << 32) |
+      bucket(static_cast<uint32_t>(robot_active(s, r)))]++;
+  TuneKey k{s->cfg.solver, s->cfg.device, {}};
+  for (auto& kv : h) {
+    k.shape.push_back(kv.first);
+    k.shape.push_back(kv.second);
+  }
+  return k;
+}
+
+double time_plan(mh_sim* s, Plan& p, int n_steps) {
+  upload_plan(s, p);
+  run_steps(s, std::max(1, n_steps / 8), 0);  // warm-up
+  hip_check(hipStreamSynchronize(s->stream), "sync");
+  auto t0 = std::chrono::steady_clock::now();
+  run_steps(s, n_steps, 0);
+  hip_check(hipStreamSynchronize(s->stream), "sync");
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+}  // namespace mh
 
 int mh_sim_autotune(mh_sim* s, int n_steps) {
   MH_TRY
@@ -773,6 +900,19 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   if (s->n_robots == 0) return MH_OK;
   if (n_steps < 1) n_steps = 1;
   mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  const mh::TuneKey key = mh::tune_key(s);
+  {
+    std::lock_guard<std::mutex> lk(mh::tune_mutex());
+    auto it = mh::tune_cache().find(key);
+    if (it != mh::tune_cache().end()) {
+      mh::Plan p;
+      if (mh::make_plan(s, it->second.lds, it->second.stream, it->second.rpb, p)) {
+        mh::upload_plan(s, p);
+        if (s->substeps == 0) s->substeps = 1000;
+        return MH_OK;
+      }
+    }
+  }
   // candidates run on a copy of the state; the population is restored after
   std::vector<float4> p0(s->n_masses), v0(s->n_masses);
   std::vector<float> st0(s->stress.n);
@@ -784,31 +924,56 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   s->cfg.use_graph = 0;
   const int saved_sub = s->substeps;
   s->substeps = 0;
+  const bool use_lds = s->cfg.kernel != MH_KERNEL_STREAM;
+  const bool use_stream = s->cfg.kernel != MH_KERNEL_LDS;
+  auto lc = use_lds ? mh::candidates(s, MH_KERNEL_LDS) : std::vector<const mh::KernelVariant*>{};
+  auto sc = use_stream ? mh::candidates(s, MH_KERNEL_STREAM) : std::vector<const mh::KernelVariant*>{};
+  lc.push_back(nullptr);  // everything streamed
+  const mh::KernelVariant* sv0 = sc.empty() ? nullptr : sc.front();
+  std::vector<int> rpbs = {s->cfg.robots_per_block};
+  if (s->cfg.robots_per_block == 0) rpbs = {0, 1, 2, 4};
   double best = std::numeric_limits<double>::infinity();
-  const mh::KernelVariant* best_v = nullptr;
-  for (const mh::KernelVariant* v : mh::candidates(s)) {
-    mh::Plan p;
-    if (!mh::make_plan(s, *v, p)) continue;
-    if (s->cfg.robots_per_block == 0 && v->kind == MH_KERNEL_LDS && p.lds > static_cast<size_t>(s->lds_limit))
-      continue;
-    mh::upload_plan(s, p);
-    mh::run_steps(s, std::max(1, n_steps / 8), 0);  // warm-up
-    mh::hip_check(hipStreamSynchronize(s->stream), "sync");
-    auto t0 = std::chrono::steady_clock::now();
-    mh::run_steps(s, n_steps, 0);
-    mh::hip_check(hipStreamSynchronize(s->stream), "sync");
-    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (dt < best) {
-      best = dt;
-      best_v = v;
+  mh::TuneChoice choice{nullptr, nullptr, 0};
+  // coordinate search: LDS variant x robots per block, then streaming variant
+  for (const mh::KernelVariant* lv : lc) {
+    for (int rpb : rpbs) {
+      if (!lv && rpb != rpbs.front()) continue;
+      mh::Plan p;
+      if (!mh::make_plan(s, lv, sv0, rpb, p)) continue;
+      // skip robots-per-block caps that produce the same packing
+      if (rpb > 0 && !p.parts.empty() && p.parts[0].v == lv &&
+          static_cast<int>(p.parts[0].packs.size()) == p.parts[0].n_robots && rpb > 1)
+        continue;
+      const double t = mh::time_plan(s, p, n_steps);
+      if (t < best) {
+        best = t;
+        choice = {lv, sv0, rpb};
+      }
     }
   }
-  if (!best_v) throw mh::Error(MH_ERR_RANGE, "no step kernel fits this population");
+  for (const mh::KernelVariant* sv : sc) {
+    if (sv == sv0) continue;
+    mh::Plan p;
+    if (!mh::make_plan(s, choice.lds, sv, choice.rpb, p)) continue;
+    bool uses_stream = false;
+    for (auto& part : p.parts) uses_stream |= part.v->kind == MH_KERNEL_STREAM;
+    if (!uses_stream) break;
+    const double t = mh::time_plan(s, p, n_steps);
+    if (t < best) {
+      best = t;
+      choice.stream = sv;
+    }
+  }
+  if (!std::isfinite(best)) throw mh::Error(MH_ERR_RANGE, "no step kernel fits this population");
   mh::Plan p;
-  mh::make_plan(s, *best_v, p);
+  mh::make_plan(s, choice.lds, choice.stream, choice.rpb, p);
   mh::upload_plan(s, p);
-  // substeps per launch: the longest launch is cheapest; keep launches
-  // bounded to ~1k steps so a caller's synchronize stays responsive
+  {
+    std::lock_guard<std::mutex> lk(mh::tune_mutex());
+    mh::tune_cache()[key] = choice;
+  }
+  // substeps per launch: long launches amortize the launch; ~1k substeps
+  // keep a caller's synchronize responsive
   s->substeps = saved_sub > 0 ? saved_sub : 1000;
   s->cfg.use_graph = use_graph;
   mh::hip_check(hipMemcpy(s->pos.p, p0.data(), p0.size() * sizeof(float4), hipMemcpyHostToDevice), "H2D");

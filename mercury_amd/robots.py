@@ -9,8 +9,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import check, f32p, load, ptr, u8p, u16p, u32p
 import ctypes
+
+from ._lib import check, f32p, i32p, load, ptr, u8p, u16p, u32p
 
 
 @dataclass
@@ -122,3 +123,90 @@ def voxel_population(material_set, n_robots, voxels_per_axis=8, p_empty=0.2, see
         enc = random_voxel_encodings(rng, voxels_per_axis, n_materials, p_empty)
         out.append(voxel_build(material_set, n, res, enc, mass=mass))
     return out
+
+
+def nn_materials(weights, sizes, pos):
+    """Material index (0-based output) per mass from a bias-free ReLU MLP on
+    its position (NNRobot::forward, NNRobot.h:41-92)."""
+    pos = np.ascontiguousarray(pos, np.float32)
+    w = np.ascontiguousarray(np.concatenate([np.asarray(x, np.float32).reshape(-1) for x in weights]))
+    sz = np.ascontiguousarray(sizes, np.int32)
+    out = np.zeros(len(pos), np.uint8)
+    check(load().mh_nn_materials(len(sizes) - 1, ptr(sz, i32p), ptr(w, f32p), ptr(pos, f32p),
+                                 len(pos), ptr(out, u8p)))
+    return out
+
+
+def knn_batch(mass_offset, pos, K, device=0):
+    """KNN rows (neighbour index, distance) for every mass, on the GPU."""
+    mo = np.ascontiguousarray(mass_offset, np.uint32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    M = int(mo[-1])
+    nbr = np.zeros((M, K), np.uint16)
+    dist = np.zeros((M, K), np.float32)
+    check(load().mh_knn_batch(device, len(mo) - 1, ptr(mo, u32p), ptr(pos, f32p), K,
+                              ptr(nbr, u16p), ptr(dist, f32p)))
+    return nbr, dist
+
+
+def knn_springs(material_set, nbr, dist, mass_encoding, min_length=1e-3):
+    n, K = nbr.shape
+    enc = np.ascontiguousarray(mass_encoding, np.uint32)
+    maxs = n * K
+    pairs = np.zeros((maxs, 2), np.uint16)
+    ln = np.zeros(maxs, np.float32)
+    mid = np.zeros(maxs, np.uint8)
+    ns = ctypes.c_int()
+    check(load().mh_knn_springs(material_set.handle, n, K, ptr(np.ascontiguousarray(nbr), u16p),
+                                ptr(np.ascontiguousarray(dist), f32p), ptr(enc, u32p),
+                                float(min_length), ptr(pairs, u16p), ptr(ln, f32p), ptr(mid, u8p),
+                                maxs, ctypes.byref(ns)))
+    s = ns.value
+    return pairs[:s].copy(), ln[:s].copy(), mid[:s].copy()
+
+
+def shift_xy(pos):
+    """ShiftX then ShiftY (VoxelRobot.cpp:30-56, NNRobot.cpp:29-55): max x
+    to 0, then min y to 0, by float additions."""
+    pos = np.array(pos, np.float32, copy=True)
+    if len(pos):
+        pos[:, 0] = (pos[:, 0] + np.float32(-pos[:, 0].max())).astype(np.float32)
+        pos[:, 1] = (pos[:, 1] + np.float32(-pos[:, 1].min())).astype(np.float32)
+    return pos
+
+
+def nn_population(material_set, n_robots, mass_range=(200, 2000), cube=0.5, K=25,
+                  hidden=(25, 25), n_materials=4, seed=0, mass=0.01, device=0):
+    """Seeded NN-robot population: per robot a mass count uniform in
+    mass_range, positions uniform in a cube of side `cube` (m), materials
+    from a random-init bias-free MLP 3 -> hidden -> n_materials on the
+    position, springs to the K nearest neighbours (computed on the GPU)."""
+    rng = np.random.default_rng(seed)
+    sizes = [3, *hidden, n_materials]
+    robots = []
+    counts = rng.integers(mass_range[0], mass_range[1] + 1, n_robots)
+    all_pos, mats = [], []
+    for r in range(n_robots):
+        m = int(counts[r])
+        pos = (rng.random((m, 3)) * cube).astype(np.float32)
+        weights = [rng.uniform(-1, 1, (o, i)).astype(np.float32)
+                   for i, o in zip(sizes[:-1], sizes[1:])]
+        # the network sees positions relative to the cube centre, like the
+        # reference's origin-centred input cloud (NNRobot.h:115-135)
+        idx = nn_materials(weights, sizes, (pos - np.float32(cube / 2)).astype(np.float32))
+        all_pos.append(pos)
+        mats.append(idx)
+    mo = np.zeros(n_robots + 1, np.uint32)
+    mo[1:] = np.cumsum(counts)
+    nbr, dist = knn_batch(mo, np.concatenate(all_pos), K, device)
+    for r in range(n_robots):
+        a, b = int(mo[r]), int(mo[r + 1])
+        base_id = mats[r].astype(np.uint32) + 1          # base material ids 1..n
+        enc = (np.uint32(1) << base_id).astype(np.uint32)
+        pairs, ln, sid = knn_springs(material_set, nbr[a:b], dist[a:b], enc)
+        e = Element(shift_xy(all_pos[r]), base_id.astype(np.uint8), pairs, ln, sid)
+        e.rest_length = ln.copy()
+        e.mass = np.full(b - a, mass, np.float32)
+        e.mass_encoding = enc
+        robots.append(e)
+    return robots

@@ -1,0 +1,102 @@
+"""Surrogate pieces: rank statistics (CPU), and the HIP MLP forward, backward
+and Adam training checked against the FP64 oracle (GPU)."""
+import numpy as np
+import pytest
+
+import oracle
+import mercury_amd as mh
+from mercury_amd import surrogate
+
+
+def test_spearman_matches_scipy_with_ties():
+    from scipy.stats import spearmanr
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        a = rng.integers(0, 20, 300).astype(float)   # many ties
+        b = a + rng.normal(0, 5, 300)
+        assert surrogate.spearman(a, b) == pytest.approx(spearmanr(a, b).correlation, abs=1e-12)
+    assert surrogate.spearman([1, 2, 3], [3, 2, 1]) == pytest.approx(-1.0)
+    assert surrogate.mae([1, 2], [2, 4]) == pytest.approx(1.5)
+
+
+def test_genome_features_shape_and_determinism():
+    ms = mh.workload_materials()
+    els = mh.voxel_population(ms, 5, voxels_per_axis=4, seed=3)
+    f1 = surrogate.genome_features(els, 4, mh.materials.WORKLOAD_ROWS)
+    f2 = surrogate.genome_features(els, 4, mh.materials.WORKLOAD_ROWS)
+    assert f1.shape == (5, 64) and np.array_equal(f1, f2) and np.isfinite(f1).all()
+    assert np.allclose(f1[:, :5].sum(1), 1.0)  # material fractions
+    assert len(np.unique(f1, axis=0)) == 5
+
+
+def bf16_round(w):
+    u = np.ascontiguousarray(w, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32)
+
+
+def oracle_layers(net, p):
+    W, b = [], []
+    for w, bb in net.layers(p):
+        W.append(bf16_round(w).astype(np.float64))
+        b.append(bb.astype(np.float64))
+    return W, b
+
+
+@pytest.mark.gpu
+def test_mlp_forward_and_gradient_match_fp64_oracle():
+    print("mlp fwd/bwd vs oracle", flush=True)
+    rng = np.random.default_rng(1)
+    net = surrogate.MLP([64, 256, 256, 1], seed=7)
+    x = rng.normal(size=(300, 64)).astype(np.float32)
+    t = rng.normal(size=(300, 1)).astype(np.float32)
+    W, b = oracle_layers(net, net.get_params())
+    y = net.forward(x)
+    yo = oracle.mlp_forward(W, b, x.astype(np.float64))
+    assert np.abs(y - yo).max() <= 1e-4 * max(1.0, np.abs(yo).max())
+    loss, g = net.loss_grad(x, t)
+    lo, gW, gb = oracle.mlp_loss_grad(W, b, x.astype(np.float64), t.astype(np.float64))
+    assert loss == pytest.approx(lo, rel=1e-4)
+    go = np.concatenate([np.concatenate([w.reshape(-1), v]) for w, v in zip(gW, gb)])
+    scale = np.abs(go).max()
+    assert np.abs(g - go).max() <= 1e-4 * scale
+    # a dropped GELU derivative or a transposed operand would be O(scale) off
+    assert np.corrcoef(g, go)[0, 1] > 0.99999
+
+
+@pytest.mark.gpu
+def test_mlp_training_learns_nonlinear_target():
+    print("mlp training", flush=True)
+    rng = np.random.default_rng(2)
+    X = rng.normal(size=(8192, 64)).astype(np.float32)
+    y = (np.sin(X[:, 0]) + X[:, 1] * X[:, 2] + 0.5 * np.abs(X[:, 3])).astype(np.float32)
+    s = surrogate.Surrogate(seed=3)
+    s.fit(X[:7000], y[:7000], epochs=40, batch=256, lr=1e-3)
+    pred = s.predict(X[7000:])
+    rho = surrogate.spearman(pred, y[7000:])
+    # a linear model cannot capture the product and |.| terms
+    A = np.c_[X[:7000], np.ones(7000)]
+    coef = np.linalg.lstsq(A, y[:7000], rcond=None)[0]
+    lin = np.c_[X[7000:], np.ones(len(X) - 7000)] @ coef
+    print("rho mlp", rho, "rho linear", surrogate.spearman(lin, y[7000:]), flush=True)
+    assert rho > 0.9 and rho > surrogate.spearman(lin, y[7000:]) + 0.1
+    assert surrogate.mae(pred, y[7000:]) < surrogate.mae(lin, y[7000:])
+
+
+@pytest.mark.gpu
+def test_adam_first_step_matches_oracle():
+    print("adam step", flush=True)
+    rng = np.random.default_rng(4)
+    net = surrogate.MLP([8, 16, 1], seed=5)
+    p0 = net.get_params()
+    x = rng.normal(size=(64, 8)).astype(np.float32)
+    t = rng.normal(size=(64, 1)).astype(np.float32)
+    _, g = net.loss_grad(x, t)
+    net.set_params(p0)
+    net.train(x, t, epochs=1, batch=64, lr=1e-2, seed=0)  # one full-batch Adam step
+    p1 = net.get_params()
+    po = p0.astype(np.float64).copy()
+    m = np.zeros_like(po)
+    v = np.zeros_like(po)
+    oracle.adam(po, g.astype(np.float64), m, v, 1e-2, 0.9, 0.999, 1e-8, 1)
+    assert np.allclose(p1, po, atol=1e-6)
