@@ -1,0 +1,398 @@
+"""Evolution loop over voxel genomes, with optional surrogate pre-screening.
+
+The caller of the evaluate path (optimize/optimizer/optimizer_impl.h:113-449,
+Evaluator_impl.h:39-107), rebuilt for the surrogate-screened workload:
+  * genomes: one material per voxel of an n^3 voxel grid (0 = empty);
+  * variation: swap crossover (uniform per-voxel swap between two parents)
+    with probability `crossover`, then per-voxel mutation with probability
+    `mutation`;
+  * ALPS (Hornby 2006, age-layered population structure): `layers` age
+    layers with polynomial age limits; layer 0 is re-seeded with random
+    genomes every `age_gap` generations; parents come from a layer and the
+    one below; each layer keeps its best `elitism` fraction and fills the
+    rest by fitness; individuals past their layer's age limit move up;
+    (the reference parses ALPS but never runs it, optimizer_impl.h:421-428);
+  * evaluation: every simulated robot counts one evaluation, as
+    Evaluator_impl.h:103-106 counts them; fitness is the COM-x displacement
+    after `duration` s on land;
+  * surrogate: scores `screen_factor` x offspring, the best `pop` are
+    simulated; retrained every `retrain_every` generations on all simulated
+    pairs; when its Spearman correlation on the last simulated batch falls
+    below `min_rho`, the next generations are simulated without screening
+    until a retrain brings it back.
+"""
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .materials import WORKLOAD_ROWS, workload_materials
+from .robots import last_layer_air, voxel_build
+from .simulator import SimConfig, Simulator
+from .surrogate import Surrogate, spearman
+
+LAND = dict(gravity=9.81, ground_k=1.0e5, mu_s=1.0, mu_k=0.8, drag=0.0, damping=0.99)
+
+
+def random_genomes(rng, n, voxels, n_materials=4, p_empty=0.2):
+    """(n, voxels^3) material indices: 0 = empty with p_empty, else 1..n_materials."""
+    g = rng.integers(1, n_materials + 1, (n, voxels ** 3)).astype(np.uint8)
+    g[rng.random((n, voxels ** 3)) < p_empty] = 0
+    return g
+
+
+def mutate(rng, g, rate, n_materials=4):
+    g = g.copy()
+    hit = rng.random(g.shape) < rate
+    g[hit] = rng.integers(0, n_materials + 1, int(hit.sum())).astype(np.uint8)
+    return g
+
+
+def swap_crossover(rng, a, b, rate):
+    """Per child: with probability `rate`, each voxel from the second parent
+    with probability 1/2; otherwise a copy of the first parent."""
+    do = rng.random(len(a)) < rate
+    swap = (rng.random(a.shape) < 0.5) & do[:, None]
+    return np.where(swap, b, a)
+
+
+def genome_to_lattice(g, voxels):
+    """Voxel materials -> encodings on the (voxels+1)^3 lattice (last layer air)."""
+    n = voxels + 1
+    idx = np.arange(n ** 3)
+    x, y, z = idx % n, (idx // n) % n, idx // (n * n)
+    inside = (x < voxels) & (y < voxels) & (z < voxels)
+    enc = np.ones((len(g), n ** 3), np.uint32)
+    vox_id = x[inside] + voxels * (y[inside] + voxels * z[inside])
+    mats = g[:, vox_id].astype(np.uint32)
+    enc[:, inside] = np.where(mats == 0, 1, np.uint32(1) << mats)
+    return enc
+
+
+def genome_features(g, voxels, rows=WORKLOAD_ROWS):
+    """64 features per genome, vectorized over the population: material
+    fractions, per-material centroids and spreads, solid-voxel moments and
+    bounding box, actuation-phase moments, ground contact, front/back
+    balance and the offset between the two actuation phases."""
+    nb = len(rows)
+    N = len(g)
+    v = voxels
+    idx = np.arange(v ** 3)
+    c = np.stack([idx % v, (idx // v) % v, idx // (v * v)], 1).astype(np.float32) / max(v - 1, 1)
+    feats = []
+    onehot = [(g == m).astype(np.float32) for m in range(nb + 1)]
+    cnt = [o.sum(1) for o in onehot]
+    feats += [x / g.shape[1] for x in cnt]                                  # nb+1
+    for m in range(1, nb + 1):
+        w = onehot[m]
+        s = np.maximum(cnt[m], 1)[:, None]
+        mean = (w @ c) / s
+        var = (w @ (c * c)) / s - mean * mean
+        feats += list(mean.T) + list(np.sqrt(np.maximum(var, 0)).T)           # 6 nb
+    solid = (g > 0).astype(np.float32)
+    ss = np.maximum(solid.sum(1), 1)[:, None]
+    mean = (solid @ c) / ss
+    var = (solid @ (c * c)) / ss - mean * mean
+    big = np.where(solid[:, :, None] > 0, c[None], np.inf)
+    lo = np.where(np.isfinite(big.min(1)), big.min(1), 0)
+    hi = np.where(solid[:, :, None] > 0, c[None], -np.inf).max(1)
+    hi = np.where(np.isfinite(hi), hi, 0)
+    feats += list(mean.T) + list(np.sqrt(np.maximum(var, 0)).T) + list(lo.T) + list(hi.T)  # 12
+    amp = np.array([0.0] + [r[1] for r in rows], np.float32)
+    ph = np.array([0.0] + [r[3] for r in rows], np.float32)
+    a = amp[g]
+    wc, ws = a * np.cos(ph[g]), a * np.sin(ph[g])
+    n_s = ss[:, 0]
+    for k in range(3):
+        feats += [(wc @ c[:, k]) / n_s, (ws @ c[:, k]) / n_s]
+    feats += [wc.sum(1) / n_s, ws.sum(1) / n_s]                               # 8
+    bottom = (c[:, 1] == 0).astype(np.float32)
+    nbot = max(bottom.sum(), 1)
+    for m in range(1, nb + 1):
+        feats.append((onehot[m] @ bottom) / nbot)
+    feats.append((solid @ bottom) / nbot)                                   # nb+1
+    front = (c[:, 0] >= 0.5).astype(np.float32)
+    for m in range(1, nb + 1):
+        feats.append((onehot[m] @ front) / np.maximum(cnt[m], 1))           # nb
+    act = [m for m in range(1, nb + 1) if amp[m] > 0][:2]
+    if len(act) == 2:
+        m0 = (onehot[act[0]] @ c) / np.maximum(cnt[act[0]], 1)[:, None]
+        m1 = (onehot[act[1]] @ c) / np.maximum(cnt[act[1]], 1)[:, None]
+        feats += list((m0 - m1).T)                                          # 3
+    out = np.stack([np.asarray(f, np.float32) for f in feats], 1)
+    if out.shape[1] < 64:
+        out = np.concatenate([out, np.zeros((N, 64 - out.shape[1]), np.float32)], 1)
+    return out[:, :64]
+
+
+class BatchSimulator:
+    """Builds and simulates genome batches; one evaluation per robot."""
+
+    def __init__(self, voxels=8, duration=1.0, dt=1e-4, spacing=0.1, mass=0.001, device=0):
+        self.voxels, self.duration = voxels, duration
+        self.ms = workload_materials()
+        self.res = 1.0 / spacing
+        self.mass = mass
+        self.sim = Simulator(self.ms, SimConfig(solver=_lib.MH_SOLVER_FORCE, time_step=dt,
+                                                device=device, **LAND))
+        self.evals = 0
+        self.sim_seconds = 0.0
+        self.tuned = False
+
+    def fitness(self, genomes):
+        enc = genome_to_lattice(genomes, self.voxels)
+        els = [voxel_build(self.ms, self.voxels + 1, self.res, e, mass=self.mass) for e in enc]
+        t0 = time.perf_counter()
+        self.sim.SetElements(els)
+        if not self.tuned:
+            self.sim.autotune(100)
+            self.tuned = True
+        self.sim.Reset()
+        self.sim.Simulate(self.duration)
+        f = self.sim.fitness(_lib.MH_FITNESS_DISPLACEMENT)
+        self.sim_seconds += time.perf_counter() - t0
+        self.evals += len(genomes)
+        return f.astype(np.float64)
+
+
+@dataclass
+class Layer:
+    genomes: np.ndarray
+    fitness: np.ndarray
+    age: np.ndarray
+
+
+@dataclass
+class RunResult:
+    evals_to_threshold: int = -1
+    seconds_to_threshold: float = -1.0
+    generations: int = 0
+    best: float = -np.inf
+    history: list = field(default_factory=list)
+    rho_history: list = field(default_factory=list)
+    fallback_generations: int = 0
+
+
+def age_limits(layers, age_gap):
+    """Polynomial ALPS age limits: age_gap * (1, 2, 4, 6, 9, 12, 16, ...), last unbounded."""
+    poly = [1, 2, 4, 6, 9, 12, 16, 20, 25, 30]
+    lim = [age_gap * poly[min(i, len(poly) - 1)] for i in range(layers - 1)]
+    return lim + [np.iinfo(np.int64).max]
+
+
+def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elitism=0.1,
+        age_gap=5, threshold=None, max_generations=40, use_surrogate=False, screen_factor=4,
+        retrain_every=5, min_rho=0.6, train_epochs=30, duration=1.0, sim=None, verbose=False):
+    """One evolution run; stops when the best fitness reaches `threshold` or
+    after max_generations.  Returns a RunResult."""
+    rng = np.random.default_rng(seed)
+    sim = sim or BatchSimulator(voxels=voxels, duration=duration)
+    evals0, t_start = sim.evals, time.perf_counter()
+    res = RunResult()
+    per = pop // layers
+    limits = age_limits(layers, age_gap)
+    data_x, data_y = [], []
+    sur = None
+    screening = use_surrogate
+
+    g0 = random_genomes(rng, pop, voxels)
+    f0 = sim.fitness(g0)
+    data_x.append(genome_features(g0, voxels))
+    data_y.append(f0)
+    order = np.argsort(-f0, kind="stable")
+    L = [Layer(g0[order[i * per:(i + 1) * per]], f0[order[i * per:(i + 1) * per]],
+               np.zeros(per, np.int64)) for i in range(layers)]
+
+    def record():
+        best = max(float(l.fitness.max()) for l in L if len(l.fitness))
+        res.best = max(res.best, best)
+        res.history.append((sim.evals - evals0, time.perf_counter() - t_start, res.best))
+        if threshold is not None and res.best >= threshold and res.evals_to_threshold < 0:
+            res.evals_to_threshold = sim.evals - evals0
+            res.seconds_to_threshold = time.perf_counter() - t_start
+        return res.evals_to_threshold >= 0
+
+    if record():
+        return res
+    for gen in range(1, max_generations + 1):
+        res.generations = gen
+        if use_surrogate and (sur is None or gen % retrain_every == 0):
+            sur = Surrogate(seed=seed + gen)
+            X = np.concatenate(data_x)[-200000:]
+            Y = np.concatenate(data_y)[-200000:]
+            sur.fit(X, Y, epochs=train_epochs, batch=512, lr=1e-3, seed=seed + gen)
+            screening = True
+        # offspring per layer: parents from the layer and the one below
+        n_child = pop * (screen_factor if (use_surrogate and screening) else 1)
+        kids, kid_age = [], []
+        for i in range(layers):
+            pool_g = np.concatenate([L[j].genomes for j in (i - 1, i) if j >= 0])
+            pool_f = np.concatenate([L[j].fitness for j in (i - 1, i) if j >= 0])
+            pool_a = np.concatenate([L[j].age for j in (i - 1, i) if j >= 0])
+            m = n_child // layers
+
+            def tournament():
+                a = rng.integers(0, len(pool_g), (m, 2))
+                return np.where(pool_f[a[:, 0]] >= pool_f[a[:, 1]], a[:, 0], a[:, 1])
+            pa, pb = tournament(), tournament()
+            child = mutate(rng, swap_crossover(rng, pool_g[pa], pool_g[pb], crossover), mutation)
+            kids.append(child)
+            kid_age.append(np.maximum(pool_a[pa], pool_a[pb]) + 1)
+        kids = np.concatenate(kids)
+        kid_age = np.concatenate(kid_age)
+        layer_of = np.repeat(np.arange(layers), n_child // layers)
+        if use_surrogate and screening:
+            pred = sur.predict(genome_features(kids, voxels))
+            keep = []
+            for i in range(layers):  # keep the top share of every layer's offspring
+                sel = np.where(layer_of == i)[0]
+                keep.append(sel[np.argsort(-pred[sel], kind="stable")[:per]])
+            keep = np.concatenate(keep)
+            kids, kid_age, layer_of, pred = kids[keep], kid_age[keep], layer_of[keep], pred[keep]
+        else:
+            res.fallback_generations += int(use_surrogate)
+        fit = sim.fitness(kids)
+        feats = genome_features(kids, voxels)
+        data_x.append(feats)
+        data_y.append(fit)
+        if use_surrogate and screening:
+            rho = spearman(pred, fit)
+            res.rho_history.append(rho)
+            if rho < min_rho:
+                screening = False
+        elif use_surrogate:
+            rho = spearman(sur.predict(feats), fit)
+            res.rho_history.append(rho)
+            screening = rho >= min_rho
+        # replacement: merge, move over-age individuals up, keep elites + best
+        for i in range(layers):
+            sel = layer_of == i
+            g = np.concatenate([L[i].genomes, kids[sel]])
+            f = np.concatenate([L[i].fitness, fit[sel]])
+            a = np.concatenate([L[i].age + 1, kid_age[sel]])
+            L[i] = Layer(g, f, a)
+        for i in range(layers - 1):
+            old = L[i].age > limits[i]
+            if old.any():
+                up = L[i + 1]
+                L[i + 1] = Layer(np.concatenate([up.genomes, L[i].genomes[old]]),
+                                 np.concatenate([up.fitness, L[i].fitness[old]]),
+                                 np.concatenate([up.age, L[i].age[old]]))
+                L[i] = Layer(L[i].genomes[~old], L[i].fitness[~old], L[i].age[~old])
+        for i in range(layers):
+            o = np.argsort(-L[i].fitness, kind="stable")
+            n_elite = max(1, int(round(elitism * per)))
+            keep = o[:max(per, n_elite)]
+            L[i] = Layer(L[i].genomes[keep], L[i].fitness[keep], L[i].age[keep])
+        if gen % age_gap == 0:  # ALPS: fresh random genomes into the youngest layer
+            g = random_genomes(rng, per, voxels)
+            f = sim.fitness(g)
+            data_x.append(genome_features(g, voxels))
+            data_y.append(f)
+            L[0] = Layer(g, f, np.zeros(per, np.int64))
+        done = record()
+        if verbose:
+            print(f"gen {gen} evals {sim.evals - evals0} best {res.best:.5f} "
+                  f"{'rho %.3f' % res.rho_history[-1] if res.rho_history else ''}", flush=True)
+        if done:
+            break
+    return res
+
+
+def surrogate_quality(n_train=200000, n_test=20000, voxels=8, duration=1.0, seed=0,
+                      epochs=30, batch=512, chunk=4096, sim=None, verbose=True):
+    """Simulate n_train + n_test random genomes, train the surrogate on the
+    first n_train and report Spearman rho and MAE on the held-out rest."""
+    rng = np.random.default_rng(seed)
+    sim = sim or BatchSimulator(voxels=voxels, duration=duration)
+    n = n_train + n_test
+    X = np.zeros((n, 64), np.float32)
+    Y = np.zeros(n, np.float64)
+    t0 = time.perf_counter()
+    for a in range(0, n, chunk):
+        g = random_genomes(rng, min(chunk, n - a), voxels)
+        Y[a:a + len(g)] = sim.fitness(g)
+        X[a:a + len(g)] = genome_features(g, voxels)
+        if verbose:
+            print(f"simulated {a + len(g)}/{n}", flush=True)
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sur = Surrogate(seed=seed)
+    loss = sur.fit(X[:n_train], Y[:n_train], epochs=epochs, batch=batch, lr=1e-3, seed=seed)
+    t_train = time.perf_counter() - t0
+    pred = sur.predict(X[n_train:])
+    from .surrogate import mae
+    return dict(n_train=n_train, n_test=n_test, spearman=spearman(pred, Y[n_train:]),
+                mae=mae(pred, Y[n_train:]), fitness_mean=float(Y.mean()), fitness_std=float(Y.std()),
+                final_train_loss=loss, data_seconds=t_data, train_seconds=t_train,
+                sim_seconds=sim.sim_seconds)
+
+
+def compare(seeds=5, threshold=None, pop=4096, max_generations=40, voxels=8, duration=1.0,
+            verbose=True, **kw):
+    """Evaluations (and wall-clock) to reach `threshold`, plain GA vs
+    surrogate-screened GA, over the given seeds."""
+    sim = BatchSimulator(voxels=voxels, duration=duration)
+    out = []
+    for s in range(seeds):
+        for use in (False, True):
+            r = run(seed=s, pop=pop, voxels=voxels, threshold=threshold, use_surrogate=use,
+                    max_generations=max_generations, sim=sim, verbose=verbose, **kw)
+            row = dict(seed=s, surrogate=use, evals_to_threshold=r.evals_to_threshold,
+                       seconds_to_threshold=r.seconds_to_threshold, generations=r.generations,
+                       best=r.best, fallback_generations=r.fallback_generations,
+                       rho=[round(x, 3) for x in r.rho_history])
+            out.append(row)
+            if verbose:
+                import json
+                print(json.dumps(row), flush=True)
+    return out
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="voxel-robot evolution with surrogate screening")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    q = sub.add_parser("quality", help="surrogate Spearman/MAE on held-out simulated genomes")
+    q.add_argument("--train", type=int, default=200000)
+    q.add_argument("--test", type=int, default=20000)
+    q.add_argument("--epochs", type=int, default=30)
+    q.add_argument("--seed", type=int, default=0)
+    c = sub.add_parser("compare", help="evaluations to threshold with and without the surrogate")
+    c.add_argument("--seeds", type=int, default=5)
+    c.add_argument("--threshold", type=float, required=True)
+    c.add_argument("--pop", type=int, default=4096)
+    c.add_argument("--generations", type=int, default=40)
+    c.add_argument("--screen-factor", type=int, default=4)
+    c.add_argument("--epochs", type=int, default=30)
+    r = sub.add_parser("run", help="one evolution run")
+    r.add_argument("--seed", type=int, default=0)
+    r.add_argument("--pop", type=int, default=4096)
+    r.add_argument("--generations", type=int, default=10)
+    r.add_argument("--surrogate", action="store_true")
+    for p in (q, c, r):
+        p.add_argument("--voxels", type=int, default=8)
+        p.add_argument("--duration", type=float, default=1.0)
+    a = ap.parse_args(argv)
+    if a.cmd == "quality":
+        res = surrogate_quality(a.train, a.test, voxels=a.voxels, duration=a.duration,
+                                seed=a.seed, epochs=a.epochs)
+        print(json.dumps(res), flush=True)
+    elif a.cmd == "compare":
+        rows = compare(a.seeds, a.threshold, pop=a.pop, max_generations=a.generations,
+                       voxels=a.voxels, duration=a.duration, screen_factor=a.screen_factor,
+                       train_epochs=a.epochs)
+        for use in (False, True):
+            e = [x["evals_to_threshold"] for x in rows if x["surrogate"] == use]
+            print(json.dumps(dict(surrogate=use, evals_to_threshold=e,
+                                  reached=sum(v >= 0 for v in e))), flush=True)
+    else:
+        res = run(seed=a.seed, pop=a.pop, voxels=a.voxels, duration=a.duration,
+                  max_generations=a.generations, use_surrogate=a.surrogate, verbose=True)
+        print(json.dumps(dict(best=res.best, history=res.history, rho=res.rho_history)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,128 @@
+"""Evolution loop pieces on the CPU: genome encoding, variation operators,
+ALPS layer bookkeeping and the surrogate fallback switch (driven by a
+stand-in fitness so no GPU is needed), plus one short GPU run."""
+import numpy as np
+import pytest
+
+import mercury_amd as mh
+from mercury_amd import evolve, surrogate
+
+
+def test_genome_to_lattice_matches_voxel_encoding_layout():
+    rng = np.random.default_rng(0)
+    g = evolve.random_genomes(rng, 3, 4)
+    enc = evolve.genome_to_lattice(g, 4)
+    n = 5
+    assert enc.shape == (3, n ** 3)
+    # the last lattice layer in every axis is air
+    assert np.array_equal(mh.robots.last_layer_air(n, enc[0]), enc[0])
+    # voxel (x, y, z) sits at lattice x + n*(y + n*z) with one bit per material
+    for r in range(3):
+        for v in range(64):
+            x, y, z = v % 4, (v // 4) % 4, v // 16
+            e = enc[r, x + n * (y + n * z)]
+            assert e == (1 if g[r, v] == 0 else 1 << int(g[r, v]))
+
+
+def test_variation_operators():
+    rng = np.random.default_rng(1)
+    a = evolve.random_genomes(rng, 200, 4)
+    b = evolve.random_genomes(rng, 200, 4)
+    assert np.array_equal(evolve.swap_crossover(rng, a, b, 0.0), a)
+    c = evolve.swap_crossover(rng, a, b, 1.0)
+    assert ((c == a) | (c == b)).all()
+    frac_b = ((c == b) & (a != b)).sum() / (a != b).sum()
+    assert 0.45 < frac_b < 0.55
+    assert np.array_equal(evolve.mutate(rng, a, 0.0), a)
+    m = evolve.mutate(rng, a, 1.0)
+    assert m.min() >= 0 and m.max() <= 4
+    changed = (evolve.mutate(rng, a, 0.05) != a).mean()
+    assert 0.02 < changed < 0.06  # 0.05 x P(new draw differs) = 0.05 x 4/5
+
+
+def test_age_limits_polynomial():
+    lim = evolve.age_limits(4, 5)
+    assert lim[:3] == [5, 10, 20] and lim[3] > 10 ** 12
+
+
+def test_vectorized_features_agree_with_element_features():
+    """The genome-array features and the built-robot features describe the
+    same material layout when Strip removes nothing."""
+    ms = mh.workload_materials()
+    g = np.full((1, 27), 1, np.uint8)
+    g[0, ::2] = 3
+    g[0, 1::4] = 4
+    enc = evolve.genome_to_lattice(g, 3)
+    el = mh.robots.voxel_build(ms, 4, 10.0, enc[0], mass=0.001)
+    fa = evolve.genome_features(g, 3)[0]
+    fb = surrogate.genome_features([el], 3, mh.materials.WORKLOAD_ROWS)[0]
+    # material fractions, per-material centroids/spreads, solid moments, bbox
+    np.testing.assert_allclose(fa[:5 + 24 + 12], fb[:5 + 24 + 12], atol=1e-5)
+
+
+class _FakeSim:
+    """Fitness = share of phase-0 actuated voxels minus share of empty ones."""
+
+    def __init__(self):
+        self.evals = 0
+
+    def fitness(self, g):
+        self.evals += len(g)
+        return (g == 3).mean(1) - (g == 0).mean(1)
+
+
+def test_plain_run_improves_and_counts_evaluations():
+    sim = _FakeSim()
+    r = evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=6, sim=sim)
+    assert r.generations == 6
+    # initial population + pop per generation + a fresh layer every age_gap
+    assert sim.evals == 256 + 6 * 256 + (6 // 5) * (256 // 4)
+    best = [h[2] for h in r.history]
+    assert all(b1 >= b0 for b0, b1 in zip(best, best[1:])) and best[-1] > best[0]
+
+
+def test_run_stops_at_threshold():
+    sim = _FakeSim()
+    r = evolve.run(seed=0, pop=256, voxels=4, layers=4, threshold=0.25, max_generations=50,
+                   sim=sim)
+    assert r.best >= 0.25 and r.evals_to_threshold == sim.evals
+
+
+class _FakeSurrogate:
+    """Predicts fitness with a fixed sign: +1 informative, -1 anti-correlated."""
+    sign = 1.0
+
+    def __init__(self, seed=0, **kw):
+        pass
+
+    def fit(self, X, y, **kw):
+        return 0.0
+
+    def predict(self, X):
+        return self.sign * (X[:, 3] - X[:, 0])  # share of material 3 minus empty share
+
+
+def test_surrogate_screening_and_rho_fallback(monkeypatch):
+    monkeypatch.setattr(evolve, "Surrogate", _FakeSurrogate)
+    _FakeSurrogate.sign = 1.0
+    good = evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=4, sim=_FakeSim(),
+                      use_surrogate=True)
+    assert good.fallback_generations == 0 and min(good.rho_history) > 0.6
+    plain = evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=4, sim=_FakeSim())
+    assert good.best >= plain.best
+    _FakeSurrogate.sign = -1.0
+    bad = evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=4, sim=_FakeSim(),
+                     use_surrogate=True)
+    # first screened generation shows rho < 0.6; the following ones are unscreened
+    assert bad.rho_history[0] < 0.6 and bad.fallback_generations >= 3
+
+
+@pytest.mark.gpu
+def test_short_gpu_evolution_run():
+    print("evolve: 2 generations, pop 256, 4^3 voxels, 0.05 s", flush=True)
+    sim = evolve.BatchSimulator(voxels=4, duration=0.05)
+    r = evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=2, sim=sim)
+    assert sim.evals == 256 * 3 and np.isfinite(r.best)
+    r2 = evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=2, use_surrogate=True,
+                    train_epochs=2, sim=sim)
+    assert len(r2.rho_history) == 2 and np.isfinite(r2.best)
