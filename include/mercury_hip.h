@@ -244,6 +244,16 @@ int mh_calibrate_lds(int device, float* gbytes_per_s);
  * reference for scaling hardware byte counters. */
 int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms);
 
+/* No reference counterpart: checks, bit for bit, the fast arithmetic the
+ * step kernels rely on against the compiler's correctly rounded operations,
+ * over the float bit patterns [begin, begin + count).  which: 0 hardware
+ * sqrt, 1 the kernels' in-range correctly rounded sqrt, 2 hardware
+ * reciprocal, 3 x against the next float (every pattern differs; a check
+ * of the checker), 4 the kernels' in-range correctly rounded reciprocal.
+ * *mismatches receives the number of differing results. */
+int mh_check_arith(int device, int which, uint32_t begin, uint32_t count,
+                   uint64_t* mismatches);
+
 #ifdef __cplusplus
 }
 #endif

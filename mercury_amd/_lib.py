@@ -104,6 +104,8 @@ _SIGS = {
     "mh_mlp_train": (c_int, [vp, f32p, f32p, c_int, c_int, c_int, c_float, c_uint64, f32p]),
     "mh_calibrate_lds": (c_int, [c_int, f32p]),
     "mh_calibrate_copy": (c_int, [c_int, ctypes.c_size_t, c_int, f32p]),
+    "mh_check_arith": (c_int, [c_int, c_int, ctypes.c_uint32, ctypes.c_uint32,
+                               ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 _lib = None

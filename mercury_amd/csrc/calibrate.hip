@@ -9,7 +9,10 @@
 //                      WRITE_SIZE counters to bytes.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "mercury_hip.h"
+#include "mh_device.h"
 #include "mh_internal.h"
 
 namespace {
@@ -42,6 +45,50 @@ __global__ void copy_f4(const float4* __restrict__ src, float4* __restrict__ dst
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
     dst[i] = src[i];
+}
+
+// Bit-exact comparison of the fast arithmetic sequences with the compiler's
+// correctly rounded operations over a contiguous range of float bit
+// patterns; one mismatch count per block.
+//   which 0: hardware sqrt          vs sqrtf
+//         1: mh::sqrt_rn_inrange    vs sqrtf
+//         2: hardware reciprocal    vs 1.0f / x
+//         3: x                      vs the next float (all differ: checks
+//                                      the counting itself)
+//         4: mh::rcp_rn_inrange     vs 1.0f / x
+__global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, uint32_t count,
+                                                   uint32_t* out) {
+  __shared__ uint32_t part[256];
+  uint32_t bad = 0;
+  const uint32_t stride = gridDim.x * 256u;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < count; i += stride) {
+    const float x = __uint_as_float(begin + i);
+    float a, b;
+    if (which == 0) {
+      a = __builtin_amdgcn_sqrtf(x);
+      b = sqrtf(x);
+    } else if (which == 1) {
+      a = mh::sqrt_rn_inrange(x);
+      b = sqrtf(x);
+    } else if (which == 2) {
+      a = __builtin_amdgcn_rcpf(x);
+      b = 1.0f / x;
+    } else if (which == 3) {
+      a = x;
+      b = __uint_as_float(__float_as_uint(x) + 1u);
+    } else {
+      a = mh::rcp_rn_inrange(x);
+      b = 1.0f / x;
+    }
+    bad += (__float_as_uint(a) != __float_as_uint(b)) ? 1u : 0u;
+  }
+  part[threadIdx.x] = bad;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = part[0];
 }
 
 int fail(hipError_t e, const char* what) {
@@ -113,6 +160,26 @@ int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms) {
   hipFree(dst);
   if (e != hipSuccess) return fail(e, "copy calibration");
   *avg_ms = ms / reps;
+  return MH_OK;
+}
+
+int mh_check_arith(int device, int which, uint32_t begin, uint32_t count, uint64_t* mismatches) {
+  if (!mismatches || which < 0 || which > 4 || count == 0 ||
+      static_cast<uint64_t>(begin) + count > 0x100000000ull)
+    return MH_ERR_ARG;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(e, "hipSetDevice");
+  const int blocks = 256 * 32;
+  uint32_t* out = nullptr;
+  if ((e = hipMalloc(&out, sizeof(uint32_t) * blocks)) != hipSuccess) return fail(e, "hipMalloc");
+  hipLaunchKernelGGL(check_arith, dim3(blocks), dim3(256), 0, nullptr, which, begin, count, out);
+  std::vector<uint32_t> h(blocks);
+  e = hipMemcpy(h.data(), out, sizeof(uint32_t) * blocks, hipMemcpyDeviceToHost);
+  hipFree(out);
+  if (e != hipSuccess) return fail(e, "arith check");
+  uint64_t total = 0;
+  for (uint32_t v : h) total += v;
+  *mismatches = total;
   return MH_OK;
 }
 

@@ -44,11 +44,73 @@ __device__ __forceinline__ float act_sin(float x) {
   return (quadrant & 2) ? -v : v;
 }
 
-// |(x, y, z)| as sqrtf((x*x + y*y) + z*z), correctly rounded sqrt.
+
+// Correctly rounded sqrt for x in [2^-96, 2^126] without the range
+// scaling and special-value handling of the general sequence: the hardware
+// square root (within 1 ulp) is corrected to the nearest float by the signs
+// of the residuals x - s*(s -+ 1 ulp), the rounding test of Tuckerman
+// (Markstein 1990, "Computation of elementary functions on the IBM RISC
+// System/6000 processor").  Checked against sqrtf for every float in range
+// by mh_check_arith.
+__device__ __forceinline__ float sqrt_rn_inrange(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
+  const float r_dn = fmaf(-s_dn, s, x);
+  const float r_up = fmaf(-s_up, s, x);
+  float r = (r_dn <= 0.f) ? s_dn : s;
+  r = (r_up > 0.f) ? s_up : r;
+  return r;
+}
+constexpr float kSqrtFastLo = 0x1p-96f;
+constexpr float kSqrtFastHi = 0x1p+126f;
+
+// Correctly rounded 1/y for y in [2^-64, 2^64]: the hardware reciprocal
+// (within 1 ulp) refined by one Newton step with a fused residual,
+// r + r (1 - y r).  Checked against 1.0f / y for every float in range by
+// mh_check_arith.
+__device__ __forceinline__ float rcp_rn_inrange(float y) {
+  const float r = __builtin_amdgcn_rcpf(y);
+  const float e = fmaf(-y, r, 1.0f);
+  return fmaf(e, r, r);
+}
+constexpr float kRcpFastLo = 0x1p-64f;
+constexpr float kRcpFastHi = 0x1p+64f;
+
+// |(x, y, z)| as sqrtf((x*x + y*y) + z*z), correctly rounded sqrt.  When
+// every lane of the wave has its sum of squares in the fast range (the
+// usual case: spring lengths are far from 2^-48 and 2^63) the short
+// sequence runs; otherwise the whole wave takes the general one.  The
+// branch is wave-uniform, so no lane pays for both.
 __device__ __forceinline__ float norm3(float x, float y, float z) {
   float s = x * x + y * y;
   s = s + z * z;
+  // s is +0, positive or NaN: one unsigned compare of the bit pattern
+  // tests kSqrtFastLo <= s <= kSqrtFastHi
+  const bool fast = __float_as_uint(s) - __float_as_uint(kSqrtFastLo) <=
+                    __float_as_uint(kSqrtFastHi) - __float_as_uint(kSqrtFastLo);
+  if (__builtin_amdgcn_ballot_w64(fast) == __builtin_amdgcn_read_exec())
+    return sqrt_rn_inrange(s);
   return sqrtf(s);
+}
+
+// d = |(x, y, z)| and inv = 1 / (d + eps), both correctly rounded.  For a
+// sum of squares in [2^-96, 2^126], d lies in [2^-48, 2^63] and d + eps
+// (eps <= 2^-40) in the fast reciprocal's range, so one wave-uniform test
+// covers both short sequences.
+__device__ __forceinline__ void length_and_inverse(float x, float y, float z, float eps,
+                                                   float& d, float& inv) {
+  float s = x * x + y * y;
+  s = s + z * z;
+  const bool fast = __float_as_uint(s) - __float_as_uint(kSqrtFastLo) <=
+                    __float_as_uint(kSqrtFastHi) - __float_as_uint(kSqrtFastLo);
+  if (__builtin_amdgcn_ballot_w64(fast) == __builtin_amdgcn_read_exec()) {
+    d = sqrt_rn_inrange(s);
+    inv = rcp_rn_inrange(d + eps);
+  } else {
+    d = sqrtf(s);
+    inv = 1.0f / (d + eps);
+  }
 }
 
 // Rest-length factor A * sin(omega * t + phi) of one material at time t.

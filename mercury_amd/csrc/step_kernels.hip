@@ -18,7 +18,7 @@
 //   XPBD  x~ = x + v dt;  lambda = -(d - L)/(2 + ((1/k)/dt)/dt);
 //         dp = lambda * (x~a - x~b)/(d + 1e-12);  x' = x~ + sum(+-dp);
 //         v' = (float)(0.99 * (double)(x' - x) / (double)dt)
-//   force c = k (L - d) (xa - xb)/(d + 1e-12);  F = sum(+-c) - m g y^ - c_d v
+//   force c = ((k (L - d)) (1 / (d + 1e-12))) (xa - xb);  F = sum(+-c) - m g y^ - c_d v
 //         + ground normal/friction;  u = v + (F/m) dt;  x' = x + u dt;
 //         v' = damping * u
 //   L = fma(Lbar, A sin(fma(omega, t, phi)), Lbar)
@@ -130,21 +130,25 @@ __device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, fl
   const float dx = pa.x - pb.x;
   const float dy = pa.y - pb.y;
   const float dz = pa.z - pb.z;
-  const float d = norm3(dx, dy, dz);
-  const float den = d + kEps;
-  const float nx = dx / den, ny = dy / den, nz = dz / den;
   if constexpr (SOLVER == MH_SOLVER_XPBD_REF) {
+    const float d = norm3(dx, dy, dz);
+    const float den = d + kEps;
+    const float nx = dx / den, ny = dy / den, nz = dz / den;
     const float C = d - L;
     lambda = -C / kk;
     cx = lambda * nx;
     cy = lambda * ny;
     cz = lambda * nz;
   } else {
-    const float f = kk * (L - d);
+    // Hooke's law with one reciprocal: f = k (L - d) * (1 / (d + eps)),
+    // c = f * delta
+    float d, inv;
+    length_and_inverse(dx, dy, dz, kEps, d, inv);
+    const float f = (kk * (L - d)) * inv;
     lambda = 0.f;
-    cx = f * nx;
-    cy = f * ny;
-    cz = f * nz;
+    cx = f * dx;
+    cy = f * dy;
+    cz = f * dz;
   }
 }
 

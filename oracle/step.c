@@ -25,8 +25,10 @@
  *
  * force (project spec; the reference snapshot has no force solver -- paper
  * p.2 Alg. 1 and the land/water environment of paper p.2 Sec. II.E):
- *   per non-air spring: L as above; d, n as above;
- *        c = (k * (L - d)) * n;  +c on a, -c on b  (ascending spring index)
+ *   per non-air spring: L and d as above;
+ *        f = (k * (L - d)) * (1 / (d + 1e-12f));  c = f * (x~a - x~b)
+ *        (Hooke's law with one correctly rounded reciprocal per spring);
+ *        +c on a, -c on b  (ascending spring index)
  *   per mass: F = sum(c)
  *        F.y = F.y - m*g
  *        F   = F - drag*v                       (if drag != 0)
@@ -280,10 +282,9 @@ static void force32_robot(job_t* j, int r) {
       float dy = x[3 * a + 1] - x[3 * b + 1];
       float dz = x[3 * a + 2] - x[3 * b + 2];
       float d = nrm3f(dx, dy, dz);
-      float den = d + ORC_EPS;
-      float nx = dx / den, ny = dy / den, nz = dz / den;
-      float f = mt->k[m] * (L - d);
-      float cx = f * nx, cy = f * ny, cz = f * nz;
+      float inv = 1.0f / (d + ORC_EPS);
+      float f = (mt->k[m] * (L - d)) * inv;
+      float cx = f * dx, cy = f * dy, cz = f * dz;
       acc[3 * a] = acc[3 * a] + cx;
       acc[3 * a + 1] = acc[3 * a + 1] + cy;
       acc[3 * a + 2] = acc[3 * a + 2] + cz;

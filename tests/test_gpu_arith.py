@@ -1,0 +1,37 @@
+"""The fast correctly rounded sqrt used by the step kernels, checked bit for
+bit against the compiler's correctly rounded sqrtf for EVERY float in its
+range (about 1.9e9 inputs), on the GPU."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import mercury_amd as mh
+from mercury_amd._lib import check
+
+
+def bits(x):
+    return int(np.float32(x).view(np.uint32))
+
+
+def mismatches(which, lo, hi):
+    n = ctypes.c_uint64()
+    check(mh.load().mh_check_arith(0, which, lo, hi - lo, ctypes.byref(n)))
+    return n.value
+
+
+@pytest.mark.gpu
+def test_fast_sqrt_is_correctly_rounded_over_its_whole_range():
+    print("arith: sqrt_rn_inrange over [2^-96, 2^126]", flush=True)
+    lo, hi = bits(2.0 ** -96), bits(2.0 ** 126) + 1
+    assert mismatches(1, lo, hi) == 0
+    # informational: how often the uncorrected hardware sqrt differs
+    print("hardware sqrt mismatches in range:", mismatches(0, lo, hi), flush=True)
+
+
+@pytest.mark.gpu
+def test_check_arith_counts_every_difference():
+    print("arith: checker self-test", flush=True)
+    lo, hi = bits(1.0), bits(4.0)
+    assert mismatches(3, lo, hi) == hi - lo
+    print("hardware rcp mismatches in [1, 4):", mismatches(2, lo, hi), flush=True)

@@ -247,3 +247,75 @@ def test_step_times_match_float_loop():
             t = np.float32(t + np.float32(dt))
             r = np.float32(r - np.float32(dt))
         assert np.array_equal(ts, np.array(exp, np.float32))
+
+
+def numpy_force_step_f32(pos, vel, mass, pairs, lbar, mat, k, R, env):
+    """Independent float32 restatement of one force-solver step: per-mass
+    gather of f * delta terms, gravity, drag, ground contact and friction."""
+    f = np.float32
+    dt, g, kg, mus, muk, c, damp = (f(env[key]) for key in
+                                    ("dt", "gravity", "ground_k", "mu_s", "mu_k", "drag", "damping"))
+    terms = {}
+    for s, (a, b) in enumerate(pairs):
+        if mat[s] == 0:
+            continue
+        L = fma32(lbar[s], R[mat[s]], lbar[s])
+        d3 = (pos[a] - pos[b]).astype(np.float32)
+        ss = f(f(d3[0] * d3[0]) + f(d3[1] * d3[1]))
+        d = f(np.sqrt(f(ss + f(d3[2] * d3[2]))))
+        fs = f(f(k[mat[s]] * f(L - d)) * f(f(1) / f(d + f(1e-12))))
+        terms[s] = (fs * d3).astype(np.float32)
+    out_p, out_v = pos.copy(), vel.copy()
+    for i in range(len(pos)):
+        F = np.zeros(3, np.float32)
+        for s, (a, b) in enumerate(pairs):
+            if s in terms and a == i:
+                F = (F + terms[s]).astype(np.float32)
+            if s in terms and b == i:
+                F = (F - terms[s]).astype(np.float32)
+        v = vel[i].copy()
+        F[1] = f(F[1] - f(mass[i] * g))
+        if c != 0:
+            F = (F - (c * v).astype(np.float32)).astype(np.float32)
+        y = pos[i, 1]
+        if kg != 0 and y < 0:
+            fn = f(kg * f(-y))
+            F[1] = f(F[1] + fn)
+            fh = f(np.sqrt(f(f(F[0] * F[0]) + f(F[2] * F[2]))))
+            if fh <= f(mus * fn):
+                F[0] = F[2] = v[0] = v[2] = f(0)
+            else:
+                sc = f(f(1) - f(f(muk * fn) / fh))
+                F[0], F[2] = f(F[0] * sc), f(F[2] * sc)
+        inv = f(f(1) / mass[i])
+        u = (v + ((F * inv).astype(np.float32) * dt).astype(np.float32)).astype(np.float32)
+        out_p[i] = (pos[i] + (u * dt).astype(np.float32)).astype(np.float32)
+        out_v[i] = (damp * u).astype(np.float32)
+    return out_p, out_v
+
+
+@pytest.mark.parametrize("drag", [0.0, 1.0])
+def test_force_mirror_matches_independent_restatement(drag):
+    rng = np.random.default_rng(11)
+    n = 10
+    pos = (rng.random((n, 3)) * 0.3).astype(np.float32)
+    pos[:4, 1] = -rng.random(4).astype(np.float32) * 1e-4   # four masses in ground contact
+    vel = (rng.normal(0, 0.05, (n, 3))).astype(np.float32)
+    pairs = [(int(a), int(b)) for a, b in rng.integers(0, n, (24, 2)) if a != b]
+    m = len(pairs)
+    lbar = (rng.random(m) * 0.2 + 0.05).astype(np.float32)
+    mat = rng.integers(0, 3, m).astype(np.uint8)
+    rows = [AIR, (5e3, 0.14, 12.566371, 0.0), (1e4, 0.0, 12.566371, 0.0)]
+    mt = table(rows)
+    env = dict(dt=1e-4, gravity=9.81, ground_k=1e5, mu_s=1.0, mu_k=0.8, drag=drag, damping=0.99)
+    t0 = np.float32(0.3)
+    k = np.array([0.0] + [r[0] for r in rows[1:]], np.float32)
+    R = np.array([np.float32(r[1]) * oracle.sinf(fma32(np.float32(r[2]), t0, np.float32(r[3])))
+                  for r in rows], np.float32)
+    mass = np.full(n, 1e-3, np.float32)
+    pop, _ = single_robot(pos, pairs, lbar, mat, mass=mass)
+    e = oracle.make_env(**env)
+    p, v = oracle.force_f32(pop, mt, e, float(t0), 1, pos, vel)
+    ep, ev = numpy_force_step_f32(pos, vel, mass, pairs, lbar, mat, k, R, env)
+    assert np.array_equal(np.asarray(p).reshape(-1, 3).view(np.uint32), ep.view(np.uint32))
+    assert np.array_equal(np.asarray(v).reshape(-1, 3).view(np.uint32), ev.view(np.uint32))
