@@ -63,6 +63,16 @@ const KernelVariant* kernel_variants(int* count);
 
 // LDS bytes a pack of (masses, springs, materials) needs in the LDS kernel;
 // the streaming kernel needs only masses and the material table.
+// LDS position slot of pack-local mass i in the LDS kernel, and the slot
+// count for n masses: one unused slot after every 8 skews consecutive
+// masses across the LDS banks.
+__host__ __device__ inline int pos_slot(int i) { return i + (i >> 3); }
+__host__ __device__ inline int pos_slots(int n) { return n + (n >> 3) + 1; }
+// Bytes of the LDS spring-term array (3 floats per slot, spring_cap + 2
+// slots), rounded up to 16 B so the float4 positions after it stay aligned.
+__host__ __device__ inline size_t lds_term_bytes(int spring_cap) {
+  return (12u * static_cast<size_t>(spring_cap + 2) + 15u) & ~static_cast<size_t>(15);
+}
 size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat);
 size_t lds_bytes_stream(int mass_cap, int n_mat);
 

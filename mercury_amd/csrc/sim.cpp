@@ -215,7 +215,8 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
       const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
       const int sp = static_cast<int>(d.n_spring) + robot_active(s, r);
       const int g = gathers + static_cast<int>(s->gather_pad[r]);
-      const bool fits = m <= mcap && sp <= scap && sp + 2 <= 32767 && g <= 65535 &&
+      // gather entries hold 3 * slot in 16 bits: slots up to spring_cap + 1
+      const bool fits = m <= mcap && sp <= scap && 3 * (sp + 1) <= 65535 && g <= 65535 &&
                         lds_bytes_lds(std::max(lp.mass_cap, m), std::max(lp.spring_cap, sp),
                                       std::max(lp.csr_cap, g), n_mat) <= static_cast<size_t>(s->lds_limit);
       if (!fits) break;
@@ -243,26 +244,33 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
 }
 
 // Device arrays that depend on the packing: pack-local spring pairs, the
-// gather lists (ascending spring index per mass; LDS parts pad every row to
-// a multiple of 4 with the zero slot), row starts and phase-2 order.
+// gather lists, row bounds and phase-2 order.  Mass i's row is two parts:
+// the springs whose first endpoint it is, then those whose second endpoint
+// it is, each in ascending spring index; row[2i], row[2i+1], row[2i+2]
+// bound them.  LDS parts store entries as 3 * slot and pad each part to a
+// multiple of 4 with the zero slot; streaming parts store the slot.
 void upload_plan(mh_sim* s, Plan& plan) {
   const size_t na = s->n_active;
   std::vector<uint32_t> spair(na), csr, row;
-  csr.reserve(2 * na + 3 * s->n_masses);
+  csr.reserve(2 * na + 6 * s->n_masses);
   std::vector<uint16_t> perm(s->n_masses);
-  std::vector<uint32_t> order, count;
+  std::vector<uint32_t> order, count, nplus, nminus, lpair;
   std::vector<PackDesc> all;
-  row.reserve(s->n_masses + 64);
+  row.reserve(2 * s->n_masses + 64);
   bool need_scratch = false;
   for (Part& part : plan.parts) {
     const bool padded = part.v->kind == MH_KERNEL_LDS;
     need_scratch |= part.v->kind == MH_KERNEL_STREAM;
-    const uint32_t zero_entry = static_cast<uint32_t>(part.spring_cap + 1) << 1;
+    const uint32_t zero_entry = 3u * static_cast<uint32_t>(part.spring_cap + 1);
+    const uint32_t scale = padded ? 3u : 1u;
+    auto pad = [padded](uint32_t n) { return padded ? (n + 3u) & ~3u : n; };
     for (PackDesc& d : part.packs) {
       d.csr_begin = static_cast<uint32_t>(csr.size());
       d.row_begin = static_cast<uint32_t>(row.size());
       const uint32_t nm = d.n_mass;
-      count.assign(nm + 1, 0);
+      nplus.assign(nm, 0);
+      nminus.assign(nm, 0);
+      lpair.resize(2 * static_cast<size_t>(d.n_spring));
       // pack-local pairs: robot-local index + robot's offset inside the pack
       int r = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
                                s->mass_off.begin()) - 1;
@@ -271,28 +279,39 @@ void upload_plan(mh_sim* s, Plan& plan) {
         while (g >= s->active_off[r + 1]) r++;
         const uint32_t off = s->mass_off[r] - d.mass_begin;
         const uint32_t a = s->act_pairs[2 * g] + off, b = s->act_pairs[2 * g + 1] + off;
-        spair[g] = a | (b << 16);
-        count[a + 1]++;
-        count[b + 1]++;
+        // LDS parts address positions by skewed slot
+        spair[g] = padded ? static_cast<uint32_t>(pos_slot(static_cast<int>(a))) |
+                                (static_cast<uint32_t>(pos_slot(static_cast<int>(b))) << 16)
+                          : a | (b << 16);
+        lpair[2 * k] = a;
+        lpair[2 * k + 1] = b;
+        nplus[a]++;
+        nminus[b]++;
       }
-      // phase-2 visiting order: longest gather lists first (ties keep order)
+      // row bounds: [2i] first-endpoint part, [2i+1] second-endpoint part
+      count.assign(2 * nm + 1, 0);
+      for (uint32_t i = 0; i < nm; i++) {
+        count[2 * i + 1] = count[2 * i] + pad(nplus[i]);
+        count[2 * i + 2] = count[2 * i + 1] + pad(nminus[i]);
+      }
+      // phase-2 visiting order: longest rows first (ties keep order)
       order.resize(nm);
       std::iota(order.begin(), order.end(), 0u);
-      std::stable_sort(order.begin(), order.end(),
-                       [&](uint32_t x, uint32_t y) { return count[x + 1] > count[y + 1]; });
+      std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        return count[2 * x + 2] - count[2 * x] > count[2 * y + 2] - count[2 * y];
+      });
       for (uint32_t i = 0; i < nm; i++) perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
+      for (uint32_t i = 0; i <= 2 * nm; i++) row.push_back(count[i]);
+      csr.resize(d.csr_begin + count[2 * nm], zero_entry);
+      std::vector<uint32_t> fill_plus(nm), fill_minus(nm);
       for (uint32_t i = 0; i < nm; i++) {
-        const uint32_t deg = count[i + 1];
-        count[i + 1] = count[i] + (padded ? (deg + 3u) & ~3u : deg);
+        fill_plus[i] = count[2 * i];
+        fill_minus[i] = count[2 * i + 1];
       }
-      for (uint32_t i = 0; i <= nm; i++) row.push_back(count[i]);
-      csr.resize(d.csr_begin + count[nm], zero_entry);
-      std::vector<uint32_t> fill(count.begin(), count.end() - 1);
       for (uint32_t k = 0; k < d.n_spring; k++) {
-        const uint32_t pr = spair[d.spring_begin + k];
-        const uint32_t a = pr & 0xffffu, b = pr >> 16;
-        csr[d.csr_begin + fill[a]++] = k << 1;
-        csr[d.csr_begin + fill[b]++] = (k << 1) | 1u;
+        const uint32_t a = lpair[2 * k], b = lpair[2 * k + 1];
+        csr[d.csr_begin + fill_plus[a]++] = scale * k;
+        csr[d.csr_begin + fill_minus[b]++] = scale * k;
       }
     }
     // launch order: most work first, so uneven packs balance across CUs
@@ -617,18 +636,19 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   }
   if (compact_to_id.empty()) compact_to_id.push_back(0);  // keeps tables non-empty
 
-  // per robot: gather entries with every row rounded up to 4
+  // per robot: gather entries with both parts of every row rounded up to 4
   std::vector<uint32_t> gpad(R, 0);
   {
-    std::vector<uint32_t> deg;
+    std::vector<uint32_t> plus, minus;
     for (int r = 0; r < R; r++) {
-      deg.assign(mo[r + 1] - mo[r], 0);
+      plus.assign(mo[r + 1] - mo[r], 0);
+      minus.assign(mo[r + 1] - mo[r], 0);
       for (uint32_t k = ao[r]; k < ao[r + 1]; k++) {
-        deg[ap[2 * k]]++;
-        deg[ap[2 * k + 1]]++;
+        plus[ap[2 * k]]++;
+        minus[ap[2 * k + 1]]++;
       }
       uint32_t g = 0;
-      for (uint32_t x : deg) g += (x + 3u) & ~3u;
+      for (size_t i = 0; i < plus.size(); i++) g += ((plus[i] + 3u) & ~3u) + ((minus[i] + 3u) & ~3u);
       gpad[r] = g;
     }
   }

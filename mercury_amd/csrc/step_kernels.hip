@@ -153,15 +153,22 @@ __device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, fl
 }
 
 // ----------------------------------------------------------------- LDS kernel
-// LDS layout: positions float4[mass_cap] | spring terms xy float2[spring_cap
-// + 2] | gather entries u16[csr_cap] | spring terms z float[spring_cap + 2] |
-// row starts u16[mass_cap + 1, even] | actuation float[n_mat].  Slot
-// spring_cap is the sink of lanes without a spring, slot spring_cap + 1
-// holds zeros and pads every gather row to a multiple of 4 entries (adding
-// +0 leaves a sum that started at +0 unchanged).  Springs are dealt to
-// waves in contiguous runs of whole 64-lane iterations, so every wave runs
-// ceil(springs / BLOCK) iterations.  Phase-2 lanes visit masses in the
-// order of p.mperm (longest gather rows first).
+// LDS layout (from LDS address 0): spring terms float[3][spring_cap + 2]
+// (12 B per slot, rounded up to 16 B) | positions float4[pos_slots(mass_cap)] |
+// gather entries u16[csr_cap] | row bounds u16[2 mass_cap + 2] |
+// actuation float[n_mat].  Slot spring_cap is the sink of lanes without a
+// spring; slot spring_cap + 1 holds zeros.  A gather entry is the word
+// offset 3 * slot of a term.  Mass i's row has two parts, [row[2i],
+// row[2i+1]) with the springs whose first endpoint it is and [row[2i+1],
+// row[2i+2]) with those whose second endpoint it is, each in ascending
+// spring index and padded to a multiple of 4 entries with the zero slot
+// (adding or subtracting +0 leaves a sum that started at +0 unchanged).
+// Springs are dealt to waves in contiguous runs of whole 64-lane
+// iterations, so every wave runs ceil(springs / BLOCK) iterations.
+// Phase-2 lanes visit masses in the order of p.mperm (longest rows first).
+// Mass i's position sits in slot i + i/8 (pos_slot): the skew spreads the
+// 16-byte position reads of a wave over the LDS banks; spring records carry
+// skewed endpoint indices.
 template <int SOLVER, int BLOCK, int NSPT, int MPT>
 __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
   constexpr int kWaves = BLOCK / 64;
@@ -170,24 +177,20 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
   const int nm = static_cast<int>(d.n_mass);
   const int ns = static_cast<int>(d.n_spring);
   const int sink = p.spring_cap, zero = p.spring_cap + 1;
-  float4* s_pos = reinterpret_cast<float4*>(smem);
-  float2* s_cxy = reinterpret_cast<float2*>(s_pos + p.mass_cap);
-  uint16_t* s_csr = reinterpret_cast<uint16_t*>(s_cxy + p.spring_cap + 2);
-  float* s_cz = reinterpret_cast<float*>(s_csr + p.csr_cap);
-  uint16_t* s_row = reinterpret_cast<uint16_t*>(s_cz + p.spring_cap + 2);
-  float* s_R = reinterpret_cast<float*>(s_row + ((p.mass_cap + 2) & ~1));
+  float* s_term = reinterpret_cast<float*>(smem);
+  float4* s_pos = reinterpret_cast<float4*>(smem + lds_term_bytes(p.spring_cap));
+  uint16_t* s_csr = reinterpret_cast<uint16_t*>(s_pos + pos_slots(p.mass_cap));
+  uint16_t* s_row = s_csr + p.csr_cap;
+  float* s_R = reinterpret_cast<float*>(s_row + 2 * p.mass_cap + 2);
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int iters = (ns + 64 * kWaves - 1) / (64 * kWaves);  // <= NSPT
   const int wave_first = wave * iters * 64;
 
-  const int n_csr = static_cast<int>(p.row[d.row_begin + nm]);
+  const int n_csr = static_cast<int>(p.row[d.row_begin + 2 * nm]);
   for (int i = tid; i < n_csr; i += BLOCK) s_csr[i] = static_cast<uint16_t>(p.csr[d.csr_begin + i]);
-  for (int i = tid; i <= nm; i += BLOCK) s_row[i] = static_cast<uint16_t>(p.row[d.row_begin + i]);
-  if (tid == 0) {
-    s_cxy[zero] = make_float2(0.f, 0.f);
-    s_cz[zero] = 0.f;
-  }
+  for (int i = tid; i <= 2 * nm; i += BLOCK) s_row[i] = static_cast<uint16_t>(p.row[d.row_begin + i]);
+  if (tid < 3) s_term[3 * zero + tid] = 0.f;
 
   // spring records -> registers; lanes past the pack's springs compute a
   // dummy spring (0,0) whose term goes to the sink slot
@@ -197,7 +200,7 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
 #pragma unroll
   for (int k = 0; k < NSPT; ++k) {
     const int s = wave_first + k * 64 + lane;
-    sp[k] = 0; lb[k] = 1.f; kk[k] = 1.f; st[k] = 0.f; mi[k] = 0; slot[k] = sink;
+    sp[k] = 0; lb[k] = 1.f; kk[k] = 1.f; st[k] = 0.f; mi[k] = 0; slot[k] = 3 * sink;
     if (k < iters && s < ns) {
       const uint32_t g = d.spring_begin + s;
       sp[k] = p.spair[g];
@@ -205,20 +208,22 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
       mi[k] = p.smat[g];
       kk[k] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[mi[k]] : p.mat_k[mi[k]];
       if (p.track_stress) st[k] = p.sstress[g];
-      slot[k] = s;
+      slot[k] = 3 * s;
     }
   }
   // masses -> registers, first positions -> LDS
   MassState ms[MPT];
-  int mid[MPT];
+  int mid[MPT], pslot[MPT];
 #pragma unroll
   for (int j = 0; j < MPT; ++j) {
     const int i = tid + j * BLOCK;
     mid[j] = -1;
+    pslot[j] = 0;
     if (i < nm) {
       mid[j] = p.mperm[d.mass_begin + i];
+      pslot[j] = pos_slot(mid[j]);
       load_mass(ms[j], p.pos, p.vel, d.mass_begin + mid[j], p);
-      s_pos[mid[j]] = lds_position<SOLVER>(ms[j]);
+      s_pos[pslot[j]] = lds_position<SOLVER>(ms[j]);
     }
   }
   float t = p.t0;
@@ -234,8 +239,9 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
         float cx, cy, cz, lambda;
         spring_term<SOLVER>(pa, pb, lb[k], s_R[mi[k]], kk[k], cx, cy, cz, lambda);
         if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) st[k] = st[k] + lambda / lb[k];
-        s_cxy[slot[k]] = make_float2(cx, cy);
-        s_cz[slot[k]] = cz;
+        s_term[slot[k]] = cx;
+        s_term[slot[k] + 1] = cy;
+        s_term[slot[k] + 2] = cz;
       }
     }
     __syncthreads();
@@ -244,26 +250,37 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
     for (int j = 0; j < MPT; ++j) {
       if (mid[j] >= 0) {
         const int i = mid[j];
-        const int e1 = s_row[i + 1];
+        const int e1 = s_row[2 * i + 1], e2 = s_row[2 * i + 2];
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        // ascending spring order, four gathers in flight; the sign is an
-        // exact multiply inside the fma, so each step is one rounding of
-        // acc +- term
-        for (int e = s_row[i]; e < e1; e += 4) {
+        // first-endpoint terms added, then second-endpoint terms
+        // subtracted, each in ascending spring order, four gathers in flight
+        for (int e = s_row[2 * i]; e < e1; e += 4) {
           const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
-          const uint32_t c0 = w.x & 0xffffu, c1 = w.x >> 16, c2 = w.y & 0xffffu, c3 = w.y >> 16;
-          const float2 q0 = s_cxy[c0 >> 1], q1 = s_cxy[c1 >> 1], q2 = s_cxy[c2 >> 1],
-                       q3 = s_cxy[c3 >> 1];
-          const float z0 = s_cz[c0 >> 1], z1 = s_cz[c1 >> 1], z2 = s_cz[c2 >> 1],
-                      z3 = s_cz[c3 >> 1];
-          const float g0 = (c0 & 1u) ? -1.0f : 1.0f, g1 = (c1 & 1u) ? -1.0f : 1.0f,
-                      g2 = (c2 & 1u) ? -1.0f : 1.0f, g3 = (c3 & 1u) ? -1.0f : 1.0f;
-          ax = fmaf(g0, q0.x, ax); ay = fmaf(g0, q0.y, ay); az = fmaf(g0, z0, az);
-          ax = fmaf(g1, q1.x, ax); ay = fmaf(g1, q1.y, ay); az = fmaf(g1, z1, az);
-          ax = fmaf(g2, q2.x, ax); ay = fmaf(g2, q2.y, ay); az = fmaf(g2, z2, az);
-          ax = fmaf(g3, q3.x, ax); ay = fmaf(g3, q3.y, ay); az = fmaf(g3, z3, az);
+          const float* q0 = s_term + (w.x & 0xffffu);
+          const float* q1 = s_term + (w.x >> 16);
+          const float* q2 = s_term + (w.y & 0xffffu);
+          const float* q3 = s_term + (w.y >> 16);
+          const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
+          const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
+          ax = ax + x0; ay = ay + y0; az = az + z0;
+          ax = ax + x1; ay = ay + y1; az = az + z1;
+          ax = ax + x2; ay = ay + y2; az = az + z2;
+          ax = ax + x3; ay = ay + y3; az = az + z3;
         }
-        s_pos[i] = integrate<SOLVER>(ms[j], ax, ay, az, p);
+        for (int e = e1; e < e2; e += 4) {
+          const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
+          const float* q0 = s_term + (w.x & 0xffffu);
+          const float* q1 = s_term + (w.x >> 16);
+          const float* q2 = s_term + (w.y & 0xffffu);
+          const float* q3 = s_term + (w.y >> 16);
+          const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
+          const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
+          ax = ax - x0; ay = ay - y0; az = az - z0;
+          ax = ax - x1; ay = ay - y1; az = az - z1;
+          ax = ax - x2; ay = ay - y2; az = az - z2;
+          ax = ax - x3; ay = ay - y3; az = az - z3;
+        }
+        s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax, ay, az, p);
       }
     }
     if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
@@ -343,15 +360,19 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
       s.mg = s.m * p.gravity;
       const float4 lp = s_pos[i];
       s.tx = lp.x; s.ty = lp.y; s.tz = lp.z;
-      const int e1 = static_cast<int>(row[i + 1]);
+      const int e1 = static_cast<int>(row[2 * i + 1]), e2 = static_cast<int>(row[2 * i + 2]);
       float ax = 0.0f, ay = 0.0f, az = 0.0f;
-      for (int e = static_cast<int>(row[i]); e < e1; ++e) {
-        const uint32_t c = csr[e];
-        const float4 q = sc[c >> 1];
-        const float sg = (c & 1u) ? -1.0f : 1.0f;
-        ax = fmaf(sg, q.x, ax);
-        ay = fmaf(sg, q.y, ay);
-        az = fmaf(sg, q.z, az);
+      for (int e = static_cast<int>(row[2 * i]); e < e1; ++e) {
+        const float4 q = sc[csr[e]];
+        ax = ax + q.x;
+        ay = ay + q.y;
+        az = az + q.z;
+      }
+      for (int e = e1; e < e2; ++e) {
+        const float4 q = sc[csr[e]];
+        ax = ax - q.x;
+        ay = ay - q.y;
+        az = az - q.z;
       }
       s_pos[i] = integrate<SOLVER>(s, ax, ay, az, p);
       p.pos[d.mass_begin + i] = make_float4(s.x, s.y, s.z, s.m);
@@ -403,10 +424,10 @@ const KernelVariant* kernel_variants(int* count) {
 }
 
 size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat) {
-  size_t b = 16u * static_cast<size_t>(mass_cap);       // float4 positions
-  b += 12u * static_cast<size_t>(spring_cap + 2);       // spring terms xy | z, sink, zero
+  size_t b = lds_term_bytes(spring_cap);                // spring terms, sink, zero
+  b += 16u * static_cast<size_t>(pos_slots(mass_cap));  // float4 positions (skewed)
   b += 2u * static_cast<size_t>(csr_cap);               // gather entries (u16)
-  b += 2u * static_cast<size_t>((mass_cap + 2) & ~1);   // row starts
+  b += 2u * static_cast<size_t>(2 * mass_cap + 2);      // row bounds
   b += 4u * static_cast<size_t>(n_mat);                 // actuation table
   return b;
 }

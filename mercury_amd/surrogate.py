@@ -10,6 +10,7 @@ import ctypes
 
 import numpy as np
 
+from . import _lib as _libmod
 from ._lib import check, f32p, i32p, load, ptr
 
 N_FEATURES = 64
@@ -115,9 +116,9 @@ class MLP:
         self._h = h
 
     def __del__(self):
-        from . import _lib
-        if getattr(self, "_h", None) is not None and _lib._lib is not None:
-            _lib._lib.mh_mlp_destroy(self._h)
+        lib = getattr(_libmod, "_lib", None)
+        if getattr(self, "_h", None) is not None and lib is not None:
+            lib.mh_mlp_destroy(self._h)
             self._h = None
 
     def n_params(self):

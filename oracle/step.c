@@ -19,8 +19,9 @@
  *   (common/simulator/Simulator.cu:428-449).
  * The reference sums corrections with shared-memory float atomics (order
  * undefined).  The canonical order here, and in the HIP kernels, is: per
- * mass, start from +0.0f and add the contributions of its springs in
- * ascending spring index.  |.| is sqrtf((x*x + y*y) + z*z) and sin is
+ * mass, start from +0.0f, add the terms of the springs whose first
+ * endpoint it is (ascending spring index), then subtract the terms of the
+ * springs whose second endpoint it is (ascending spring index).  |.| is sqrtf((x*x + y*y) + z*z) and sin is
  * orc_sinf (CUDA's norm3df/sinf are outside the reference tree).
  *
  * force (project spec; the reference snapshot has no force solver -- paper
@@ -28,7 +29,7 @@
  *   per non-air spring: L and d as above;
  *        f = (k * (L - d)) * (1 / (d + 1e-12f));  c = f * (x~a - x~b)
  *        (Hooke's law with one correctly rounded reciprocal per spring);
- *        +c on a, -c on b  (ascending spring index)
+ *        +c on a, -c on b  (canonical order above)
  *   per mass: F = sum(c)
  *        F.y = F.y - m*g
  *        F   = F - drag*v                       (if drag != 0)
@@ -135,6 +136,28 @@ static int run_jobs(job_t proto, int nthreads) {
   return 0;
 }
 
+/* Canonical summation order of the FP32 steps: per mass, from +0.0f, add
+ * the terms of the springs whose first endpoint it is, in ascending spring
+ * index, then subtract the terms of the springs whose second endpoint it
+ * is, in ascending spring index.  Air springs contribute nothing. */
+static void gather_terms(const orc_topology* tp, uint32_t sb, uint32_t ns,
+                         const float* term, float* acc) {
+  for (uint32_t s = 0; s < ns; s++) {
+    if (tp->mat[sb + s] == 0) continue;
+    uint32_t a = tp->pairs[2 * (sb + s)];
+    acc[3 * a] = acc[3 * a] + term[3 * s];
+    acc[3 * a + 1] = acc[3 * a + 1] + term[3 * s + 1];
+    acc[3 * a + 2] = acc[3 * a + 2] + term[3 * s + 2];
+  }
+  for (uint32_t s = 0; s < ns; s++) {
+    if (tp->mat[sb + s] == 0) continue;
+    uint32_t b = tp->pairs[2 * (sb + s) + 1];
+    acc[3 * b] = acc[3 * b] - term[3 * s];
+    acc[3 * b + 1] = acc[3 * b + 1] - term[3 * s + 1];
+    acc[3 * b + 2] = acc[3 * b + 2] - term[3 * s + 2];
+  }
+}
+
 /* ------------------------------------------------------------ xpbd fp32 */
 static void xpbd32_robot(job_t* j, int r) {
   const orc_topology* tp = j->topo;
@@ -147,6 +170,7 @@ static void xpbd32_robot(job_t* j, int r) {
   float* st = j->stress ? (float*)j->stress + sb : NULL;
   float* xt = (float*)malloc(sizeof(float) * 3 * nm);
   float* acc = (float*)malloc(sizeof(float) * 3 * nm);
+  float* term = (float*)malloc(sizeof(float) * 3 * (ns + 1));
   float* K = (float*)malloc(sizeof(float) * (size_t)mt->n);
   float* R = (float*)malloc(sizeof(float) * (size_t)mt->n);
   for (int m = 0; m < mt->n; m++) {
@@ -182,13 +206,9 @@ static void xpbd32_robot(job_t* j, int r) {
       float lambda = -C / K[m];
       float px = lambda * nx, py = lambda * ny, pz = lambda * nz;
       if (st) st[s] = st[s] + lambda / lb;
-      acc[3 * a] = acc[3 * a] + px;
-      acc[3 * a + 1] = acc[3 * a + 1] + py;
-      acc[3 * a + 2] = acc[3 * a + 2] + pz;
-      acc[3 * b] = acc[3 * b] - px;
-      acc[3 * b + 1] = acc[3 * b + 1] - py;
-      acc[3 * b + 2] = acc[3 * b + 2] - pz;
+      term[3 * s] = px; term[3 * s + 1] = py; term[3 * s + 2] = pz;
     }
+    gather_terms(tp, sb, ns, term, acc);
     for (uint32_t i = 0; i < 3 * nm; i++) {
       float xn = xt[i] + acc[i];
       float diff = xn - x[i];
@@ -197,7 +217,7 @@ static void xpbd32_robot(job_t* j, int r) {
     }
     t = t + dt;
   }
-  free(xt); free(acc); free(K); free(R);
+  free(xt); free(acc); free(term); free(K); free(R);
 }
 
 /* ------------------------------------------------------------ xpbd fp64 */
@@ -264,6 +284,7 @@ static void force32_robot(job_t* j, int r) {
   float* v = (float*)j->vel + 3 * (size_t)mb;
   const float* ms = tp->mass + mb;
   float* acc = (float*)malloc(sizeof(float) * 3 * nm);
+  float* term = (float*)malloc(sizeof(float) * 3 * (ns + 1));
   float* R = (float*)malloc(sizeof(float) * (size_t)mt->n);
   float t = j->t0;
   for (int step = 0; step < j->nsteps; step++) {
@@ -284,14 +305,9 @@ static void force32_robot(job_t* j, int r) {
       float d = nrm3f(dx, dy, dz);
       float inv = 1.0f / (d + ORC_EPS);
       float f = (mt->k[m] * (L - d)) * inv;
-      float cx = f * dx, cy = f * dy, cz = f * dz;
-      acc[3 * a] = acc[3 * a] + cx;
-      acc[3 * a + 1] = acc[3 * a + 1] + cy;
-      acc[3 * a + 2] = acc[3 * a + 2] + cz;
-      acc[3 * b] = acc[3 * b] - cx;
-      acc[3 * b + 1] = acc[3 * b + 1] - cy;
-      acc[3 * b + 2] = acc[3 * b + 2] - cz;
+      term[3 * s] = f * dx; term[3 * s + 1] = f * dy; term[3 * s + 2] = f * dz;
     }
+    gather_terms(tp, sb, ns, term, acc);
     for (uint32_t i = 0; i < nm; i++) {
       float m = ms[i];
       float fx = acc[3 * i], fy = acc[3 * i + 1], fz = acc[3 * i + 2];
@@ -329,7 +345,7 @@ static void force32_robot(job_t* j, int r) {
     }
     t = t + dt;
   }
-  free(acc); free(R);
+  free(acc); free(term); free(R);
 }
 
 /* ----------------------------------------------------------- force fp64 */

@@ -205,9 +205,10 @@ def numpy_xpbd_step_f32(pos, vel, pairs, lbar, mat, K, R, dt):
     out_p, out_v = pos.copy(), vel.copy()
     for i in range(n):
         acc = np.zeros(3, np.float32)
-        for s, (a, b) in enumerate(pairs):
+        for s, (a, b) in enumerate(pairs):      # first endpoint: add
             if s in terms and a == i:
                 acc = (acc + terms[s]).astype(np.float32)
+        for s, (a, b) in enumerate(pairs):      # then second endpoint: subtract
             if s in terms and b == i:
                 acc = (acc - terms[s]).astype(np.float32)
         xn = (xt[i] + acc).astype(np.float32)
@@ -268,9 +269,10 @@ def numpy_force_step_f32(pos, vel, mass, pairs, lbar, mat, k, R, env):
     out_p, out_v = pos.copy(), vel.copy()
     for i in range(len(pos)):
         F = np.zeros(3, np.float32)
-        for s, (a, b) in enumerate(pairs):
+        for s, (a, b) in enumerate(pairs):      # first endpoint: add
             if s in terms and a == i:
                 F = (F + terms[s]).astype(np.float32)
+        for s, (a, b) in enumerate(pairs):      # then second endpoint: subtract
             if s in terms and b == i:
                 F = (F - terms[s]).astype(np.float32)
         v = vel[i].copy()
