@@ -28,9 +28,12 @@ import mercury_amd as mh  # noqa: E402
 
 LAND = dict(gravity=9.81, ground_k=1.0e5, mu_s=1.0, mu_k=0.8, drag=0.0, damping=0.99)
 WATER = dict(gravity=0.0, ground_k=0.0, mu_s=0.0, mu_k=0.0, drag=1.0, damping=0.99)
-# streaming kernel, HBM bytes per spring-update: record 9 B + term write
-# 16 B + two gathers of (entry 4 B + term 16 B)
+# streaming kernel with terms in global memory, HBM bytes per spring-update:
+# record 9 B + term write 16 B + two gathers of (entry 4 B + term 16 B)
 STREAM_BYTES_PER_SPRING = 65.0
+# chunked streaming kernel (terms in LDS chunks), HBM bytes per
+# spring-update: record 9 B + two u16 gather entries 4 B
+CHUNK_HBM_BYTES_PER_SPRING = 13.0
 HBM_PEAK_GBS = 8000.0
 # LDS bytes per spring-update of the LDS-resident kernel, per robot size:
 # 57 B per spring (two endpoint positions 24 B, spring record 9 B, result
@@ -191,11 +194,20 @@ def main():
     mh._lib.check(mh.load().mh_calibrate_lds(0, ctypes.byref(lds_peak)))
     lds_peak = float(lds_peak.value)
     steps_per_launch = n_sub / max(launches, 1)
-    if info.kernel == 1:  # LDS kernel only
+    hbm_extra = None
+    if info.kernel == 1 or info.stream_chunk > 0:
+        # LDS-resident kernel, or chunked streaming kernel whose terms also
+        # live in LDS: LDS bytes over all springs
         bound, peak, unit = "lds", lds_peak, "GB/s"
         per_launch = (LDS_BYTES_PER_SPRING * n_springs + LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
         per_unit = LDS_BYTES_PER_SPRING + LDS_BYTES_PER_MASS * n_masses / n_springs
-    else:  # the streaming kernel carries the robots over the LDS budget
+        if info.stream_chunk > 0 and kernel_ms > 0:
+            hb = CHUNK_HBM_BYTES_PER_SPRING * info.n_stream_springs * steps_per_launch
+            hbm_extra = {"hbm_bytes_per_spring_update": CHUNK_HBM_BYTES_PER_SPRING,
+                         "hbm_achieved_gbs": hb / (kernel_ms * 1e-3) / 1e9,
+                         "hbm_frac": hb / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "stream_chunk_springs": info.stream_chunk}
+    else:  # the global-memory streaming kernel carries the big robots
         bound, peak, unit = "hbm", HBM_PEAK_GBS, "GB/s"
         per_launch = STREAM_BYTES_PER_SPRING * info.n_stream_springs * steps_per_launch
         per_unit = STREAM_BYTES_PER_SPRING
@@ -218,7 +230,8 @@ def main():
             "workload": workload,
             "robots": args.robots, "springs_total": n_springs, "active_springs": n_active,
             "masses_total": n_masses, "substeps_per_eval": n_sub, "solver": args.solver,
-            "kernel": {1: "lds", 2: "stream", 3: "lds+stream"}.get(info.kernel, "?"), "block": info.block_size,
+            "kernel": ({1: "lds", 2: "stream", 3: "lds+stream"}.get(info.kernel, "?") +
+                       (" (chunked)" if info.stream_chunk > 0 else "")), "block": info.block_size,
             "blocks": info.n_blocks, "lds_bytes_per_block": info.lds_bytes,
             "substeps_per_launch": info.substeps_per_launch, "hipgraph": True,
             "lds_robots": info.n_lds_robots, "stream_robots": info.n_stream_robots,
@@ -237,6 +250,7 @@ def main():
             "kernel_ms_per_launch": kernel_ms,
             "launches_per_step": launches,
             "algorithmic_bytes_per_spring_update": per_unit,
+            **({"streamed_part": hbm_extra} if hbm_extra else {}),
         },
         "host_seconds": {"topology_build": t_build, "upload": t_upload},
     }

@@ -170,6 +170,8 @@ typedef struct {
   int max_masses_per_block, max_springs_per_block;
   int n_lds_robots, n_stream_robots, stream_block_size, robots_per_block;
   int n_stream_springs;  /* active springs handled by the streaming kernel */
+  int stream_chunk;      /* springs per LDS chunk of the streaming kernel;
+                            0 = terms kept in global memory */
 } mh_sim_info;
 int mh_sim_get_info(const mh_sim* sim, mh_sim_info* info);
 /* Times the candidate launch plans (LDS variant x robots per block, then

@@ -31,6 +31,8 @@ struct StepParams {
                              // LDS plans pad rows to 4 with the zero slot
   const uint32_t* row;       // pack-local gather offsets, n_mass + 1 per pack
   const uint16_t* mperm;     // per pack: phase-2 visiting order of its masses
+  const uint16_t* ccsr;      // chunked kernel: per chunk, entries 3 * (spring - chunk start)
+  const uint16_t* crow;      // chunked kernel: per chunk, 2 n_mass + 1 row bounds
   float* scratch;            // streaming kernel: float4 per active spring
   const float* mat_k;        // spring constant
   const float* mat_kx;       // xpbd: 2 + ((1/k)/dt)/dt
@@ -39,7 +41,8 @@ struct StepParams {
   const float* mat_phi;
   int n_mat;
   int mass_cap;              // LDS capacity in masses (layout stride)
-  int spring_cap;            // LDS capacity in springs (layout stride)
+  int spring_cap;            // LDS capacity in springs (layout stride); the
+                             // chunked kernel's springs per chunk
   int csr_cap;               // LDS capacity in gather entries (multiple of 4)
   int n_steps;
   float dt;
@@ -52,6 +55,7 @@ struct StepParams {
 struct KernelVariant {
   int solver;     // MH_SOLVER_*
   int kind;       // MH_KERNEL_LDS / MH_KERNEL_STREAM
+  int chunked;    // STREAM kind: 1 = spring terms in LDS chunks, 0 = in global memory
   int block;      // threads per workgroup
   int nspt;       // springs per thread held in registers (LDS kind)
   int mpt;        // masses per thread

@@ -104,7 +104,7 @@ struct mh_sim {
   mh::DevBuf<float4> pos, vel;
   mh::DevBuf<float> lbar, stress, scratch, com;
   mh::DevBuf<uint8_t> smat, mass_id;
-  mh::DevBuf<uint16_t> mperm;
+  mh::DevBuf<uint16_t> mperm, ccsr, crow;
   mh::DevBuf<uint32_t> spair, csr, row, robot_begin;
   mh::DevBuf<mh::PackDesc> packs;
   mh::DevBuf<float> mat_k, mat_kx, mat_dL0, mat_omega, mat_phi;
@@ -173,8 +173,22 @@ int robot_active(const mh_sim* s, int r) { return static_cast<int>(s->active_off
 
 bool fits_lds_alone(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
   const int m = robot_masses(s, r), sp = robot_active(s, r), g = static_cast<int>(s->gather_pad[r]);
-  return m <= v.block * v.mpt && sp <= v.block * v.nspt && sp + 2 <= 32767 && g <= 65535 &&
+  return m <= v.block * v.mpt && sp <= v.block * v.nspt && 3 * (sp + 1) <= 65535 && g <= 65535 &&
          lds_bytes_lds(std::max(m, 1), std::max(sp, 1), g, n_mat) <= static_cast<size_t>(s->lds_limit);
+}
+
+// Springs per chunk for the chunked kernel and a robot of m masses: the
+// largest candidate whose LDS layout (chunk terms, skewed positions, one
+// chunk's gather entries -- at most 2C plus padding of 3 per row part --
+// and row bounds) fits; 0 if none does.
+int chunk_springs(const mh_sim* s, const KernelVariant& v, int m, int n_mat) {
+  if (m > v.block * v.mpt) return 0;
+  for (int C : {8192, 6144, 5120, 4096, 3072, 2048, 1536, 1024, 512}) {
+    const int entries = 2 * C + 6 * m;
+    if (entries > 65535 || 3 * (C + 1) > 65535) continue;
+    if (lds_bytes_lds(std::max(m, 1), C, entries, n_mat) <= static_cast<size_t>(s->lds_limit)) return C;
+  }
+  return 0;
 }
 
 // Greedy packing of consecutive LDS-eligible robots into workgroups of the
@@ -193,7 +207,10 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
   while (r < s->n_robots) {
     if (!lv || !fits_lds_alone(s, *lv, r, n_mat)) {
       const int m = robot_masses(s, r);
-      if (!sv || lds_bytes_stream(std::max(m, 1), n_mat) > static_cast<size_t>(s->lds_limit)) return false;
+      if (!sv) return false;
+      if (sv->chunked ? chunk_springs(s, *sv, m, n_mat) == 0
+                      : lds_bytes_stream(std::max(m, 1), n_mat) > static_cast<size_t>(s->lds_limit))
+        return false;
       PackDesc d{};
       d.mass_begin = s->mass_off[r];
       d.n_mass = static_cast<uint32_t>(m);
@@ -237,10 +254,74 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
     plan.parts.push_back(std::move(lp));
   }
   if (!stp.packs.empty()) {
-    stp.lds = lds_bytes_stream(std::max(stp.mass_cap, 1), n_mat);
+    if (sv->chunked) {
+      // one chunk size for the part: the one that fits its largest robot
+      stp.spring_cap = chunk_springs(s, *sv, std::max(stp.mass_cap, 1), n_mat);
+      stp.csr_cap = 2 * stp.spring_cap + 6 * std::max(stp.mass_cap, 1);
+      stp.lds = lds_bytes_lds(std::max(stp.mass_cap, 1), stp.spring_cap, stp.csr_cap, n_mat);
+    } else {
+      stp.lds = lds_bytes_stream(std::max(stp.mass_cap, 1), n_mat);
+    }
     plan.parts.push_back(std::move(stp));
   }
   return true;
+}
+
+// Chunked part (one robot per pack): skewed spring pairs, and per chunk of
+// part.spring_cap springs the u16 gather entries 3 * (spring - chunk start)
+// and 2 n_mass + 1 chunk-relative row bounds, each row part in ascending
+// spring index padded to 4 with the zero slot.
+void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::vector<uint16_t>& ccsr,
+                    std::vector<uint16_t>& crow) {
+  const uint32_t C = static_cast<uint32_t>(part.spring_cap);
+  const uint16_t zero_entry = static_cast<uint16_t>(3u * (C + 1));
+  auto pad = [](uint32_t n) { return (n + 3u) & ~3u; };
+  std::vector<uint32_t> lpair, count, fill_plus, fill_minus;
+  for (PackDesc& d : part.packs) {
+    d.csr_begin = static_cast<uint32_t>(ccsr.size());
+    d.row_begin = static_cast<uint32_t>(crow.size());
+    const uint32_t nm = d.n_mass, ns = d.n_spring;
+    lpair.resize(2 * static_cast<size_t>(ns));
+    for (uint32_t k = 0; k < ns; k++) {
+      const uint32_t g = d.spring_begin + k;
+      const uint32_t a = s->act_pairs[2 * g], b = s->act_pairs[2 * g + 1];
+      lpair[2 * k] = a;
+      lpair[2 * k + 1] = b;
+      spair[g] = static_cast<uint32_t>(pos_slot(static_cast<int>(a))) |
+                 (static_cast<uint32_t>(pos_slot(static_cast<int>(b))) << 16);
+    }
+    const uint32_t n_chunks = ns > 0 ? (ns + C - 1) / C : 1;
+    for (uint32_t c = 0; c < n_chunks; c++) {
+      const uint32_t s0 = c * C, s1 = std::min(ns, s0 + C);
+      count.assign(2 * nm + 1, 0);
+      for (uint32_t k = s0; k < s1; k++) {
+        count[2 * lpair[2 * k] + 1]++;      // first-endpoint part size
+        count[2 * lpair[2 * k + 1] + 2]++;  // second-endpoint part size
+      }
+      for (uint32_t i = 0; i < nm; i++) {
+        count[2 * i + 1] = count[2 * i] + pad(count[2 * i + 1]);
+        count[2 * i + 2] = count[2 * i + 1] + pad(count[2 * i + 2]);
+      }
+      if (count[2 * nm] > static_cast<uint32_t>(part.csr_cap))
+        throw Error(MH_ERR_INTERNAL, "chunk gather list over its LDS capacity");
+      const size_t base = ccsr.size();
+      ccsr.resize(base + count[2 * nm], zero_entry);
+      fill_plus.resize(nm);
+      fill_minus.resize(nm);
+      for (uint32_t i = 0; i < nm; i++) {
+        fill_plus[i] = count[2 * i];
+        fill_minus[i] = count[2 * i + 1];
+      }
+      for (uint32_t k = s0; k < s1; k++) {
+        ccsr[base + fill_plus[lpair[2 * k]]++] = static_cast<uint16_t>(3u * (k - s0));
+        ccsr[base + fill_minus[lpair[2 * k + 1]]++] = static_cast<uint16_t>(3u * (k - s0));
+      }
+      for (uint32_t i = 0; i <= 2 * nm; i++) crow.push_back(static_cast<uint16_t>(count[i]));
+    }
+  }
+  std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
+    return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
+  });
 }
 
 // Device arrays that depend on the packing: pack-local spring pairs, the
@@ -258,66 +339,71 @@ void upload_plan(mh_sim* s, Plan& plan) {
   std::vector<PackDesc> all;
   row.reserve(2 * s->n_masses + 64);
   bool need_scratch = false;
+  std::vector<uint16_t> ccsr, crow;
   for (Part& part : plan.parts) {
     const bool padded = part.v->kind == MH_KERNEL_LDS;
-    need_scratch |= part.v->kind == MH_KERNEL_STREAM;
-    const uint32_t zero_entry = 3u * static_cast<uint32_t>(part.spring_cap + 1);
-    const uint32_t scale = padded ? 3u : 1u;
-    auto pad = [padded](uint32_t n) { return padded ? (n + 3u) & ~3u : n; };
-    for (PackDesc& d : part.packs) {
-      d.csr_begin = static_cast<uint32_t>(csr.size());
-      d.row_begin = static_cast<uint32_t>(row.size());
-      const uint32_t nm = d.n_mass;
-      nplus.assign(nm, 0);
-      nminus.assign(nm, 0);
-      lpair.resize(2 * static_cast<size_t>(d.n_spring));
-      // pack-local pairs: robot-local index + robot's offset inside the pack
-      int r = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
-                               s->mass_off.begin()) - 1;
-      for (uint32_t k = 0; k < d.n_spring; k++) {
-        const uint32_t g = d.spring_begin + k;
-        while (g >= s->active_off[r + 1]) r++;
-        const uint32_t off = s->mass_off[r] - d.mass_begin;
-        const uint32_t a = s->act_pairs[2 * g] + off, b = s->act_pairs[2 * g + 1] + off;
-        // LDS parts address positions by skewed slot
-        spair[g] = padded ? static_cast<uint32_t>(pos_slot(static_cast<int>(a))) |
-                                (static_cast<uint32_t>(pos_slot(static_cast<int>(b))) << 16)
-                          : a | (b << 16);
-        lpair[2 * k] = a;
-        lpair[2 * k + 1] = b;
-        nplus[a]++;
-        nminus[b]++;
+    need_scratch |= part.v->kind == MH_KERNEL_STREAM && !part.v->chunked;
+    if (part.v->chunked) {
+      upload_chunked(s, part, spair, ccsr, crow);
+    } else {
+      const uint32_t zero_entry = 3u * static_cast<uint32_t>(part.spring_cap + 1);
+      const uint32_t scale = padded ? 3u : 1u;
+      auto pad = [padded](uint32_t n) { return padded ? (n + 3u) & ~3u : n; };
+      for (PackDesc& d : part.packs) {
+        d.csr_begin = static_cast<uint32_t>(csr.size());
+        d.row_begin = static_cast<uint32_t>(row.size());
+        const uint32_t nm = d.n_mass;
+        nplus.assign(nm, 0);
+        nminus.assign(nm, 0);
+        lpair.resize(2 * static_cast<size_t>(d.n_spring));
+        // pack-local pairs: robot-local index + robot's offset inside the pack
+        int r = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
+                                 s->mass_off.begin()) - 1;
+        for (uint32_t k = 0; k < d.n_spring; k++) {
+          const uint32_t g = d.spring_begin + k;
+          while (g >= s->active_off[r + 1]) r++;
+          const uint32_t off = s->mass_off[r] - d.mass_begin;
+          const uint32_t a = s->act_pairs[2 * g] + off, b = s->act_pairs[2 * g + 1] + off;
+          // LDS parts address positions by skewed slot
+          spair[g] = padded ? static_cast<uint32_t>(pos_slot(static_cast<int>(a))) |
+                                  (static_cast<uint32_t>(pos_slot(static_cast<int>(b))) << 16)
+                            : a | (b << 16);
+          lpair[2 * k] = a;
+          lpair[2 * k + 1] = b;
+          nplus[a]++;
+          nminus[b]++;
+        }
+        // row bounds: [2i] first-endpoint part, [2i+1] second-endpoint part
+        count.assign(2 * nm + 1, 0);
+        for (uint32_t i = 0; i < nm; i++) {
+          count[2 * i + 1] = count[2 * i] + pad(nplus[i]);
+          count[2 * i + 2] = count[2 * i + 1] + pad(nminus[i]);
+        }
+        // phase-2 visiting order: longest rows first (ties keep order)
+        order.resize(nm);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+          return count[2 * x + 2] - count[2 * x] > count[2 * y + 2] - count[2 * y];
+        });
+        for (uint32_t i = 0; i < nm; i++) perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
+        for (uint32_t i = 0; i <= 2 * nm; i++) row.push_back(count[i]);
+        csr.resize(d.csr_begin + count[2 * nm], zero_entry);
+        std::vector<uint32_t> fill_plus(nm), fill_minus(nm);
+        for (uint32_t i = 0; i < nm; i++) {
+          fill_plus[i] = count[2 * i];
+          fill_minus[i] = count[2 * i + 1];
+        }
+        for (uint32_t k = 0; k < d.n_spring; k++) {
+          const uint32_t a = lpair[2 * k], b = lpair[2 * k + 1];
+          csr[d.csr_begin + fill_plus[a]++] = scale * k;
+          csr[d.csr_begin + fill_minus[b]++] = scale * k;
+        }
       }
-      // row bounds: [2i] first-endpoint part, [2i+1] second-endpoint part
-      count.assign(2 * nm + 1, 0);
-      for (uint32_t i = 0; i < nm; i++) {
-        count[2 * i + 1] = count[2 * i] + pad(nplus[i]);
-        count[2 * i + 2] = count[2 * i + 1] + pad(nminus[i]);
-      }
-      // phase-2 visiting order: longest rows first (ties keep order)
-      order.resize(nm);
-      std::iota(order.begin(), order.end(), 0u);
-      std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        return count[2 * x + 2] - count[2 * x] > count[2 * y + 2] - count[2 * y];
+      // launch order: most work first, so uneven packs balance across CUs
+      std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
+        return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
       });
-      for (uint32_t i = 0; i < nm; i++) perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
-      for (uint32_t i = 0; i <= 2 * nm; i++) row.push_back(count[i]);
-      csr.resize(d.csr_begin + count[2 * nm], zero_entry);
-      std::vector<uint32_t> fill_plus(nm), fill_minus(nm);
-      for (uint32_t i = 0; i < nm; i++) {
-        fill_plus[i] = count[2 * i];
-        fill_minus[i] = count[2 * i + 1];
-      }
-      for (uint32_t k = 0; k < d.n_spring; k++) {
-        const uint32_t a = lpair[2 * k], b = lpair[2 * k + 1];
-        csr[d.csr_begin + fill_plus[a]++] = scale * k;
-        csr[d.csr_begin + fill_minus[b]++] = scale * k;
-      }
     }
-    // launch order: most work first, so uneven packs balance across CUs
-    std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
-      return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
-    });
     part.pack_offset = static_cast<uint32_t>(all.size());
     all.insert(all.end(), part.packs.begin(), part.packs.end());
     if (part.lds > 64 * 1024)
@@ -329,6 +415,8 @@ void upload_plan(mh_sim* s, Plan& plan) {
   s->row.upload(row.data(), row.size(), s->stream);
   s->packs.upload(all.data(), all.size(), s->stream);
   s->mperm.upload(perm.data(), perm.size(), s->stream);
+  s->ccsr.upload(ccsr.data(), ccsr.size(), s->stream);
+  s->crow.upload(crow.data(), crow.size(), s->stream);
   if (need_scratch) s->scratch.alloc(4 * std::max<size_t>(na, 1));
   hip_check(hipStreamSynchronize(s->stream), "plan upload");
   s->plan = std::move(plan);
@@ -348,8 +436,9 @@ std::vector<const KernelVariant*> candidates(const mh_sim* s, int kind) {
   return out;
 }
 
-// Default plan: the first LDS variant (table order, largest blocks first)
-// that takes every robot it can, the first streaming variant for the rest.
+// Default plan: the LDS variant (table order, largest blocks first) that
+// takes the most robots, and for the rest the first streaming variant
+// (chunked ones first) that fits them.
 void choose_default_plan(mh_sim* s) {
   const bool use_lds = s->cfg.kernel != MH_KERNEL_STREAM;
   const bool use_stream = s->cfg.kernel != MH_KERNEL_LDS;
@@ -368,8 +457,9 @@ void choose_default_plan(mh_sim* s) {
     }
   }
   Plan p;
-  if (!make_plan(s, best, sv, s->cfg.robots_per_block, p))
-    throw Error(MH_ERR_RANGE, "no step kernel fits this population (robot too large for LDS)");
+  bool ok = make_plan(s, best, sv, s->cfg.robots_per_block, p);
+  for (size_t i = 1; !ok && i < sc.size(); i++) ok = make_plan(s, best, sc[i], s->cfg.robots_per_block, p);
+  if (!ok) throw Error(MH_ERR_RANGE, "no step kernel fits this population (robot too large for LDS)");
   upload_plan(s, p);
 }
 
@@ -385,6 +475,8 @@ StepParams make_params(mh_sim* s, const Part& part, float t0, int n_steps, int t
   p.csr = s->csr.p;
   p.row = s->row.p;
   p.mperm = s->mperm.p;
+  p.ccsr = s->ccsr.p;
+  p.crow = s->crow.p;
   p.scratch = s->scratch.p;
   p.mat_k = s->mat_k.p;
   p.mat_kx = s->mat_kx.p;
@@ -847,6 +939,7 @@ int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
       info->stream_block_size = part.v->block;
       info->n_stream_robots = part.n_robots;
       for (const mh::PackDesc& d : part.packs) info->n_stream_springs += static_cast<int>(d.n_spring);
+      info->stream_chunk = part.v->chunked ? part.spring_cap : 0;
     }
   }
   info->robots_per_block = s->plan.rpb;
@@ -949,7 +1042,15 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   auto lc = use_lds ? mh::candidates(s, MH_KERNEL_LDS) : std::vector<const mh::KernelVariant*>{};
   auto sc = use_stream ? mh::candidates(s, MH_KERNEL_STREAM) : std::vector<const mh::KernelVariant*>{};
   lc.push_back(nullptr);  // everything streamed
+  // streaming variant for the LDS search: the first that can take every robot
   const mh::KernelVariant* sv0 = sc.empty() ? nullptr : sc.front();
+  for (const mh::KernelVariant* sv : sc) {
+    mh::Plan p;
+    if (mh::make_plan(s, nullptr, sv, 0, p)) {
+      sv0 = sv;
+      break;
+    }
+  }
   std::vector<int> rpbs = {s->cfg.robots_per_block};
   if (s->cfg.robots_per_block == 0) rpbs = {0, 1, 2, 4};
   double best = std::numeric_limits<double>::infinity();

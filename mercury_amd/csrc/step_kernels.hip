@@ -162,7 +162,8 @@ __device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, fl
 // row[2i+1]) with the springs whose first endpoint it is and [row[2i+1],
 // row[2i+2]) with those whose second endpoint it is, each in ascending
 // spring index and padded to a multiple of 4 entries with the zero slot
-// (adding or subtracting +0 leaves a sum that started at +0 unchanged).
+// (adding +0 leaves a sum that started at +0 unchanged); the mass receives
+// P - M of the two part sums.
 // Springs are dealt to waves in contiguous runs of whole 64-lane
 // iterations, so every wave runs ceil(springs / BLOCK) iterations.
 // Phase-2 lanes visit masses in the order of p.mperm (longest rows first).
@@ -251,9 +252,10 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
       if (mid[j] >= 0) {
         const int i = mid[j];
         const int e1 = s_row[2 * i + 1], e2 = s_row[2 * i + 2];
-        float ax = 0.0f, ay = 0.0f, az = 0.0f;
-        // first-endpoint terms added, then second-endpoint terms
-        // subtracted, each in ascending spring order, four gathers in flight
+        float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+        // P (first-endpoint terms) and M (second-endpoint terms), each
+        // summed in ascending spring order, four gathers in flight; the
+        // mass receives P - M
         for (int e = s_row[2 * i]; e < e1; e += 4) {
           const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
           const float* q0 = s_term + (w.x & 0xffffu);
@@ -275,12 +277,12 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
           const float* q3 = s_term + (w.y >> 16);
           const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
           const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
-          ax = ax - x0; ay = ay - y0; az = az - z0;
-          ax = ax - x1; ay = ay - y1; az = az - z1;
-          ax = ax - x2; ay = ay - y2; az = az - z2;
-          ax = ax - x3; ay = ay - y3; az = az - z3;
+          bx = bx + x0; by = by + y0; bz = bz + z0;
+          bx = bx + x1; by = by + y1; bz = bz + z1;
+          bx = bx + x2; by = by + y2; bz = bz + z2;
+          bx = bx + x3; by = by + y3; bz = bz + z3;
         }
-        s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax, ay, az, p);
+        s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax - bx, ay - by, az - bz, p);
       }
     }
     if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
@@ -361,7 +363,7 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
       const float4 lp = s_pos[i];
       s.tx = lp.x; s.ty = lp.y; s.tz = lp.z;
       const int e1 = static_cast<int>(row[2 * i + 1]), e2 = static_cast<int>(row[2 * i + 2]);
-      float ax = 0.0f, ay = 0.0f, az = 0.0f;
+      float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
       for (int e = static_cast<int>(row[2 * i]); e < e1; ++e) {
         const float4 q = sc[csr[e]];
         ax = ax + q.x;
@@ -370,17 +372,157 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
       }
       for (int e = e1; e < e2; ++e) {
         const float4 q = sc[csr[e]];
-        ax = ax - q.x;
-        ay = ay - q.y;
-        az = az - q.z;
+        bx = bx + q.x;
+        by = by + q.y;
+        bz = bz + q.z;
       }
-      s_pos[i] = integrate<SOLVER>(s, ax, ay, az, p);
+      s_pos[i] = integrate<SOLVER>(s, ax - bx, ay - by, az - bz, p);
       p.pos[d.mass_begin + i] = make_float4(s.x, s.y, s.z, s.m);
       p.vel[d.mass_begin + i] = make_float4(s.vx, s.vy, s.vz, 0.f);
     }
     if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
     t = tn;
     __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------- chunked kernel
+// For robots whose springs do not fit in LDS at once.  Positions stay in LDS
+// (skewed slots, as in step_lds) and each lane keeps MPT masses in
+// registers; every substep walks the robot's springs in chunks of
+// p.spring_cap: phase 1 streams the chunk's spring records from global
+// memory, writes the terms to LDS and copies the chunk's gather lists and
+// row bounds (u16, prepared on the host) into LDS; after a barrier each
+// lane adds the chunk's first-endpoint terms to P and second-endpoint terms
+// to M of its masses, in ascending spring order; a barrier ends the chunk.
+// After the last chunk the masses integrate with P - M.  LDS layout as in
+// step_lds with spring_cap = springs per chunk.
+template <int SOLVER, int BLOCK, int MPT>
+__global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PackDesc d = p.packs[blockIdx.x];
+  const int nm = static_cast<int>(d.n_mass);
+  const int ns = static_cast<int>(d.n_spring);
+  const int C = p.spring_cap;
+  const int n_chunks = ns > 0 ? (ns + C - 1) / C : 1;
+  const int zero = C + 1;
+  float* s_term = reinterpret_cast<float*>(smem);
+  float4* s_pos = reinterpret_cast<float4*>(smem + lds_term_bytes(C));
+  uint16_t* s_csr = reinterpret_cast<uint16_t*>(s_pos + pos_slots(p.mass_cap));
+  uint16_t* s_row = s_csr + p.csr_cap;
+  float* s_R = reinterpret_cast<float*>(s_row + 2 * p.mass_cap + 2);
+  const int tid = threadIdx.x;
+  const uint32_t* spair = p.spair + d.spring_begin;
+  const float* slbar = p.slbar + d.spring_begin;
+  const uint8_t* smat = p.smat + d.spring_begin;
+  const uint16_t* ccsr = p.ccsr + d.csr_begin;
+  const uint16_t* crow = p.crow + d.row_begin;
+  if (tid < 3) s_term[3 * zero + tid] = 0.f;
+
+  MassState ms[MPT];
+  int pslot[MPT];
+#pragma unroll
+  for (int j = 0; j < MPT; ++j) {
+    const int i = tid + j * BLOCK;
+    pslot[j] = pos_slot(i);
+    if (i < nm) {
+      load_mass(ms[j], p.pos, p.vel, d.mass_begin + i, p);
+      s_pos[pslot[j]] = lds_position<SOLVER>(ms[j]);
+    }
+  }
+  float t = p.t0;
+  if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], t);
+  __syncthreads();
+
+  for (int step = 0; step < p.n_steps; ++step) {
+    float P[MPT][3], M[MPT][3];
+#pragma unroll
+    for (int j = 0; j < MPT; ++j) {
+      P[j][0] = P[j][1] = P[j][2] = 0.0f;
+      M[j][0] = M[j][1] = M[j][2] = 0.0f;
+    }
+    const uint16_t* rc = crow;  // this chunk's 2 nm + 1 row bounds
+    int e_base = 0;             // this chunk's first entry in ccsr
+    const float tn = t + p.dt;
+    for (int c = 0; c < n_chunks; ++c) {
+      const int s0 = c * C;
+      const int s1 = min(ns, s0 + C);
+      for (int s = s0 + tid; s < s1; s += BLOCK) {
+        const uint32_t pr = spair[s];
+        const int m = smat[s];
+        const float kk = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[m] : p.mat_k[m];
+        float cx, cy, cz, lambda;
+        spring_term<SOLVER>(s_pos[pr & 0xffffu], s_pos[pr >> 16], slbar[s], s_R[m], kk, cx, cy,
+                            cz, lambda);
+        if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
+          float* stp = p.sstress + d.spring_begin + s;
+          *stp = *stp + lambda / slbar[s];
+        }
+        const int o = 3 * (s - s0);
+        s_term[o] = cx;
+        s_term[o + 1] = cy;
+        s_term[o + 2] = cz;
+      }
+      const int n_e = rc[2 * nm];
+      for (int i = tid; i < n_e; i += BLOCK) s_csr[i] = ccsr[e_base + i];
+      for (int i = tid; i <= 2 * nm; i += BLOCK) s_row[i] = rc[i];
+      __syncthreads();
+      const bool last = c == n_chunks - 1;
+#pragma unroll
+      for (int j = 0; j < MPT; ++j) {
+        const int i = tid + j * BLOCK;
+        if (i < nm) {
+          const int e1 = s_row[2 * i + 1], e2 = s_row[2 * i + 2];
+          float ax = P[j][0], ay = P[j][1], az = P[j][2];
+          for (int e = s_row[2 * i]; e < e1; e += 4) {
+            const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
+            const float* q0 = s_term + (w.x & 0xffffu);
+            const float* q1 = s_term + (w.x >> 16);
+            const float* q2 = s_term + (w.y & 0xffffu);
+            const float* q3 = s_term + (w.y >> 16);
+            const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
+            const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
+            ax = ax + x0; ay = ay + y0; az = az + z0;
+            ax = ax + x1; ay = ay + y1; az = az + z1;
+            ax = ax + x2; ay = ay + y2; az = az + z2;
+            ax = ax + x3; ay = ay + y3; az = az + z3;
+          }
+          P[j][0] = ax; P[j][1] = ay; P[j][2] = az;
+          float bx = M[j][0], by = M[j][1], bz = M[j][2];
+          for (int e = e1; e < e2; e += 4) {
+            const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
+            const float* q0 = s_term + (w.x & 0xffffu);
+            const float* q1 = s_term + (w.x >> 16);
+            const float* q2 = s_term + (w.y & 0xffffu);
+            const float* q3 = s_term + (w.y >> 16);
+            const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
+            const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
+            bx = bx + x0; by = by + y0; bz = bz + z0;
+            bx = bx + x1; by = by + y1; bz = bz + z1;
+            bx = bx + x2; by = by + y2; bz = bz + z2;
+            bx = bx + x3; by = by + y3; bz = bz + z3;
+          }
+          M[j][0] = bx; M[j][1] = by; M[j][2] = bz;
+          if (last) s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax - bx, ay - by, az - bz, p);
+        }
+      }
+      if (last && tid < p.n_mat)
+        s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
+      e_base += n_e;
+      rc += 2 * nm + 1;
+      __syncthreads();
+    }
+    t = tn;
+  }
+
+#pragma unroll
+  for (int j = 0; j < MPT; ++j) {
+    const int i = tid + j * BLOCK;
+    if (i < nm) {
+      const uint32_t gi = d.mass_begin + i;
+      p.pos[gi] = make_float4(ms[j].x, ms[j].y, ms[j].z, ms[j].m);
+      p.vel[gi] = make_float4(ms[j].vx, ms[j].vy, ms[j].vz, 0.f);
+    }
   }
 }
 
@@ -405,13 +547,16 @@ __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint3
 }
 
 #define MH_LDS(SOLVER, B, S, M) \
-  {SOLVER, MH_KERNEL_LDS, B, S, M, reinterpret_cast<const void*>(&step_lds<SOLVER, B, S, M>)}
+  {SOLVER, MH_KERNEL_LDS, 0, B, S, M, reinterpret_cast<const void*>(&step_lds<SOLVER, B, S, M>)}
+#define MH_CHUNK(SOLVER, B, M) \
+  {SOLVER, MH_KERNEL_STREAM, 1, B, 0, M, reinterpret_cast<const void*>(&step_chunk<SOLVER, B, M>)}
 #define MH_STREAM(SOLVER, B) \
-  {SOLVER, MH_KERNEL_STREAM, B, 0, 0, reinterpret_cast<const void*>(&step_stream<SOLVER, B>)}
+  {SOLVER, MH_KERNEL_STREAM, 0, B, 0, 0, reinterpret_cast<const void*>(&step_stream<SOLVER, B>)}
 #define MH_LDS_SET(SOLVER)                                                         \
   MH_LDS(SOLVER, 1024, 8, 1), MH_LDS(SOLVER, 1024, 8, 2), MH_LDS(SOLVER, 1024, 4, 1), \
       MH_LDS(SOLVER, 512, 8, 2), MH_LDS(SOLVER, 512, 16, 2), \
       MH_LDS(SOLVER, 512, 16, 4), MH_LDS(SOLVER, 256, 16, 4), MH_LDS(SOLVER, 256, 8, 2), \
+      MH_CHUNK(SOLVER, 1024, 2), MH_CHUNK(SOLVER, 1024, 4), MH_CHUNK(SOLVER, 512, 4), \
       MH_STREAM(SOLVER, 1024), MH_STREAM(SOLVER, 512), MH_STREAM(SOLVER, 256)
 
 const KernelVariant kVariants[] = {MH_LDS_SET(MH_SOLVER_XPBD_REF), MH_LDS_SET(MH_SOLVER_FORCE)};

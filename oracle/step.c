@@ -19,9 +19,9 @@
  *   (common/simulator/Simulator.cu:428-449).
  * The reference sums corrections with shared-memory float atomics (order
  * undefined).  The canonical order here, and in the HIP kernels, is: per
- * mass, start from +0.0f, add the terms of the springs whose first
- * endpoint it is (ascending spring index), then subtract the terms of the
- * springs whose second endpoint it is (ascending spring index).  |.| is sqrtf((x*x + y*y) + z*z) and sin is
+ * mass, P = sum from +0.0f of the terms of the springs whose first
+ * endpoint it is (ascending spring index), M = the same over the springs
+ * whose second endpoint it is, and the mass receives P - M.  |.| is sqrtf((x*x + y*y) + z*z) and sin is
  * orc_sinf (CUDA's norm3df/sinf are outside the reference tree).
  *
  * force (project spec; the reference snapshot has no force solver -- paper
@@ -136,12 +136,13 @@ static int run_jobs(job_t proto, int nthreads) {
   return 0;
 }
 
-/* Canonical summation order of the FP32 steps: per mass, from +0.0f, add
- * the terms of the springs whose first endpoint it is, in ascending spring
- * index, then subtract the terms of the springs whose second endpoint it
- * is, in ascending spring index.  Air springs contribute nothing. */
-static void gather_terms(const orc_topology* tp, uint32_t sb, uint32_t ns,
-                         const float* term, float* acc) {
+/* Canonical summation order of the FP32 steps: per mass, P = the sum from
+ * +0.0f of the terms of the springs whose first endpoint it is, in
+ * ascending spring index; M = the same for the springs whose second
+ * endpoint it is; the mass receives P - M.  Air springs contribute nothing.
+ * acc must hold zeros on entry. */
+static void gather_terms(const orc_topology* tp, uint32_t sb, uint32_t ns, uint32_t nm,
+                         const float* term, float* acc, float* accm) {
   for (uint32_t s = 0; s < ns; s++) {
     if (tp->mat[sb + s] == 0) continue;
     uint32_t a = tp->pairs[2 * (sb + s)];
@@ -149,13 +150,15 @@ static void gather_terms(const orc_topology* tp, uint32_t sb, uint32_t ns,
     acc[3 * a + 1] = acc[3 * a + 1] + term[3 * s + 1];
     acc[3 * a + 2] = acc[3 * a + 2] + term[3 * s + 2];
   }
+  for (uint32_t i = 0; i < 3 * nm; i++) accm[i] = 0.0f;
   for (uint32_t s = 0; s < ns; s++) {
     if (tp->mat[sb + s] == 0) continue;
     uint32_t b = tp->pairs[2 * (sb + s) + 1];
-    acc[3 * b] = acc[3 * b] - term[3 * s];
-    acc[3 * b + 1] = acc[3 * b + 1] - term[3 * s + 1];
-    acc[3 * b + 2] = acc[3 * b + 2] - term[3 * s + 2];
+    accm[3 * b] = accm[3 * b] + term[3 * s];
+    accm[3 * b + 1] = accm[3 * b + 1] + term[3 * s + 1];
+    accm[3 * b + 2] = accm[3 * b + 2] + term[3 * s + 2];
   }
+  for (uint32_t i = 0; i < 3 * nm; i++) acc[i] = acc[i] - accm[i];
 }
 
 /* ------------------------------------------------------------ xpbd fp32 */
@@ -171,6 +174,7 @@ static void xpbd32_robot(job_t* j, int r) {
   float* xt = (float*)malloc(sizeof(float) * 3 * nm);
   float* acc = (float*)malloc(sizeof(float) * 3 * nm);
   float* term = (float*)malloc(sizeof(float) * 3 * (ns + 1));
+  float* accm = (float*)malloc(sizeof(float) * 3 * nm);
   float* K = (float*)malloc(sizeof(float) * (size_t)mt->n);
   float* R = (float*)malloc(sizeof(float) * (size_t)mt->n);
   for (int m = 0; m < mt->n; m++) {
@@ -208,7 +212,7 @@ static void xpbd32_robot(job_t* j, int r) {
       if (st) st[s] = st[s] + lambda / lb;
       term[3 * s] = px; term[3 * s + 1] = py; term[3 * s + 2] = pz;
     }
-    gather_terms(tp, sb, ns, term, acc);
+    gather_terms(tp, sb, ns, nm, term, acc, accm);
     for (uint32_t i = 0; i < 3 * nm; i++) {
       float xn = xt[i] + acc[i];
       float diff = xn - x[i];
@@ -217,7 +221,7 @@ static void xpbd32_robot(job_t* j, int r) {
     }
     t = t + dt;
   }
-  free(xt); free(acc); free(term); free(K); free(R);
+  free(xt); free(acc); free(term); free(accm); free(K); free(R);
 }
 
 /* ------------------------------------------------------------ xpbd fp64 */
@@ -285,6 +289,7 @@ static void force32_robot(job_t* j, int r) {
   const float* ms = tp->mass + mb;
   float* acc = (float*)malloc(sizeof(float) * 3 * nm);
   float* term = (float*)malloc(sizeof(float) * 3 * (ns + 1));
+  float* accm = (float*)malloc(sizeof(float) * 3 * nm);
   float* R = (float*)malloc(sizeof(float) * (size_t)mt->n);
   float t = j->t0;
   for (int step = 0; step < j->nsteps; step++) {
@@ -307,7 +312,7 @@ static void force32_robot(job_t* j, int r) {
       float f = (mt->k[m] * (L - d)) * inv;
       term[3 * s] = f * dx; term[3 * s + 1] = f * dy; term[3 * s + 2] = f * dz;
     }
-    gather_terms(tp, sb, ns, term, acc);
+    gather_terms(tp, sb, ns, nm, term, acc, accm);
     for (uint32_t i = 0; i < nm; i++) {
       float m = ms[i];
       float fx = acc[3 * i], fy = acc[3 * i + 1], fz = acc[3 * i + 2];
@@ -345,7 +350,7 @@ static void force32_robot(job_t* j, int r) {
     }
     t = t + dt;
   }
-  free(acc); free(term); free(R);
+  free(acc); free(term); free(accm); free(R);
 }
 
 /* ----------------------------------------------------------- force fp64 */

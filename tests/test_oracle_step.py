@@ -204,13 +204,15 @@ def numpy_xpbd_step_f32(pos, vel, pairs, lbar, mat, K, R, dt):
         terms[s] = (lam * nrm).astype(np.float32)
     out_p, out_v = pos.copy(), vel.copy()
     for i in range(n):
-        acc = np.zeros(3, np.float32)
-        for s, (a, b) in enumerate(pairs):      # first endpoint: add
+        P = np.zeros(3, np.float32)
+        M = np.zeros(3, np.float32)
+        for s, (a, b) in enumerate(pairs):      # first endpoint
             if s in terms and a == i:
-                acc = (acc + terms[s]).astype(np.float32)
-        for s, (a, b) in enumerate(pairs):      # then second endpoint: subtract
+                P = (P + terms[s]).astype(np.float32)
+        for s, (a, b) in enumerate(pairs):      # second endpoint
             if s in terms and b == i:
-                acc = (acc - terms[s]).astype(np.float32)
+                M = (M + terms[s]).astype(np.float32)
+        acc = (P - M).astype(np.float32)
         xn = (xt[i] + acc).astype(np.float32)
         out_v[i] = ((0.99 * (xn - pos[i]).astype(np.float64)) / np.float64(f(dt))).astype(np.float32)
         out_p[i] = xn
@@ -268,13 +270,15 @@ def numpy_force_step_f32(pos, vel, mass, pairs, lbar, mat, k, R, env):
         terms[s] = (fs * d3).astype(np.float32)
     out_p, out_v = pos.copy(), vel.copy()
     for i in range(len(pos)):
-        F = np.zeros(3, np.float32)
-        for s, (a, b) in enumerate(pairs):      # first endpoint: add
+        P = np.zeros(3, np.float32)
+        M = np.zeros(3, np.float32)
+        for s, (a, b) in enumerate(pairs):      # first endpoint
             if s in terms and a == i:
-                F = (F + terms[s]).astype(np.float32)
-        for s, (a, b) in enumerate(pairs):      # then second endpoint: subtract
+                P = (P + terms[s]).astype(np.float32)
+        for s, (a, b) in enumerate(pairs):      # second endpoint
             if s in terms and b == i:
-                F = (F - terms[s]).astype(np.float32)
+                M = (M + terms[s]).astype(np.float32)
+        F = (P - M).astype(np.float32)
         v = vel[i].copy()
         F[1] = f(F[1] - f(mass[i] * g))
         if c != 0:
