@@ -87,6 +87,18 @@ int mh_voxel_build(const mh_material_set* set, int nx, int ny, int nz,
                    uint8_t* spring_material_id, uint32_t* spring_encoding,
                    int max_springs, int* n_springs,
                    uint32_t* stripped_encoding);
+/* VoxelRobot::BatchBuild (VoxelRobot.cpp:291-308): mh_voxel_build for
+ * n_robots lattices (voxel_encoding holds n_robots * nx*ny*nz encodings) on
+ * `threads` host threads (0 = all), written as one concatenated population:
+ * masses n_robots * nx*ny*nz (positions, material ids), springs per robot
+ * at [spring_offset[r], spring_offset[r+1]) in each robot's DFS order.
+ * spring_capacity bounds the total; mh_voxel_max_springs gives the per-robot
+ * bound. */
+int mh_voxel_build_batch(const mh_material_set* set, int n_robots, int nx, int ny, int nz,
+                         float resolution, const uint32_t* voxel_encoding, int threads,
+                         float* mass_pos, uint8_t* mass_material_id, uint32_t* spring_offset,
+                         uint16_t* spring_pairs, float* spring_length,
+                         uint8_t* spring_material_id, int64_t spring_capacity);
 /* Mean lengths |p[m0] - p[m1]| as SoftBody::Reset computes them. */
 int mh_rest_lengths(const float* pos, const uint16_t* pairs, int n_springs,
                     float* out);

@@ -68,6 +68,8 @@ _SIGS = {
     "mh_voxel_max_springs": (c_int, [c_int, c_int, c_int]),
     "mh_voxel_build": (c_int, [vp, c_int, c_int, c_int, c_float, u32p, f32p, u8p, u16p, f32p,
                                u8p, u32p, c_int, ctypes.POINTER(c_int), u32p]),
+    "mh_voxel_build_batch": (c_int, [vp, c_int, c_int, c_int, c_int, c_float, u32p, c_int, f32p,
+                                     u8p, u32p, u16p, f32p, u8p, ctypes.c_int64]),
     "mh_rest_lengths": (c_int, [f32p, u16p, c_int, f32p]),
     "mh_baseline": (c_int, [f32p, u8p, c_int, f32p, f32p]),
     "mh_config_default": (None, [ctypes.POINTER(Config)]),

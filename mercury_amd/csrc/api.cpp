@@ -1,7 +1,9 @@
 // api.cpp -- C entry points for errors, materials, voxel topology, baseline.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <thread>
 
 #include "mh_internal.h"
 
@@ -133,6 +135,60 @@ int mh_voxel_build(const mh_material_set* set, int nx, int ny, int nz, float res
   std::memcpy(spring_material_id, out.spring_id.data(), ns);
   if (spring_encoding) std::memcpy(spring_encoding, out.spring_enc.data(), sizeof(uint32_t) * ns);
   if (stripped_encoding) std::memcpy(stripped_encoding, out.stripped_enc.data(), sizeof(uint32_t) * n);
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_voxel_build_batch(const mh_material_set* set, int n_robots, int nx, int ny, int nz,
+                         float resolution, const uint32_t* voxel_encoding, int threads,
+                         float* mass_pos, uint8_t* mass_material_id, uint32_t* spring_offset,
+                         uint16_t* spring_pairs, float* spring_length, uint8_t* spring_material_id,
+                         int64_t spring_capacity) {
+  MH_TRY
+  if (!set || n_robots < 0 || (n_robots > 0 && (!voxel_encoding || !mass_pos || !mass_material_id ||
+                                                !spring_offset || !spring_pairs || !spring_length ||
+                                                !spring_material_id)))
+    throw mh::Error(MH_ERR_ARG, "null argument");
+  if (!(resolution > 0.f)) throw mh::Error(MH_ERR_ARG, "resolution must be > 0");
+  if (nx < 1 || ny < 1 || nz < 1 || static_cast<long>(nx) * ny * nz > 65536)
+    throw mh::Error(MH_ERR_ARG, "lattice must have 1..65536 points");
+  const size_t nv = static_cast<size_t>(nx) * ny * nz;
+  std::vector<mh::VoxelOut> out(static_cast<size_t>(n_robots));
+  int nt = threads > 0 ? threads : static_cast<int>(std::thread::hardware_concurrency());
+  nt = std::max(1, std::min(nt, std::max(n_robots, 1)));
+  std::vector<std::string> errors(static_cast<size_t>(nt));
+  std::atomic<int> next{0};
+  auto work = [&](int t) {
+    try {
+      for (int r = next++; r < n_robots; r = next++)
+        mh::voxel_build(*set, nx, ny, nz, resolution, voxel_encoding + nv * r, out[r]);
+    } catch (const std::exception& e) {
+      errors[t] = e.what();
+      next = n_robots;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; t++) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  for (const std::string& e : errors)
+    if (!e.empty()) throw mh::Error(MH_ERR_ARG, e);
+  spring_offset[0] = 0;
+  for (int r = 0; r < n_robots; r++) {
+    const size_t ns = out[r].length.size();
+    if (spring_offset[r] + ns > static_cast<uint64_t>(spring_capacity))
+      throw mh::Error(MH_ERR_RANGE, "spring_capacity too small");
+    spring_offset[r + 1] = spring_offset[r] + static_cast<uint32_t>(ns);
+  }
+  for (int r = 0; r < n_robots; r++) {
+    const mh::VoxelOut& o = out[r];
+    const size_t s0 = spring_offset[r], ns = o.length.size();
+    std::memcpy(mass_pos + 3 * nv * r, o.pos.data(), sizeof(float) * 3 * nv);
+    std::memcpy(mass_material_id + nv * r, o.mass_id.data(), nv);
+    std::memcpy(spring_pairs + 2 * s0, o.pairs.data(), sizeof(uint16_t) * 2 * ns);
+    std::memcpy(spring_length + s0, o.length.data(), sizeof(float) * ns);
+    std::memcpy(spring_material_id + s0, o.spring_id.data(), ns);
+  }
   return MH_OK;
   MH_CATCH
 }

@@ -28,7 +28,7 @@ import numpy as np
 
 from . import _lib
 from .materials import WORKLOAD_ROWS, workload_materials
-from .robots import last_layer_air, voxel_build
+from .robots import last_layer_air, voxel_build, voxel_build_batch
 from .simulator import SimConfig, Simulator
 from .surrogate import Surrogate, spearman
 
@@ -129,8 +129,10 @@ def genome_features(g, voxels, rows=WORKLOAD_ROWS):
 class BatchSimulator:
     """Builds and simulates genome batches; one evaluation per robot."""
 
-    def __init__(self, voxels=8, duration=1.0, dt=1e-4, spacing=0.1, mass=0.001, device=0):
+    def __init__(self, voxels=8, duration=1.0, dt=1e-4, spacing=0.1, mass=0.001, device=0,
+                 build_threads=0):
         self.voxels, self.duration = voxels, duration
+        self.build_threads = build_threads
         self.ms = workload_materials()
         self.res = 1.0 / spacing
         self.mass = mass
@@ -142,9 +144,10 @@ class BatchSimulator:
 
     def fitness(self, genomes):
         enc = genome_to_lattice(genomes, self.voxels)
-        els = [voxel_build(self.ms, self.voxels + 1, self.res, e, mass=self.mass) for e in enc]
+        pop = voxel_build_batch(self.ms, self.voxels + 1, self.res, enc, mass=self.mass,
+                                threads=self.build_threads)
         t0 = time.perf_counter()
-        self.sim.SetElements(els)
+        self.sim.set_population(pop)
         if not self.tuned:
             self.sim.autotune(100)
             self.tuned = True

@@ -72,6 +72,30 @@ def voxel_build(material_set, n, resolution, voxel_encoding, mass=1.0):
     return e
 
 
+def voxel_build_batch(material_set, n, resolution, voxel_encodings, mass=1.0, threads=0):
+    """VoxelRobot::BatchBuild: many n^3 lattices built by native threads into
+    one concatenated population (dict of arrays, as Simulator.set_population
+    takes them)."""
+    L = load()
+    nv = n ** 3
+    enc = np.ascontiguousarray(voxel_encodings, np.uint32).reshape(-1, nv)
+    R = len(enc)
+    cap = max(1, R * L.mh_voxel_max_springs(n, n, n))
+    pos = np.zeros((R * nv, 3), np.float32)
+    mid = np.zeros(R * nv, np.uint8)
+    so = np.zeros(R + 1, np.uint32)
+    pairs = np.zeros((cap, 2), np.uint16)
+    ln = np.zeros(cap, np.float32)
+    sid = np.zeros(cap, np.uint8)
+    check(L.mh_voxel_build_batch(material_set.handle, R, n, n, n, float(resolution), ptr(enc, u32p),
+                                 int(threads), ptr(pos, f32p), ptr(mid, u8p), ptr(so, u32p),
+                                 ptr(pairs, u16p), ptr(ln, f32p), ptr(sid, u8p), cap))
+    S = int(so[-1])
+    return dict(mass_offset=(np.arange(R + 1, dtype=np.uint32) * nv), spring_offset=so, pos=pos,
+                vel=np.zeros_like(pos), mass=np.full(R * nv, mass, np.float32), mass_material=mid,
+                pairs=pairs[:S], mean_length=ln[:S], spring_material=sid[:S])
+
+
 def rest_lengths(pos, pairs):
     """SoftBody::Reset mean lengths from the current proto positions."""
     pos = np.ascontiguousarray(pos, np.float32)

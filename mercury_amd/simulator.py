@@ -96,6 +96,7 @@ class Simulator:
                                    ctypes.byref(h)))
         self._h = h
         self._elements = []
+        self._n_robots = 0
         self._arrays = None
 
     def close(self):
@@ -112,17 +113,25 @@ class Simulator:
 
     def SetElements(self, elements):
         """Simulator::SetElements (Simulator.cu:206-367): copy to the device."""
-        a = pack(elements)
+        self.set_population(pack(elements))
         self._elements = list(elements)
-        self._arrays = a
-        pop = Population(len(elements), ptr(a["mass_offset"], u32p), ptr(a["spring_offset"], u32p),
+        mo, so = self._arrays["mass_offset"], self._arrays["spring_offset"]
+        return [ElementTracker(i, int(mo[i]), int(mo[i + 1]), int(so[i]), int(so[i + 1]))
+                for i in range(len(elements))]
+
+    def set_population(self, a):
+        """Upload a packed population (the dict pack() or
+        robots.voxel_build_batch() returns) without Element objects."""
+        n = len(a["mass_offset"]) - 1
+        a = {k: np.ascontiguousarray(v) for k, v in a.items()}
+        pop = Population(n, ptr(a["mass_offset"], u32p), ptr(a["spring_offset"], u32p),
                          ptr(a["pos"], f32p), ptr(a["vel"], f32p), ptr(a["mass"], f32p),
                          ptr(a["mass_material"], u8p), ptr(a["pairs"], u16p),
                          ptr(a["mean_length"], f32p), ptr(a["spring_material"], u8p))
         check(load().mh_sim_set_population(self._h, ctypes.byref(pop)))
-        mo, so = a["mass_offset"], a["spring_offset"]
-        return [ElementTracker(i, int(mo[i]), int(mo[i + 1]), int(so[i]), int(so[i + 1]))
-                for i in range(len(elements))]
+        self._arrays = a
+        self._n_robots = n
+        self._elements = []
 
     # ------------------------------------------------------------- stepping
     def Simulate(self, sim_duration, trackStresses=False):
@@ -161,14 +170,14 @@ class Simulator:
 
     # ------------------------------------------------------------- scoring
     def fitness(self, mode=_lib.MH_FITNESS_REFERENCE):
-        out = np.zeros(max(len(self._elements), 1), np.float32)
+        out = np.zeros(max(self._n_robots, 1), np.float32)
         check(load().mh_sim_fitness(self._h, int(mode), ptr(out, f32p)))
-        return out[:len(self._elements)]
+        return out[:self._n_robots]
 
     def com(self):
-        out = np.zeros((max(len(self._elements), 1), 3), np.float32)
+        out = np.zeros((max(self._n_robots, 1), 3), np.float32)
         check(load().mh_sim_com(self._h, ptr(out, f32p)))
-        return out[:len(self._elements)]
+        return out[:self._n_robots]
 
     # --------------------------------------------------------------- time
     def Reset(self):

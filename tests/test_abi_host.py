@@ -112,3 +112,17 @@ def test_voxel_build_rejects_bad_input():
         mh.voxel_build(ms, 3, 10.0, np.full(27, 1 << 9, np.uint32))  # bit outside the set
     with pytest.raises(mh.MhError):
         mh.voxel_build(ms, 3, 0.0, np.full(27, 2, np.uint32))       # resolution 0
+
+
+def test_voxel_build_batch_matches_single_builds():
+    """VoxelRobot::BatchBuild: the threaded batch gives the per-robot builds,
+    concatenated, for any thread count."""
+    ms = mh.workload_materials()
+    rng = np.random.default_rng(5)
+    encs = np.stack([mh.robots.random_voxel_encodings(rng, 4, 4, 0.3) for _ in range(37)])
+    encs[3] = 1  # an all-air robot: no springs
+    ref = mh.pack([mh.robots.voxel_build(ms, 5, 10.0, e, mass=0.002) for e in encs])
+    for threads in (1, 3, 0):
+        got = mh.robots.voxel_build_batch(ms, 5, 10.0, encs, mass=0.002, threads=threads)
+        for k in ref:
+            assert np.array_equal(ref[k], got[k]), (threads, k)
