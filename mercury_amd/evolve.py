@@ -15,11 +15,13 @@ Evaluator_impl.h:39-107), rebuilt for the surrogate-screened workload:
   * evaluation: every simulated robot counts one evaluation, as
     Evaluator_impl.h:103-106 counts them; fitness is the COM-x displacement
     after `duration` s on land;
-  * surrogate: scores `screen_factor` x offspring, the best `pop` are
-    simulated; retrained every `retrain_every` generations on all simulated
-    pairs; when its Spearman correlation on the last simulated batch falls
-    below `min_rho`, the next generations are simulated without screening
-    until a retrain brings it back.
+  * surrogate: scores `screen_factor` x offspring; of each layer's share
+    the best (1 - audit) by prediction and a uniform `audit` sample of the
+    rest are simulated; retrained every `retrain_every` generations on all
+    simulated pairs; when its Spearman correlation on the audit sample
+    falls below `min_rho`, the next generations are simulated without
+    screening (still scored, to watch the correlation) until it recovers
+    or a retrain brings it back.
 """
 import time
 from dataclasses import dataclass, field
@@ -186,7 +188,8 @@ def age_limits(layers, age_gap):
 
 def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elitism=0.1,
         age_gap=5, threshold=None, max_generations=40, use_surrogate=False, screen_factor=4,
-        retrain_every=5, min_rho=0.6, train_epochs=30, duration=1.0, sim=None, verbose=False):
+        retrain_every=5, min_rho=0.6, train_epochs=30, duration=1.0, sim=None, verbose=False,
+        audit=0.1):
     """One evolution run; stops when the best fitness reaches `threshold` or
     after max_generations.  Returns a RunResult."""
     rng = np.random.default_rng(seed)
@@ -247,11 +250,21 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         layer_of = np.repeat(np.arange(layers), n_child // layers)
         if use_surrogate and screening:
             pred = sur.predict(genome_features(kids, voxels))
-            keep = []
-            for i in range(layers):  # keep the top share of every layer's offspring
+            keep, audited = [], []
+            n_audit = int(round(audit * per))
+            for i in range(layers):
+                # the top share of every layer's offspring by prediction, plus
+                # a uniform sample of the rest that measures the surrogate
                 sel = np.where(layer_of == i)[0]
-                keep.append(sel[np.argsort(-pred[sel], kind="stable")[:per]])
+                order = sel[np.argsort(-pred[sel], kind="stable")]
+                top = order[:per - n_audit]
+                rest = order[per - n_audit:]
+                aud = rng.choice(rest, size=min(n_audit, len(rest)), replace=False) if n_audit else rest[:0]
+                keep += [top, aud]
+                audited.append(np.ones(len(aud), bool))
+                audited.insert(-1, np.zeros(len(top), bool))
             keep = np.concatenate(keep)
+            audited = np.concatenate(audited)
             kids, kid_age, layer_of, pred = kids[keep], kid_age[keep], layer_of[keep], pred[keep]
         else:
             res.fallback_generations += int(use_surrogate)
@@ -260,7 +273,9 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         data_x.append(feats)
         data_y.append(fit)
         if use_surrogate and screening:
-            rho = spearman(pred, fit)
+            # with an audit sample, rank quality is judged on it alone (the
+            # screened top share is range-restricted)
+            rho = spearman(pred[audited], fit[audited]) if audited.sum() >= 8 else spearman(pred, fit)
             res.rho_history.append(rho)
             if rho < min_rho:
                 screening = False
@@ -332,20 +347,32 @@ def surrogate_quality(n_train=200000, n_test=20000, voxels=8, duration=1.0, seed
                 sim_seconds=sim.sim_seconds)
 
 
+def evals_to(history, threshold):
+    """(evaluations, seconds) at which a run's best fitness first reached
+    threshold, from RunResult.history; (-1, -1) if it never did."""
+    for ev, sec, best in history:
+        if best >= threshold:
+            return ev, sec
+    return -1, -1.0
+
+
 def compare(seeds=5, threshold=None, pop=4096, max_generations=40, voxels=8, duration=1.0,
-            verbose=True, **kw):
-    """Evaluations (and wall-clock) to reach `threshold`, plain GA vs
-    surrogate-screened GA, over the given seeds."""
+            verbose=True, seed_start=0, **kw):
+    """Plain GA vs surrogate-screened GA over the given seeds, with the same
+    seeds and evaluation counting.  With a threshold each run stops when its
+    best fitness reaches it; without one every run goes max_generations and
+    evaluations-to-threshold follow from the histories (evals_to)."""
     sim = BatchSimulator(voxels=voxels, duration=duration)
     out = []
-    for s in range(seeds):
+    for s in range(seed_start, seed_start + seeds):
         for use in (False, True):
             r = run(seed=s, pop=pop, voxels=voxels, threshold=threshold, use_surrogate=use,
                     max_generations=max_generations, sim=sim, verbose=verbose, **kw)
             row = dict(seed=s, surrogate=use, evals_to_threshold=r.evals_to_threshold,
                        seconds_to_threshold=r.seconds_to_threshold, generations=r.generations,
                        best=r.best, fallback_generations=r.fallback_generations,
-                       rho=[round(x, 3) for x in r.rho_history])
+                       rho=[round(x, 3) for x in r.rho_history],
+                       history=[(int(e), round(t, 2), float(b)) for e, t, b in r.history])
             out.append(row)
             if verbose:
                 import json
@@ -365,7 +392,8 @@ def main(argv=None):
     q.add_argument("--seed", type=int, default=0)
     c = sub.add_parser("compare", help="evaluations to threshold with and without the surrogate")
     c.add_argument("--seeds", type=int, default=5)
-    c.add_argument("--threshold", type=float, required=True)
+    c.add_argument("--threshold", type=float, default=None)
+    c.add_argument("--seed-start", type=int, default=0)
     c.add_argument("--pop", type=int, default=4096)
     c.add_argument("--generations", type=int, default=40)
     c.add_argument("--screen-factor", type=int, default=4)
@@ -386,7 +414,7 @@ def main(argv=None):
     elif a.cmd == "compare":
         rows = compare(a.seeds, a.threshold, pop=a.pop, max_generations=a.generations,
                        voxels=a.voxels, duration=a.duration, screen_factor=a.screen_factor,
-                       train_epochs=a.epochs)
+                       train_epochs=a.epochs, seed_start=a.seed_start)
         for use in (False, True):
             e = [x["evals_to_threshold"] for x in rows if x["surrogate"] == use]
             print(json.dumps(dict(surrogate=use, evals_to_threshold=e,
