@@ -250,6 +250,27 @@ int mh_mlp_train(mh_mlp* net, const float* x, const float* target, int n,
                  int batch, int n_epochs, float lr, uint64_t seed,
                  float* final_loss);
 
+/* No reference counterpart: a convolutional feature extractor in front of
+ * the surrogate MLP, for genomes of 8^3 voxel materials (0 = empty, 1..4;
+ * voxel (x, y, z) at x + 8 (y + 8 z)).  conv 3x3x3 5->32 GELU, conv 32->32
+ * GELU, 2x2x2 average pool, conv 32->64 GELU, mean over positions (64
+ * features), dense 64->256 GELU ->256 GELU ->1.  bf16 weights with an fp32
+ * master copy, fp32 accumulation, deterministic reductions; parameters are
+ * laid out layer by layer as W [out x in] then b, with conv weights
+ * indexed [out][k * C_in + c], k = (dz+1)*9 + (dy+1)*3 + (dx+1). */
+typedef struct mh_cnn mh_cnn;
+int mh_cnn_create(int device, uint64_t seed, mh_cnn** out);
+int mh_cnn_destroy(mh_cnn* net);
+int mh_cnn_param_count(const mh_cnn* net);
+int mh_cnn_get_params(const mh_cnn* net, float* out);
+int mh_cnn_set_params(mh_cnn* net, const float* in);
+int mh_cnn_forward(mh_cnn* net, const uint8_t* genomes, int n, float* y);
+int mh_cnn_loss_grad(mh_cnn* net, const uint8_t* genomes, const float* target,
+                     int n, float* loss, float* grad);
+int mh_cnn_train(mh_cnn* net, const uint8_t* genomes, const float* target,
+                 int n, int batch, int n_epochs, float lr, uint64_t seed,
+                 float* final_loss);
+
 /* ------------------------------------------------ measurement utilities */
 /* No reference counterpart: figures for the roofline of the step kernels.
  * LDS read bandwidth of the device (GB/s), from a ds_read_b128 kernel. */

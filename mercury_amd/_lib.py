@@ -104,6 +104,14 @@ _SIGS = {
     "mh_mlp_forward": (c_int, [vp, f32p, c_int, f32p]),
     "mh_mlp_loss_grad": (c_int, [vp, f32p, f32p, c_int, f32p, f32p]),
     "mh_mlp_train": (c_int, [vp, f32p, f32p, c_int, c_int, c_int, c_float, c_uint64, f32p]),
+    "mh_cnn_create": (c_int, [c_int, ctypes.c_uint64, ctypes.POINTER(vp)]),
+    "mh_cnn_destroy": (c_int, [vp]),
+    "mh_cnn_param_count": (c_int, [vp]),
+    "mh_cnn_get_params": (c_int, [vp, f32p]),
+    "mh_cnn_set_params": (c_int, [vp, f32p]),
+    "mh_cnn_forward": (c_int, [vp, u8p, c_int, f32p]),
+    "mh_cnn_loss_grad": (c_int, [vp, u8p, f32p, c_int, f32p, f32p]),
+    "mh_cnn_train": (c_int, [vp, u8p, f32p, c_int, c_int, c_int, c_float, ctypes.c_uint64, f32p]),
     "mh_calibrate_lds": (c_int, [c_int, f32p]),
     "mh_calibrate_copy": (c_int, [c_int, ctypes.c_size_t, c_int, f32p]),
     "mh_check_arith": (c_int, [c_int, c_int, ctypes.c_uint32, ctypes.c_uint32,

@@ -1,0 +1,561 @@
+// cnn.hip -- convolutional surrogate for voxel genomes: a learned 64-d
+// feature extractor in front of the surrogate MLP (64 -> 256 -> 256 -> 1).
+//
+// Input: a genome, one material index (0 = empty, 1..4) per voxel of an 8^3
+// grid, voxel (x, y, z) at x + 8 (y + 8 z).  Network:
+//   conv1  3x3x3, 5 (one-hot materials) -> 32, zero padding, GELU
+//   conv2  3x3x3, 32 -> 32, GELU
+//   pool   2x2x2 average, 8^3 -> 4^3
+//   conv3  3x3x3, 32 -> 64, GELU
+//   mean over the 4^3 positions -> 64 features
+//   dense  64 -> 256 GELU -> 256 GELU -> 1
+// Convolutions run as GEMMs over im2col rows (one row per output position,
+// column k*C + c for kernel offset k = (dz+1)*9 + (dy+1)*3 + (dx+1)); the
+// backward pass gathers column gradients back to positions in a fixed
+// order (col2im), weight gradients reduce over the batch with split-K
+// partials summed in a fixed order, so training is deterministic.  Weights
+// are bf16 with an fp32 master copy for Adam; activations and
+// accumulation are fp32.  Loss = mean 0.5 (y - target)^2.
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "mercury_hip.h"
+#include "mh_internal.h"
+#include "mh_nn_kernels.h"
+
+namespace {
+
+constexpr int kG = 8, kP1 = kG * kG * kG;        // genome grid and positions
+constexpr int kG2 = kG / 2, kP2 = kG2 * kG2 * kG2;
+constexpr int kC0 = 5, kC1 = 32, kC2 = 32, kC3 = 64, kH = 256;
+constexpr int kK1 = 27 * kC0, kK2 = 27 * kC1, kK3 = 27 * kC2;
+constexpr int kSplit = 64;                        // split-K partials for weight gradients
+
+__device__ __forceinline__ bool inside(int z, int y, int x, int D) {
+  return z >= 0 && z < D && y >= 0 && y < D && x >= 0 && x < D;
+}
+
+// col[r][k*5 + c] = 1 if the voxel at position(r) + offset(k) lies in the
+// grid and holds material c; r = n * 512 + pos.  One thread per (r, k).
+__global__ void im2col_onehot(const uint8_t* g, int B, float* col) {
+  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<size_t>(B) * kP1 * 27) return;
+  const size_t r = t / 27;
+  const int k = static_cast<int>(t % 27);
+  const int n = static_cast<int>(r / kP1), pos = static_cast<int>(r % kP1);
+  const int x = pos % kG, y = (pos / kG) % kG, z = pos / (kG * kG);
+  const int zz = z + k / 9 - 1, yy = y + (k / 3) % 3 - 1, xx = x + k % 3 - 1;
+  int m = -1;
+  if (inside(zz, yy, xx, kG)) m = g[static_cast<size_t>(n) * kP1 + (zz * kG + yy) * kG + xx];
+  float* out = col + r * kK1 + k * kC0;
+#pragma unroll
+  for (int c = 0; c < kC0; ++c) out[c] = (m == c) ? 1.f : 0.f;
+}
+
+// col[r][k*C + c] = act[n, pos(r) + offset(k), c] (0 outside the D^3 grid);
+// one thread per output element.
+__global__ void im2col_f(const float* act, int B, int D, int C, float* col) {
+  const int P = D * D * D;
+  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<size_t>(B) * P * 27 * C) return;
+  const int c = static_cast<int>(t % C);
+  const size_t rk = t / C;
+  const int k = static_cast<int>(rk % 27);
+  const size_t r = rk / 27;
+  const int n = static_cast<int>(r / P), pos = static_cast<int>(r % P);
+  const int x = pos % D, y = (pos / D) % D, z = pos / (D * D);
+  const int zz = z + k / 9 - 1, yy = y + (k / 3) % 3 - 1, xx = x + k % 3 - 1;
+  col[t] = inside(zz, yy, xx, D) ? act[(static_cast<size_t>(n) * P + (zz * D + yy) * D + xx) * C + c] : 0.f;
+}
+
+// d[n, pos, c] = sum over k of dcol[n, pos - offset(k), k*C + c] (k
+// ascending, positions inside the grid), times gelu'(Z[n, pos, c]) when Z
+// is given.
+__global__ void col2im_f(const float* dcol, const float* Z, int B, int D, int C, float* d) {
+  const int P = D * D * D;
+  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<size_t>(B) * P * C) return;
+  const int c = static_cast<int>(t % C);
+  const size_t r = t / C;
+  const int n = static_cast<int>(r / P), pos = static_cast<int>(r % P);
+  const int x = pos % D, y = (pos / D) % D, z = pos / (D * D);
+  float s = 0.f;
+  for (int k = 0; k < 27; ++k) {
+    const int zz = z - (k / 9 - 1), yy = y - ((k / 3) % 3 - 1), xx = x - (k % 3 - 1);
+    if (inside(zz, yy, xx, D))
+      s += dcol[(static_cast<size_t>(n) * P + (zz * D + yy) * D + xx) * (27 * C) + k * C + c];
+  }
+  d[t] = Z ? s * gelu_grad(Z[t]) : s;
+}
+
+// 2x2x2 average pooling 8^3 -> 4^3: the 8 inputs summed z, y, x ascending
+__global__ void pool2_fwd(const float* a, int B, float* p) {
+  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<size_t>(B) * kP2 * kC2) return;
+  const int c = static_cast<int>(t % kC2);
+  const size_t r = t / kC2;
+  const int n = static_cast<int>(r / kP2), q = static_cast<int>(r % kP2);
+  const int X = q % kG2, Y = (q / kG2) % kG2, Z = q / (kG2 * kG2);
+  float s = 0.f;
+  for (int dz = 0; dz < 2; ++dz)
+    for (int dy = 0; dy < 2; ++dy)
+      for (int dx = 0; dx < 2; ++dx)
+        s += a[(static_cast<size_t>(n) * kP1 + ((2 * Z + dz) * kG + 2 * Y + dy) * kG + 2 * X + dx) * kC2 + c];
+  p[t] = 0.125f * s;
+}
+
+// pooling backward fused with conv2's activation: dZ2 = dp / 8 * gelu'(Z2)
+__global__ void pool2_bwd_gelu(const float* dp, const float* Z, int B, float* dZ) {
+  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<size_t>(B) * kP1 * kC2) return;
+  const int c = static_cast<int>(t % kC2);
+  const size_t r = t / kC2;
+  const int n = static_cast<int>(r / kP1), pos = static_cast<int>(r % kP1);
+  const int x = pos % kG, y = (pos / kG) % kG, z = pos / (kG * kG);
+  const int q = ((z / 2) * kG2 + y / 2) * kG2 + x / 2;
+  dZ[t] = 0.125f * dp[(static_cast<size_t>(n) * kP2 + q) * kC2 + c] * gelu_grad(Z[t]);
+}
+
+// mean over the 4^3 positions (ascending) -> features [B][64]
+__global__ void gap_fwd(const float* a, int B, float* f) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B * kC3) return;
+  const int n = t / kC3, c = t % kC3;
+  float s = 0.f;
+  for (int q = 0; q < kP2; ++q) s += a[(static_cast<size_t>(n) * kP2 + q) * kC3 + c];
+  f[t] = s * (1.0f / kP2);
+}
+
+// mean backward fused with conv3's activation: dZ3 = df / 64 * gelu'(Z3)
+__global__ void gap_bwd_gelu(const float* df, const float* Z, int B, float* dZ) {
+  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+  if (t >= static_cast<size_t>(B) * kP2 * kC3) return;
+  const int c = static_cast<int>(t % kC3);
+  const int n = static_cast<int>(t / (static_cast<size_t>(kP2) * kC3));
+  dZ[t] = df[n * kC3 + c] * (1.0f / kP2) * gelu_grad(Z[t]);
+}
+
+// Weight gradient with split-K: part[s][i][j] = sum over rows r in split s
+// (ascending) of D[r][i] * X[r][j]; D is [M x Ni], X is [M x Nj].
+__global__ __launch_bounds__(256) void wgrad_split(const float* __restrict__ Dm, int Ni,
+                                                   const float* __restrict__ X, int Nj, int M, int rows,
+                                                   float* __restrict__ part) {
+  __shared__ float sd[kTk][kTile + 1];
+  __shared__ float sx[kTk][kTile + 1];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
+  const int r_begin = blockIdx.z * rows, r_end = min(M, r_begin + rows);
+  float acc[4][4] = {};
+  for (int r0 = r_begin; r0 < r_end; r0 += kTk) {
+    for (int e = threadIdx.x; e < kTile * kTk; e += 256) {
+      const int ii = e % kTile, rr = e / kTile;
+      const int r = r0 + rr;
+      const int i = i0 + ii, j = j0 + ii;
+      sd[rr][ii] = (r < r_end && i < Ni) ? Dm[static_cast<size_t>(r) * Ni + i] : 0.f;
+      sx[rr][ii] = (r < r_end && j < Nj) ? X[static_cast<size_t>(r) * Nj + j] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < kTk; ++rr) {
+      float a[4], b[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = sd[rr][ty * 4 + q];
+        b[q] = sx[rr][tx * 4 + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) acc[q][w] = fmaf(a[q], b[w], acc[q][w]);
+    }
+    __syncthreads();
+  }
+  float* out = part + static_cast<size_t>(blockIdx.z) * Ni * Nj;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const int i = i0 + ty * 4 + q, j = j0 + tx * 4 + w;
+      if (i < Ni && j < Nj) out[static_cast<size_t>(i) * Nj + j] = acc[q][w];
+    }
+}
+
+// out[e] = sum over splits s (ascending) of part[s][e]
+__global__ void sum_splits(const float* part, int n, int splits, float* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = 0.f;
+  for (int k = 0; k < splits; ++k) s += part[static_cast<size_t>(k) * n + e];
+  out[e] = s;
+}
+
+// column sums with split rows: part[s][c] = sum over rows of split s
+__global__ void colsum_split(const float* X, int M, int N, int rows, float* part) {
+  const int c = threadIdx.x + blockIdx.x * blockDim.x;
+  if (c >= N) return;
+  const int r_begin = blockIdx.y * rows, r_end = min(M, r_begin + rows);
+  float s = 0.f;
+  for (int r = r_begin; r < r_end; ++r) s += X[static_cast<size_t>(r) * N + c];
+  part[static_cast<size_t>(blockIdx.y) * N + c] = s;
+}
+
+__global__ void gather_genomes(const uint8_t* src, const int* idx, int n, uint8_t* dst) {
+  const int r = blockIdx.x;
+  if (r >= n) return;
+  for (int j = threadIdx.x; j < kP1; j += blockDim.x)
+    dst[static_cast<size_t>(r) * kP1 + j] = src[static_cast<size_t>(idx[r]) * kP1 + j];
+}
+
+__global__ void gather_scalar(const float* src, const int* idx, int n, float* dst) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[r] = src[idx[r]];
+}
+
+unsigned blocks_for(size_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+struct LayerShape {
+  int out, in;  // weight [out x in], bias [out]
+};
+constexpr LayerShape kLayers[] = {{kC1, kK1}, {kC2, kK2}, {kC3, kK3}, {kH, kC3}, {kH, kH}, {1, kH}};
+constexpr int kNL = 6;
+
+}  // namespace
+
+struct mh_cnn {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t w_off[kNL] = {}, b_off[kNL] = {};
+  size_t n_params = 0;
+  Dev<float> params, grads, m, v, part;
+  Dev<__hip_bfloat16> w16;
+  int cap = 0;
+  // activations (B rows of genomes)
+  Dev<uint8_t> g;
+  Dev<float> col1, z1, a1, col2, z2, a2, p2, col3, z3, a3, feat, zh1, ah1, zh2, ah2, y, t;
+  // gradients
+  Dev<float> dy, terms, dzh2, dzh1, dfeat, dz3, dcol3, dp2, dz2, dcol2, dz1;
+  Dev<uint8_t> data_g;
+  Dev<float> data_t;
+  Dev<int> idx;
+  int adam_t = 0;
+  ~mh_cnn() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  const __hip_bfloat16* W(int l) const { return w16.p + w_off[l]; }
+  const float* b(int l) const { return params.p + b_off[l]; }
+};
+
+namespace {
+
+void ensure(mh_cnn* net, int B) {
+  if (B <= net->cap) return;
+  const size_t R1 = static_cast<size_t>(B) * kP1, R2 = static_cast<size_t>(B) * kP2;
+  net->g.ensure(static_cast<size_t>(B) * kP1);
+  net->col1.ensure(R1 * kK1);
+  net->z1.ensure(R1 * kC1);
+  net->a1.ensure(R1 * kC1);
+  net->col2.ensure(R1 * kK2);
+  net->z2.ensure(R1 * kC2);
+  net->a2.ensure(R1 * kC2);
+  net->p2.ensure(R2 * kC2);
+  net->col3.ensure(R2 * kK3);
+  net->z3.ensure(R2 * kC3);
+  net->a3.ensure(R2 * kC3);
+  net->feat.ensure(static_cast<size_t>(B) * kC3);
+  net->zh1.ensure(static_cast<size_t>(B) * kH);
+  net->ah1.ensure(static_cast<size_t>(B) * kH);
+  net->zh2.ensure(static_cast<size_t>(B) * kH);
+  net->ah2.ensure(static_cast<size_t>(B) * kH);
+  net->y.ensure(B);
+  net->t.ensure(B);
+  net->dy.ensure(B);
+  net->terms.ensure(B);
+  net->dzh2.ensure(static_cast<size_t>(B) * kH);
+  net->dzh1.ensure(static_cast<size_t>(B) * kH);
+  net->dfeat.ensure(static_cast<size_t>(B) * kC3);
+  net->dz3.ensure(R2 * kC3);
+  net->dcol3.ensure(R2 * kK3);
+  net->dp2.ensure(R2 * kC2);
+  net->dz2.ensure(R1 * kC2);
+  net->dcol2.ensure(R1 * kK2);
+  net->dz1.ensure(R1 * kC1);
+  net->cap = B;
+}
+
+// out[M x N] = act(in[M x K] W^T + b): EPI 1 stores Z and gelu, EPI 2 plain
+template <int EPI>
+void dense(mh_cnn* net, int l, const float* in, int M, float* out, float* Z) {
+  const int N = kLayers[l].out, K = kLayers[l].in;
+  hipLaunchKernelGGL((gemm<float, __hip_bfloat16, false, true, EPI>), grid_for(M, N), dim3(256), 0, net->stream,
+                     in, K, net->W(l), K, out, N, M, N, K, net->b(l), Z, static_cast<const float*>(nullptr));
+}
+
+void forward(mh_cnn* net, int B) {
+  const size_t R1 = static_cast<size_t>(B) * kP1, R2 = static_cast<size_t>(B) * kP2;
+  hipStream_t s = net->stream;
+  hipLaunchKernelGGL(im2col_onehot, dim3(blocks_for(R1 * 27)), dim3(256), 0, s, net->g.p, B, net->col1.p);
+  dense<1>(net, 0, net->col1.p, static_cast<int>(R1), net->a1.p, net->z1.p);
+  hipLaunchKernelGGL(im2col_f, dim3(blocks_for(R1 * kK2)), dim3(256), 0, s, net->a1.p, B, kG, kC1, net->col2.p);
+  dense<1>(net, 1, net->col2.p, static_cast<int>(R1), net->a2.p, net->z2.p);
+  hipLaunchKernelGGL(pool2_fwd, dim3(blocks_for(R2 * kC2)), dim3(256), 0, s, net->a2.p, B, net->p2.p);
+  hipLaunchKernelGGL(im2col_f, dim3(blocks_for(R2 * kK3)), dim3(256), 0, s, net->p2.p, B, kG2, kC2, net->col3.p);
+  dense<1>(net, 2, net->col3.p, static_cast<int>(R2), net->a3.p, net->z3.p);
+  hipLaunchKernelGGL(gap_fwd, dim3(blocks_for(static_cast<size_t>(B) * kC3)), dim3(256), 0, s, net->a3.p, B,
+                     net->feat.p);
+  dense<1>(net, 3, net->feat.p, B, net->ah1.p, net->zh1.p);
+  dense<1>(net, 4, net->ah1.p, B, net->ah2.p, net->zh2.p);
+  dense<2>(net, 5, net->ah2.p, B, net->y.p, nullptr);
+  hipcheck(hipGetLastError(), "cnn forward");
+}
+
+// gradient of layer l's weights and bias from dZ [M x out] and its input
+// X [M x in], reduced over rows in kSplit fixed-order partials
+void layer_grads(mh_cnn* net, int l, const float* dZ, const float* X, int M) {
+  const int No = kLayers[l].out, Ni = kLayers[l].in;
+  const int rows = (M + kSplit - 1) / kSplit;
+  hipStream_t s = net->stream;
+  hipLaunchKernelGGL(wgrad_split, dim3((Ni + kTile - 1) / kTile, (No + kTile - 1) / kTile, kSplit), dim3(256),
+                     0, s, dZ, No, X, Ni, M, rows, net->part.p);
+  hipLaunchKernelGGL(sum_splits, dim3(blocks_for(static_cast<size_t>(No) * Ni)), dim3(256), 0, s, net->part.p,
+                     No * Ni, kSplit, net->grads.p + net->w_off[l]);
+  hipLaunchKernelGGL(colsum_split, dim3((No + 255) / 256, kSplit), dim3(256), 0, s, dZ, M, No, rows,
+                     net->part.p);
+  hipLaunchKernelGGL(sum_splits, dim3(blocks_for(No)), dim3(256), 0, s, net->part.p, No, kSplit,
+                     net->grads.p + net->b_off[l]);
+}
+
+// dIn [M x in] = dZ [M x out] W (EPI 3: times gelu'(Zprev), EPI 0 plain)
+template <int EPI>
+void dense_back(mh_cnn* net, int l, const float* dZ, int M, float* dIn, const float* Zprev) {
+  const int No = kLayers[l].out, Ni = kLayers[l].in;
+  hipLaunchKernelGGL((gemm<float, __hip_bfloat16, false, false, EPI>), grid_for(M, Ni), dim3(256), 0, net->stream,
+                     dZ, No, net->W(l), Ni, dIn, Ni, M, Ni, No, static_cast<const float*>(nullptr),
+                     static_cast<float*>(nullptr), Zprev);
+}
+
+void backward(mh_cnn* net, int B) {
+  const size_t R1 = static_cast<size_t>(B) * kP1, R2 = static_cast<size_t>(B) * kP2;
+  hipStream_t s = net->stream;
+  layer_grads(net, 5, net->dy.p, net->ah2.p, B);
+  dense_back<3>(net, 5, net->dy.p, B, net->dzh2.p, net->zh2.p);
+  layer_grads(net, 4, net->dzh2.p, net->ah1.p, B);
+  dense_back<3>(net, 4, net->dzh2.p, B, net->dzh1.p, net->zh1.p);
+  layer_grads(net, 3, net->dzh1.p, net->feat.p, B);
+  dense_back<0>(net, 3, net->dzh1.p, B, net->dfeat.p, nullptr);
+  hipLaunchKernelGGL(gap_bwd_gelu, dim3(blocks_for(R2 * kC3)), dim3(256), 0, s, net->dfeat.p, net->z3.p, B,
+                     net->dz3.p);
+  layer_grads(net, 2, net->dz3.p, net->col3.p, static_cast<int>(R2));
+  dense_back<0>(net, 2, net->dz3.p, static_cast<int>(R2), net->dcol3.p, nullptr);
+  hipLaunchKernelGGL(col2im_f, dim3(blocks_for(R2 * kC2)), dim3(256), 0, s, net->dcol3.p,
+                     static_cast<const float*>(nullptr), B, kG2, kC2, net->dp2.p);
+  hipLaunchKernelGGL(pool2_bwd_gelu, dim3(blocks_for(R1 * kC2)), dim3(256), 0, s, net->dp2.p, net->z2.p, B,
+                     net->dz2.p);
+  layer_grads(net, 1, net->dz2.p, net->col2.p, static_cast<int>(R1));
+  dense_back<0>(net, 1, net->dz2.p, static_cast<int>(R1), net->dcol2.p, nullptr);
+  hipLaunchKernelGGL(col2im_f, dim3(blocks_for(R1 * kC1)), dim3(256), 0, s, net->dcol2.p, net->z1.p, B, kG, kC1,
+                     net->dz1.p);
+  layer_grads(net, 0, net->dz1.p, net->col1.p, static_cast<int>(R1));
+  hipcheck(hipGetLastError(), "cnn backward");
+}
+
+// loss terms and dy = (y - t) / B for the batch in net->g / net->t
+float loss_backward(mh_cnn* net, int B, bool want_loss) {
+  forward(net, B);
+  hipLaunchKernelGGL(loss_grad, dim3(blocks_for(B)), dim3(256), 0, net->stream, net->y.p, net->t.p, B, 1.0f / B,
+                     net->dy.p, net->terms.p);
+  backward(net, B);
+  if (!want_loss) return 0.f;
+  std::vector<float> terms(B);
+  hipcheck(hipMemcpyAsync(terms.data(), net->terms.p, B * sizeof(float), hipMemcpyDeviceToHost, net->stream), "D2H");
+  hipcheck(hipStreamSynchronize(net->stream), "sync");
+  double sum = 0.0;
+  for (float v : terms) sum += v;
+  return static_cast<float>(sum / B);
+}
+
+int fail(const mh::Error& e) {
+  mh::set_error(e.what());
+  return e.code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_cnn_create(int device, uint64_t seed, mh_cnn** out) {
+  try {
+    if (!out) throw mh::Error(MH_ERR_ARG, "null argument");
+    auto net = std::make_unique<mh_cnn>();
+    net->device = device;
+    hipcheck(hipSetDevice(device), "hipSetDevice");
+    hipcheck(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking), "stream");
+    size_t off = 0, max_w = 0;
+    for (int l = 0; l < kNL; l++) {
+      net->w_off[l] = off;
+      off += static_cast<size_t>(kLayers[l].out) * kLayers[l].in;
+      net->b_off[l] = off;
+      off += kLayers[l].out;
+      max_w = std::max(max_w, static_cast<size_t>(kLayers[l].out) * kLayers[l].in);
+    }
+    net->n_params = off;
+    // He-uniform weights (bound sqrt(6 / fan_in)), zero biases
+    std::mt19937_64 rng(seed);
+    std::vector<float> p(off, 0.f);
+    for (int l = 0; l < kNL; l++) {
+      std::uniform_real_distribution<float> u(-std::sqrt(6.0f / kLayers[l].in), std::sqrt(6.0f / kLayers[l].in));
+      for (size_t i = 0; i < static_cast<size_t>(kLayers[l].out) * kLayers[l].in; i++) p[net->w_off[l] + i] = u(rng);
+    }
+    net->params.ensure(off);
+    net->grads.ensure(off);
+    net->m.ensure(off);
+    net->v.ensure(off);
+    net->w16.ensure(off);
+    net->part.ensure(static_cast<size_t>(kSplit) * max_w);
+    std::vector<float> zeros(off, 0.f);
+    hipcheck(hipMemcpy(net->params.p, p.data(), off * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    hipcheck(hipMemcpy(net->m.p, zeros.data(), off * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    hipcheck(hipMemcpy(net->v.p, zeros.data(), off * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    hipLaunchKernelGGL(to_bf16, dim3(blocks_for(off)), dim3(256), 0, net->stream, net->params.p, net->w16.p,
+                       static_cast<int>(off));
+    hipcheck(hipStreamSynchronize(net->stream), "init");
+    *out = net.release();
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_cnn_destroy(mh_cnn* net) {
+  delete net;
+  return MH_OK;
+}
+
+int mh_cnn_param_count(const mh_cnn* net) { return net ? static_cast<int>(net->n_params) : MH_ERR_ARG; }
+
+int mh_cnn_get_params(const mh_cnn* net, float* out) {
+  try {
+    if (!net || !out) throw mh::Error(MH_ERR_ARG, "null argument");
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    hipcheck(hipMemcpy(out, net->params.p, net->n_params * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_cnn_set_params(mh_cnn* net, const float* in) {
+  try {
+    if (!net || !in) throw mh::Error(MH_ERR_ARG, "null argument");
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    hipcheck(hipMemcpy(net->params.p, in, net->n_params * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    std::vector<float> zeros(net->n_params, 0.f);
+    hipcheck(hipMemcpy(net->m.p, zeros.data(), net->n_params * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    hipcheck(hipMemcpy(net->v.p, zeros.data(), net->n_params * sizeof(float), hipMemcpyHostToDevice), "H2D");
+    net->adam_t = 0;
+    hipLaunchKernelGGL(to_bf16, dim3(blocks_for(net->n_params)), dim3(256), 0, net->stream, net->params.p,
+                       net->w16.p, static_cast<int>(net->n_params));
+    hipcheck(hipStreamSynchronize(net->stream), "set params");
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_cnn_forward(mh_cnn* net, const uint8_t* genomes, int n, float* y) {
+  try {
+    if (!net || n < 0 || (n > 0 && (!genomes || !y))) throw mh::Error(MH_ERR_ARG, "null argument");
+    if (n == 0) return MH_OK;
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    const int chunk = 256;
+    ensure(net, std::min(n, chunk));
+    for (int b0 = 0; b0 < n; b0 += chunk) {
+      const int B = std::min(chunk, n - b0);
+      hipcheck(hipMemcpyAsync(net->g.p, genomes + static_cast<size_t>(b0) * kP1, static_cast<size_t>(B) * kP1,
+                              hipMemcpyHostToDevice, net->stream), "H2D");
+      forward(net, B);
+      hipcheck(hipMemcpyAsync(y + b0, net->y.p, B * sizeof(float), hipMemcpyDeviceToHost, net->stream), "D2H");
+    }
+    hipcheck(hipStreamSynchronize(net->stream), "sync");
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_cnn_loss_grad(mh_cnn* net, const uint8_t* genomes, const float* target, int n, float* loss, float* grad) {
+  try {
+    if (!net || !genomes || !target || n < 1) throw mh::Error(MH_ERR_ARG, "null argument");
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    ensure(net, n);
+    hipcheck(hipMemcpyAsync(net->g.p, genomes, static_cast<size_t>(n) * kP1, hipMemcpyHostToDevice, net->stream),
+             "H2D");
+    hipcheck(hipMemcpyAsync(net->t.p, target, n * sizeof(float), hipMemcpyHostToDevice, net->stream), "H2D");
+    const float l = loss_backward(net, n, true);
+    if (loss) *loss = l;
+    if (grad) hipcheck(hipMemcpy(grad, net->grads.p, net->n_params * sizeof(float), hipMemcpyDeviceToHost), "D2H");
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_cnn_train(mh_cnn* net, const uint8_t* genomes, const float* target, int n, int batch, int n_epochs, float lr,
+                 uint64_t seed, float* final_loss) {
+  try {
+    if (!net || !genomes || !target || n < 1 || batch < 1 || n_epochs < 0) throw mh::Error(MH_ERR_ARG, "bad arguments");
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    batch = std::min(batch, n);
+    ensure(net, batch);
+    net->data_g.ensure(static_cast<size_t>(n) * kP1);
+    net->data_t.ensure(n);
+    net->idx.ensure(n);
+    hipcheck(hipMemcpyAsync(net->data_g.p, genomes, static_cast<size_t>(n) * kP1, hipMemcpyHostToDevice,
+                            net->stream), "H2D");
+    hipcheck(hipMemcpyAsync(net->data_t.p, target, n * sizeof(float), hipMemcpyHostToDevice, net->stream), "H2D");
+    std::mt19937_64 rng(seed);
+    std::vector<int> perm(n);
+    double epoch_loss = 0.0;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int ep = 0; ep < n_epochs; ep++) {
+      for (int i = 0; i < n; i++) perm[i] = i;
+      std::shuffle(perm.begin(), perm.end(), rng);
+      hipcheck(hipMemcpyAsync(net->idx.p, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, net->stream), "H2D");
+      const bool last = ep == n_epochs - 1;
+      epoch_loss = 0.0;
+      int batches = 0;
+      for (int b0 = 0; b0 + batch <= n; b0 += batch) {
+        hipLaunchKernelGGL(gather_genomes, dim3(batch), dim3(128), 0, net->stream, net->data_g.p, net->idx.p + b0,
+                           batch, net->g.p);
+        hipLaunchKernelGGL(gather_scalar, dim3(blocks_for(batch)), dim3(256), 0, net->stream, net->data_t.p,
+                           net->idx.p + b0, batch, net->t.p);
+        const float l = loss_backward(net, batch, last);
+        net->adam_t++;
+        const float bc1 = 1.f - std::pow(b1, static_cast<float>(net->adam_t));
+        const float bc2 = 1.f - std::pow(b2, static_cast<float>(net->adam_t));
+        hipLaunchKernelGGL(adam_step, dim3(blocks_for(net->n_params)), dim3(256), 0, net->stream, net->params.p,
+                           net->grads.p, net->m.p, net->v.p, net->w16.p, static_cast<int>(net->n_params), lr, b1,
+                           b2, eps, bc1, bc2);
+        hipcheck(hipGetLastError(), "train step");
+        if (last) {
+          epoch_loss += l;
+          batches++;
+        }
+      }
+      if (batches) epoch_loss /= batches;
+    }
+    hipcheck(hipStreamSynchronize(net->stream), "sync");
+    if (final_loss) *final_loss = static_cast<float>(epoch_loss);
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+}  // extern "C"

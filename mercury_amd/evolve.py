@@ -32,7 +32,7 @@ from . import _lib
 from .materials import WORKLOAD_ROWS, workload_materials
 from .robots import last_layer_air, voxel_build, voxel_build_batch
 from .simulator import SimConfig, Simulator
-from .surrogate import Surrogate, spearman
+from .surrogate import GenomeSurrogate, Surrogate, spearman
 
 LAND = dict(gravity=9.81, ground_k=1.0e5, mu_s=1.0, mu_k=0.8, drag=0.0, damping=0.99)
 
@@ -161,6 +161,31 @@ class BatchSimulator:
         return f.astype(np.float64)
 
 
+class GenomeModel:
+    """Surrogate on genomes: kind "mlp" feeds genome_features to the MLP,
+    kind "cnn" feeds the voxel grid to the convolutional network."""
+
+    def __init__(self, kind, voxels, seed=0, epochs=None):
+        if kind not in ("mlp", "cnn"):
+            raise ValueError("surrogate kind must be 'mlp' or 'cnn'")
+        if kind == "cnn" and voxels != 8:
+            raise ValueError("the convolutional surrogate takes 8^3 genomes")
+        self.kind, self.voxels = kind, voxels
+        self.epochs = epochs if epochs is not None else (30 if kind == "mlp" else 8)
+        self.model = Surrogate(seed=seed) if kind == "mlp" else GenomeSurrogate(seed=seed)
+
+    def fit(self, genomes, y, seed=0):
+        if self.kind == "mlp":
+            return self.model.fit(genome_features(genomes, self.voxels), y, epochs=self.epochs, batch=512,
+                                  lr=1e-3, seed=seed)
+        return self.model.fit(genomes, y, epochs=self.epochs, batch=256, lr=1e-3, seed=seed)
+
+    def predict(self, genomes):
+        if self.kind == "mlp":
+            return self.model.predict(genome_features(genomes, self.voxels))
+        return self.model.predict(genomes)
+
+
 @dataclass
 class Layer:
     genomes: np.ndarray
@@ -188,8 +213,8 @@ def age_limits(layers, age_gap):
 
 def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elitism=0.1,
         age_gap=5, threshold=None, max_generations=40, use_surrogate=False, screen_factor=4,
-        retrain_every=5, min_rho=0.6, train_epochs=30, duration=1.0, sim=None, verbose=False,
-        audit=0.1):
+        retrain_every=5, min_rho=0.6, train_epochs=None, duration=1.0, sim=None, verbose=False,
+        audit=0.1, surrogate_kind="mlp", max_train=200000):
     """One evolution run; stops when the best fitness reaches `threshold` or
     after max_generations.  Returns a RunResult."""
     rng = np.random.default_rng(seed)
@@ -204,7 +229,7 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
 
     g0 = random_genomes(rng, pop, voxels)
     f0 = sim.fitness(g0)
-    data_x.append(genome_features(g0, voxels))
+    data_x.append(g0)
     data_y.append(f0)
     order = np.argsort(-f0, kind="stable")
     L = [Layer(g0[order[i * per:(i + 1) * per]], f0[order[i * per:(i + 1) * per]],
@@ -224,10 +249,10 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
     for gen in range(1, max_generations + 1):
         res.generations = gen
         if use_surrogate and (sur is None or gen % retrain_every == 0):
-            sur = Surrogate(seed=seed + gen)
-            X = np.concatenate(data_x)[-200000:]
-            Y = np.concatenate(data_y)[-200000:]
-            sur.fit(X, Y, epochs=train_epochs, batch=512, lr=1e-3, seed=seed + gen)
+            sur = GenomeModel(surrogate_kind, voxels, seed=seed + gen, epochs=train_epochs)
+            X = np.concatenate(data_x)[-max_train:]
+            Y = np.concatenate(data_y)[-max_train:]
+            sur.fit(X, Y, seed=seed + gen)
             screening = True
         # offspring per layer: parents from the layer and the one below
         n_child = pop * (screen_factor if (use_surrogate and screening) else 1)
@@ -249,7 +274,7 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         kid_age = np.concatenate(kid_age)
         layer_of = np.repeat(np.arange(layers), n_child // layers)
         if use_surrogate and screening:
-            pred = sur.predict(genome_features(kids, voxels))
+            pred = sur.predict(kids)
             keep, audited = [], []
             n_audit = int(round(audit * per))
             for i in range(layers):
@@ -269,8 +294,7 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         else:
             res.fallback_generations += int(use_surrogate)
         fit = sim.fitness(kids)
-        feats = genome_features(kids, voxels)
-        data_x.append(feats)
+        data_x.append(kids)
         data_y.append(fit)
         if use_surrogate and screening:
             # with an audit sample, rank quality is judged on it alone (the
@@ -280,7 +304,7 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
             if rho < min_rho:
                 screening = False
         elif use_surrogate:
-            rho = spearman(sur.predict(feats), fit)
+            rho = spearman(sur.predict(kids), fit)
             res.rho_history.append(rho)
             screening = rho >= min_rho
         # replacement: merge, move over-age individuals up, keep elites + best
@@ -306,7 +330,7 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         if gen % age_gap == 0:  # ALPS: fresh random genomes into the youngest layer
             g = random_genomes(rng, per, voxels)
             f = sim.fitness(g)
-            data_x.append(genome_features(g, voxels))
+            data_x.append(g)
             data_y.append(f)
             L[0] = Layer(g, f, np.zeros(per, np.int64))
         done = record()
@@ -319,32 +343,35 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
 
 
 def surrogate_quality(n_train=200000, n_test=20000, voxels=8, duration=1.0, seed=0,
-                      epochs=30, batch=512, chunk=4096, sim=None, verbose=True):
-    """Simulate n_train + n_test random genomes, train the surrogate on the
-    first n_train and report Spearman rho and MAE on the held-out rest."""
+                      kinds=("mlp", "cnn"), epochs=None, chunk=4096, sim=None, verbose=True):
+    """Simulate n_train + n_test random genomes, train each surrogate kind on
+    the first n_train and report Spearman rho and MAE on the held-out rest."""
     rng = np.random.default_rng(seed)
     sim = sim or BatchSimulator(voxels=voxels, duration=duration)
     n = n_train + n_test
-    X = np.zeros((n, 64), np.float32)
+    G = np.zeros((n, voxels ** 3), np.uint8)
     Y = np.zeros(n, np.float64)
     t0 = time.perf_counter()
     for a in range(0, n, chunk):
         g = random_genomes(rng, min(chunk, n - a), voxels)
         Y[a:a + len(g)] = sim.fitness(g)
-        X[a:a + len(g)] = genome_features(g, voxels)
+        G[a:a + len(g)] = g
         if verbose:
             print(f"simulated {a + len(g)}/{n}", flush=True)
-    t_data = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    sur = Surrogate(seed=seed)
-    loss = sur.fit(X[:n_train], Y[:n_train], epochs=epochs, batch=batch, lr=1e-3, seed=seed)
-    t_train = time.perf_counter() - t0
-    pred = sur.predict(X[n_train:])
+    out = dict(n_train=n_train, n_test=n_test, fitness_mean=float(Y.mean()), fitness_std=float(Y.std()),
+               data_seconds=time.perf_counter() - t0, sim_seconds=sim.sim_seconds)
     from .surrogate import mae
-    return dict(n_train=n_train, n_test=n_test, spearman=spearman(pred, Y[n_train:]),
-                mae=mae(pred, Y[n_train:]), fitness_mean=float(Y.mean()), fitness_std=float(Y.std()),
-                final_train_loss=loss, data_seconds=t_data, train_seconds=t_train,
-                sim_seconds=sim.sim_seconds)
+    for kind in kinds:
+        t0 = time.perf_counter()
+        sur = GenomeModel(kind, voxels, seed=seed, epochs=epochs)
+        loss = sur.fit(G[:n_train], Y[:n_train], seed=seed)
+        t_train = time.perf_counter() - t0
+        pred = sur.predict(G[n_train:])
+        out[kind] = dict(spearman=spearman(pred, Y[n_train:]), mae=mae(pred, Y[n_train:]),
+                         final_train_loss=loss, train_seconds=t_train, epochs=sur.epochs)
+        if verbose:
+            print(kind, out[kind], flush=True)
+    return out
 
 
 def evals_to(history, threshold):
@@ -388,7 +415,8 @@ def main(argv=None):
     q = sub.add_parser("quality", help="surrogate Spearman/MAE on held-out simulated genomes")
     q.add_argument("--train", type=int, default=200000)
     q.add_argument("--test", type=int, default=20000)
-    q.add_argument("--epochs", type=int, default=30)
+    q.add_argument("--epochs", type=int, default=None)
+    q.add_argument("--kinds", default="mlp,cnn")
     q.add_argument("--seed", type=int, default=0)
     c = sub.add_parser("compare", help="evaluations to threshold with and without the surrogate")
     c.add_argument("--seeds", type=int, default=5)
@@ -397,7 +425,8 @@ def main(argv=None):
     c.add_argument("--pop", type=int, default=4096)
     c.add_argument("--generations", type=int, default=40)
     c.add_argument("--screen-factor", type=int, default=4)
-    c.add_argument("--epochs", type=int, default=30)
+    c.add_argument("--epochs", type=int, default=None)
+    c.add_argument("--surrogate", choices=("mlp", "cnn"), default="mlp")
     r = sub.add_parser("run", help="one evolution run")
     r.add_argument("--seed", type=int, default=0)
     r.add_argument("--pop", type=int, default=4096)
@@ -409,12 +438,12 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "quality":
         res = surrogate_quality(a.train, a.test, voxels=a.voxels, duration=a.duration,
-                                seed=a.seed, epochs=a.epochs)
+                                seed=a.seed, kinds=tuple(a.kinds.split(",")), epochs=a.epochs)
         print(json.dumps(res), flush=True)
     elif a.cmd == "compare":
         rows = compare(a.seeds, a.threshold, pop=a.pop, max_generations=a.generations,
                        voxels=a.voxels, duration=a.duration, screen_factor=a.screen_factor,
-                       train_epochs=a.epochs, seed_start=a.seed_start)
+                       train_epochs=a.epochs, seed_start=a.seed_start, surrogate_kind=a.surrogate)
         for use in (False, True):
             e = [x["evals_to_threshold"] for x in rows if x["surrogate"] == use]
             print(json.dumps(dict(surrogate=use, evals_to_threshold=e,

@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib as _libmod
-from ._lib import check, f32p, i32p, load, ptr
+from ._lib import check, f32p, i32p, load, ptr, u8p
 
 N_FEATURES = 64
 
@@ -189,3 +189,84 @@ class Surrogate:
     def predict(self, X):
         X = np.asarray(X, np.float32)
         return self.net.forward((X - self.mu) / self.sd)[:, 0] * self.ysd + self.ymu
+
+
+class CNN:
+    """mh_cnn handle: the convolutional genome network (8^3 voxel materials
+    -> 64 learned features -> MLP 64-256-256-1)."""
+
+    LAYERS = [(32, 135), (32, 864), (64, 864), (256, 64), (256, 256), (1, 256)]
+
+    def __init__(self, seed=0, device=0):
+        h = ctypes.c_void_p()
+        check(load().mh_cnn_create(device, int(seed), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        lib = getattr(_libmod, "_lib", None)
+        if getattr(self, "_h", None) is not None and lib is not None:
+            lib.mh_cnn_destroy(self._h)
+            self._h = None
+
+    def n_params(self):
+        return load().mh_cnn_param_count(self._h)
+
+    def get_params(self):
+        out = np.zeros(self.n_params(), np.float32)
+        check(load().mh_cnn_get_params(self._h, ptr(out, f32p)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(load().mh_cnn_set_params(self._h, ptr(p, f32p)))
+
+    def layers(self, p=None):
+        p = self.get_params() if p is None else p
+        out, o = [], 0
+        for n_out, n_in in self.LAYERS:
+            W = p[o:o + n_out * n_in].reshape(n_out, n_in)
+            o += n_out * n_in
+            out.append((W, p[o:o + n_out]))
+            o += n_out
+        return out
+
+    def forward(self, genomes):
+        g = np.ascontiguousarray(genomes, np.uint8).reshape(-1, 512)
+        y = np.zeros(len(g), np.float32)
+        check(load().mh_cnn_forward(self._h, ptr(g, u8p), len(g), ptr(y, f32p)))
+        return y
+
+    def loss_grad(self, genomes, t):
+        g = np.ascontiguousarray(genomes, np.uint8).reshape(-1, 512)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1)
+        loss = ctypes.c_float()
+        grad = np.zeros(self.n_params(), np.float32)
+        check(load().mh_cnn_loss_grad(self._h, ptr(g, u8p), ptr(t, f32p), len(g), ctypes.byref(loss),
+                                      ptr(grad, f32p)))
+        return float(loss.value), grad
+
+    def train(self, genomes, t, epochs=10, batch=256, lr=1e-3, seed=0):
+        g = np.ascontiguousarray(genomes, np.uint8).reshape(-1, 512)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1)
+        loss = ctypes.c_float()
+        check(load().mh_cnn_train(self._h, ptr(g, u8p), ptr(t, f32p), len(g), int(batch), int(epochs),
+                                  float(lr), int(seed), ctypes.byref(loss)))
+        return float(loss.value)
+
+
+class GenomeSurrogate:
+    """Fitness of 8^3 voxel genomes predicted by the CNN (standardized
+    targets); same fit/predict interface as Surrogate, on genomes instead
+    of feature rows."""
+
+    def __init__(self, seed=0, device=0):
+        self.net = CNN(seed=seed, device=device)
+
+    def fit(self, genomes, y, epochs=10, batch=256, lr=1e-3, seed=0):
+        y = np.asarray(y, np.float64)
+        self.ymu, self.ysd = float(y.mean()), float(y.std() + 1e-12)
+        return self.net.train(genomes, (y - self.ymu) / self.ysd, epochs=epochs, batch=batch, lr=lr,
+                              seed=seed)
+
+    def predict(self, genomes):
+        return self.net.forward(genomes).astype(np.float64) * self.ysd + self.ymu
