@@ -384,7 +384,7 @@ def evals_to(history, threshold):
 
 
 def compare(seeds=5, threshold=None, pop=4096, max_generations=40, voxels=8, duration=1.0,
-            verbose=True, seed_start=0, **kw):
+            verbose=True, seed_start=0, modes=(False, True), **kw):
     """Plain GA vs surrogate-screened GA over the given seeds, with the same
     seeds and evaluation counting.  With a threshold each run stops when its
     best fitness reaches it; without one every run goes max_generations and
@@ -392,7 +392,7 @@ def compare(seeds=5, threshold=None, pop=4096, max_generations=40, voxels=8, dur
     sim = BatchSimulator(voxels=voxels, duration=duration)
     out = []
     for s in range(seed_start, seed_start + seeds):
-        for use in (False, True):
+        for use in modes:
             r = run(seed=s, pop=pop, voxels=voxels, threshold=threshold, use_surrogate=use,
                     max_generations=max_generations, sim=sim, verbose=verbose, **kw)
             row = dict(seed=s, surrogate=use, evals_to_threshold=r.evals_to_threshold,
@@ -427,6 +427,7 @@ def main(argv=None):
     c.add_argument("--screen-factor", type=int, default=4)
     c.add_argument("--epochs", type=int, default=None)
     c.add_argument("--surrogate", choices=("mlp", "cnn"), default="mlp")
+    c.add_argument("--only", choices=("both", "plain", "surrogate"), default="both")
     r = sub.add_parser("run", help="one evolution run")
     r.add_argument("--seed", type=int, default=0)
     r.add_argument("--pop", type=int, default=4096)
@@ -443,7 +444,8 @@ def main(argv=None):
     elif a.cmd == "compare":
         rows = compare(a.seeds, a.threshold, pop=a.pop, max_generations=a.generations,
                        voxels=a.voxels, duration=a.duration, screen_factor=a.screen_factor,
-                       train_epochs=a.epochs, seed_start=a.seed_start, surrogate_kind=a.surrogate)
+                       train_epochs=a.epochs, seed_start=a.seed_start, surrogate_kind=a.surrogate,
+                       modes={"both": (False, True), "plain": (False,), "surrogate": (True,)}[a.only])
         for use in (False, True):
             e = [x["evals_to_threshold"] for x in rows if x["surrogate"] == use]
             print(json.dumps(dict(surrogate=use, evals_to_threshold=e,
