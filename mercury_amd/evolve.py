@@ -407,6 +407,23 @@ def compare(seeds=5, threshold=None, pop=4096, max_generations=40, voxels=8, dur
     return out
 
 
+def report(rows, thresholds):
+    """Mean evaluations (over the seeds that reached it) to each threshold,
+    per loop, from compare() rows; returns {label: {threshold: (mean, reached, seeds)}}."""
+    out = {}
+    for row in rows:
+        label = row.get("label") or ("surrogate" if row["surrogate"] else "plain")
+        out.setdefault(label, []).append(row)
+    table = {}
+    for label, rs in out.items():
+        table[label] = {}
+        for thr in thresholds:
+            ev = [evals_to(r["history"], thr)[0] for r in rs]
+            hit = [e for e in ev if e >= 0]
+            table[label][thr] = (float(np.mean(hit)) if hit else float("nan"), len(hit), len(rs))
+    return table
+
+
 def main(argv=None):
     import argparse
     import json
@@ -428,6 +445,10 @@ def main(argv=None):
     c.add_argument("--epochs", type=int, default=None)
     c.add_argument("--surrogate", choices=("mlp", "cnn"), default="mlp")
     c.add_argument("--only", choices=("both", "plain", "surrogate"), default="both")
+    rp = sub.add_parser("report", help="evaluations-to-threshold table from compare logs")
+    rp.add_argument("logs", nargs="+", help="[LABEL=]path of a compare log (JSON lines); rows are "
+                    "labelled LABEL:plain / LABEL:surrogate")
+    rp.add_argument("--thresholds", default="0.07,0.075,0.08,0.085,0.09,0.095,0.1")
     r = sub.add_parser("run", help="one evolution run")
     r.add_argument("--seed", type=int, default=0)
     r.add_argument("--pop", type=int, default=4096)
@@ -437,6 +458,26 @@ def main(argv=None):
         p.add_argument("--voxels", type=int, default=8)
         p.add_argument("--duration", type=float, default=1.0)
     a = ap.parse_args(argv)
+    if a.cmd == "report":
+        rows = []
+        for spec in a.logs:
+            label, path = spec.split("=", 1) if "=" in spec else (None, spec)
+            with open(path) as fh:
+                for line in fh:
+                    if line.startswith('{"seed"'):
+                        row = json.loads(line)
+                        kind = "surrogate" if row["surrogate"] else "plain"
+                        row["label"] = f"{label}:{kind}" if label else kind
+                        rows.append(row)
+        thr = [float(x) for x in a.thresholds.split(",")]
+        table = report(rows, thr)
+        labels = list(table)
+        print("| threshold | " + " | ".join(f"{l}: mean evals (seeds reached)" for l in labels) + " |")
+        print("|---" * (len(labels) + 1) + "|")
+        for t in thr:
+            cells = [f"{table[l][t][0]:.0f} ({table[l][t][1]}/{table[l][t][2]})" for l in labels]
+            print(f"| {t} | " + " | ".join(cells) + " |")
+        return
     if a.cmd == "quality":
         res = surrogate_quality(a.train, a.test, voxels=a.voxels, duration=a.duration,
                                 seed=a.seed, kinds=tuple(a.kinds.split(",")), epochs=a.epochs)

@@ -126,3 +126,13 @@ def test_short_gpu_evolution_run():
     r2 = evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=2, use_surrogate=True,
                     train_epochs=2, sim=sim)
     assert len(r2.rho_history) == 2 and np.isfinite(r2.best)
+
+
+def test_evals_to_and_report():
+    h = [(4096, 1.0, 0.05), (8192, 2.0, 0.07), (12288, 3.0, 0.09)]
+    assert evolve.evals_to(h, 0.06) == (8192, 2.0)
+    assert evolve.evals_to(h, 0.2) == (-1, -1.0)
+    rows = [dict(surrogate=False, history=h), dict(surrogate=True, history=h[1:]),
+            dict(surrogate=False, history=h[:1])]
+    t = evolve.report(rows, [0.07, 0.09])
+    assert t["plain"][0.07] == (8192.0, 1, 2) and t["surrogate"][0.09] == (12288.0, 1, 1)
