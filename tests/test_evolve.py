@@ -136,3 +136,14 @@ def test_evals_to_and_report():
             dict(surrogate=False, history=h[:1])]
     t = evolve.report(rows, [0.07, 0.09])
     assert t["plain"][0.07] == (8192.0, 1, 2) and t["surrogate"][0.09] == (12288.0, 1, 1)
+
+
+@pytest.mark.gpu
+def test_short_gpu_evolution_run_cnn_surrogate():
+    print("evolve: cnn surrogate, 2 generations, pop 256, 8^3 voxels, 0.05 s", flush=True)
+    sim = evolve.BatchSimulator(voxels=8, duration=0.05)
+    r = evolve.run(seed=1, pop=256, voxels=8, layers=4, max_generations=2, use_surrogate=True,
+                   surrogate_kind="cnn", train_epochs=1, sim=sim)
+    assert len(r.rho_history) == 2 and np.isfinite(r.best)
+    with pytest.raises(ValueError):
+        evolve.GenomeModel("cnn", 4)
