@@ -91,6 +91,21 @@ def cpu_baseline(els, solver, dt, steps, threads, env_kw):
     return springs * steps / secs, secs
 
 
+def measured_traffic(workload):
+    """HBM bytes per step-kernel launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a copy kernel of known size), if one exists."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        entry = json.load(fh).get(workload)
+    if not entry:
+        return None
+    return {"bytes_per_launch": entry["bytes_per_launch"],
+            "source": "profiles/traffic.json: " + entry["summary"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +227,7 @@ def main():
         per_launch = STREAM_BYTES_PER_SPRING * info.n_stream_springs * steps_per_launch
         per_unit = STREAM_BYTES_PER_SPRING
     achieved = per_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
+    traffic = measured_traffic(args.workload)
 
     out = {
         "metric": metric,
@@ -244,7 +260,8 @@ def main():
             "peak": peak,
             "unit": unit,
             "frac": (achieved / peak) if achieved else None,
-            "traffic": None,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            **({"traffic_source": traffic["source"]} if traffic else {}),
             "peak_kind": ("measured ds_read_b128 bandwidth (mh_calibrate_lds); nominal "
                           f"{LDS_NOMINAL_GBS:.0f}") if bound == "lds" else "MI355X HBM3E",
             "kernel_ms_per_launch": kernel_ms,
