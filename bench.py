@@ -40,6 +40,11 @@ HBM_PEAK_GBS = 8000.0
 # write 12 B and gather read 12 B) + 48 B per mass (pos/vel read + write).
 LDS_BYTES_PER_SPRING = 57.0
 LDS_BYTES_PER_MASS = 48.0
+# mass-parallel kernel: both endpoints evaluate a spring, each reading the
+# other endpoint's position (12 B), the rest length (4 B) and the material's
+# stiffness and actuation (8 B); per mass its own position read and write
+MASS_LDS_BYTES_PER_SPRING = 48.0
+MASS_LDS_BYTES_PER_MASS = 32.0
 # MI355X nominal LDS bandwidth: 256 CUs x 128 B/clk x 2.4 GHz
 LDS_NOMINAL_GBS = 256 * 128 * 2.4
 
@@ -210,7 +215,11 @@ def main():
     lds_peak = float(lds_peak.value)
     steps_per_launch = n_sub / max(launches, 1)
     hbm_extra = None
-    if info.kernel == 1 or info.stream_chunk > 0:
+    if info.kernel & mh.MH_KERNEL_MASS and not info.kernel & mh.MH_KERNEL_STREAM:
+        bound, peak, unit = "lds", lds_peak, "GB/s"
+        per_launch = (MASS_LDS_BYTES_PER_SPRING * n_springs + MASS_LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
+        per_unit = MASS_LDS_BYTES_PER_SPRING + MASS_LDS_BYTES_PER_MASS * n_masses / n_springs
+    elif info.kernel == 1 or info.stream_chunk > 0:
         # LDS-resident kernel, or chunked streaming kernel whose terms also
         # live in LDS: LDS bytes over all springs
         bound, peak, unit = "lds", lds_peak, "GB/s"
@@ -246,7 +255,7 @@ def main():
             "workload": workload,
             "robots": args.robots, "springs_total": n_springs, "active_springs": n_active,
             "masses_total": n_masses, "substeps_per_eval": n_sub, "solver": args.solver,
-            "kernel": ({1: "lds", 2: "stream", 3: "lds+stream"}.get(info.kernel, "?") +
+            "kernel": ({1: "lds", 2: "stream", 3: "lds+stream", 4: "mass", 6: "mass+stream"}.get(info.kernel, "?") +
                        (" (chunked)" if info.stream_chunk > 0 else "")), "block": info.block_size,
             "blocks": info.n_blocks, "lds_bytes_per_block": info.lds_bytes,
             "substeps_per_launch": info.substeps_per_launch, "hipgraph": True,

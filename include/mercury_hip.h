@@ -108,7 +108,12 @@ int mh_baseline(const float* pos, const uint8_t* mass_material_id,
 
 /* ------------------------------------------------------------ simulator */
 enum { MH_SOLVER_XPBD_REF = 0, MH_SOLVER_FORCE = 1 };
-enum { MH_KERNEL_AUTO = 0, MH_KERNEL_LDS = 1, MH_KERNEL_STREAM = 2 };
+/* Step kernel families, as a bit set (MH_KERNEL_AUTO = all):
+ *   LDS     robots resident in LDS, spring terms gathered per mass;
+ *   STREAM  robots over the LDS budget, spring records streamed per substep;
+ *   MASS    robots resident in LDS, one lane per mass evaluating each of
+ *           its springs (both endpoints evaluate a spring). */
+enum { MH_KERNEL_AUTO = 0, MH_KERNEL_LDS = 1, MH_KERNEL_STREAM = 2, MH_KERNEL_MASS = 4 };
 enum { MH_FITNESS_REFERENCE = 0, MH_FITNESS_DISPLACEMENT = 1 };
 
 typedef struct {
@@ -120,8 +125,8 @@ typedef struct {
   float drag;          /* force solver: linear drag coefficient N s/m */
   float damping;       /* force solver: velocity factor per step */
   int device;          /* HIP device ordinal */
-  int kernel;          /* MH_KERNEL_* */
-  int block_size;      /* 0 = autotune; else 256, 512 or 1024 */
+  int kernel;          /* bit set of MH_KERNEL_* families; 0 = all */
+  int block_size;      /* 0 = any; else 256, 512, 768 or 1024 threads */
   int robots_per_block;/* 0 = as many as fit; else an upper bound */
   int substeps_per_launch; /* 0 = autotune */
   int use_graph;       /* capture each simulate's launches in a hipGraph */
@@ -173,9 +178,9 @@ int mh_sim_reset(mh_sim* sim);               /* simulated time := 0 */
 float mh_sim_total_time(const mh_sim* sim);
 float mh_sim_get_time_step(const mh_sim* sim);
 int mh_sim_set_time_step(mh_sim* sim, float dt);
-/* Population counts and the launch plan chosen for it.  kernel is a bit
- * set of MH_KERNEL_LDS / MH_KERNEL_STREAM (robots over the LDS budget run
- * in the streaming kernel, concurrently with the LDS kernel's robots). */
+/* Population counts and the launch plan chosen for it.  kernel is the bit
+ * set of MH_KERNEL_* families in use (robots over the LDS budget run in the
+ * streaming kernel, concurrently with the LDS-resident part). */
 typedef struct {
   int n_robots, n_masses, n_springs, n_active_springs;
   int kernel, block_size, n_blocks, substeps_per_launch, lds_bytes;

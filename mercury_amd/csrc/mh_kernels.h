@@ -31,6 +31,9 @@ struct StepParams {
                              // LDS plans pad rows to 4 with the zero slot
   const uint32_t* row;       // pack-local gather offsets, n_mass + 1 per pack
   const uint16_t* mperm;     // per pack: phase-2 visiting order of its masses
+  const uint32_t* mrec;      // mass-parallel kernel: per-mass spring records
+  const uint8_t* mcnt;       // mass-parallel kernel: record counts per mass (2 each)
+  const float* lbtab;        // mass-parallel kernel: per-pack rest-length tables
   const uint16_t* ccsr;      // chunked kernel: per chunk, entries 3 * (spring - chunk start)
   const uint16_t* crow;      // chunked kernel: per chunk, 2 n_mass + 1 row bounds
   float* scratch;            // streaming kernel: float4 per active spring
@@ -54,10 +57,11 @@ struct StepParams {
 // A compiled step-kernel variant.
 struct KernelVariant {
   int solver;     // MH_SOLVER_*
-  int kind;       // MH_KERNEL_LDS / MH_KERNEL_STREAM
+  int kind;       // MH_KERNEL_LDS / MH_KERNEL_STREAM / MH_KERNEL_MASS
   int chunked;    // STREAM kind: 1 = spring terms in LDS chunks, 0 = in global memory
   int block;      // threads per workgroup
-  int nspt;       // springs per thread held in registers (LDS kind)
+  int nspt;       // springs per thread held in registers (LDS kind); records
+                  // per endpoint side (MASS kind)
   int mpt;        // masses per thread
   const void* fn; // kernel symbol
 };
@@ -72,6 +76,8 @@ const KernelVariant* kernel_variants(int* count);
 // masses across the LDS banks.
 __host__ __device__ inline int pos_slot(int i) { return i + (i >> 3); }
 __host__ __device__ inline int pos_slots(int n) { return n + (n >> 3) + 1; }
+// Mass-parallel kernel: distinct rest lengths per pack (7-bit index)
+constexpr int kMassLbarMax = 128;
 // Bytes of the LDS spring-term array (3 floats per slot, spring_cap + 2
 // slots), rounded up to 16 B so the float4 positions after it stay aligned.
 __host__ __device__ inline size_t lds_term_bytes(int spring_cap) {
@@ -79,6 +85,7 @@ __host__ __device__ inline size_t lds_term_bytes(int spring_cap) {
 }
 size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat);
 size_t lds_bytes_stream(int mass_cap, int n_mat);
+size_t lds_bytes_mass(int mass_cap, int n_mat);
 
 hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes,
                        hipStream_t stream, const StepParams& p);

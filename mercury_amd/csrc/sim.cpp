@@ -92,9 +92,13 @@ struct mh_sim {
   int n_robots = 0;
   std::vector<uint32_t> mass_off, spring_off, active_off;  // per robot prefix offsets
   std::vector<uint32_t> gather_pad;  // per robot: sum over masses of deg rounded up to 4
+  std::vector<uint8_t> max_side;     // per robot: most springs one mass has on one side (capped 255)
+  std::vector<std::vector<uint32_t>> lbar_set;  // per robot: distinct rest lengths (bits, sorted;
+                                                // emptied past the mass kernel's table size)
   std::vector<uint16_t> act_pairs;   // active springs, robot-local
   std::vector<uint32_t> active_src;  // active spring -> original spring index
   std::vector<uint8_t> act_mat;      // compact material index
+  std::vector<float> act_lbar;       // mean length per active spring
   std::vector<uint8_t> compact_to_id;
   std::vector<float> base_com, base_len;
   size_t n_masses = 0, n_springs = 0, n_active = 0;
@@ -105,6 +109,9 @@ struct mh_sim {
   mh::DevBuf<float> lbar, stress, scratch, com;
   mh::DevBuf<uint8_t> smat, mass_id;
   mh::DevBuf<uint16_t> mperm, ccsr, crow;
+  mh::DevBuf<uint32_t> mrec;
+  mh::DevBuf<uint8_t> mcnt;
+  mh::DevBuf<float> lbtab;
   mh::DevBuf<uint32_t> spair, csr, row, robot_begin;
   mh::DevBuf<mh::PackDesc> packs;
   mh::DevBuf<float> mat_k, mat_kx, mat_dL0, mat_omega, mat_phi;
@@ -177,6 +184,19 @@ bool fits_lds_alone(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
          lds_bytes_lds(std::max(m, 1), std::max(sp, 1), g, n_mat) <= static_cast<size_t>(s->lds_limit);
 }
 
+bool fits_mass(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
+  const int m = robot_masses(s, r);
+  // a robot with springs needs its rest lengths in the table (an empty set
+  // marks one with too many distinct values)
+  const bool table_ok = robot_active(s, r) == 0 || !s->lbar_set[r].empty();
+  return m <= v.block && s->max_side[r] <= v.nspt && table_ok && n_mat <= 256 &&
+         lds_bytes_mass(std::max(m, 1), n_mat) <= static_cast<size_t>(s->lds_limit);
+}
+
+bool fits_resident(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
+  return v.kind == MH_KERNEL_MASS ? fits_mass(s, v, r, n_mat) : fits_lds_alone(s, v, r, n_mat);
+}
+
 // Springs per chunk for the chunked kernel and a robot of m masses: the
 // largest candidate whose LDS layout (chunk terms, skewed positions, one
 // chunk's gather entries -- at most 2C plus padding of 3 per row part --
@@ -204,8 +224,9 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
   lp.v = lv;
   stp.v = sv;
   int r = 0;
+  std::vector<uint32_t> lbar_union, merged;
   while (r < s->n_robots) {
-    if (!lv || !fits_lds_alone(s, *lv, r, n_mat)) {
+    if (!lv || !fits_resident(s, *lv, r, n_mat)) {
       const int m = robot_masses(s, r);
       if (!sv) return false;
       if (sv->chunked ? chunk_springs(s, *sv, m, n_mat) == 0
@@ -221,6 +242,32 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
       stp.packs.push_back(d);
       stp.n_robots++;
       r++;
+      continue;
+    }
+    if (lv->kind == MH_KERNEL_MASS) {
+      // consecutive robots while their masses fill one workgroup and their
+      // rest lengths share one table
+      PackDesc d{};
+      d.mass_begin = s->mass_off[r];
+      d.spring_begin = s->active_off[r];
+      lbar_union.clear();
+      int count = 0;
+      while (r < s->n_robots && count < cap && fits_mass(s, *lv, r, n_mat)) {
+        const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
+        merged.clear();
+        std::set_union(lbar_union.begin(), lbar_union.end(), s->lbar_set[r].begin(), s->lbar_set[r].end(),
+                       std::back_inserter(merged));
+        if (m > lv->block || static_cast<int>(merged.size()) > kMassLbarMax) break;
+        lbar_union.swap(merged);
+        d.n_mass = static_cast<uint32_t>(m);
+        d.n_spring += static_cast<uint32_t>(robot_active(s, r));
+        r++;
+        count++;
+      }
+      lp.mass_cap = std::max(lp.mass_cap, static_cast<int>(d.n_mass));
+      lp.spring_cap = std::max(lp.spring_cap, static_cast<int>(d.n_spring));
+      lp.packs.push_back(d);
+      lp.n_robots += count;
       continue;
     }
     PackDesc d{};
@@ -250,7 +297,9 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
     lp.n_robots += count;
   }
   if (!lp.packs.empty()) {
-    lp.lds = lds_bytes_lds(std::max(lp.mass_cap, 1), std::max(lp.spring_cap, 1), lp.csr_cap, n_mat);
+    lp.lds = lv->kind == MH_KERNEL_MASS
+                 ? lds_bytes_mass(std::max(lp.mass_cap, 1), n_mat)
+                 : lds_bytes_lds(std::max(lp.mass_cap, 1), std::max(lp.spring_cap, 1), lp.csr_cap, n_mat);
     plan.parts.push_back(std::move(lp));
   }
   if (!stp.packs.empty()) {
@@ -324,6 +373,66 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
   });
 }
 
+// Mass-parallel part: per pack a rest-length table and per mass its
+// first-endpoint and second-endpoint records in ascending spring index,
+// laid out [part][k][mass] (part 2: pack-local spring index of the part-0
+// records), with the two counts per mass.  A record is the other
+// endpoint's skewed LDS slot | material << 16 | rest-length index << 24.
+void upload_mass(mh_sim* s, Part& part, std::vector<uint32_t>& mrec, std::vector<uint8_t>& mcnt,
+                 std::vector<float>& lbtab) {
+  const int P = part.v->nspt;
+  std::vector<uint32_t> table, merged, fill_p, fill_m;
+  for (PackDesc& d : part.packs) {
+    const uint32_t nm = d.n_mass, ns = d.n_spring;
+    const int r0 = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
+                                    s->mass_off.begin()) - 1;
+    // the pack's rest-length table: union of its robots' sets, sorted
+    table.clear();
+    for (int q = r0; q < s->n_robots && s->mass_off[q] < d.mass_begin + nm; q++) {
+      merged.clear();
+      std::set_union(table.begin(), table.end(), s->lbar_set[q].begin(), s->lbar_set[q].end(),
+                     std::back_inserter(merged));
+      table.swap(merged);
+    }
+    d.pad0 = static_cast<uint32_t>(lbtab.size());
+    d.pad1 = static_cast<uint32_t>(table.size());
+    for (uint32_t bits : table) {
+      float f;
+      std::memcpy(&f, &bits, 4);
+      lbtab.push_back(f);
+    }
+    d.csr_begin = static_cast<uint32_t>(mrec.size());
+    d.row_begin = static_cast<uint32_t>(mcnt.size());
+    mrec.resize(mrec.size() + static_cast<size_t>(3 * P) * nm, 0u);
+    uint32_t* out = mrec.data() + d.csr_begin;
+    fill_p.assign(nm, 0);
+    fill_m.assign(nm, 0);
+    int q = r0;
+    for (uint32_t k = 0; k < ns; k++) {  // ascending spring index
+      const uint32_t g = d.spring_begin + k;
+      while (g >= s->active_off[q + 1]) q++;
+      const uint32_t off = s->mass_off[q] - d.mass_begin;
+      const uint32_t a = s->act_pairs[2 * g] + off, b = s->act_pairs[2 * g + 1] + off;
+      uint32_t bits;
+      std::memcpy(&bits, &s->act_lbar[g], 4);
+      const uint32_t li = static_cast<uint32_t>(std::lower_bound(table.begin(), table.end(), bits) - table.begin());
+      const uint32_t tail = (static_cast<uint32_t>(s->act_mat[g]) << 16) | (li << 24);
+      if (fill_p[a] >= static_cast<uint32_t>(P) || fill_m[b] >= static_cast<uint32_t>(P))
+        throw Error(MH_ERR_INTERNAL, "mass record lists over the kernel's capacity");
+      out[static_cast<size_t>(2 * P + fill_p[a]) * nm + a] = k;
+      out[static_cast<size_t>(fill_p[a]++) * nm + a] = static_cast<uint32_t>(pos_slot(static_cast<int>(b))) | tail;
+      out[static_cast<size_t>(P + fill_m[b]++) * nm + b] = static_cast<uint32_t>(pos_slot(static_cast<int>(a))) | tail;
+    }
+    for (uint32_t i = 0; i < nm; i++) {
+      mcnt.push_back(static_cast<uint8_t>(fill_p[i]));
+      mcnt.push_back(static_cast<uint8_t>(fill_m[i]));
+    }
+  }
+  std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
+    return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
+  });
+}
+
 // Device arrays that depend on the packing: pack-local spring pairs, the
 // gather lists, row bounds and phase-2 order.  Mass i's row is two parts:
 // the springs whose first endpoint it is, then those whose second endpoint
@@ -340,10 +449,15 @@ void upload_plan(mh_sim* s, Plan& plan) {
   row.reserve(2 * s->n_masses + 64);
   bool need_scratch = false;
   std::vector<uint16_t> ccsr, crow;
+  std::vector<uint32_t> mrec;
+  std::vector<uint8_t> mcnt;
+  std::vector<float> lbtab;
   for (Part& part : plan.parts) {
     const bool padded = part.v->kind == MH_KERNEL_LDS;
     need_scratch |= part.v->kind == MH_KERNEL_STREAM && !part.v->chunked;
-    if (part.v->chunked) {
+    if (part.v->kind == MH_KERNEL_MASS) {
+      upload_mass(s, part, mrec, mcnt, lbtab);
+    } else if (part.v->chunked) {
       upload_chunked(s, part, spair, ccsr, crow);
     } else {
       const uint32_t zero_entry = 3u * static_cast<uint32_t>(part.spring_cap + 1);
@@ -417,19 +531,26 @@ void upload_plan(mh_sim* s, Plan& plan) {
   s->mperm.upload(perm.data(), perm.size(), s->stream);
   s->ccsr.upload(ccsr.data(), ccsr.size(), s->stream);
   s->crow.upload(crow.data(), crow.size(), s->stream);
+  s->mrec.upload(mrec.data(), mrec.size(), s->stream);
+  s->mcnt.upload(mcnt.data(), mcnt.size(), s->stream);
+  s->lbtab.upload(lbtab.data(), lbtab.size(), s->stream);
   if (need_scratch) s->scratch.alloc(4 * std::max<size_t>(na, 1));
   hip_check(hipStreamSynchronize(s->stream), "plan upload");
   s->plan = std::move(plan);
   drop_graph(s);
 }
 
-std::vector<const KernelVariant*> candidates(const mh_sim* s, int kind) {
+// Compiled variants of the given families (bit set) allowed by the config,
+// in table order.
+std::vector<const KernelVariant*> candidates(const mh_sim* s, int kinds) {
   int nv = 0;
   const KernelVariant* all = kernel_variants(&nv);
+  const int allowed = s->cfg.kernel == MH_KERNEL_AUTO ? (MH_KERNEL_LDS | MH_KERNEL_STREAM | MH_KERNEL_MASS)
+                                                      : s->cfg.kernel;
   std::vector<const KernelVariant*> out;
   for (int i = 0; i < nv; i++) {
     const KernelVariant& v = all[i];
-    if (v.solver != s->cfg.solver || v.kind != kind) continue;
+    if (v.solver != s->cfg.solver || !(v.kind & kinds & allowed)) continue;
     if (s->cfg.block_size && v.block != s->cfg.block_size) continue;
     out.push_back(&v);
   }
@@ -440,17 +561,15 @@ std::vector<const KernelVariant*> candidates(const mh_sim* s, int kind) {
 // takes the most robots, and for the rest the first streaming variant
 // (chunked ones first) that fits them.
 void choose_default_plan(mh_sim* s) {
-  const bool use_lds = s->cfg.kernel != MH_KERNEL_STREAM;
-  const bool use_stream = s->cfg.kernel != MH_KERNEL_LDS;
-  auto lc = use_lds ? candidates(s, MH_KERNEL_LDS) : std::vector<const KernelVariant*>{};
-  auto sc = use_stream ? candidates(s, MH_KERNEL_STREAM) : std::vector<const KernelVariant*>{};
+  auto lc = candidates(s, MH_KERNEL_LDS | MH_KERNEL_MASS);
+  auto sc = candidates(s, MH_KERNEL_STREAM);
   const KernelVariant* sv = sc.empty() ? nullptr : sc.front();
   const int n_mat = static_cast<int>(s->compact_to_id.size());
   const KernelVariant* best = nullptr;
   int best_fit = -1;
   for (const KernelVariant* v : lc) {
     int fit = 0;
-    for (int r = 0; r < s->n_robots; r++) fit += fits_lds_alone(s, *v, r, n_mat) ? 1 : 0;
+    for (int r = 0; r < s->n_robots; r++) fit += fits_resident(s, *v, r, n_mat) ? 1 : 0;
     if (fit > best_fit) {
       best_fit = fit;
       best = v;
@@ -477,6 +596,9 @@ StepParams make_params(mh_sim* s, const Part& part, float t0, int n_steps, int t
   p.mperm = s->mperm.p;
   p.ccsr = s->ccsr.p;
   p.crow = s->crow.p;
+  p.mrec = s->mrec.p;
+  p.mcnt = s->mcnt.p;
+  p.lbtab = s->lbtab.p;
   p.scratch = s->scratch.p;
   p.mat_k = s->mat_k.p;
   p.mat_kx = s->mat_kx.p;
@@ -638,9 +760,10 @@ int mh_sim_create(const mh_config* cfg, const mh_material_set* set, mh_sim** out
   if (cfg->solver != MH_SOLVER_XPBD_REF && cfg->solver != MH_SOLVER_FORCE)
     throw mh::Error(MH_ERR_ARG, "unknown solver");
   if (!(cfg->dt > 0.f)) throw mh::Error(MH_ERR_ARG, "dt must be > 0");
-  if (cfg->block_size && cfg->block_size != 256 && cfg->block_size != 512 && cfg->block_size != 1024)
-    throw mh::Error(MH_ERR_ARG, "block_size must be 0, 256, 512 or 1024");
-  if (cfg->kernel < MH_KERNEL_AUTO || cfg->kernel > MH_KERNEL_STREAM)
+  if (cfg->block_size && cfg->block_size != 256 && cfg->block_size != 512 && cfg->block_size != 768 &&
+      cfg->block_size != 1024)
+    throw mh::Error(MH_ERR_ARG, "block_size must be 0, 256, 512, 768 or 1024");
+  if (cfg->kernel < MH_KERNEL_AUTO || cfg->kernel > (MH_KERNEL_LDS | MH_KERNEL_STREAM | MH_KERNEL_MASS))
     throw mh::Error(MH_ERR_ARG, "unknown kernel kind");
   int ndev = 0;
   mh::hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
@@ -728,20 +851,36 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   }
   if (compact_to_id.empty()) compact_to_id.push_back(0);  // keeps tables non-empty
 
-  // per robot: gather entries with both parts of every row rounded up to 4
+  // per robot: gather entries with both parts of every row rounded up to 4;
+  // the most springs one mass has on one side; distinct rest lengths (the
+  // set is left empty when it exceeds the mass-parallel kernel's table)
   std::vector<uint32_t> gpad(R, 0);
+  std::vector<uint8_t> mside(R, 0);
+  std::vector<std::vector<uint32_t>> lsets(R);
   {
     std::vector<uint32_t> plus, minus;
     for (int r = 0; r < R; r++) {
       plus.assign(mo[r + 1] - mo[r], 0);
       minus.assign(mo[r + 1] - mo[r], 0);
+      std::vector<uint32_t>& ls = lsets[r];
       for (uint32_t k = ao[r]; k < ao[r + 1]; k++) {
         plus[ap[2 * k]]++;
         minus[ap[2 * k + 1]]++;
+        uint32_t bits;
+        std::memcpy(&bits, &alb[k], 4);
+        if (ls.size() <= static_cast<size_t>(mh::kMassLbarMax)) {
+          auto it = std::lower_bound(ls.begin(), ls.end(), bits);
+          if (it == ls.end() || *it != bits) ls.insert(it, bits);
+        }
       }
-      uint32_t g = 0;
-      for (size_t i = 0; i < plus.size(); i++) g += ((plus[i] + 3u) & ~3u) + ((minus[i] + 3u) & ~3u);
+      if (ls.size() > static_cast<size_t>(mh::kMassLbarMax)) ls.clear();
+      uint32_t g = 0, side = 0;
+      for (size_t i = 0; i < plus.size(); i++) {
+        g += ((plus[i] + 3u) & ~3u) + ((minus[i] + 3u) & ~3u);
+        side = std::max(side, std::max(plus[i], minus[i]));
+      }
       gpad[r] = g;
+      mside[r] = static_cast<uint8_t>(std::min<uint32_t>(side, 255));
     }
   }
 
@@ -767,7 +906,10 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   s->spring_off = so;
   s->active_off = ao;
   s->gather_pad = std::move(gpad);
+  s->max_side = std::move(mside);
+  s->lbar_set = std::move(lsets);
   s->act_pairs = std::move(ap);
+  s->act_lbar = alb;
   s->active_src = std::move(src);
   s->act_mat = std::move(amat);
   s->compact_to_id = std::move(compact_to_id);
@@ -929,7 +1071,7 @@ int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
   for (const mh::Part& part : s->plan.parts) {
     info->kernel |= part.v->kind;
     info->n_blocks += static_cast<int>(part.packs.size());
-    if (part.v->kind == MH_KERNEL_LDS) {
+    if (part.v->kind != MH_KERNEL_STREAM) {
       info->block_size = part.v->block;
       info->lds_bytes = static_cast<int>(part.lds);
       info->max_masses_per_block = part.mass_cap;
@@ -1037,10 +1179,8 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   s->cfg.use_graph = 0;
   const int saved_sub = s->substeps;
   s->substeps = 0;
-  const bool use_lds = s->cfg.kernel != MH_KERNEL_STREAM;
-  const bool use_stream = s->cfg.kernel != MH_KERNEL_LDS;
-  auto lc = use_lds ? mh::candidates(s, MH_KERNEL_LDS) : std::vector<const mh::KernelVariant*>{};
-  auto sc = use_stream ? mh::candidates(s, MH_KERNEL_STREAM) : std::vector<const mh::KernelVariant*>{};
+  auto lc = mh::candidates(s, MH_KERNEL_LDS | MH_KERNEL_MASS);
+  auto sc = mh::candidates(s, MH_KERNEL_STREAM);
   lc.push_back(nullptr);  // everything streamed
   // streaming variant for the LDS search: the first that can take every robot
   const mh::KernelVariant* sv0 = sc.empty() ? nullptr : sc.front();

@@ -526,6 +526,142 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
   }
 }
 
+// ------------------------------------------------------- mass-parallel kernel
+// One lane per mass; every spring is evaluated by both of its endpoints
+// (the same operands in the same order, so both get the same bits), which
+// removes the spring-term buffer and gather lists from LDS and leaves one
+// barrier per substep.  LDS holds two position buffers (read one, write the
+// other; skewed slots as in step_lds, plus a far-away ghost slot in the
+// last slot that padding records point at), the pack's rest-length table
+// (at most 128 values), the per-material stiffness and two actuation
+// tables.  Each lane keeps its records in registers: MAXP first-endpoint
+// records then MAXP second-endpoint records, each in ascending spring
+// index; a record is other-slot | material << 16 | rest-length index << 24.
+// Records live in p.mrec as [part][k][mass] (part 0: first endpoint, part
+// 1: second endpoint, part 2: the pack-local spring index of part-0
+// records for the stress output); p.mcnt holds the two counts per mass.
+template <int SOLVER, int BLOCK, int MAXP>
+__global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PackDesc d = p.packs[blockIdx.x];
+  const int nm = static_cast<int>(d.n_mass);
+  const int slots = pos_slots(p.mass_cap);
+  const int ghost = slots - 1;
+  float4* s_pos0 = reinterpret_cast<float4*>(smem);
+  float4* s_pos1 = s_pos0 + slots;
+  float* s_lb = reinterpret_cast<float*>(s_pos1 + slots);
+  float* s_k = s_lb + kMassLbarMax;
+  float* s_R = s_k + p.n_mat;
+  const int tid = threadIdx.x;
+  const bool live = tid < nm;
+  const uint32_t* rec = p.mrec + d.csr_begin;
+
+  MassState ms;
+  uint32_t rp[MAXP], rm[MAXP];
+  float st[MAXP];
+  int np = 0, nq = 0;
+  const int pslot = live ? pos_slot(tid) : ghost;
+  const uint32_t pad = static_cast<uint32_t>(ghost);
+#pragma unroll
+  for (int k = 0; k < MAXP; ++k) {
+    rp[k] = pad;
+    rm[k] = pad;
+    st[k] = 0.f;
+  }
+  if (live) {
+    load_mass(ms, p.pos, p.vel, d.mass_begin + tid, p);
+    np = p.mcnt[d.row_begin + 2 * tid];
+    nq = p.mcnt[d.row_begin + 2 * tid + 1];
+#pragma unroll
+    for (int k = 0; k < MAXP; ++k) {
+      if (k < np) rp[k] = rec[static_cast<size_t>(k) * nm + tid];
+      if (k < nq) rm[k] = rec[static_cast<size_t>(MAXP + k) * nm + tid];
+      if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress && k < np)
+        st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]];
+    }
+    s_pos0[pslot] = lds_position<SOLVER>(ms);
+  }
+  if (tid == 0) {
+    s_pos0[ghost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
+    s_pos1[ghost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
+  }
+  for (int i = tid; i < static_cast<int>(d.pad1); i += BLOCK) s_lb[i] = p.lbtab[d.pad0 + i];
+  for (int i = tid; i < p.n_mat; i += BLOCK) s_k[i] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[i] : p.mat_k[i];
+  float t = p.t0;
+  if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], t);
+  // wave-wide loop bounds
+  int np_w = np, nq_w = nq;
+  for (int off = 32; off > 0; off >>= 1) {
+    np_w = max(np_w, __shfl_xor(np_w, off));
+    nq_w = max(nq_w, __shfl_xor(nq_w, off));
+  }
+  __syncthreads();
+
+  int cur = 0;
+  for (int step = 0; step < p.n_steps; ++step) {
+    const float4* P0 = cur ? s_pos1 : s_pos0;
+    float4* P1 = cur ? s_pos0 : s_pos1;
+    const float* R = s_R + cur * p.n_mat;
+    // idle lanes sit at the origin, 1e3*sqrt(3) from the ghost, so their
+    // padding terms stay in the fast arithmetic range
+    const float4 own = live ? P0[pslot] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+#pragma unroll
+    for (int k = 0; k < MAXP; ++k) {
+      if (k < np_w) {
+        // opaque to the optimizer: decoding the record per substep costs a
+        // few VALU, hoisting every decoded field out of the step loop would
+        // cost registers
+        uint32_t r = rp[k];
+        asm volatile("" : "+v"(r));
+        const int m = (r >> 16) & 0xff;
+        const float lb = s_lb[r >> 24];
+        float cx, cy, cz, lambda;
+        spring_term<SOLVER>(own, P0[r & 0xffffu], lb, R[m], s_k[m], cx, cy, cz, lambda);
+        if (k < np) {
+          ax = ax + cx; ay = ay + cy; az = az + cz;
+          if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) st[k] = st[k] + lambda / lb;
+        }
+      }
+      // one record at a time: keeps the unrolled loop's registers bounded
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int k = 0; k < MAXP; ++k) {
+      if (k < nq_w) {
+        uint32_t r = rm[k];
+        asm volatile("" : "+v"(r));
+        const int m = (r >> 16) & 0xff;
+        const float lb = s_lb[r >> 24];
+        float cx, cy, cz, lambda;
+        spring_term<SOLVER>(P0[r & 0xffffu], own, lb, R[m], s_k[m], cx, cy, cz, lambda);
+        if (k < nq) {
+          bx = bx + cx; by = by + cy; bz = bz + cz;
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    const float tn = t + p.dt;
+    if (live) P1[pslot] = integrate<SOLVER>(ms, ax - bx, ay - by, az - bz, p);
+    if (tid < p.n_mat)
+      s_R[(cur ^ 1) * p.n_mat + tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
+    t = tn;
+    cur ^= 1;
+    __syncthreads();
+  }
+
+  if (live) {
+    const uint32_t gi = d.mass_begin + tid;
+    p.pos[gi] = make_float4(ms.x, ms.y, ms.z, ms.m);
+    p.vel[gi] = make_float4(ms.vx, ms.vy, ms.vz, 0.f);
+    if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
+#pragma unroll
+      for (int k = 0; k < MAXP; ++k)
+        if (k < np) p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]] = st[k];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ COM
 __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint32_t* begin,
                           int n_robots, float* out) {
@@ -548,12 +684,15 @@ __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint3
 
 #define MH_LDS(SOLVER, B, S, M) \
   {SOLVER, MH_KERNEL_LDS, 0, B, S, M, reinterpret_cast<const void*>(&step_lds<SOLVER, B, S, M>)}
+#define MH_MASS(SOLVER, B, P) \
+  {SOLVER, MH_KERNEL_MASS, 0, B, P, 1, reinterpret_cast<const void*>(&step_mass<SOLVER, B, P>)}
 #define MH_CHUNK(SOLVER, B, M) \
   {SOLVER, MH_KERNEL_STREAM, 1, B, 0, M, reinterpret_cast<const void*>(&step_chunk<SOLVER, B, M>)}
 #define MH_STREAM(SOLVER, B) \
   {SOLVER, MH_KERNEL_STREAM, 0, B, 0, 0, reinterpret_cast<const void*>(&step_stream<SOLVER, B>)}
 #define MH_LDS_SET(SOLVER)                                                         \
-  MH_LDS(SOLVER, 1024, 8, 1), MH_LDS(SOLVER, 1024, 8, 2), MH_LDS(SOLVER, 1024, 4, 1), \
+  MH_MASS(SOLVER, 768, 13), MH_MASS(SOLVER, 1024, 13), MH_MASS(SOLVER, 512, 16), \
+      MH_MASS(SOLVER, 256, 16), MH_LDS(SOLVER, 1024, 8, 1), MH_LDS(SOLVER, 1024, 8, 2), MH_LDS(SOLVER, 1024, 4, 1), \
       MH_LDS(SOLVER, 512, 8, 2), MH_LDS(SOLVER, 512, 16, 2), \
       MH_LDS(SOLVER, 512, 16, 4), MH_LDS(SOLVER, 256, 16, 4), MH_LDS(SOLVER, 256, 8, 2), \
       MH_CHUNK(SOLVER, 1024, 2), MH_CHUNK(SOLVER, 1024, 4), MH_CHUNK(SOLVER, 512, 4), \
@@ -575,6 +714,12 @@ size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat) {
   b += 2u * static_cast<size_t>(2 * mass_cap + 2);      // row bounds
   b += 4u * static_cast<size_t>(n_mat);                 // actuation table
   return b;
+}
+
+size_t lds_bytes_mass(int mass_cap, int n_mat) {
+  // two position buffers (skewed slots + ghost), rest lengths, stiffness,
+  // two actuation tables
+  return 2u * 16u * static_cast<size_t>(pos_slots(mass_cap)) + 4u * kMassLbarMax + 12u * static_cast<size_t>(n_mat);
 }
 
 size_t lds_bytes_stream(int mass_cap, int n_mat) {
