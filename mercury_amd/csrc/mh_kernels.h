@@ -76,8 +76,10 @@ const KernelVariant* kernel_variants(int* count);
 // masses across the LDS banks.
 __host__ __device__ inline int pos_slot(int i) { return i + (i >> 3); }
 __host__ __device__ inline int pos_slots(int n) { return n + (n >> 3) + 1; }
-// Mass-parallel kernel: distinct rest lengths per pack (7-bit index)
-constexpr int kMassLbarMax = 128;
+// Mass-parallel kernel: distinct rest lengths per pack and materials (its
+// LDS tables hold 64 of each, the last one reserved for padding records)
+constexpr int kMassLbarMax = 63;
+constexpr int kMassMatMax = 63;
 // Bytes of the LDS spring-term array (3 floats per slot, spring_cap + 2
 // slots), rounded up to 16 B so the float4 positions after it stay aligned.
 __host__ __device__ inline size_t lds_term_bytes(int spring_cap) {
@@ -85,7 +87,7 @@ __host__ __device__ inline size_t lds_term_bytes(int spring_cap) {
 }
 size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat);
 size_t lds_bytes_stream(int mass_cap, int n_mat);
-size_t lds_bytes_mass(int mass_cap, int n_mat);
+size_t lds_bytes_mass(int block, int n_mat);
 
 hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes,
                        hipStream_t stream, const StepParams& p);

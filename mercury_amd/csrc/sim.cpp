@@ -93,6 +93,8 @@ struct mh_sim {
   std::vector<uint32_t> mass_off, spring_off, active_off;  // per robot prefix offsets
   std::vector<uint32_t> gather_pad;  // per robot: sum over masses of deg rounded up to 4
   std::vector<uint8_t> max_side;     // per robot: most springs one mass has on one side (capped 255)
+  std::vector<std::vector<uint32_t>> mat_set;   // per robot: compact materials of its springs (sorted;
+                                                // empty when over the mass-parallel kernel's table)
   std::vector<std::vector<uint32_t>> lbar_set;  // per robot: distinct rest lengths (bits, sorted;
                                                 // emptied past the mass kernel's table size)
   std::vector<uint16_t> act_pairs;   // active springs, robot-local
@@ -186,11 +188,11 @@ bool fits_lds_alone(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
 
 bool fits_mass(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
   const int m = robot_masses(s, r);
-  // a robot with springs needs its rest lengths in the table (an empty set
-  // marks one with too many distinct values)
-  const bool table_ok = robot_active(s, r) == 0 || !s->lbar_set[r].empty();
-  return m <= v.block && s->max_side[r] <= v.nspt && table_ok && n_mat <= 256 &&
-         lds_bytes_mass(std::max(m, 1), n_mat) <= static_cast<size_t>(s->lds_limit);
+  // a robot with springs needs its rest lengths and materials in the
+  // tables (an empty set marks one with too many distinct values)
+  const bool table_ok = robot_active(s, r) == 0 || (!s->lbar_set[r].empty() && !s->mat_set[r].empty());
+  return m <= v.block && s->max_side[r] <= v.nspt && table_ok &&
+         lds_bytes_mass(v.block, n_mat) <= static_cast<size_t>(s->lds_limit);
 }
 
 bool fits_resident(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
@@ -224,7 +226,7 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
   lp.v = lv;
   stp.v = sv;
   int r = 0;
-  std::vector<uint32_t> lbar_union, merged;
+  std::vector<uint32_t> lbar_union, mat_union, merged, merged_m;
   while (r < s->n_robots) {
     if (!lv || !fits_resident(s, *lv, r, n_mat)) {
       const int m = robot_masses(s, r);
@@ -246,19 +248,26 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
     }
     if (lv->kind == MH_KERNEL_MASS) {
       // consecutive robots while their masses fill one workgroup and their
-      // rest lengths share one table
+      // rest lengths and materials share one table each
       PackDesc d{};
       d.mass_begin = s->mass_off[r];
       d.spring_begin = s->active_off[r];
       lbar_union.clear();
+      mat_union.clear();
       int count = 0;
       while (r < s->n_robots && count < cap && fits_mass(s, *lv, r, n_mat)) {
         const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
         merged.clear();
         std::set_union(lbar_union.begin(), lbar_union.end(), s->lbar_set[r].begin(), s->lbar_set[r].end(),
                        std::back_inserter(merged));
-        if (m > lv->block || static_cast<int>(merged.size()) > kMassLbarMax) break;
+        merged_m.clear();
+        std::set_union(mat_union.begin(), mat_union.end(), s->mat_set[r].begin(), s->mat_set[r].end(),
+                       std::back_inserter(merged_m));
+        if (m > lv->block || static_cast<int>(merged.size()) > kMassLbarMax ||
+            static_cast<int>(merged_m.size()) > kMassMatMax)
+          break;
         lbar_union.swap(merged);
+        mat_union.swap(merged_m);
         d.n_mass = static_cast<uint32_t>(m);
         d.n_spring += static_cast<uint32_t>(robot_active(s, r));
         r++;
@@ -298,7 +307,7 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
   }
   if (!lp.packs.empty()) {
     lp.lds = lv->kind == MH_KERNEL_MASS
-                 ? lds_bytes_mass(std::max(lp.mass_cap, 1), n_mat)
+                 ? lds_bytes_mass(lv->block, n_mat)
                  : lds_bytes_lds(std::max(lp.mass_cap, 1), std::max(lp.spring_cap, 1), lp.csr_cap, n_mat);
     plan.parts.push_back(std::move(lp));
   }
@@ -373,30 +382,46 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
   });
 }
 
-// Mass-parallel part: per pack a rest-length table and per mass its
-// first-endpoint and second-endpoint records in ascending spring index,
-// laid out [part][k][mass] (part 2: pack-local spring index of the part-0
-// records), with the two counts per mass.  A record is the other
-// endpoint's skewed LDS slot | material << 16 | rest-length index << 24.
-void upload_mass(mh_sim* s, Part& part, std::vector<uint32_t>& mrec, std::vector<uint8_t>& mcnt,
-                 std::vector<float>& lbtab) {
+// Mass-parallel part: per pack a table of its rest lengths followed by its
+// materials' compact indices (as u32 bits; d.pad0 = offset, d.pad1 =
+// rest-length count | material count << 16), the lane order of its masses
+// (most records first, in perm), and per lane its mass's
+// first-endpoint and second-endpoint records in ascending spring index, laid
+// out [part][k][lane] (part 2: pack-local spring index of the part-0
+// records), with the two counts per lane.  A record holds LDS byte offsets:
+// the other endpoint's skewed slot * 16 | pack material index * 4 << 16 |
+// rest-length index * 4 << 24.
+void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector<uint32_t>& mrec,
+                 std::vector<uint8_t>& mcnt, std::vector<float>& lbtab) {
   const int P = part.v->nspt;
-  std::vector<uint32_t> table, merged, fill_p, fill_m;
+  std::vector<uint32_t> table, mtable, merged, fill_p, fill_m, order, lane;
   for (PackDesc& d : part.packs) {
     const uint32_t nm = d.n_mass, ns = d.n_spring;
     const int r0 = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
                                     s->mass_off.begin()) - 1;
-    // the pack's rest-length table: union of its robots' sets, sorted
+    // the pack's tables: unions of its robots' sets, sorted
     table.clear();
+    mtable.clear();
     for (int q = r0; q < s->n_robots && s->mass_off[q] < d.mass_begin + nm; q++) {
       merged.clear();
       std::set_union(table.begin(), table.end(), s->lbar_set[q].begin(), s->lbar_set[q].end(),
                      std::back_inserter(merged));
       table.swap(merged);
+      merged.clear();
+      std::set_union(mtable.begin(), mtable.end(), s->mat_set[q].begin(), s->mat_set[q].end(),
+                     std::back_inserter(merged));
+      mtable.swap(merged);
     }
+    if (table.size() > static_cast<size_t>(kMassLbarMax) || mtable.size() > static_cast<size_t>(kMassMatMax))
+      throw Error(MH_ERR_INTERNAL, "mass pack tables over the kernel's capacity");
     d.pad0 = static_cast<uint32_t>(lbtab.size());
-    d.pad1 = static_cast<uint32_t>(table.size());
+    d.pad1 = static_cast<uint32_t>(table.size()) | static_cast<uint32_t>(mtable.size()) << 16;
     for (uint32_t bits : table) {
+      float f;
+      std::memcpy(&f, &bits, 4);
+      lbtab.push_back(f);
+    }
+    for (uint32_t bits : mtable) {
       float f;
       std::memcpy(&f, &bits, 4);
       lbtab.push_back(f);
@@ -405,27 +430,54 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint32_t>& mrec, std::vector
     d.row_begin = static_cast<uint32_t>(mcnt.size());
     mrec.resize(mrec.size() + static_cast<size_t>(3 * P) * nm, 0u);
     uint32_t* out = mrec.data() + d.csr_begin;
+    // record counts per mass, then the lane order: most first-endpoint
+    // records first, then most second-endpoint records (ties keep order)
     fill_p.assign(nm, 0);
     fill_m.assign(nm, 0);
     int q = r0;
+    for (uint32_t k = 0; k < ns; k++) {
+      const uint32_t g = d.spring_begin + k;
+      while (g >= s->active_off[q + 1]) q++;
+      const uint32_t off = s->mass_off[q] - d.mass_begin;
+      fill_p[s->act_pairs[2 * g] + off]++;
+      fill_m[s->act_pairs[2 * g + 1] + off]++;
+    }
+    order.resize(nm);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+      return fill_p[x] != fill_p[y] ? fill_p[x] > fill_p[y] : fill_m[x] > fill_m[y];
+    });
+    lane.resize(nm);
+    for (uint32_t i = 0; i < nm; i++) {
+      perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
+      lane[order[i]] = i;
+    }
+    for (uint32_t i = 0; i < nm; i++) {
+      if (fill_p[i] > static_cast<uint32_t>(P) || fill_m[i] > static_cast<uint32_t>(P))
+        throw Error(MH_ERR_INTERNAL, "mass record lists over the kernel's capacity");
+    }
+    fill_p.assign(nm, 0);
+    fill_m.assign(nm, 0);
+    q = r0;
     for (uint32_t k = 0; k < ns; k++) {  // ascending spring index
       const uint32_t g = d.spring_begin + k;
       while (g >= s->active_off[q + 1]) q++;
       const uint32_t off = s->mass_off[q] - d.mass_begin;
       const uint32_t a = s->act_pairs[2 * g] + off, b = s->act_pairs[2 * g + 1] + off;
+      const uint32_t la = lane[a], lb = lane[b];
       uint32_t bits;
       std::memcpy(&bits, &s->act_lbar[g], 4);
       const uint32_t li = static_cast<uint32_t>(std::lower_bound(table.begin(), table.end(), bits) - table.begin());
-      const uint32_t tail = (static_cast<uint32_t>(s->act_mat[g]) << 16) | (li << 24);
-      if (fill_p[a] >= static_cast<uint32_t>(P) || fill_m[b] >= static_cast<uint32_t>(P))
-        throw Error(MH_ERR_INTERNAL, "mass record lists over the kernel's capacity");
-      out[static_cast<size_t>(2 * P + fill_p[a]) * nm + a] = k;
-      out[static_cast<size_t>(fill_p[a]++) * nm + a] = static_cast<uint32_t>(pos_slot(static_cast<int>(b))) | tail;
-      out[static_cast<size_t>(P + fill_m[b]++) * nm + b] = static_cast<uint32_t>(pos_slot(static_cast<int>(a))) | tail;
+      const uint32_t mi = static_cast<uint32_t>(
+          std::lower_bound(mtable.begin(), mtable.end(), static_cast<uint32_t>(s->act_mat[g])) - mtable.begin());
+      const uint32_t tail = ((4u * mi) << 16) | ((4u * li) << 24);
+      out[static_cast<size_t>(2 * P + fill_p[a]) * nm + la] = k;
+      out[static_cast<size_t>(fill_p[a]++) * nm + la] = 16u * static_cast<uint32_t>(pos_slot(static_cast<int>(b))) | tail;
+      out[static_cast<size_t>(P + fill_m[b]++) * nm + lb] = 16u * static_cast<uint32_t>(pos_slot(static_cast<int>(a))) | tail;
     }
     for (uint32_t i = 0; i < nm; i++) {
-      mcnt.push_back(static_cast<uint8_t>(fill_p[i]));
-      mcnt.push_back(static_cast<uint8_t>(fill_m[i]));
+      mcnt.push_back(static_cast<uint8_t>(fill_p[order[i]]));
+      mcnt.push_back(static_cast<uint8_t>(fill_m[order[i]]));
     }
   }
   std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
@@ -456,7 +508,7 @@ void upload_plan(mh_sim* s, Plan& plan) {
     const bool padded = part.v->kind == MH_KERNEL_LDS;
     need_scratch |= part.v->kind == MH_KERNEL_STREAM && !part.v->chunked;
     if (part.v->kind == MH_KERNEL_MASS) {
-      upload_mass(s, part, mrec, mcnt, lbtab);
+      upload_mass(s, part, perm, mrec, mcnt, lbtab);
     } else if (part.v->chunked) {
       upload_chunked(s, part, spair, ccsr, crow);
     } else {
@@ -852,28 +904,35 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   if (compact_to_id.empty()) compact_to_id.push_back(0);  // keeps tables non-empty
 
   // per robot: gather entries with both parts of every row rounded up to 4;
-  // the most springs one mass has on one side; distinct rest lengths (the
-  // set is left empty when it exceeds the mass-parallel kernel's table)
+  // the most springs one mass has on one side; distinct rest lengths and
+  // materials (each set is left empty when it exceeds the mass-parallel
+  // kernel's tables)
   std::vector<uint32_t> gpad(R, 0);
   std::vector<uint8_t> mside(R, 0);
-  std::vector<std::vector<uint32_t>> lsets(R);
+  std::vector<std::vector<uint32_t>> lsets(R), msets(R);
   {
     std::vector<uint32_t> plus, minus;
     for (int r = 0; r < R; r++) {
       plus.assign(mo[r + 1] - mo[r], 0);
       minus.assign(mo[r + 1] - mo[r], 0);
       std::vector<uint32_t>& ls = lsets[r];
+      std::vector<uint32_t>& mset = msets[r];
+      auto add = [](std::vector<uint32_t>& set, uint32_t v, int cap) {
+        if (set.size() <= static_cast<size_t>(cap)) {
+          auto it = std::lower_bound(set.begin(), set.end(), v);
+          if (it == set.end() || *it != v) set.insert(it, v);
+        }
+      };
       for (uint32_t k = ao[r]; k < ao[r + 1]; k++) {
         plus[ap[2 * k]]++;
         minus[ap[2 * k + 1]]++;
         uint32_t bits;
         std::memcpy(&bits, &alb[k], 4);
-        if (ls.size() <= static_cast<size_t>(mh::kMassLbarMax)) {
-          auto it = std::lower_bound(ls.begin(), ls.end(), bits);
-          if (it == ls.end() || *it != bits) ls.insert(it, bits);
-        }
+        add(ls, bits, mh::kMassLbarMax);
+        add(mset, amat[k], mh::kMassMatMax);
       }
       if (ls.size() > static_cast<size_t>(mh::kMassLbarMax)) ls.clear();
+      if (mset.size() > static_cast<size_t>(mh::kMassMatMax)) mset.clear();
       uint32_t g = 0, side = 0;
       for (size_t i = 0; i < plus.size(); i++) {
         g += ((plus[i] + 3u) & ~3u) + ((minus[i] + 3u) & ~3u);
@@ -908,6 +967,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   s->gather_pad = std::move(gpad);
   s->max_side = std::move(mside);
   s->lbar_set = std::move(lsets);
+  s->mat_set = std::move(msets);
   s->act_pairs = std::move(ap);
   s->act_lbar = alb;
   s->active_src = std::move(src);

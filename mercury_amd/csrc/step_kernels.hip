@@ -530,28 +530,99 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
 // One lane per mass; every spring is evaluated by both of its endpoints
 // (the same operands in the same order, so both get the same bits), which
 // removes the spring-term buffer and gather lists from LDS and leaves one
-// barrier per substep.  LDS holds two position buffers (read one, write the
-// other; skewed slots as in step_lds, plus a far-away ghost slot in the
-// last slot that padding records point at), the pack's rest-length table
-// (at most 128 values), the per-material stiffness and two actuation
-// tables.  Each lane keeps its records in registers: MAXP first-endpoint
-// records then MAXP second-endpoint records, each in ascending spring
-// index; a record is other-slot | material << 16 | rest-length index << 24.
-// Records live in p.mrec as [part][k][mass] (part 0: first endpoint, part
-// 1: second endpoint, part 2: the pack-local spring index of part-0
-// records for the stress output); p.mcnt holds the two counts per mass.
+// barrier per substep.  LDS (fixed layout, so table and buffer bases are
+// instruction offsets): rest lengths float[64] | stiffness float[64] |
+// actuation float[2][64] | actuation parameters (dL0, omega, phi)
+// float[3][64] | positions float4[2][pos_slots(BLOCK)] (read one
+// buffer, write the other; skewed slots as in step_lds; the last slot of
+// each buffer is a far-away ghost that padding records point at).  Lanes
+// take masses in the order of p.mperm (most records first, so a wave's
+// lists have similar lengths).  Each lane keeps its records in registers:
+// MAXP first-endpoint records then MAXP second-endpoint records, each in
+// ascending spring index, padded to the wave's longest list with records of
+// the null material (last table entry), whose terms are signed zeros; a
+// record holds byte offsets: other endpoint's
+// slot * 16 | material * 4 << 16 | rest-length index * 4 << 24.  Records
+// live in p.mrec as [part][k][lane] (part 2: the pack-local spring index of
+// the part-0 records, for the stress output); p.mcnt holds the two counts
+// per lane.
+constexpr int kMassTab = 64;               // rest lengths, materials per table
+constexpr int kMassNull = kMassTab - 1;    // table entry of padding records
+constexpr int kMassLb = 0, kMassK = 256, kMassR = 512, kMassAct = 1024, kMassPos = 2048;
+
+template <int BLOCK>
+constexpr int mass_pos_bytes() { return 16 * (BLOCK + BLOCK / 8 + 1); }
+
+// One substep reading position buffer CUR and writing the other.
+template <int SOLVER, int BLOCK, int MAXP, int CUR>
+__device__ __forceinline__ void mass_substep(unsigned char* smem, const StepParams& p, MassState& ms,
+                                             uint32_t (&rp)[MAXP], uint32_t (&rm)[MAXP],
+                                             float (&st)[MAXP], int np, int nq, int np_w, int nq_w,
+                                             bool live, int pslot, int n_pm, float t) {
+  constexpr int kP0 = kMassPos + CUR * mass_pos_bytes<BLOCK>();
+  constexpr int kP1 = kMassPos + (1 - CUR) * mass_pos_bytes<BLOCK>();
+  constexpr int kR0 = kMassR + CUR * 4 * kMassTab;
+  constexpr int kR1 = kMassR + (1 - CUR) * 4 * kMassTab;
+  // idle lanes sit at the origin, 1e3*sqrt(3) from the ghost, so their
+  // padding terms stay in the fast arithmetic range
+  const float4 own = live ? *reinterpret_cast<const float4*>(smem + kP0 + 16 * pslot)
+                          : make_float4(0.f, 0.f, 0.f, 0.f);
+  float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+#pragma unroll
+  for (int k = 0; k < MAXP; ++k) {
+    if (k < np_w) {
+      // opaque to the optimizer: decoding the record per substep costs a
+      // few VALU, hoisting every decoded field out of the step loop would
+      // cost registers
+      asm volatile("" : "+v"(rp[k]));
+      const uint32_t r = rp[k];
+      const uint32_t mo = (r >> 16) & 0xffu;
+      const float lb = *reinterpret_cast<const float*>(smem + kMassLb + (r >> 24));
+      const float4 other = *reinterpret_cast<const float4*>(smem + kP0 + (r & 0xffffu));
+      float cx, cy, cz, lambda;
+      spring_term<SOLVER>(own, other, lb, *reinterpret_cast<const float*>(smem + kR0 + mo),
+                          *reinterpret_cast<const float*>(smem + kMassK + mo), cx, cy, cz, lambda);
+      // padding records add a signed zero, which leaves a sum that
+      // started at +0 unchanged
+      ax = ax + cx; ay = ay + cy; az = az + cz;
+      if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress && k < np) st[k] = st[k] + lambda / lb;
+    }
+    // one record at a time: keeps the unrolled loop's registers bounded
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int k = 0; k < MAXP; ++k) {
+    if (k < nq_w) {
+      asm volatile("" : "+v"(rm[k]));
+      const uint32_t r = rm[k];
+      const uint32_t mo = (r >> 16) & 0xffu;
+      const float lb = *reinterpret_cast<const float*>(smem + kMassLb + (r >> 24));
+      const float4 other = *reinterpret_cast<const float4*>(smem + kP0 + (r & 0xffffu));
+      float cx, cy, cz, lambda;
+      spring_term<SOLVER>(other, own, lb, *reinterpret_cast<const float*>(smem + kR0 + mo),
+                          *reinterpret_cast<const float*>(smem + kMassK + mo), cx, cy, cz, lambda);
+      bx = bx + cx; by = by + cy; bz = bz + cz;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  if (live)
+    *reinterpret_cast<float4*>(smem + kP1 + 16 * pslot) = integrate<SOLVER>(ms, ax - bx, ay - by, az - bz, p);
+  const int tid = threadIdx.x;
+  if (tid < n_pm) {
+    const float* act = reinterpret_cast<const float*>(smem + kMassAct);
+    *reinterpret_cast<float*>(smem + kR1 + 4 * tid) =
+        actuation(act[tid], act[kMassTab + tid], act[2 * kMassTab + tid], t + p.dt);
+  }
+  __syncthreads();
+}
+
 template <int SOLVER, int BLOCK, int MAXP>
 __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   const PackDesc d = p.packs[blockIdx.x];
   const int nm = static_cast<int>(d.n_mass);
-  const int slots = pos_slots(p.mass_cap);
-  const int ghost = slots - 1;
-  float4* s_pos0 = reinterpret_cast<float4*>(smem);
-  float4* s_pos1 = s_pos0 + slots;
-  float* s_lb = reinterpret_cast<float*>(s_pos1 + slots);
-  float* s_k = s_lb + kMassLbarMax;
-  float* s_R = s_k + p.n_mat;
+  constexpr int kSlots = BLOCK + BLOCK / 8 + 1;
+  constexpr int kGhost = kSlots - 1;
   const int tid = threadIdx.x;
   const bool live = tid < nm;
   const uint32_t* rec = p.mrec + d.csr_begin;
@@ -559,9 +630,9 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   MassState ms;
   uint32_t rp[MAXP], rm[MAXP];
   float st[MAXP];
-  int np = 0, nq = 0;
-  const int pslot = live ? pos_slot(tid) : ghost;
-  const uint32_t pad = static_cast<uint32_t>(ghost);
+  int np = 0, nq = 0, mass = 0;
+  // padding record: the ghost, with the null material and rest length
+  const uint32_t pad = 16u * kGhost | (4u * kMassNull) << 16 | (4u * kMassNull) << 24;
 #pragma unroll
   for (int k = 0; k < MAXP; ++k) {
     rp[k] = pad;
@@ -569,7 +640,8 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
     st[k] = 0.f;
   }
   if (live) {
-    load_mass(ms, p.pos, p.vel, d.mass_begin + tid, p);
+    mass = p.mperm[d.mass_begin + tid];
+    load_mass(ms, p.pos, p.vel, d.mass_begin + mass, p);
     np = p.mcnt[d.row_begin + 2 * tid];
     nq = p.mcnt[d.row_begin + 2 * tid + 1];
 #pragma unroll
@@ -579,16 +651,40 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
       if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress && k < np)
         st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]];
     }
-    s_pos0[pslot] = lds_position<SOLVER>(ms);
+  }
+  const int pslot = live ? pos_slot(mass) : kGhost;
+  float4* pos0 = reinterpret_cast<float4*>(smem + kMassPos);
+  float4* pos1 = reinterpret_cast<float4*>(smem + kMassPos + mass_pos_bytes<BLOCK>());
+  if (live) pos0[pslot] = lds_position<SOLVER>(ms);
+  if (tid == 0) {
+    pos0[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
+    pos1[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
+  }
+  float* s_lb = reinterpret_cast<float*>(smem + kMassLb);
+  float* s_k = reinterpret_cast<float*>(smem + kMassK);
+  float* s_R = reinterpret_cast<float*>(smem + kMassR);
+  float* s_act = reinterpret_cast<float*>(smem + kMassAct);
+  // the pack's tables: d.pad1 = rest-length count | material count << 16;
+  // the compact material indices follow the rest lengths in p.lbtab
+  const int n_lb = static_cast<int>(d.pad1 & 0xffffu), n_pm = static_cast<int>(d.pad1 >> 16);
+  for (int i = tid; i < n_lb; i += BLOCK) s_lb[i] = p.lbtab[d.pad0 + i];
+  float t = p.t0;
+  if (tid < n_pm) {  // n_pm < kMassTab <= BLOCK
+    const uint32_t c = __float_as_uint(p.lbtab[d.pad0 + n_lb + tid]);
+    s_k[tid] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[c] : p.mat_k[c];
+    s_act[tid] = p.mat_dL0[c];
+    s_act[kMassTab + tid] = p.mat_omega[c];
+    s_act[2 * kMassTab + tid] = p.mat_phi[c];
+    s_R[tid] = actuation(p.mat_dL0[c], p.mat_omega[c], p.mat_phi[c], t);
   }
   if (tid == 0) {
-    s_pos0[ghost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
-    s_pos1[ghost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
+    // null entries: a term with stiffness 0 (force) or compliance-divisor
+    // +inf (xpbd) is a signed zero
+    s_lb[kMassNull] = 1.0f;
+    s_k[kMassNull] = (SOLVER == MH_SOLVER_XPBD_REF) ? __builtin_inff() : 0.0f;
+    s_R[kMassNull] = 0.0f;
+    s_R[kMassTab + kMassNull] = 0.0f;
   }
-  for (int i = tid; i < static_cast<int>(d.pad1); i += BLOCK) s_lb[i] = p.lbtab[d.pad0 + i];
-  for (int i = tid; i < p.n_mat; i += BLOCK) s_k[i] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[i] : p.mat_k[i];
-  float t = p.t0;
-  if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], t);
   // wave-wide loop bounds
   int np_w = np, nq_w = nq;
   for (int off = 32; off > 0; off >>= 1) {
@@ -597,61 +693,19 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   }
   __syncthreads();
 
-  int cur = 0;
-  for (int step = 0; step < p.n_steps; ++step) {
-    const float4* P0 = cur ? s_pos1 : s_pos0;
-    float4* P1 = cur ? s_pos0 : s_pos1;
-    const float* R = s_R + cur * p.n_mat;
-    // idle lanes sit at the origin, 1e3*sqrt(3) from the ghost, so their
-    // padding terms stay in the fast arithmetic range
-    const float4 own = live ? P0[pslot] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
-#pragma unroll
-    for (int k = 0; k < MAXP; ++k) {
-      if (k < np_w) {
-        // opaque to the optimizer: decoding the record per substep costs a
-        // few VALU, hoisting every decoded field out of the step loop would
-        // cost registers
-        uint32_t r = rp[k];
-        asm volatile("" : "+v"(r));
-        const int m = (r >> 16) & 0xff;
-        const float lb = s_lb[r >> 24];
-        float cx, cy, cz, lambda;
-        spring_term<SOLVER>(own, P0[r & 0xffffu], lb, R[m], s_k[m], cx, cy, cz, lambda);
-        if (k < np) {
-          ax = ax + cx; ay = ay + cy; az = az + cz;
-          if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) st[k] = st[k] + lambda / lb;
-        }
-      }
-      // one record at a time: keeps the unrolled loop's registers bounded
-      __builtin_amdgcn_sched_barrier(0);
-    }
-#pragma unroll
-    for (int k = 0; k < MAXP; ++k) {
-      if (k < nq_w) {
-        uint32_t r = rm[k];
-        asm volatile("" : "+v"(r));
-        const int m = (r >> 16) & 0xff;
-        const float lb = s_lb[r >> 24];
-        float cx, cy, cz, lambda;
-        spring_term<SOLVER>(P0[r & 0xffffu], own, lb, R[m], s_k[m], cx, cy, cz, lambda);
-        if (k < nq) {
-          bx = bx + cx; by = by + cy; bz = bz + cz;
-        }
-      }
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    const float tn = t + p.dt;
-    if (live) P1[pslot] = integrate<SOLVER>(ms, ax - bx, ay - by, az - bz, p);
-    if (tid < p.n_mat)
-      s_R[(cur ^ 1) * p.n_mat + tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
-    t = tn;
-    cur ^= 1;
-    __syncthreads();
+  // substeps in pairs, so each buffer's base is a constant offset
+  int step = 0;
+  for (; step + 1 < p.n_steps; step += 2) {
+    mass_substep<SOLVER, BLOCK, MAXP, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
+    t = t + p.dt;
+    mass_substep<SOLVER, BLOCK, MAXP, 1>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
+    t = t + p.dt;
   }
+  if (step < p.n_steps)
+    mass_substep<SOLVER, BLOCK, MAXP, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
 
   if (live) {
-    const uint32_t gi = d.mass_begin + tid;
+    const uint32_t gi = d.mass_begin + mass;
     p.pos[gi] = make_float4(ms.x, ms.y, ms.z, ms.m);
     p.vel[gi] = make_float4(ms.vx, ms.vy, ms.vz, 0.f);
     if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
@@ -716,10 +770,11 @@ size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat) {
   return b;
 }
 
-size_t lds_bytes_mass(int mass_cap, int n_mat) {
-  // two position buffers (skewed slots + ghost), rest lengths, stiffness,
-  // two actuation tables
-  return 2u * 16u * static_cast<size_t>(pos_slots(mass_cap)) + 4u * kMassLbarMax + 12u * static_cast<size_t>(n_mat);
+size_t lds_bytes_mass(int block, int n_mat) {
+  // fixed tables (rest lengths, stiffness, two actuation tables, actuation
+  // parameters) and two position buffers sized for the block
+  (void)n_mat;
+  return static_cast<size_t>(kMassPos) + 2u * 16u * static_cast<size_t>(block + block / 8 + 1);
 }
 
 size_t lds_bytes_stream(int mass_cap, int n_mat) {
