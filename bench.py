@@ -108,6 +108,7 @@ def measured_traffic(workload):
     if not entry:
         return None
     return {"bytes_per_launch": entry["bytes_per_launch"],
+            "valu_instr_per_launch": entry.get("valu_instr_per_launch"),
             "source": "profiles/traffic.json: " + entry["summary"]}
 
 
@@ -237,6 +238,23 @@ def main():
         per_unit = STREAM_BYTES_PER_SPRING
     achieved = per_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
     traffic = measured_traffic(args.workload)
+    # vector-ALU issue: the step kernel's VALU wave-instructions per launch
+    # (SQ_INSTS_VALU from the committed counter summary) over its measured
+    # launch time, against the device's calibrated v_fma_f32 issue rate
+    valu = None
+    if traffic and traffic.get("valu_instr_per_launch") and kernel_ms > 0:
+        rates = []
+        for kind in (0, 1):
+            r = ctypes.c_float()
+            mh._lib.check(mh.load().mh_calibrate_valu(0, kind, ctypes.byref(r)))
+            rates.append(float(r.value))
+        vpeak = ctypes.c_float(max(rates))
+        vach = traffic["valu_instr_per_launch"] / (kernel_ms * 1e-3) / 1e9
+        valu = {"achieved": vach, "peak": float(vpeak.value), "unit": "1e9 wave64 VALU instr/s",
+                "frac": vach / float(vpeak.value),
+                "instr_per_launch": traffic["valu_instr_per_launch"],
+                "peak_kind": "measured FP32 VALU issue rate, max of v_fma_f32 and v_mul_f32 "
+                             "chains (mh_calibrate_valu)"}
 
     out = {
         "metric": metric,
@@ -277,6 +295,7 @@ def main():
             "launches_per_step": launches,
             "algorithmic_bytes_per_spring_update": per_unit,
             **({"streamed_part": hbm_extra} if hbm_extra else {}),
+            **({"valu": valu} if valu else {}),
         },
         "host_seconds": {"topology_build": t_build, "upload": t_upload},
     }

@@ -283,13 +283,19 @@ int mh_calibrate_lds(int device, float* gbytes_per_s);
 /* Average time (ms) of a plain float4 copy of `bytes` bytes, the known-size
  * reference for scaling hardware byte counters. */
 int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms);
+/* FP32 VALU issue rate of the device in 10^9 wave64 instructions per second
+ * (independent chains, 8 waves per SIMD; kind 0 v_fma_f32, kind 1
+ * v_mul_f32): the peak for kernels bound by vector-ALU issue. */
+int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s);
 
 /* No reference counterpart: checks, bit for bit, the fast arithmetic the
  * step kernels rely on against the compiler's correctly rounded operations,
  * over the float bit patterns [begin, begin + count).  which: 0 hardware
  * sqrt, 1 the kernels' in-range correctly rounded sqrt, 2 hardware
  * reciprocal, 3 x against the next float (every pattern differs; a check
- * of the checker), 4 the kernels' in-range correctly rounded reciprocal.
+ * of the checker), 4 the kernels' in-range correctly rounded reciprocal;
+ * 5 / 6 count inputs whose hardware sqrt lies above / below sqrtf, 7 / 8
+ * the same for the hardware reciprocal (positive inputs).
  * *mismatches receives the number of differing results. */
 int mh_check_arith(int device, int which, uint32_t begin, uint32_t count,
                    uint64_t* mismatches);

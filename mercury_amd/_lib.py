@@ -114,6 +114,7 @@ _SIGS = {
     "mh_cnn_train": (c_int, [vp, u8p, f32p, c_int, c_int, c_int, c_float, ctypes.c_uint64, f32p]),
     "mh_calibrate_lds": (c_int, [c_int, f32p]),
     "mh_calibrate_copy": (c_int, [c_int, ctypes.c_size_t, c_int, f32p]),
+    "mh_calibrate_valu": (c_int, [c_int, c_int, f32p]),
     "mh_check_arith": (c_int, [c_int, c_int, ctypes.c_uint32, ctypes.c_uint32,
                                ctypes.POINTER(ctypes.c_uint64)]),
 }

@@ -7,6 +7,11 @@
 //   mh_calibrate_copy: a plain float4 copy of a known byte count, the
 //                      reference that scales rocprof's FETCH_SIZE /
 //                      WRITE_SIZE counters to bytes.
+//   mh_calibrate_valu: FP32 VALU issue rate (wave64 v_fma_f32 or v_mul_f32
+//                      per second, 8 independent chains per lane, 8 waves
+//                      per SIMD),
+//                      the peak the VALU-bound step kernels are measured
+//                      against.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -41,6 +46,24 @@ __global__ __launch_bounds__(1024) void lds_read_bw(float* out, int iters) {
   if (acc.x == -1.f) out[blockIdx.x * 1024 + tid] = acc.x + acc.y + acc.z + acc.w;
 }
 
+// kind 0: v_fma_f32 chains; kind 1: v_mul_f32 chains (two sources)
+template <int KIND>
+__global__ __launch_bounds__(256) void valu_rate(float* out, int iters, float a, float b) {
+  float x0 = threadIdx.x, x1 = x0 + 1.f, x2 = x0 + 2.f, x3 = x0 + 3.f;
+  float x4 = x0 + 4.f, x5 = x0 + 5.f, x6 = x0 + 6.f, x7 = x0 + 7.f;
+  for (int it = 0; it < iters; ++it) {
+    if (KIND == 0) {
+      x0 = fmaf(x0, a, b); x1 = fmaf(x1, a, b); x2 = fmaf(x2, a, b); x3 = fmaf(x3, a, b);
+      x4 = fmaf(x4, a, b); x5 = fmaf(x5, a, b); x6 = fmaf(x6, a, b); x7 = fmaf(x7, a, b);
+    } else {
+      x0 = x0 * a; x1 = x1 * a; x2 = x2 * a; x3 = x3 * a;
+      x4 = x4 * a; x5 = x5 * a; x6 = x6 * a; x7 = x7 * a;
+    }
+  }
+  const float s = ((x0 + x1) + (x2 + x3)) + ((x4 + x5) + (x6 + x7));
+  if (s == -1.f) out[blockIdx.x * 256 + threadIdx.x] = s;
+}
+
 __global__ void copy_f4(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
   for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -56,6 +79,11 @@ __global__ void copy_f4(const float4* __restrict__ src, float4* __restrict__ dst
 //         3: x                      vs the next float (all differ: checks
 //                                      the counting itself)
 //         4: mh::rcp_rn_inrange     vs 1.0f / x
+//         5: hardware sqrt above sqrtf (count of such inputs)
+//         6: hardware sqrt below sqrtf
+//         7: hardware reciprocal above 1.0f / x
+//         8: hardware reciprocal below 1.0f / x
+//   (5-8 count one direction of the error for positive x)
 __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, uint32_t count,
                                                    uint32_t* out) {
   __shared__ uint32_t part[256];
@@ -76,11 +104,22 @@ __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, ui
     } else if (which == 3) {
       a = x;
       b = __uint_as_float(__float_as_uint(x) + 1u);
-    } else {
+    } else if (which == 4) {
       a = mh::rcp_rn_inrange(x);
       b = 1.0f / x;
+    } else if (which <= 6) {
+      a = __builtin_amdgcn_sqrtf(x);
+      b = sqrtf(x);
+    } else {
+      a = __builtin_amdgcn_rcpf(x);
+      b = 1.0f / x;
     }
-    bad += (__float_as_uint(a) != __float_as_uint(b)) ? 1u : 0u;
+    if (which <= 4)
+      bad += (__float_as_uint(a) != __float_as_uint(b)) ? 1u : 0u;
+    else if (which == 5 || which == 7)
+      bad += (a > b) ? 1u : 0u;
+    else
+      bad += (a < b) ? 1u : 0u;
   }
   part[threadIdx.x] = bad;
   __syncthreads();
@@ -130,6 +169,38 @@ int mh_calibrate_lds(int device, float* gbytes_per_s) {
   return MH_OK;
 }
 
+int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s) {
+  if (!giga_instr_per_s || kind < 0 || kind > 1) return MH_ERR_ARG;
+  auto kernel = kind == 0 ? valu_rate<0> : valu_rate<1>;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return fail(e, "hipSetDevice");
+  int cus = 0;
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  // 8 blocks of 4 waves per CU: 8 waves per SIMD
+  const int blocks = (cus > 0 ? cus : 256) * 8;
+  const int iters = 1 << 14;
+  float* out = nullptr;
+  if ((e = hipMalloc(&out, sizeof(float) * static_cast<size_t>(blocks) * 256)) != hipSuccess)
+    return fail(e, "hipMalloc");
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, nullptr, out, 64, 0.999f, 1e-3f);
+  hipEventRecord(a, nullptr);
+  hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, nullptr, out, iters, 0.999f, 1e-3f);
+  hipEventRecord(b, nullptr);
+  e = hipEventSynchronize(b);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  hipFree(out);
+  if (e != hipSuccess) return fail(e, "valu calibration");
+  const double instr = static_cast<double>(blocks) * 4.0 * iters * 8.0;  // wave instructions
+  *giga_instr_per_s = static_cast<float>(instr / (ms * 1e-3) / 1e9);
+  return MH_OK;
+}
+
 int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms) {
   if (!avg_ms || bytes < 16 || reps < 1) return MH_ERR_ARG;
   hipError_t e = hipSetDevice(device);
@@ -164,7 +235,7 @@ int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms) {
 }
 
 int mh_check_arith(int device, int which, uint32_t begin, uint32_t count, uint64_t* mismatches) {
-  if (!mismatches || which < 0 || which > 4 || count == 0 ||
+  if (!mismatches || which < 0 || which > 8 || count == 0 ||
       static_cast<uint64_t>(begin) + count > 0x100000000ull)
     return MH_ERR_ARG;
   hipError_t e = hipSetDevice(device);

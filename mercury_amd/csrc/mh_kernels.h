@@ -64,6 +64,7 @@ struct KernelVariant {
                   // per endpoint side (MASS kind)
   int mpt;        // masses per thread
   const void* fn; // kernel symbol
+  const void* fn_stress = nullptr;  // variant used when stress is tracked (null: fn)
 };
 
 // All compiled variants (table lives in step_kernels.hip).

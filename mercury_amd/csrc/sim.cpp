@@ -572,9 +572,11 @@ void upload_plan(mh_sim* s, Plan& plan) {
     }
     part.pack_offset = static_cast<uint32_t>(all.size());
     all.insert(all.end(), part.packs.begin(), part.packs.end());
-    if (part.lds > 64 * 1024)
-      (void)hipFuncSetAttribute(part.v->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                static_cast<int>(part.lds));
+    if (part.lds > 64 * 1024) {
+      for (const void* fn : {part.v->fn, part.v->fn_stress})
+        if (fn)
+          (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(part.lds));
+    }
   }
   s->spair.upload(spair.data(), na, s->stream);
   s->csr.upload(csr.data(), csr.size(), s->stream);

@@ -554,7 +554,7 @@ template <int BLOCK>
 constexpr int mass_pos_bytes() { return 16 * (BLOCK + BLOCK / 8 + 1); }
 
 // One substep reading position buffer CUR and writing the other.
-template <int SOLVER, int BLOCK, int MAXP, int CUR>
+template <int SOLVER, int BLOCK, int MAXP, bool STRESS, int CUR>
 __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepParams& p, MassState& ms,
                                              uint32_t (&rp)[MAXP], uint32_t (&rm)[MAXP],
                                              float (&st)[MAXP], int np, int nq, int np_w, int nq_w,
@@ -585,7 +585,7 @@ __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepPara
       // padding records add a signed zero, which leaves a sum that
       // started at +0 unchanged
       ax = ax + cx; ay = ay + cy; az = az + cz;
-      if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress && k < np) st[k] = st[k] + lambda / lb;
+      if (STRESS && k < np) st[k] = st[k] + lambda / lb;
     }
     // one record at a time: keeps the unrolled loop's registers bounded
     __builtin_amdgcn_sched_barrier(0);
@@ -616,7 +616,9 @@ __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepPara
   __syncthreads();
 }
 
-template <int SOLVER, int BLOCK, int MAXP>
+// STRESS (xpbd only) accumulates per-spring stress; the variant without it
+// frees MAXP registers per lane.
+template <int SOLVER, int BLOCK, int MAXP, bool STRESS>
 __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   const PackDesc d = p.packs[blockIdx.x];
@@ -648,7 +650,7 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
     for (int k = 0; k < MAXP; ++k) {
       if (k < np) rp[k] = rec[static_cast<size_t>(k) * nm + tid];
       if (k < nq) rm[k] = rec[static_cast<size_t>(MAXP + k) * nm + tid];
-      if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress && k < np)
+      if (STRESS && k < np)
         st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]];
     }
   }
@@ -696,19 +698,19 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   // substeps in pairs, so each buffer's base is a constant offset
   int step = 0;
   for (; step + 1 < p.n_steps; step += 2) {
-    mass_substep<SOLVER, BLOCK, MAXP, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
+    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
     t = t + p.dt;
-    mass_substep<SOLVER, BLOCK, MAXP, 1>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
+    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 1>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
     t = t + p.dt;
   }
   if (step < p.n_steps)
-    mass_substep<SOLVER, BLOCK, MAXP, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
+    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
 
   if (live) {
     const uint32_t gi = d.mass_begin + mass;
     p.pos[gi] = make_float4(ms.x, ms.y, ms.z, ms.m);
     p.vel[gi] = make_float4(ms.vx, ms.vy, ms.vz, 0.f);
-    if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
+    if (STRESS) {
 #pragma unroll
       for (int k = 0; k < MAXP; ++k)
         if (k < np) p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]] = st[k];
@@ -738,8 +740,9 @@ __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint3
 
 #define MH_LDS(SOLVER, B, S, M) \
   {SOLVER, MH_KERNEL_LDS, 0, B, S, M, reinterpret_cast<const void*>(&step_lds<SOLVER, B, S, M>)}
-#define MH_MASS(SOLVER, B, P) \
-  {SOLVER, MH_KERNEL_MASS, 0, B, P, 1, reinterpret_cast<const void*>(&step_mass<SOLVER, B, P>)}
+#define MH_MASS(SOLVER, B, P)                                                         \
+  {SOLVER, MH_KERNEL_MASS, 0, B, P, 1, reinterpret_cast<const void*>(&step_mass<SOLVER, B, P, false>), \
+   reinterpret_cast<const void*>(&step_mass<SOLVER, B, P, (SOLVER == MH_SOLVER_XPBD_REF)>)}
 #define MH_CHUNK(SOLVER, B, M) \
   {SOLVER, MH_KERNEL_STREAM, 1, B, 0, M, reinterpret_cast<const void*>(&step_chunk<SOLVER, B, M>)}
 #define MH_STREAM(SOLVER, B) \
@@ -785,7 +788,8 @@ hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes, hi
                        const StepParams& p) {
   StepParams args = p;
   void* kargs[] = {&args};
-  return hipLaunchKernel(v.fn, dim3(n_packs), dim3(v.block), kargs, lds_bytes, stream);
+  const void* fn = (p.track_stress && v.fn_stress) ? v.fn_stress : v.fn;
+  return hipLaunchKernel(fn, dim3(n_packs), dim3(v.block), kargs, lds_bytes, stream);
 }
 
 hipError_t launch_com(const float4* pos, const uint8_t* mass_id, const uint32_t* robot_mass_begin,
