@@ -42,7 +42,9 @@ LDS_BYTES_PER_SPRING = 57.0
 LDS_BYTES_PER_MASS = 48.0
 # mass-parallel kernel: both endpoints evaluate a spring, each reading the
 # other endpoint's position (12 B), the rest length (4 B) and the material's
-# stiffness and actuation (8 B); per mass its own position read and write
+# stiffness and actuation (8 B); per mass its own position read and write.
+# The gather flavour moves the same bytes: the first endpoint reads 24 B and
+# writes the 12 B term, the second endpoint reads the term.
 MASS_LDS_BYTES_PER_SPRING = 48.0
 MASS_LDS_BYTES_PER_MASS = 32.0
 # MI355X nominal LDS bandwidth: 256 CUs x 128 B/clk x 2.4 GHz
@@ -108,7 +110,6 @@ def measured_traffic(workload):
     if not entry:
         return None
     return {"bytes_per_launch": entry["bytes_per_launch"],
-            "valu_instr_per_launch": entry.get("valu_instr_per_launch"),
             "source": "profiles/traffic.json: " + entry["summary"]}
 
 
@@ -216,7 +217,7 @@ def main():
     lds_peak = float(lds_peak.value)
     steps_per_launch = n_sub / max(launches, 1)
     hbm_extra = None
-    if info.kernel & mh.MH_KERNEL_MASS and not info.kernel & mh.MH_KERNEL_STREAM:
+    if info.kernel & (mh.MH_KERNEL_MASS | mh.MH_KERNEL_MASS_GATHER) and not info.kernel & mh.MH_KERNEL_STREAM:
         bound, peak, unit = "lds", lds_peak, "GB/s"
         per_launch = (MASS_LDS_BYTES_PER_SPRING * n_springs + MASS_LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
         per_unit = MASS_LDS_BYTES_PER_SPRING + MASS_LDS_BYTES_PER_MASS * n_masses / n_springs
@@ -238,23 +239,7 @@ def main():
         per_unit = STREAM_BYTES_PER_SPRING
     achieved = per_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
     traffic = measured_traffic(args.workload)
-    # vector-ALU issue: the step kernel's VALU wave-instructions per launch
-    # (SQ_INSTS_VALU from the committed counter summary) over its measured
-    # launch time, against the device's calibrated v_fma_f32 issue rate
-    valu = None
-    if traffic and traffic.get("valu_instr_per_launch") and kernel_ms > 0:
-        rates = []
-        for kind in (0, 1):
-            r = ctypes.c_float()
-            mh._lib.check(mh.load().mh_calibrate_valu(0, kind, ctypes.byref(r)))
-            rates.append(float(r.value))
-        vpeak = ctypes.c_float(max(rates))
-        vach = traffic["valu_instr_per_launch"] / (kernel_ms * 1e-3) / 1e9
-        valu = {"achieved": vach, "peak": float(vpeak.value), "unit": "1e9 wave64 VALU instr/s",
-                "frac": vach / float(vpeak.value),
-                "instr_per_launch": traffic["valu_instr_per_launch"],
-                "peak_kind": "measured FP32 VALU issue rate, max of v_fma_f32 and v_mul_f32 "
-                             "chains (mh_calibrate_valu)"}
+
 
     out = {
         "metric": metric,
@@ -273,7 +258,8 @@ def main():
             "workload": workload,
             "robots": args.robots, "springs_total": n_springs, "active_springs": n_active,
             "masses_total": n_masses, "substeps_per_eval": n_sub, "solver": args.solver,
-            "kernel": ({1: "lds", 2: "stream", 3: "lds+stream", 4: "mass", 6: "mass+stream"}.get(info.kernel, "?") +
+            "kernel": ("+".join(n for bit, n in ((8, "mass-gather"), (4, "mass"), (1, "lds"), (2, "stream"))
+                                if info.kernel & bit) +
                        (" (chunked)" if info.stream_chunk > 0 else "")), "block": info.block_size,
             "blocks": info.n_blocks, "lds_bytes_per_block": info.lds_bytes,
             "substeps_per_launch": info.substeps_per_launch, "hipgraph": True,
@@ -295,7 +281,6 @@ def main():
             "launches_per_step": launches,
             "algorithmic_bytes_per_spring_update": per_unit,
             **({"streamed_part": hbm_extra} if hbm_extra else {}),
-            **({"valu": valu} if valu else {}),
         },
         "host_seconds": {"topology_build": t_build, "upload": t_upload},
     }

@@ -112,8 +112,12 @@ enum { MH_SOLVER_XPBD_REF = 0, MH_SOLVER_FORCE = 1 };
  *   LDS     robots resident in LDS, spring terms gathered per mass;
  *   STREAM  robots over the LDS budget, spring records streamed per substep;
  *   MASS    robots resident in LDS, one lane per mass evaluating each of
- *           its springs (both endpoints evaluate a spring). */
-enum { MH_KERNEL_AUTO = 0, MH_KERNEL_LDS = 1, MH_KERNEL_STREAM = 2, MH_KERNEL_MASS = 4 };
+ *           its springs (both endpoints evaluate a spring);
+ *   MASS_GATHER  robots resident in LDS, one lane per mass; a spring is
+ *           evaluated by its first endpoint and its second endpoint reads
+ *           the stored term. */
+enum { MH_KERNEL_AUTO = 0, MH_KERNEL_LDS = 1, MH_KERNEL_STREAM = 2, MH_KERNEL_MASS = 4,
+       MH_KERNEL_MASS_GATHER = 8 };
 enum { MH_FITNESS_REFERENCE = 0, MH_FITNESS_DISPLACEMENT = 1 };
 
 typedef struct {
@@ -285,7 +289,8 @@ int mh_calibrate_lds(int device, float* gbytes_per_s);
 int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms);
 /* FP32 VALU issue rate of the device in 10^9 wave64 instructions per second
  * (independent chains, 8 waves per SIMD; kind 0 v_fma_f32, kind 1
- * v_mul_f32): the peak for kernels bound by vector-ALU issue. */
+ * v_mul_f32, kind 2 v_fma_f32 with twice the chains): the peak for kernels
+ * bound by vector-ALU issue. */
 int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s);
 
 /* No reference counterpart: checks, bit for bit, the fast arithmetic the

@@ -46,21 +46,21 @@ __global__ __launch_bounds__(1024) void lds_read_bw(float* out, int iters) {
   if (acc.x == -1.f) out[blockIdx.x * 1024 + tid] = acc.x + acc.y + acc.z + acc.w;
 }
 
-// kind 0: v_fma_f32 chains; kind 1: v_mul_f32 chains (two sources)
+// kind 0: v_fma_f32 chains; kind 1: v_mul_f32 chains (two sources);
+// kind 2: v_fma_f32, twice the chains (more independent work per wave)
 template <int KIND>
 __global__ __launch_bounds__(256) void valu_rate(float* out, int iters, float a, float b) {
-  float x0 = threadIdx.x, x1 = x0 + 1.f, x2 = x0 + 2.f, x3 = x0 + 3.f;
-  float x4 = x0 + 4.f, x5 = x0 + 5.f, x6 = x0 + 6.f, x7 = x0 + 7.f;
+  constexpr int N = KIND == 2 ? 16 : 8;
+  float x[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = static_cast<float>(threadIdx.x) + static_cast<float>(i);
   for (int it = 0; it < iters; ++it) {
-    if (KIND == 0) {
-      x0 = fmaf(x0, a, b); x1 = fmaf(x1, a, b); x2 = fmaf(x2, a, b); x3 = fmaf(x3, a, b);
-      x4 = fmaf(x4, a, b); x5 = fmaf(x5, a, b); x6 = fmaf(x6, a, b); x7 = fmaf(x7, a, b);
-    } else {
-      x0 = x0 * a; x1 = x1 * a; x2 = x2 * a; x3 = x3 * a;
-      x4 = x4 * a; x5 = x5 * a; x6 = x6 * a; x7 = x7 * a;
-    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = (KIND == 1) ? x[i] * a : fmaf(x[i], a, b);
   }
-  const float s = ((x0 + x1) + (x2 + x3)) + ((x4 + x5) + (x6 + x7));
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < N; ++i) s = s + x[i];
   if (s == -1.f) out[blockIdx.x * 256 + threadIdx.x] = s;
 }
 
@@ -144,25 +144,25 @@ int mh_calibrate_lds(int device, float* gbytes_per_s) {
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return fail(e, "hipSetDevice");
   int cus = 0;
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   const int blocks = (cus > 0 ? cus : 256) * 8;
   const int iters = 4096;
   float* out = nullptr;
   if ((e = hipMalloc(&out, sizeof(float) * static_cast<size_t>(blocks) * 1024)) != hipSuccess)
     return fail(e, "hipMalloc");
   hipEvent_t a, b;
-  hipEventCreate(&a);
-  hipEventCreate(&b);
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
   hipLaunchKernelGGL(lds_read_bw, dim3(blocks), dim3(1024), 0, nullptr, out, 16);
-  hipEventRecord(a, nullptr);
+  (void)hipEventRecord(a, nullptr);
   hipLaunchKernelGGL(lds_read_bw, dim3(blocks), dim3(1024), 0, nullptr, out, iters);
-  hipEventRecord(b, nullptr);
+  (void)hipEventRecord(b, nullptr);
   e = hipEventSynchronize(b);
   float ms = 0.f;
-  hipEventElapsedTime(&ms, a, b);
-  hipEventDestroy(a);
-  hipEventDestroy(b);
-  hipFree(out);
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(out);
   if (e != hipSuccess) return fail(e, "lds calibration");
   const double bytes = static_cast<double>(blocks) * 1024.0 * iters * 4 * 16;
   *gbytes_per_s = static_cast<float>(bytes / (ms * 1e-3) / 1e9);
@@ -170,12 +170,12 @@ int mh_calibrate_lds(int device, float* gbytes_per_s) {
 }
 
 int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s) {
-  if (!giga_instr_per_s || kind < 0 || kind > 1) return MH_ERR_ARG;
-  auto kernel = kind == 0 ? valu_rate<0> : valu_rate<1>;
+  if (!giga_instr_per_s || kind < 0 || kind > 2) return MH_ERR_ARG;
+  auto kernel = kind == 0 ? valu_rate<0> : kind == 1 ? valu_rate<1> : valu_rate<2>;
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return fail(e, "hipSetDevice");
   int cus = 0;
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
   // 8 blocks of 4 waves per CU: 8 waves per SIMD
   const int blocks = (cus > 0 ? cus : 256) * 8;
   const int iters = 1 << 14;
@@ -183,20 +183,21 @@ int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s) {
   if ((e = hipMalloc(&out, sizeof(float) * static_cast<size_t>(blocks) * 256)) != hipSuccess)
     return fail(e, "hipMalloc");
   hipEvent_t a, b;
-  hipEventCreate(&a);
-  hipEventCreate(&b);
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
   hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, nullptr, out, 64, 0.999f, 1e-3f);
-  hipEventRecord(a, nullptr);
+  (void)hipEventRecord(a, nullptr);
   hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, nullptr, out, iters, 0.999f, 1e-3f);
-  hipEventRecord(b, nullptr);
+  (void)hipEventRecord(b, nullptr);
   e = hipEventSynchronize(b);
   float ms = 0.f;
-  hipEventElapsedTime(&ms, a, b);
-  hipEventDestroy(a);
-  hipEventDestroy(b);
-  hipFree(out);
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(out);
   if (e != hipSuccess) return fail(e, "valu calibration");
-  const double instr = static_cast<double>(blocks) * 4.0 * iters * 8.0;  // wave instructions
+  const double chains = kind == 2 ? 16.0 : 8.0;
+  const double instr = static_cast<double>(blocks) * 4.0 * iters * chains;  // wave instructions
   *giga_instr_per_s = static_cast<float>(instr / (ms * 1e-3) / 1e9);
   return MH_OK;
 }
@@ -209,26 +210,26 @@ int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms) {
   float4 *src = nullptr, *dst = nullptr;
   if ((e = hipMalloc(&src, n * 16)) != hipSuccess) return fail(e, "hipMalloc");
   if ((e = hipMalloc(&dst, n * 16)) != hipSuccess) {
-    hipFree(src);
+    (void)hipFree(src);
     return fail(e, "hipMalloc");
   }
-  hipMemset(src, 0, n * 16);
+  (void)hipMemset(src, 0, n * 16);
   hipEvent_t a, b;
-  hipEventCreate(&a);
-  hipEventCreate(&b);
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
   const int blocks = 256 * 16;
   hipLaunchKernelGGL(copy_f4, dim3(blocks), dim3(256), 0, nullptr, src, dst, n);
-  hipEventRecord(a, nullptr);
+  (void)hipEventRecord(a, nullptr);
   for (int r = 0; r < reps; r++)
     hipLaunchKernelGGL(copy_f4, dim3(blocks), dim3(256), 0, nullptr, src, dst, n);
-  hipEventRecord(b, nullptr);
+  (void)hipEventRecord(b, nullptr);
   e = hipEventSynchronize(b);
   float ms = 0.f;
-  hipEventElapsedTime(&ms, a, b);
-  hipEventDestroy(a);
-  hipEventDestroy(b);
-  hipFree(src);
-  hipFree(dst);
+  (void)hipEventElapsedTime(&ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(src);
+  (void)hipFree(dst);
   if (e != hipSuccess) return fail(e, "copy calibration");
   *avg_ms = ms / reps;
   return MH_OK;
@@ -246,7 +247,7 @@ int mh_check_arith(int device, int which, uint32_t begin, uint32_t count, uint64
   hipLaunchKernelGGL(check_arith, dim3(blocks), dim3(256), 0, nullptr, which, begin, count, out);
   std::vector<uint32_t> h(blocks);
   e = hipMemcpy(h.data(), out, sizeof(uint32_t) * blocks, hipMemcpyDeviceToHost);
-  hipFree(out);
+  (void)hipFree(out);
   if (e != hipSuccess) return fail(e, "arith check");
   uint64_t total = 0;
   for (uint32_t v : h) total += v;

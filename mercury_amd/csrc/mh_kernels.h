@@ -35,7 +35,8 @@ struct StepParams {
   const uint8_t* mcnt;       // mass-parallel kernel: record counts per mass (2 each)
   const float* lbtab;        // mass-parallel kernel: per-pack rest-length tables
   const uint16_t* ccsr;      // chunked kernel: per chunk, entries 3 * (spring - chunk start)
-  const uint16_t* crow;      // chunked kernel: per chunk, 2 n_mass + 1 row bounds
+  const uint16_t* crow;      // chunked kernel: per chunk, 2 n_mass + 1 row bounds (padded to quads)
+  const uint32_t* ctab;      // chunked kernel: per pack (at d.pad0), entry offset of each chunk + end
   float* scratch;            // streaming kernel: float4 per active spring
   const float* mat_k;        // spring constant
   const float* mat_kx;       // xpbd: 2 + ((1/k)/dt)/dt
@@ -58,7 +59,8 @@ struct StepParams {
 struct KernelVariant {
   int solver;     // MH_SOLVER_*
   int kind;       // MH_KERNEL_LDS / MH_KERNEL_STREAM / MH_KERNEL_MASS
-  int chunked;    // STREAM kind: 1 = spring terms in LDS chunks, 0 = in global memory
+  int chunked;    // STREAM kind: 1 = spring terms in LDS chunks, 0 = in global memory;
+                  // MASS kind: 1 = terms computed once and gathered (step_massg)
   int block;      // threads per workgroup
   int nspt;       // springs per thread held in registers (LDS kind); records
                   // per endpoint side (MASS kind)
@@ -77,10 +79,22 @@ const KernelVariant* kernel_variants(int* count);
 // masses across the LDS banks.
 __host__ __device__ inline int pos_slot(int i) { return i + (i >> 3); }
 __host__ __device__ inline int pos_slots(int n) { return n + (n >> 3) + 1; }
-// Mass-parallel kernel: distinct rest lengths per pack and materials (its
+// Mass-parallel kernels: distinct rest lengths per pack and materials (its
 // LDS tables hold 64 of each, the last one reserved for padding records)
 constexpr int kMassLbarMax = 63;
 constexpr int kMassMatMax = 63;
+// Their fixed LDS layout (byte offsets): rest lengths float[64] | stiffness
+// float[64] | actuation float[2][64] | actuation parameters float[3][64]
+// (step_mass; step_massg keeps a zero term at kMassZero there) | positions
+constexpr int kMassTab = 64;
+constexpr int kMassNull = kMassTab - 1;  // table entry of padding records
+constexpr int kMassLb = 0, kMassK = 256, kMassR = 512, kMassAct = 1024, kMassZero = 1792, kMassPos = 2048;
+// step_massg: byte offset of the term slots (after one position buffer)
+// and of the term of (lane, k)
+__host__ __device__ constexpr int massg_term_base(int block) { return kMassPos + 16 * (block + block / 8 + 1); }
+__host__ __device__ constexpr uint32_t massg_term_offset(int block, int maxp, uint32_t lane, uint32_t k) {
+  return static_cast<uint32_t>(massg_term_base(block)) + 12u * (static_cast<uint32_t>(maxp) * lane + k);
+}
 // Bytes of the LDS spring-term array (3 floats per slot, spring_cap + 2
 // slots), rounded up to 16 B so the float4 positions after it stay aligned.
 __host__ __device__ inline size_t lds_term_bytes(int spring_cap) {
@@ -89,6 +103,14 @@ __host__ __device__ inline size_t lds_term_bytes(int spring_cap) {
 size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat);
 size_t lds_bytes_stream(int mass_cap, int n_mat);
 size_t lds_bytes_mass(int block, int n_mat);
+// Chunked streaming kernel: per lane and chunk at most kChunkSpl springs,
+// kChunkEpl quads of gather entries and kChunkRpl quads of row bounds are in
+// flight; a robot has at most kChunkMax chunks.  Row bounds per chunk:
+// 2 n_mass + 1 entries padded to whole quads.
+constexpr int kChunkSpl = 8, kChunkEpl = 8, kChunkRpl = 2, kChunkMax = 256;
+__host__ __device__ constexpr int chunk_row_quads(int n_mass) { return (2 * n_mass + 1 + 3) / 4; }
+size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat);
+size_t lds_bytes_massg(int block, int maxp);
 
 hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes,
                        hipStream_t stream, const StepParams& p);

@@ -111,7 +111,7 @@ struct mh_sim {
   mh::DevBuf<float> lbar, stress, scratch, com;
   mh::DevBuf<uint8_t> smat, mass_id;
   mh::DevBuf<uint16_t> mperm, ccsr, crow;
-  mh::DevBuf<uint32_t> mrec;
+  mh::DevBuf<uint32_t> mrec, ctab;
   mh::DevBuf<uint8_t> mcnt;
   mh::DevBuf<float> lbtab;
   mh::DevBuf<uint32_t> spair, csr, row, robot_begin;
@@ -191,24 +191,38 @@ bool fits_mass(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
   // a robot with springs needs its rest lengths and materials in the
   // tables (an empty set marks one with too many distinct values)
   const bool table_ok = robot_active(s, r) == 0 || (!s->lbar_set[r].empty() && !s->mat_set[r].empty());
-  return m <= v.block && s->max_side[r] <= v.nspt && table_ok &&
-         lds_bytes_mass(v.block, n_mat) <= static_cast<size_t>(s->lds_limit);
+  const size_t lds = v.chunked ? lds_bytes_massg(v.block, v.nspt) : lds_bytes_mass(v.block, n_mat);
+  return m <= v.block && s->max_side[r] <= v.nspt && table_ok && lds <= static_cast<size_t>(s->lds_limit);
 }
+
+// MH_KERNEL_* family bit of a variant (the gather flavour of the
+// mass-parallel kernel is a family of its own).
+int family(const KernelVariant& v) {
+  return v.kind == MH_KERNEL_MASS && v.chunked ? MH_KERNEL_MASS_GATHER : v.kind;
+}
+constexpr int kResident = MH_KERNEL_LDS | MH_KERNEL_MASS | MH_KERNEL_MASS_GATHER;
+constexpr int kAllFamilies = kResident | MH_KERNEL_STREAM;
 
 bool fits_resident(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
   return v.kind == MH_KERNEL_MASS ? fits_mass(s, v, r, n_mat) : fits_lds_alone(s, v, r, n_mat);
 }
+
+// Gather-entry capacity of a chunk of C springs for a robot of m masses: 2C
+// entries plus padding of up to 3 per row part, in whole quads.
+int chunk_csr_cap(int C, int m) { return (2 * C + 6 * m + 3) & ~3; }
 
 // Springs per chunk for the chunked kernel and a robot of m masses: the
 // largest candidate whose LDS layout (chunk terms, skewed positions, one
 // chunk's gather entries -- at most 2C plus padding of 3 per row part --
 // and row bounds) fits; 0 if none does.
 int chunk_springs(const mh_sim* s, const KernelVariant& v, int m, int n_mat) {
-  if (m > v.block * v.mpt) return 0;
+  if (m > v.block * v.mpt || chunk_row_quads(m) > kChunkRpl * v.block) return 0;
   for (int C : {8192, 6144, 5120, 4096, 3072, 2048, 1536, 1024, 512}) {
-    const int entries = 2 * C + 6 * m;
+    const int entries = chunk_csr_cap(C, m);
     if (entries > 65535 || 3 * (C + 1) > 65535) continue;
-    if (lds_bytes_lds(std::max(m, 1), C, entries, n_mat) <= static_cast<size_t>(s->lds_limit)) return C;
+    // what one chunk puts in flight per lane
+    if (C > kChunkSpl * v.block || entries / 4 > kChunkEpl * v.block) continue;
+    if (lds_bytes_chunk(std::max(m, 1), C, entries, n_mat) <= static_cast<size_t>(s->lds_limit)) return C;
   }
   return 0;
 }
@@ -307,7 +321,7 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
   }
   if (!lp.packs.empty()) {
     lp.lds = lv->kind == MH_KERNEL_MASS
-                 ? lds_bytes_mass(lv->block, n_mat)
+                 ? (lv->chunked ? lds_bytes_massg(lv->block, lv->nspt) : lds_bytes_mass(lv->block, n_mat))
                  : lds_bytes_lds(std::max(lp.mass_cap, 1), std::max(lp.spring_cap, 1), lp.csr_cap, n_mat);
     plan.parts.push_back(std::move(lp));
   }
@@ -315,8 +329,10 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
     if (sv->chunked) {
       // one chunk size for the part: the one that fits its largest robot
       stp.spring_cap = chunk_springs(s, *sv, std::max(stp.mass_cap, 1), n_mat);
-      stp.csr_cap = 2 * stp.spring_cap + 6 * std::max(stp.mass_cap, 1);
-      stp.lds = lds_bytes_lds(std::max(stp.mass_cap, 1), stp.spring_cap, stp.csr_cap, n_mat);
+      stp.csr_cap = chunk_csr_cap(stp.spring_cap, std::max(stp.mass_cap, 1));
+      stp.lds = lds_bytes_chunk(std::max(stp.mass_cap, 1), stp.spring_cap, stp.csr_cap, n_mat);
+      for (const PackDesc& d : stp.packs)
+        if ((d.n_spring + stp.spring_cap - 1) / stp.spring_cap > static_cast<uint32_t>(kChunkMax)) return false;
     } else {
       stp.lds = lds_bytes_stream(std::max(stp.mass_cap, 1), n_mat);
     }
@@ -327,10 +343,11 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
 
 // Chunked part (one robot per pack): skewed spring pairs, and per chunk of
 // part.spring_cap springs the u16 gather entries 3 * (spring - chunk start)
-// and 2 n_mass + 1 chunk-relative row bounds, each row part in ascending
-// spring index padded to 4 with the zero slot.
+// and 2 n_mass + 1 chunk-relative row bounds (padded to whole quads), each
+// row part in ascending spring index padded to 4 with the zero slot; per
+// pack (d.pad0) the entry offset of every chunk and the end.
 void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::vector<uint16_t>& ccsr,
-                    std::vector<uint16_t>& crow) {
+                    std::vector<uint16_t>& crow, std::vector<uint32_t>& ctab) {
   const uint32_t C = static_cast<uint32_t>(part.spring_cap);
   const uint16_t zero_entry = static_cast<uint16_t>(3u * (C + 1));
   auto pad = [](uint32_t n) { return (n + 3u) & ~3u; };
@@ -349,7 +366,10 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
                  (static_cast<uint32_t>(pos_slot(static_cast<int>(b))) << 16);
     }
     const uint32_t n_chunks = ns > 0 ? (ns + C - 1) / C : 1;
+    d.pad0 = static_cast<uint32_t>(ctab.size());
+    d.pad1 = n_chunks;
     for (uint32_t c = 0; c < n_chunks; c++) {
+      ctab.push_back(static_cast<uint32_t>(ccsr.size()) - d.csr_begin);
       const uint32_t s0 = c * C, s1 = std::min(ns, s0 + C);
       count.assign(2 * nm + 1, 0);
       for (uint32_t k = s0; k < s1; k++) {
@@ -375,7 +395,10 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
         ccsr[base + fill_minus[lpair[2 * k + 1]]++] = static_cast<uint16_t>(3u * (k - s0));
       }
       for (uint32_t i = 0; i <= 2 * nm; i++) crow.push_back(static_cast<uint16_t>(count[i]));
+      for (uint32_t i = 2 * nm + 1; i < 4u * static_cast<uint32_t>(chunk_row_quads(static_cast<int>(nm))); i++)
+        crow.push_back(static_cast<uint16_t>(count[2 * nm]));
     }
+    ctab.push_back(static_cast<uint32_t>(ccsr.size()) - d.csr_begin);
   }
   std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
     return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
@@ -390,10 +413,13 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
 // out [part][k][lane] (part 2: pack-local spring index of the part-0
 // records), with the two counts per lane.  A record holds LDS byte offsets:
 // the other endpoint's skewed slot * 16 | pack material index * 4 << 16 |
-// rest-length index * 4 << 24.
+// rest-length index * 4 << 24.  For the gather kernel a second-endpoint
+// record is instead the LDS offset of the term slot its first endpoint
+// fills.
 void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector<uint32_t>& mrec,
                  std::vector<uint8_t>& mcnt, std::vector<float>& lbtab) {
   const int P = part.v->nspt;
+  const bool gather = part.v->chunked != 0;
   std::vector<uint32_t> table, mtable, merged, fill_p, fill_m, order, lane;
   for (PackDesc& d : part.packs) {
     const uint32_t nm = d.n_mass, ns = d.n_spring;
@@ -471,9 +497,14 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
       const uint32_t mi = static_cast<uint32_t>(
           std::lower_bound(mtable.begin(), mtable.end(), static_cast<uint32_t>(s->act_mat[g])) - mtable.begin());
       const uint32_t tail = ((4u * mi) << 16) | ((4u * li) << 24);
-      out[static_cast<size_t>(2 * P + fill_p[a]) * nm + la] = k;
-      out[static_cast<size_t>(fill_p[a]++) * nm + la] = 16u * static_cast<uint32_t>(pos_slot(static_cast<int>(b))) | tail;
-      out[static_cast<size_t>(P + fill_m[b]++) * nm + lb] = 16u * static_cast<uint32_t>(pos_slot(static_cast<int>(a))) | tail;
+      const uint32_t ka = fill_p[a]++;
+      out[static_cast<size_t>(2 * P + ka) * nm + la] = k;
+      out[static_cast<size_t>(ka) * nm + la] = 16u * static_cast<uint32_t>(pos_slot(static_cast<int>(b))) | tail;
+      // second endpoint: the same spring's record, or (gather kernel) the
+      // LDS offset of the term its first endpoint stores
+      out[static_cast<size_t>(P + fill_m[b]++) * nm + lb] =
+          gather ? massg_term_offset(part.v->block, P, la, ka)
+                 : (16u * static_cast<uint32_t>(pos_slot(static_cast<int>(a))) | tail);
     }
     for (uint32_t i = 0; i < nm; i++) {
       mcnt.push_back(static_cast<uint8_t>(fill_p[order[i]]));
@@ -501,6 +532,7 @@ void upload_plan(mh_sim* s, Plan& plan) {
   row.reserve(2 * s->n_masses + 64);
   bool need_scratch = false;
   std::vector<uint16_t> ccsr, crow;
+  std::vector<uint32_t> ctab;
   std::vector<uint32_t> mrec;
   std::vector<uint8_t> mcnt;
   std::vector<float> lbtab;
@@ -510,7 +542,7 @@ void upload_plan(mh_sim* s, Plan& plan) {
     if (part.v->kind == MH_KERNEL_MASS) {
       upload_mass(s, part, perm, mrec, mcnt, lbtab);
     } else if (part.v->chunked) {
-      upload_chunked(s, part, spair, ccsr, crow);
+      upload_chunked(s, part, spair, ccsr, crow, ctab);
     } else {
       const uint32_t zero_entry = 3u * static_cast<uint32_t>(part.spring_cap + 1);
       const uint32_t scale = padded ? 3u : 1u;
@@ -585,6 +617,7 @@ void upload_plan(mh_sim* s, Plan& plan) {
   s->mperm.upload(perm.data(), perm.size(), s->stream);
   s->ccsr.upload(ccsr.data(), ccsr.size(), s->stream);
   s->crow.upload(crow.data(), crow.size(), s->stream);
+  s->ctab.upload(ctab.data(), ctab.size(), s->stream);
   s->mrec.upload(mrec.data(), mrec.size(), s->stream);
   s->mcnt.upload(mcnt.data(), mcnt.size(), s->stream);
   s->lbtab.upload(lbtab.data(), lbtab.size(), s->stream);
@@ -599,12 +632,11 @@ void upload_plan(mh_sim* s, Plan& plan) {
 std::vector<const KernelVariant*> candidates(const mh_sim* s, int kinds) {
   int nv = 0;
   const KernelVariant* all = kernel_variants(&nv);
-  const int allowed = s->cfg.kernel == MH_KERNEL_AUTO ? (MH_KERNEL_LDS | MH_KERNEL_STREAM | MH_KERNEL_MASS)
-                                                      : s->cfg.kernel;
+  const int allowed = s->cfg.kernel == MH_KERNEL_AUTO ? kAllFamilies : s->cfg.kernel;
   std::vector<const KernelVariant*> out;
   for (int i = 0; i < nv; i++) {
     const KernelVariant& v = all[i];
-    if (v.solver != s->cfg.solver || !(v.kind & kinds & allowed)) continue;
+    if (v.solver != s->cfg.solver || !(family(v) & kinds & allowed)) continue;
     if (s->cfg.block_size && v.block != s->cfg.block_size) continue;
     out.push_back(&v);
   }
@@ -615,7 +647,7 @@ std::vector<const KernelVariant*> candidates(const mh_sim* s, int kinds) {
 // takes the most robots, and for the rest the first streaming variant
 // (chunked ones first) that fits them.
 void choose_default_plan(mh_sim* s) {
-  auto lc = candidates(s, MH_KERNEL_LDS | MH_KERNEL_MASS);
+  auto lc = candidates(s, kResident);
   auto sc = candidates(s, MH_KERNEL_STREAM);
   const KernelVariant* sv = sc.empty() ? nullptr : sc.front();
   const int n_mat = static_cast<int>(s->compact_to_id.size());
@@ -650,6 +682,7 @@ StepParams make_params(mh_sim* s, const Part& part, float t0, int n_steps, int t
   p.mperm = s->mperm.p;
   p.ccsr = s->ccsr.p;
   p.crow = s->crow.p;
+  p.ctab = s->ctab.p;
   p.mrec = s->mrec.p;
   p.mcnt = s->mcnt.p;
   p.lbtab = s->lbtab.p;
@@ -817,7 +850,7 @@ int mh_sim_create(const mh_config* cfg, const mh_material_set* set, mh_sim** out
   if (cfg->block_size && cfg->block_size != 256 && cfg->block_size != 512 && cfg->block_size != 768 &&
       cfg->block_size != 1024)
     throw mh::Error(MH_ERR_ARG, "block_size must be 0, 256, 512, 768 or 1024");
-  if (cfg->kernel < MH_KERNEL_AUTO || cfg->kernel > (MH_KERNEL_LDS | MH_KERNEL_STREAM | MH_KERNEL_MASS))
+  if (cfg->kernel < MH_KERNEL_AUTO || cfg->kernel > mh::kAllFamilies)
     throw mh::Error(MH_ERR_ARG, "unknown kernel kind");
   int ndev = 0;
   mh::hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
@@ -1131,7 +1164,7 @@ int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
   info->n_springs = static_cast<int>(s->n_springs);
   info->n_active_springs = static_cast<int>(s->n_active);
   for (const mh::Part& part : s->plan.parts) {
-    info->kernel |= part.v->kind;
+    info->kernel |= mh::family(*part.v);
     info->n_blocks += static_cast<int>(part.packs.size());
     if (part.v->kind != MH_KERNEL_STREAM) {
       info->block_size = part.v->block;
@@ -1241,7 +1274,7 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   s->cfg.use_graph = 0;
   const int saved_sub = s->substeps;
   s->substeps = 0;
-  auto lc = mh::candidates(s, MH_KERNEL_LDS | MH_KERNEL_MASS);
+  auto lc = mh::candidates(s, mh::kResident);
   auto sc = mh::candidates(s, MH_KERNEL_STREAM);
   lc.push_back(nullptr);  // everything streamed
   // streaming variant for the LDS search: the first that can take every robot

@@ -390,13 +390,18 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
 // For robots whose springs do not fit in LDS at once.  Positions stay in LDS
 // (skewed slots, as in step_lds) and each lane keeps MPT masses in
 // registers; every substep walks the robot's springs in chunks of
-// p.spring_cap: phase 1 streams the chunk's spring records from global
-// memory, writes the terms to LDS and copies the chunk's gather lists and
-// row bounds (u16, prepared on the host) into LDS; after a barrier each
-// lane adds the chunk's first-endpoint terms to P and second-endpoint terms
-// to M of its masses, in ascending spring order; a barrier ends the chunk.
-// After the last chunk the masses integrate with P - M.  LDS layout as in
-// step_lds with spring_cap = springs per chunk.
+// p.spring_cap.  A chunk's inputs -- its spring records, its u16 gather
+// lists and its row bounds (both prepared on the host, 8-byte aligned) --
+// are loaded into registers one chunk ahead, so the global-memory latency
+// of chunk c + 1 overlaps chunk c's work.  Phase 1 copies the chunk's gather
+// lists and row bounds into LDS and writes its spring terms to LDS; after a
+// barrier, phase 2 adds the chunk's first-endpoint terms to P and
+// second-endpoint terms to M of each lane's masses, in ascending spring
+// order; a barrier ends the chunk.  After the last chunk the masses
+// integrate with P - M.  LDS: terms | positions | gather entries
+// u16[csr_cap] | row bounds u16[4 chunk_row_quads(mass_cap)] | actuation
+// float[n_mat] | stiffness float[n_mat] | chunk entry offsets
+// u32[kChunkMax + 1] (from p.ctab at d.pad0).
 template <int SOLVER, int BLOCK, int MPT>
 __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -406,11 +411,14 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
   const int C = p.spring_cap;
   const int n_chunks = ns > 0 ? (ns + C - 1) / C : 1;
   const int zero = C + 1;
+  const int rq = chunk_row_quads(nm);
   float* s_term = reinterpret_cast<float*>(smem);
   float4* s_pos = reinterpret_cast<float4*>(smem + lds_term_bytes(C));
   uint16_t* s_csr = reinterpret_cast<uint16_t*>(s_pos + pos_slots(p.mass_cap));
   uint16_t* s_row = s_csr + p.csr_cap;
-  float* s_R = reinterpret_cast<float*>(s_row + 2 * p.mass_cap + 2);
+  float* s_R = reinterpret_cast<float*>(s_row + 4 * chunk_row_quads(p.mass_cap));
+  float* s_K = s_R + p.n_mat;
+  uint32_t* s_eoff = reinterpret_cast<uint32_t*>(s_K + p.n_mat);
   const int tid = threadIdx.x;
   const uint32_t* spair = p.spair + d.spring_begin;
   const float* slbar = p.slbar + d.spring_begin;
@@ -418,6 +426,8 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
   const uint16_t* ccsr = p.ccsr + d.csr_begin;
   const uint16_t* crow = p.crow + d.row_begin;
   if (tid < 3) s_term[3 * zero + tid] = 0.f;
+  for (int i = tid; i <= n_chunks; i += BLOCK) s_eoff[i] = p.ctab[d.pad0 + i];
+  for (int i = tid; i < p.n_mat; i += BLOCK) s_K[i] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[i] : p.mat_k[i];
 
   MassState ms[MPT];
   int pslot[MPT];
@@ -434,6 +444,38 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
   if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], t);
   __syncthreads();
 
+  // the next chunk's inputs, in flight while the current one is worked on
+  uint32_t pr[kChunkSpl], mt[kChunkSpl];
+  float lbv[kChunkSpl];
+  uint2 ce[kChunkEpl], rw[kChunkRpl];
+  auto fetch = [&](int c) {
+    const int s0 = c * C, s1 = min(ns, s0 + C);
+#pragma unroll
+    for (int j = 0; j < kChunkSpl; ++j) {
+      const int s = s0 + tid + j * BLOCK;
+      if (s < s1) {
+        pr[j] = spair[s];
+        lbv[j] = slbar[s];
+        mt[j] = smat[s];
+      }
+    }
+    const int e0 = static_cast<int>(s_eoff[c]);
+    const int ne4 = (static_cast<int>(s_eoff[c + 1]) - e0) >> 2;
+    const uint2* src = reinterpret_cast<const uint2*>(ccsr + e0);
+#pragma unroll
+    for (int j = 0; j < kChunkEpl; ++j) {
+      const int i = tid + j * BLOCK;
+      if (i < ne4) ce[j] = src[i];
+    }
+    const uint2* rsrc = reinterpret_cast<const uint2*>(crow + static_cast<size_t>(4 * rq) * c);
+#pragma unroll
+    for (int j = 0; j < kChunkRpl; ++j) {
+      const int i = tid + j * BLOCK;
+      if (i < rq) rw[j] = rsrc[i];
+    }
+  };
+  fetch(0);
+
   for (int step = 0; step < p.n_steps; ++step) {
     float P[MPT][3], M[MPT][3];
 #pragma unroll
@@ -441,31 +483,42 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
       P[j][0] = P[j][1] = P[j][2] = 0.0f;
       M[j][0] = M[j][1] = M[j][2] = 0.0f;
     }
-    const uint16_t* rc = crow;  // this chunk's 2 nm + 1 row bounds
-    int e_base = 0;             // this chunk's first entry in ccsr
     const float tn = t + p.dt;
     for (int c = 0; c < n_chunks; ++c) {
       const int s0 = c * C;
       const int s1 = min(ns, s0 + C);
-      for (int s = s0 + tid; s < s1; s += BLOCK) {
-        const uint32_t pr = spair[s];
-        const int m = smat[s];
-        const float kk = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[m] : p.mat_k[m];
-        float cx, cy, cz, lambda;
-        spring_term<SOLVER>(s_pos[pr & 0xffffu], s_pos[pr >> 16], slbar[s], s_R[m], kk, cx, cy,
-                            cz, lambda);
-        if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
-          float* stp = p.sstress + d.spring_begin + s;
-          *stp = *stp + lambda / slbar[s];
-        }
-        const int o = 3 * (s - s0);
-        s_term[o] = cx;
-        s_term[o + 1] = cy;
-        s_term[o + 2] = cz;
+      const int ne4 = (static_cast<int>(s_eoff[c + 1]) - static_cast<int>(s_eoff[c])) >> 2;
+#pragma unroll
+      for (int j = 0; j < kChunkEpl; ++j) {
+        const int i = tid + j * BLOCK;
+        if (i < ne4) reinterpret_cast<uint2*>(s_csr)[i] = ce[j];
       }
-      const int n_e = rc[2 * nm];
-      for (int i = tid; i < n_e; i += BLOCK) s_csr[i] = ccsr[e_base + i];
-      for (int i = tid; i <= 2 * nm; i += BLOCK) s_row[i] = rc[i];
+#pragma unroll
+      for (int j = 0; j < kChunkRpl; ++j) {
+        const int i = tid + j * BLOCK;
+        if (i < rq) reinterpret_cast<uint2*>(s_row)[i] = rw[j];
+      }
+#pragma unroll
+      for (int j = 0; j < kChunkSpl; ++j) {
+        const int s = s0 + tid + j * BLOCK;
+        if (s < s1) {
+          const int m = static_cast<int>(mt[j]);
+          float cx, cy, cz, lambda;
+          spring_term<SOLVER>(s_pos[pr[j] & 0xffffu], s_pos[pr[j] >> 16], lbv[j], s_R[m], s_K[m], cx, cy,
+                              cz, lambda);
+          if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
+            float* stp = p.sstress + d.spring_begin + s;
+            *stp = *stp + lambda / lbv[j];
+          }
+          const int o = 3 * (s - s0);
+          s_term[o] = cx;
+          s_term[o + 1] = cy;
+          s_term[o + 2] = cz;
+        }
+      }
+      // the following chunk (the first one again after the last: the next
+      // substep starts with it)
+      fetch(c + 1 < n_chunks ? c + 1 : 0);
       __syncthreads();
       const bool last = c == n_chunks - 1;
 #pragma unroll
@@ -508,8 +561,6 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
       }
       if (last && tid < p.n_mat)
         s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
-      e_base += n_e;
-      rc += 2 * nm + 1;
       __syncthreads();
     }
     t = tn;
@@ -546,10 +597,6 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
 // live in p.mrec as [part][k][lane] (part 2: the pack-local spring index of
 // the part-0 records, for the stress output); p.mcnt holds the two counts
 // per lane.
-constexpr int kMassTab = 64;               // rest lengths, materials per table
-constexpr int kMassNull = kMassTab - 1;    // table entry of padding records
-constexpr int kMassLb = 0, kMassK = 256, kMassR = 512, kMassAct = 1024, kMassPos = 2048;
-
 template <int BLOCK>
 constexpr int mass_pos_bytes() { return 16 * (BLOCK + BLOCK / 8 + 1); }
 
@@ -718,6 +765,136 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   }
 }
 
+// ------------------------------------------------ mass-parallel gather kernel
+// As step_mass, but each spring is evaluated once, by its first endpoint.
+// Phase 1: a lane computes its first-endpoint terms, adds them to P and
+// stores each in its own LDS term slots (12 B per (lane, k), lane-major).
+// Barrier.  Phase 2: the lane adds the terms of the springs whose second
+// endpoint it is to M, reading them from their first endpoints' slots,
+// integrates with P - M and writes its position in place; the next
+// actuation table is written in place too.  Barrier.  Two barriers per
+// substep, half the spring arithmetic of step_mass.  LDS: the tables of
+// step_mass with one actuation table and a zero term at kMassZero |
+// positions float4[pos_slots(BLOCK)] | terms.  First-endpoint records as in
+// step_mass; a second-endpoint record is the LDS byte offset of its term
+// (padding: the zero term).
+template <int SOLVER, int BLOCK, int MAXP, bool STRESS>
+__global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PackDesc d = p.packs[blockIdx.x];
+  const int nm = static_cast<int>(d.n_mass);
+  constexpr int kSlots = BLOCK + BLOCK / 8 + 1;
+  constexpr int kGhost = kSlots - 1;
+  constexpr int kTerm = massg_term_base(BLOCK);
+  const int tid = threadIdx.x;
+  const bool live = tid < nm;
+  const uint32_t* rec = p.mrec + d.csr_begin;
+
+  MassState ms;
+  uint32_t rp[MAXP], rm[MAXP];
+  float st[MAXP];
+  int np = 0, nq = 0, mass = 0;
+  const uint32_t pad = 16u * kGhost | (4u * kMassNull) << 16 | (4u * kMassNull) << 24;
+#pragma unroll
+  for (int k = 0; k < MAXP; ++k) {
+    rp[k] = pad;
+    rm[k] = kMassZero;
+    st[k] = 0.f;
+  }
+  if (live) {
+    mass = p.mperm[d.mass_begin + tid];
+    load_mass(ms, p.pos, p.vel, d.mass_begin + mass, p);
+    np = p.mcnt[d.row_begin + 2 * tid];
+    nq = p.mcnt[d.row_begin + 2 * tid + 1];
+#pragma unroll
+    for (int k = 0; k < MAXP; ++k) {
+      if (k < np) rp[k] = rec[static_cast<size_t>(k) * nm + tid];
+      if (k < nq) rm[k] = rec[static_cast<size_t>(MAXP + k) * nm + tid];
+      if (STRESS && k < np) st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]];
+    }
+  }
+  const int pslot = live ? pos_slot(mass) : kGhost;
+  float4* pos = reinterpret_cast<float4*>(smem + kMassPos);
+  if (live) pos[pslot] = lds_position<SOLVER>(ms);
+  float* s_lb = reinterpret_cast<float*>(smem + kMassLb);
+  float* s_k = reinterpret_cast<float*>(smem + kMassK);
+  float* s_R = reinterpret_cast<float*>(smem + kMassR);
+  const int n_lb = static_cast<int>(d.pad1 & 0xffffu), n_pm = static_cast<int>(d.pad1 >> 16);
+  for (int i = tid; i < n_lb; i += BLOCK) s_lb[i] = p.lbtab[d.pad0 + i];
+  float t = p.t0;
+  float a_dl = 0.f, a_om = 0.f, a_ph = 0.f;  // lanes < n_pm: their material's actuation
+  if (tid < n_pm) {  // n_pm < kMassTab <= BLOCK
+    const uint32_t c = __float_as_uint(p.lbtab[d.pad0 + n_lb + tid]);
+    s_k[tid] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[c] : p.mat_k[c];
+    a_dl = p.mat_dL0[c];
+    a_om = p.mat_omega[c];
+    a_ph = p.mat_phi[c];
+    s_R[tid] = actuation(a_dl, a_om, a_ph, t);
+  }
+  if (tid == 0) {
+    pos[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
+    s_lb[kMassNull] = 1.0f;
+    s_k[kMassNull] = (SOLVER == MH_SOLVER_XPBD_REF) ? __builtin_inff() : 0.0f;
+    s_R[kMassNull] = 0.0f;
+    float* z = reinterpret_cast<float*>(smem + kMassZero);
+    z[0] = z[1] = z[2] = 0.0f;
+  }
+  int np_w = np, nq_w = nq;
+  for (int off = 32; off > 0; off >>= 1) {
+    np_w = max(np_w, __shfl_xor(np_w, off));
+    nq_w = max(nq_w, __shfl_xor(nq_w, off));
+  }
+  unsigned char* mine = smem + kTerm + 12 * MAXP * tid;
+  __syncthreads();
+
+  for (int step = 0; step < p.n_steps; ++step) {
+    const float4 own = live ? pos[pslot] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll
+    for (int k = 0; k < MAXP; ++k) {
+      if (k < np_w) {
+        const uint32_t r = rp[k];
+        const uint32_t mo = (r >> 16) & 0xffu;
+        const float lb = *reinterpret_cast<const float*>(smem + kMassLb + (r >> 24));
+        const float4 other = *reinterpret_cast<const float4*>(smem + kMassPos + (r & 0xffffu));
+        float cx, cy, cz, lambda;
+        spring_term<SOLVER>(own, other, lb, *reinterpret_cast<const float*>(smem + kMassR + mo),
+                            *reinterpret_cast<const float*>(smem + kMassK + mo), cx, cy, cz, lambda);
+        ax = ax + cx; ay = ay + cy; az = az + cz;
+        float* slot = reinterpret_cast<float*>(mine + 12 * k);
+        slot[0] = cx;
+        slot[1] = cy;
+        slot[2] = cz;
+        if (STRESS && k < np) st[k] = st[k] + lambda / lb;
+      }
+    }
+    __syncthreads();
+    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+#pragma unroll
+    for (int k = 0; k < MAXP; ++k) {
+      if (k < nq_w) {
+        const float* q = reinterpret_cast<const float*>(smem + rm[k]);
+        bx = bx + q[0]; by = by + q[1]; bz = bz + q[2];
+      }
+    }
+    if (live) pos[pslot] = integrate<SOLVER>(ms, ax - bx, ay - by, az - bz, p);
+    if (tid < n_pm) s_R[tid] = actuation(a_dl, a_om, a_ph, t + p.dt);
+    t = t + p.dt;
+    __syncthreads();
+  }
+
+  if (live) {
+    const uint32_t gi = d.mass_begin + mass;
+    p.pos[gi] = make_float4(ms.x, ms.y, ms.z, ms.m);
+    p.vel[gi] = make_float4(ms.vx, ms.vy, ms.vz, 0.f);
+    if (STRESS) {
+#pragma unroll
+      for (int k = 0; k < MAXP; ++k)
+        if (k < np) p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]] = st[k];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ COM
 __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint32_t* begin,
                           int n_robots, float* out) {
@@ -743,16 +920,20 @@ __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint3
 #define MH_MASS(SOLVER, B, P)                                                         \
   {SOLVER, MH_KERNEL_MASS, 0, B, P, 1, reinterpret_cast<const void*>(&step_mass<SOLVER, B, P, false>), \
    reinterpret_cast<const void*>(&step_mass<SOLVER, B, P, (SOLVER == MH_SOLVER_XPBD_REF)>)}
+#define MH_MASSG(SOLVER, B, P)                                                        \
+  {SOLVER, MH_KERNEL_MASS, 1, B, P, 1, reinterpret_cast<const void*>(&step_massg<SOLVER, B, P, false>), \
+   reinterpret_cast<const void*>(&step_massg<SOLVER, B, P, (SOLVER == MH_SOLVER_XPBD_REF)>)}
 #define MH_CHUNK(SOLVER, B, M) \
   {SOLVER, MH_KERNEL_STREAM, 1, B, 0, M, reinterpret_cast<const void*>(&step_chunk<SOLVER, B, M>)}
 #define MH_STREAM(SOLVER, B) \
   {SOLVER, MH_KERNEL_STREAM, 0, B, 0, 0, reinterpret_cast<const void*>(&step_stream<SOLVER, B>)}
 #define MH_LDS_SET(SOLVER)                                                         \
-  MH_MASS(SOLVER, 768, 13), MH_MASS(SOLVER, 1024, 13), MH_MASS(SOLVER, 512, 16), \
+  MH_MASSG(SOLVER, 768, 13), MH_MASSG(SOLVER, 512, 16), MH_MASSG(SOLVER, 256, 16),     \
+      MH_MASS(SOLVER, 768, 13), MH_MASS(SOLVER, 1024, 13), MH_MASS(SOLVER, 512, 16),   \
       MH_MASS(SOLVER, 256, 16), MH_LDS(SOLVER, 1024, 8, 1), MH_LDS(SOLVER, 1024, 8, 2), MH_LDS(SOLVER, 1024, 4, 1), \
       MH_LDS(SOLVER, 512, 8, 2), MH_LDS(SOLVER, 512, 16, 2), \
       MH_LDS(SOLVER, 512, 16, 4), MH_LDS(SOLVER, 256, 16, 4), MH_LDS(SOLVER, 256, 8, 2), \
-      MH_CHUNK(SOLVER, 1024, 2), MH_CHUNK(SOLVER, 1024, 4), MH_CHUNK(SOLVER, 512, 4), \
+      MH_CHUNK(SOLVER, 1024, 2), MH_CHUNK(SOLVER, 512, 4),                            \
       MH_STREAM(SOLVER, 1024), MH_STREAM(SOLVER, 512), MH_STREAM(SOLVER, 256)
 
 const KernelVariant kVariants[] = {MH_LDS_SET(MH_SOLVER_XPBD_REF), MH_LDS_SET(MH_SOLVER_FORCE)};
@@ -778,6 +959,21 @@ size_t lds_bytes_mass(int block, int n_mat) {
   // parameters) and two position buffers sized for the block
   (void)n_mat;
   return static_cast<size_t>(kMassPos) + 2u * 16u * static_cast<size_t>(block + block / 8 + 1);
+}
+
+size_t lds_bytes_massg(int block, int maxp) {
+  // tables, one position buffer, 12 B of terms per (lane, record)
+  return static_cast<size_t>(massg_term_base(block)) + 12u * static_cast<size_t>(maxp) * block;
+}
+
+size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat) {
+  size_t b = lds_term_bytes(spring_cap);                          // spring terms, sink, zero
+  b += 16u * static_cast<size_t>(pos_slots(mass_cap));            // float4 positions (skewed)
+  b += 2u * static_cast<size_t>(csr_cap);                         // gather entries (u16)
+  b += 8u * static_cast<size_t>(chunk_row_quads(mass_cap));       // row bounds
+  b += 8u * static_cast<size_t>(n_mat);                           // actuation, stiffness
+  b += 4u * static_cast<size_t>(kChunkMax + 1);                   // chunk entry offsets
+  return b;
 }
 
 size_t lds_bytes_stream(int mass_cap, int n_mat) {
