@@ -180,6 +180,11 @@ int mh_sim_fitness(mh_sim* sim, int mode, float* out);
 int mh_sim_com(mh_sim* sim, float* out);
 int mh_sim_reset(mh_sim* sim);               /* simulated time := 0 */
 float mh_sim_total_time(const mh_sim* sim);
+/* No reference counterpart: sets the simulated time (the actuation clock),
+ * which mh_sim_set_population resets to 0.  Devo cycles use it to continue
+ * the clock across a population re-upload, as the reference's in-place
+ * spring replacement (Simulator::Devo, Simulator.cu:640-668) does. */
+int mh_sim_set_total_time(mh_sim* sim, float t);
 float mh_sim_get_time_step(const mh_sim* sim);
 int mh_sim_set_time_step(mh_sim* sim, float dt);
 /* Population counts and the launch plan chosen for it.  kernel is the bit

@@ -84,6 +84,7 @@ _SIGS = {
     "mh_sim_com": (c_int, [vp, f32p]),
     "mh_sim_reset": (c_int, [vp]),
     "mh_sim_total_time": (c_float, [vp]),
+    "mh_sim_set_total_time": (c_int, [vp, c_float]),
     "mh_sim_get_time_step": (c_float, [vp]),
     "mh_sim_set_time_step": (c_int, [vp, c_float]),
     "mh_sim_get_info": (c_int, [vp, ctypes.POINTER(SimInfo)]),

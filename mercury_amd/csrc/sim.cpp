@@ -1140,6 +1140,12 @@ int mh_sim_reset(mh_sim* s) {
 }
 
 float mh_sim_total_time(const mh_sim* s) { return s ? s->total_time : 0.f; }
+
+int mh_sim_set_total_time(mh_sim* s, float t) {
+  if (!s || !std::isfinite(t)) return MH_ERR_ARG;
+  s->total_time = t;
+  return MH_OK;
+}
 float mh_sim_get_time_step(const mh_sim* s) { return s ? s->cfg.dt : 0.f; }
 
 int mh_sim_set_time_step(mh_sim* s, float dt) {

@@ -192,6 +192,11 @@ class Simulator:
     def getTotalTime(self):
         return float(load().mh_sim_total_time(self._h))
 
+    def setTotalTime(self, t):
+        """Continue the actuation clock at t (no reference counterpart: the
+        population upload restarts it at 0)."""
+        check(load().mh_sim_set_total_time(self._h, float(t)))
+
     # -------------------------------------------------------------- tuning
     def info(self):
         i = SimInfo()

@@ -1,0 +1,91 @@
+"""Devo: stress-sorted spring replacement (Simulator::Devo,
+Simulator.cu:640-668; replaceSprings, devo_kernel.cu:81-165).  CPU checks of
+the replacement rule against the oracle's material tables and a float64
+length formula; GPU checks of the devo cycles inside the evaluator."""
+import numpy as np
+import pytest
+
+import mercury_amd as mh
+import oracle
+from mercury_amd import devo, evolve
+from mercury_amd.evaluate import Evaluator
+from mercury_amd.materials import WORKLOAD_ROWS
+
+
+def robot(seed=0, voxels=4):
+    ms = mh.workload_materials()
+    g = evolve.random_genomes(np.random.default_rng(seed), 1, voxels)
+    return ms, mh.voxel_build(ms, voxels + 1, 10.0, evolve.genome_to_lattice(g, voxels)[0], mass=0.001)
+
+
+def test_replaces_the_most_stressed_springs():
+    ms, el = robot()
+    rng = np.random.default_rng(1)
+    stress = rng.normal(size=el.n_springs).astype(np.float32)
+    stress[:40] = 5.0                      # ties: the lowest spring indices rank first
+    pos = el.pos + rng.normal(scale=1e-3, size=el.pos.shape).astype(np.float32)
+    t = 0.37
+    new, order = devo.devo_robot(el, stress, pos, t, 32, np.random.default_rng(7), devo.MaterialLookup(ms))
+    assert np.array_equal(order, np.arange(32))
+    keep = np.setdiff1d(np.arange(el.n_springs), order)
+    assert np.array_equal(new.pairs[keep], el.pairs[keep])
+    assert np.array_equal(new.mean_length[keep].view(np.uint32), el.mean_length[keep].view(np.uint32))
+    assert np.array_equal(new.spring_material[keep], el.spring_material[keep])
+    p = new.pairs[order].astype(int)
+    assert (p[:, 0] != p[:, 1]).all() and p.max() < el.n_masses
+    # material: decode(enc_a | enc_b) from the oracle's composite tables
+    base = oracle.make_base(WORKLOAD_ROWS)
+    by_id, _, by_enc = oracle.composite_tables(base)
+    cc = len(by_enc)
+    mass_enc = by_id["encoding"][el.mass_material]
+    for k, (a, b) in zip(order, p):
+        e = int(mass_enc[a]) | int(mass_enc[b])
+        want = by_enc[oracle.encoded_idx(e, cc)]
+        assert new.spring_material[k] == want.id
+        # rest length |x_a - x_b| / (1 + dL0 sin(omega t + phi)), float64 reference
+        d = np.linalg.norm(pos[a].astype(np.float64) - pos[b].astype(np.float64))
+        L = d / (1.0 + want.dL0 * np.sin(want.omega * t + want.phi))
+        assert new.mean_length[k] == pytest.approx(L, rel=2e-6)
+    # the same generator state gives the same pairs; another gives others
+    again, _ = devo.devo_robot(el, stress, pos, t, 32, np.random.default_rng(7), devo.MaterialLookup(ms))
+    other, _ = devo.devo_robot(el, stress, pos, t, 32, np.random.default_rng(8), devo.MaterialLookup(ms))
+    assert np.array_equal(again.pairs, new.pairs) and not np.array_equal(other.pairs, new.pairs)
+
+
+def test_nothing_replaced():
+    ms, el = robot(2)
+    new, order = devo.devo_robot(el, np.zeros(el.n_springs), el.pos, 0.0, 0, np.random.default_rng(0),
+                                 devo.MaterialLookup(ms))
+    assert len(order) == 0 and np.array_equal(new.pairs, el.pairs)
+
+
+def test_random_pairs_are_distinct():
+    p = devo.random_pairs(np.random.default_rng(3), 5000, 2)
+    assert ((p[:, 0] == 0) & (p[:, 1] == 1) | (p[:, 0] == 1) & (p[:, 1] == 0)).all()
+
+
+def test_rest_from_proto_is_the_reset_length():
+    _, el = robot(4)
+    r = devo.rest_from_proto(el, el.pos)
+    assert np.array_equal(r.mean_length.view(np.uint32), mh.robots.rest_lengths(el.pos, el.pairs).view(np.uint32))
+    assert np.array_equal(r.rest_length, r.mean_length)
+
+
+@pytest.mark.gpu
+def test_devo_cycles_in_the_evaluator():
+    print("devo: 4 robots 4^3, 2 cycles of 0.02 s, 16 springs", flush=True)
+    ms = mh.workload_materials()
+    robots = mh.voxel_population(ms, 4, voxels_per_axis=4, seed=6)
+    cfg = mh.SimConfig(solver=mh.MH_SOLVER_FORCE)
+    plain = Evaluator(ms, cfg, evaluation_time=0.03).batch_evaluate(robots)
+    # replacing nothing leaves the evaluation untouched
+    none = Evaluator(ms, cfg, evaluation_time=0.03, devo_cycles=2, devo_time=0.02, replaced_springs=0)
+    assert np.array_equal(none.batch_evaluate(robots), plain)
+    ev = Evaluator(ms, cfg, evaluation_time=0.03, devo_cycles=2, devo_time=0.02, replaced_springs=16, seed=3)
+    fit = ev.batch_evaluate(robots)
+    changed = [int((a.pairs != b.pairs).any(axis=1).sum()) for a, b in zip(ev.last_devo, robots)]
+    assert all(0 < c <= 32 for c in changed), changed
+    # the evaluation is that of the devo'd topology, and the cycles are reproducible
+    assert np.array_equal(Evaluator(ms, cfg, evaluation_time=0.03).batch_evaluate(ev.last_devo), fit)
+    ev2 = Evaluator(ms, cfg, evaluation_time=0.03, devo_cycles=2, devo_time=0.02, replaced_springs=16, seed=3)
+    assert np.array_equal(ev2.batch_evaluate(robots), fit)

@@ -3,12 +3,20 @@ import numpy as np
 import pytest
 
 import mercury_amd as mh
+from mercury_amd.devo import rest_from_proto
 from mercury_amd.evaluate import Evaluator
 
 
-def test_devo_cycles_rejected():
-    with pytest.raises(NotImplementedError):
-        Evaluator(None, devo_cycles=1)
+def test_evaluator_uploads_reset_robots():
+    """BatchEvaluate resets every solution before uploading it
+    (Evaluator_impl.h:52-53): its springs' lengths are recomputed from the
+    proto positions, which moves some of the built lengths by an ulp."""
+    ms = mh.workload_materials()
+    r = mh.voxel_population(ms, 1, voxels_per_axis=4, seed=5)[0]
+    reset = rest_from_proto(r, r.pos)
+    assert np.array_equal(reset.mean_length.view(np.uint32),
+                          mh.robots.rest_lengths(r.pos, r.pairs).view(np.uint32))
+    assert np.abs(reset.mean_length - r.mean_length).max() <= 1e-6
 
 
 @pytest.mark.gpu
@@ -26,7 +34,7 @@ def test_batch_evaluate_equals_two_phase_reference_flow():
     assert ev.eval_count == 6 and fit[2] == 0
 
     sim = mh.Simulator(ms, cfg)
-    live = [r for r, v in zip(robots, valid) if v]
+    live = [rest_from_proto(r, r.pos) for r, v in zip(robots, valid) if v]   # R.Reset()
     sim.Reset()
     sim.SetElements(live)
     sim.Simulate(0.03)
