@@ -305,7 +305,10 @@ int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s);
  * reciprocal, 3 x against the next float (every pattern differs; a check
  * of the checker), 4 the kernels' in-range correctly rounded reciprocal;
  * 5 / 6 count inputs whose hardware sqrt lies above / below sqrtf, 7 / 8
- * the same for the hardware reciprocal (positive inputs).
+ * the same for the hardware reciprocal (positive inputs); 9 the kernels'
+ * fast division against a / b on pseudo-random pairs (pattern i of
+ * [begin, begin + count) gives one pair; |a| in [2^-79, 2^63) or 0,
+ * b in [2^-48, 2^40)).
  * *mismatches receives the number of differing results. */
 int mh_check_arith(int device, int which, uint32_t begin, uint32_t count,
                    uint64_t* mismatches);

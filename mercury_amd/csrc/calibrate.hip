@@ -84,6 +84,10 @@ __global__ void copy_f4(const float4* __restrict__ src, float4* __restrict__ dst
 //         7: hardware reciprocal above 1.0f / x
 //         8: hardware reciprocal below 1.0f / x
 //   (5-8 count one direction of the error for positive x)
+//         9: mh::div_rn_from_rcp(a, b, rcp_rn_inrange(b)) vs a / b for a
+//            pseudo-random pair drawn from the pattern (|a| in [2^-79, 2^63)
+//            or 0, b in [2^-48, 2^40), random signs of a; pairs whose
+//            quotient is not a normal float are skipped)
 __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, uint32_t count,
                                                    uint32_t* out) {
   __shared__ uint32_t part[256];
@@ -107,6 +111,24 @@ __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, ui
     } else if (which == 4) {
       a = mh::rcp_rn_inrange(x);
       b = 1.0f / x;
+    } else if (which == 9) {
+      // splitmix64 of the pattern: exponents and mantissas of a and b
+      uint64_t z = (static_cast<uint64_t>(begin + i) + 0x9e3779b97f4a7c15ull) * 0xbf58476d1ce4e5b9ull;
+      z = (z ^ (z >> 31)) * 0x94d049bb133111ebull;
+      z ^= z >> 29;
+      const int ea = -79 + static_cast<int>((z & 0xff) % 142);
+      const int eb = -48 + static_cast<int>(((z >> 8) & 0xff) % 88);
+      const uint32_t ma = static_cast<uint32_t>(z >> 16) & 0x7fffffu;
+      const uint32_t mb = static_cast<uint32_t>(z >> 39) & 0x7fffffu;
+      float x = __uint_as_float((static_cast<uint32_t>(ea + 127) << 23) | ma);
+      if ((z >> 62) & 1u) x = -x;
+      if (((z >> 8) & 0xff) == 0xff) x = 0.0f;
+      const float y = __uint_as_float((static_cast<uint32_t>(eb + 127) << 23) | mb);
+      b = x / y;
+      const float ab = fabsf(b);
+      a = (ab == 0.0f && x == 0.0f) || (ab >= 0x1p-126f && ab < 0x1p127f)
+              ? mh::div_rn_from_rcp(x, y, mh::rcp_rn_inrange(y))
+              : b;
     } else if (which <= 6) {
       a = __builtin_amdgcn_sqrtf(x);
       b = sqrtf(x);
@@ -116,6 +138,8 @@ __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, ui
     }
     if (which <= 4)
       bad += (__float_as_uint(a) != __float_as_uint(b)) ? 1u : 0u;
+    else if (which == 9)  // a zero quotient may differ in sign
+      bad += (__float_as_uint(a) != __float_as_uint(b) && !(a == 0.0f && b == 0.0f)) ? 1u : 0u;
     else if (which == 5 || which == 7)
       bad += (a > b) ? 1u : 0u;
     else
@@ -236,7 +260,7 @@ int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms) {
 }
 
 int mh_check_arith(int device, int which, uint32_t begin, uint32_t count, uint64_t* mismatches) {
-  if (!mismatches || which < 0 || which > 8 || count == 0 ||
+  if (!mismatches || which < 0 || which > 9 || count == 0 ||
       static_cast<uint64_t>(begin) + count > 0x100000000ull)
     return MH_ERR_ARG;
   hipError_t e = hipSetDevice(device);

@@ -113,6 +113,19 @@ __device__ __forceinline__ void length_and_inverse(float x, float y, float z, fl
   }
 }
 
+// Correctly rounded a / b from y = RN(1/b): q0 = a y, then two fused
+// corrections q + (a - b q) y.  The first leaves q faithful, so its
+// remainder is exact and the second rounds correctly (Markstein's theorem
+// on fused multiply-add division).  Valid when a is 0 or |a| >= 2^-79, b and
+// a / b are normal and below 2^64, so no remainder falls below 2^-126.
+__device__ __forceinline__ float div_rn_from_rcp(float a, float b, float y) {
+  const float q0 = a * y;
+  const float r0 = fmaf(-b, q0, a);
+  const float q1 = fmaf(r0, y, q0);
+  const float r1 = fmaf(-b, q1, a);
+  return fmaf(r1, y, q1);
+}
+
 // Rest-length factor A * sin(omega * t + phi) of one material at time t.
 __device__ __forceinline__ float actuation(float dL0, float omega, float phi, float t) {
   return dL0 * act_sin(fmaf(omega, t, phi));

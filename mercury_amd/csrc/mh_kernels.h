@@ -52,6 +52,8 @@ struct StepParams {
   float dt;
   float t0;
   int track_stress;
+  int fast_div;              // xpbd, gather kernel: every material stiffness, rest length
+                             // and amplitude lies in the fast division's range
   float gravity, ground_k, mu_s, mu_k, drag, damping;
 };
 

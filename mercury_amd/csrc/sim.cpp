@@ -105,6 +105,8 @@ struct mh_sim {
   std::vector<float> base_com, base_len;
   size_t n_masses = 0, n_springs = 0, n_active = 0;
   bool has_population = false;
+  bool fast_div_mats = false;   // xpbd divisors and amplitudes in the fast division's range
+  bool fast_div_lbar = false;   // active rest lengths in its range
 
   // device state
   mh::DevBuf<float4> pos, vel;
@@ -170,6 +172,11 @@ void upload_materials(mh_sim* s) {
     w[i] = m.omega;
     ph[i] = m.phi;
   }
+  // the gather kernel's fast xpbd division needs divisors in [2^-40, 2^40]
+  // and amplitudes of at most 1/2
+  s->fast_div_mats = true;
+  for (int i = 0; i < n; i++)
+    s->fast_div_mats = s->fast_div_mats && kx[i] >= 0x1p-40f && kx[i] <= 0x1p40f && std::fabs(a[i]) <= 0.5f;
   s->mat_k.upload(k.data(), n, s->stream);
   s->mat_kx.upload(kx.data(), n, s->stream);
   s->mat_dL0.upload(a.data(), n, s->stream);
@@ -700,6 +707,7 @@ StepParams make_params(mh_sim* s, const Part& part, float t0, int n_steps, int t
   p.dt = s->cfg.dt;
   p.t0 = t0;
   p.track_stress = track;
+  p.fast_div = s->fast_div_mats && s->fast_div_lbar;
   p.gravity = s->cfg.gravity;
   p.ground_k = s->cfg.ground_k;
   p.mu_s = s->cfg.mu_s;
@@ -1005,6 +1013,8 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   s->mat_set = std::move(msets);
   s->act_pairs = std::move(ap);
   s->act_lbar = alb;
+  s->fast_div_lbar = true;
+  for (float L : alb) s->fast_div_lbar = s->fast_div_lbar && L >= 0x1p-40f && L <= 0x1p20f;
   s->active_src = std::move(src);
   s->act_mat = std::move(amat);
   s->compact_to_id = std::move(compact_to_id);

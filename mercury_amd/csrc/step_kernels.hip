@@ -765,6 +765,55 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   }
 }
 
+// XPBD term of the gather kernel.  fast_ok says that the workgroup's
+// positions and the pack's materials lie where div_rn_from_rcp is exact
+// (coordinates 0 or of magnitude in [2^-56, 2^20]; stiffness divisor in
+// [2^-40, 2^40], rest lengths in [2^-40, 2^20], amplitude at most 1/2); the
+// wave then takes the short sequence if every lane's squared length is also
+// in the fast sqrt range.  ky = RN(1/kk); 0 marks the null material, whose
+// term is a signed zero.  Same results as spring_term<XPBD> bit for bit.
+__device__ __forceinline__ void xpbd_term(float4 pa, float4 pb, float lbar, float R, float kk, float ky,
+                                          bool fast_ok, float& cx, float& cy, float& cz, float& lambda) {
+  const float L = fmaf(lbar, R, lbar);
+  const float dx = pa.x - pb.x;
+  const float dy = pa.y - pb.y;
+  const float dz = pa.z - pb.z;
+  float s = dx * dx + dy * dy;
+  s = s + dz * dz;
+  const bool fast = fast_ok && __float_as_uint(s) - __float_as_uint(kSqrtFastLo) <=
+                                   __float_as_uint(kSqrtFastHi) - __float_as_uint(kSqrtFastLo);
+  float nx, ny, nz;
+  if (__builtin_amdgcn_ballot_w64(fast) == __builtin_amdgcn_read_exec()) {
+    const float d = sqrt_rn_inrange(s);
+    const float den = d + kEps;
+    const float y = rcp_rn_inrange(den);
+    nx = div_rn_from_rcp(dx, den, y);
+    ny = div_rn_from_rcp(dy, den, y);
+    nz = div_rn_from_rcp(dz, den, y);
+    lambda = div_rn_from_rcp(-(d - L), kk, ky);
+  } else {
+    const float d = sqrtf(s);
+    const float den = d + kEps;
+    nx = dx / den;
+    ny = dy / den;
+    nz = dz / den;
+    lambda = ky == 0.0f ? 0.0f : -(d - L) / kk;
+  }
+  cx = lambda * nx;
+  cy = lambda * ny;
+  cz = lambda * nz;
+}
+
+// Position of a mass in the fast division's range: each coordinate 0 or of
+// magnitude in [2^-56, 2^20] (NaN is not).
+__device__ __forceinline__ bool position_in_div_range(float4 q) {
+  auto ok = [](float x) {
+    const float a = fabsf(x);
+    return a == 0.0f || (a >= 0x1p-56f && a <= 0x1p20f);
+  };
+  return ok(q.x) && ok(q.y) && ok(q.z);
+}
+
 // ------------------------------------------------ mass-parallel gather kernel
 // As step_mass, but each spring is evaluated once, by its first endpoint.
 // Phase 1: a lane computes its first-endpoint terms, adds them to P and
@@ -777,7 +826,10 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
 // step_mass with one actuation table and a zero term at kMassZero |
 // positions float4[pos_slots(BLOCK)] | terms.  First-endpoint records as in
 // step_mass; a second-endpoint record is the LDS byte offset of its term
-// (padding: the zero term).
+// (padding: the zero term).  XPBD: the reciprocals of the stiffness divisors
+// sit where step_mass keeps its actuation parameters, and two flags (at
+// kMassZero + 16, one per substep parity) record whether any position
+// written for that substep left the fast division's range.
 template <int SOLVER, int BLOCK, int MAXP, bool STRESS>
 __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -823,9 +875,12 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   for (int i = tid; i < n_lb; i += BLOCK) s_lb[i] = p.lbtab[d.pad0 + i];
   float t = p.t0;
   float a_dl = 0.f, a_om = 0.f, a_ph = 0.f;  // lanes < n_pm: their material's actuation
+  float* s_ky = reinterpret_cast<float*>(smem + kMassAct);
+  uint32_t* s_flag = reinterpret_cast<uint32_t*>(smem + kMassZero + 16);
   if (tid < n_pm) {  // n_pm < kMassTab <= BLOCK
     const uint32_t c = __float_as_uint(p.lbtab[d.pad0 + n_lb + tid]);
     s_k[tid] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[c] : p.mat_k[c];
+    if (SOLVER == MH_SOLVER_XPBD_REF) s_ky[tid] = 1.0f / p.mat_kx[c];
     a_dl = p.mat_dL0[c];
     a_om = p.mat_omega[c];
     a_ph = p.mat_phi[c];
@@ -834,10 +889,13 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   if (tid == 0) {
     pos[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
     s_lb[kMassNull] = 1.0f;
-    s_k[kMassNull] = (SOLVER == MH_SOLVER_XPBD_REF) ? __builtin_inff() : 0.0f;
+    s_k[kMassNull] = (SOLVER == MH_SOLVER_XPBD_REF) ? 1.0f : 0.0f;
+    if (SOLVER == MH_SOLVER_XPBD_REF) s_ky[kMassNull] = 0.0f;
     s_R[kMassNull] = 0.0f;
     float* z = reinterpret_cast<float*>(smem + kMassZero);
     z[0] = z[1] = z[2] = 0.0f;
+    s_flag[0] = 0u;
+    s_flag[1] = 0u;
   }
   int np_w = np, nq_w = nq;
   for (int off = 32; off > 0; off >>= 1) {
@@ -845,10 +903,19 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
     nq_w = max(nq_w, __shfl_xor(nq_w, off));
   }
   unsigned char* mine = smem + kTerm + 12 * MAXP * tid;
+  if (SOLVER == MH_SOLVER_XPBD_REF) {
+    __syncthreads();  // flags cleared
+    if (live && !position_in_div_range(lds_position<SOLVER>(ms))) atomicOr(&s_flag[0], 1u);
+  }
   __syncthreads();
 
   for (int step = 0; step < p.n_steps; ++step) {
     const float4 own = live ? pos[pslot] : make_float4(0.f, 0.f, 0.f, 0.f);
+    bool fast_ok = false;
+    if (SOLVER == MH_SOLVER_XPBD_REF) {
+      fast_ok = p.fast_div && s_flag[step & 1] == 0u;
+      if (tid == 0) s_flag[(step + 1) & 1] = 0u;  // read last in the previous substep
+    }
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
 #pragma unroll
     for (int k = 0; k < MAXP; ++k) {
@@ -858,8 +925,13 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
         const float lb = *reinterpret_cast<const float*>(smem + kMassLb + (r >> 24));
         const float4 other = *reinterpret_cast<const float4*>(smem + kMassPos + (r & 0xffffu));
         float cx, cy, cz, lambda;
-        spring_term<SOLVER>(own, other, lb, *reinterpret_cast<const float*>(smem + kMassR + mo),
-                            *reinterpret_cast<const float*>(smem + kMassK + mo), cx, cy, cz, lambda);
+        if constexpr (SOLVER == MH_SOLVER_XPBD_REF)
+          xpbd_term(own, other, lb, *reinterpret_cast<const float*>(smem + kMassR + mo),
+                    *reinterpret_cast<const float*>(smem + kMassK + mo),
+                    *reinterpret_cast<const float*>(smem + kMassAct + mo), fast_ok, cx, cy, cz, lambda);
+        else
+          spring_term<SOLVER>(own, other, lb, *reinterpret_cast<const float*>(smem + kMassR + mo),
+                              *reinterpret_cast<const float*>(smem + kMassK + mo), cx, cy, cz, lambda);
         ax = ax + cx; ay = ay + cy; az = az + cz;
         float* slot = reinterpret_cast<float*>(mine + 12 * k);
         slot[0] = cx;
@@ -877,7 +949,11 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
         bx = bx + q[0]; by = by + q[1]; bz = bz + q[2];
       }
     }
-    if (live) pos[pslot] = integrate<SOLVER>(ms, ax - bx, ay - by, az - bz, p);
+    if (live) {
+      const float4 np4 = integrate<SOLVER>(ms, ax - bx, ay - by, az - bz, p);
+      pos[pslot] = np4;
+      if (SOLVER == MH_SOLVER_XPBD_REF && !position_in_div_range(np4)) atomicOr(&s_flag[(step + 1) & 1], 1u);
+    }
     if (tid < n_pm) s_R[tid] = actuation(a_dl, a_om, a_ph, t + p.dt);
     t = t + p.dt;
     __syncthreads();

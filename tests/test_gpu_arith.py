@@ -35,3 +35,13 @@ def test_check_arith_counts_every_difference():
     lo, hi = bits(1.0), bits(4.0)
     assert mismatches(3, lo, hi) == hi - lo
     print("hardware rcp mismatches in [1, 4):", mismatches(2, lo, hi), flush=True)
+
+
+@pytest.mark.gpu
+def test_fast_division_is_correctly_rounded():
+    """mh::div_rn_from_rcp (reciprocal + two fused corrections) against IEEE
+    a / b on 2^31 pseudo-random pairs from the range the XPBD gather kernel
+    admits to its short path (numerators 0 or of magnitude [2^-79, 2^63),
+    divisors [2^-48, 2^40))."""
+    print("arith: fast division, 2^31 pairs", flush=True)
+    assert mismatches(9, 0, 1 << 31) == 0
