@@ -10,14 +10,17 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <thread>
 #include <tuple>
 
 #include "mh_internal.h"
@@ -29,6 +32,36 @@ namespace {
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess)
     throw Error(MH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// fn(i) for i in [0, n) on up to 16 host threads; items are independent.
+// The first exception thrown by any item is rethrown after all threads end.
+template <typename F>
+void parallel_for(size_t n, F&& fn) {
+  size_t nt = std::min<size_t>(std::max(1u, std::thread::hardware_concurrency()), 16);
+  nt = std::min(nt, n / 8 + 1);  // small batches stay on the calling thread
+  if (nt <= 1) {
+    for (size_t i = 0; i < n; i++) fn(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::exception_ptr err;
+  std::mutex mu;
+  auto work = [&]() {
+    for (size_t i = next++; i < n; i = next++) {
+      try {
+        fn(i);
+      } catch (...) {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!err) err = std::current_exception();
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  if (err) std::rethrow_exception(err);
 }
 
 template <typename T>
@@ -427,15 +460,20 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
                  std::vector<uint8_t>& mcnt, std::vector<float>& lbtab) {
   const int P = part.v->nspt;
   const bool gather = part.v->chunked != 0;
-  std::vector<uint32_t> table, mtable, merged, fill_p, fill_m, order, lane;
-  for (PackDesc& d : part.packs) {
-    const uint32_t nm = d.n_mass, ns = d.n_spring;
-    const int r0 = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
-                                    s->mass_off.begin()) - 1;
-    // the pack's tables: unions of its robots' sets, sorted
-    table.clear();
-    mtable.clear();
-    for (int q = r0; q < s->n_robots && s->mass_off[q] < d.mass_begin + nm; q++) {
+  const size_t n_packs = part.packs.size();
+  auto first_robot = [s](const PackDesc& d) {
+    return static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
+                            s->mass_off.begin()) - 1;
+  };
+  // pass 1 (packs in parallel): the pack's tables, unions of its robots'
+  // sets, sorted
+  std::vector<std::vector<uint32_t>> tables(n_packs), mtables(n_packs);
+  parallel_for(n_packs, [&](size_t pi) {
+    const PackDesc& d = part.packs[pi];
+    std::vector<uint32_t>& table = tables[pi];
+    std::vector<uint32_t>& mtable = mtables[pi];
+    std::vector<uint32_t> merged;
+    for (int q = first_robot(d); q < s->n_robots && s->mass_off[q] < d.mass_begin + d.n_mass; q++) {
       merged.clear();
       std::set_union(table.begin(), table.end(), s->lbar_set[q].begin(), s->lbar_set[q].end(),
                      std::back_inserter(merged));
@@ -447,26 +485,36 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
     }
     if (table.size() > static_cast<size_t>(kMassLbarMax) || mtable.size() > static_cast<size_t>(kMassMatMax))
       throw Error(MH_ERR_INTERNAL, "mass pack tables over the kernel's capacity");
-    d.pad0 = static_cast<uint32_t>(lbtab.size());
-    d.pad1 = static_cast<uint32_t>(table.size()) | static_cast<uint32_t>(mtable.size()) << 16;
-    for (uint32_t bits : table) {
-      float f;
-      std::memcpy(&f, &bits, 4);
-      lbtab.push_back(f);
-    }
-    for (uint32_t bits : mtable) {
-      float f;
-      std::memcpy(&f, &bits, 4);
-      lbtab.push_back(f);
-    }
-    d.csr_begin = static_cast<uint32_t>(mrec.size());
-    d.row_begin = static_cast<uint32_t>(mcnt.size());
-    mrec.resize(mrec.size() + static_cast<size_t>(3 * P) * nm, 0u);
+  });
+  // offsets of every pack's tables, records and counts
+  size_t lb_end = lbtab.size(), rec_end = mrec.size(), cnt_end = mcnt.size();
+  for (size_t pi = 0; pi < n_packs; pi++) {
+    PackDesc& d = part.packs[pi];
+    d.pad0 = static_cast<uint32_t>(lb_end);
+    d.pad1 = static_cast<uint32_t>(tables[pi].size()) | static_cast<uint32_t>(mtables[pi].size()) << 16;
+    lb_end += tables[pi].size() + mtables[pi].size();
+    d.csr_begin = static_cast<uint32_t>(rec_end);
+    rec_end += static_cast<size_t>(3 * P) * d.n_mass;
+    d.row_begin = static_cast<uint32_t>(cnt_end);
+    cnt_end += 2u * d.n_mass;
+  }
+  lbtab.resize(lb_end);
+  mrec.resize(rec_end, 0u);
+  mcnt.resize(cnt_end);
+  // pass 2 (packs in parallel): tables, lane order and records
+  parallel_for(n_packs, [&](size_t pi) {
+    const PackDesc& d = part.packs[pi];
+    const std::vector<uint32_t>& table = tables[pi];
+    const std::vector<uint32_t>& mtable = mtables[pi];
+    const uint32_t nm = d.n_mass, ns = d.n_spring;
+    const int r0 = first_robot(d);
+    float* tab = lbtab.data() + d.pad0;
+    for (size_t t = 0; t < table.size(); t++) std::memcpy(&tab[t], &table[t], 4);
+    for (size_t t = 0; t < mtable.size(); t++) std::memcpy(&tab[table.size() + t], &mtable[t], 4);
     uint32_t* out = mrec.data() + d.csr_begin;
     // record counts per mass, then the lane order: most first-endpoint
     // records first, then most second-endpoint records (ties keep order)
-    fill_p.assign(nm, 0);
-    fill_m.assign(nm, 0);
+    std::vector<uint32_t> fill_p(nm, 0), fill_m(nm, 0), order(nm), lane(nm);
     int q = r0;
     for (uint32_t k = 0; k < ns; k++) {
       const uint32_t g = d.spring_begin + k;
@@ -475,12 +523,10 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
       fill_p[s->act_pairs[2 * g] + off]++;
       fill_m[s->act_pairs[2 * g + 1] + off]++;
     }
-    order.resize(nm);
     std::iota(order.begin(), order.end(), 0u);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
       return fill_p[x] != fill_p[y] ? fill_p[x] > fill_p[y] : fill_m[x] > fill_m[y];
     });
-    lane.resize(nm);
     for (uint32_t i = 0; i < nm; i++) {
       perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
       lane[order[i]] = i;
@@ -489,9 +535,14 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
       if (fill_p[i] > static_cast<uint32_t>(P) || fill_m[i] > static_cast<uint32_t>(P))
         throw Error(MH_ERR_INTERNAL, "mass record lists over the kernel's capacity");
     }
-    fill_p.assign(nm, 0);
-    fill_m.assign(nm, 0);
+    // table positions by value: rest lengths (bit patterns) and materials
+    auto index_of = [](const std::vector<uint32_t>& t, uint32_t v) {
+      return static_cast<uint32_t>(std::lower_bound(t.begin(), t.end(), v) - t.begin());
+    };
+    std::fill(fill_p.begin(), fill_p.end(), 0u);
+    std::fill(fill_m.begin(), fill_m.end(), 0u);
     q = r0;
+    uint32_t last_bits = ~0u, last_li = 0;
     for (uint32_t k = 0; k < ns; k++) {  // ascending spring index
       const uint32_t g = d.spring_begin + k;
       while (g >= s->active_off[q + 1]) q++;
@@ -500,9 +551,12 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
       const uint32_t la = lane[a], lb = lane[b];
       uint32_t bits;
       std::memcpy(&bits, &s->act_lbar[g], 4);
-      const uint32_t li = static_cast<uint32_t>(std::lower_bound(table.begin(), table.end(), bits) - table.begin());
-      const uint32_t mi = static_cast<uint32_t>(
-          std::lower_bound(mtable.begin(), mtable.end(), static_cast<uint32_t>(s->act_mat[g])) - mtable.begin());
+      if (bits != last_bits) {
+        last_bits = bits;
+        last_li = index_of(table, bits);
+      }
+      const uint32_t li = last_li;
+      const uint32_t mi = index_of(mtable, static_cast<uint32_t>(s->act_mat[g]));
       const uint32_t tail = ((4u * mi) << 16) | ((4u * li) << 24);
       const uint32_t ka = fill_p[a]++;
       out[static_cast<size_t>(2 * P + ka) * nm + la] = k;
@@ -513,11 +567,12 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
           gather ? massg_term_offset(part.v->block, P, la, ka)
                  : (16u * static_cast<uint32_t>(pos_slot(static_cast<int>(a))) | tail);
     }
+    uint8_t* cnt = mcnt.data() + d.row_begin;
     for (uint32_t i = 0; i < nm; i++) {
-      mcnt.push_back(static_cast<uint8_t>(fill_p[order[i]]));
-      mcnt.push_back(static_cast<uint8_t>(fill_m[order[i]]));
+      cnt[2 * i] = static_cast<uint8_t>(fill_p[order[i]]);
+      cnt[2 * i + 1] = static_cast<uint8_t>(fill_m[order[i]]);
     }
-  }
+  });
   std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
     return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
   });
@@ -953,11 +1008,10 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   std::vector<uint32_t> gpad(R, 0);
   std::vector<uint8_t> mside(R, 0);
   std::vector<std::vector<uint32_t>> lsets(R), msets(R);
-  {
-    std::vector<uint32_t> plus, minus;
-    for (int r = 0; r < R; r++) {
-      plus.assign(mo[r + 1] - mo[r], 0);
-      minus.assign(mo[r + 1] - mo[r], 0);
+  mh::parallel_for(static_cast<size_t>(R), [&](size_t ri) {
+    const int r = static_cast<int>(ri);
+    {
+      std::vector<uint32_t> plus(mo[r + 1] - mo[r], 0), minus(mo[r + 1] - mo[r], 0);
       std::vector<uint32_t>& ls = lsets[r];
       std::vector<uint32_t>& mset = msets[r];
       auto add = [](std::vector<uint32_t>& set, uint32_t v, int cap) {
@@ -984,7 +1038,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
       gpad[r] = g;
       mside[r] = static_cast<uint8_t>(std::min<uint32_t>(side, 255));
     }
-  }
+  });
 
   // masses: float4 (xyz, mass) and (v, 0)
   std::vector<float4> p4(M), v4(M);
