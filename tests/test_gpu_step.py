@@ -72,6 +72,7 @@ def test_uneven_population_bitexact():
                                  dict(block_size=256), dict(block_size=512),
                                  dict(robots_per_block=1), dict(kernel=mh.MH_KERNEL_STREAM),
                                  dict(kernel=mh.MH_KERNEL_STREAM, block_size=256),
+                                 dict(kernel=mh.MH_KERNEL_STREAM, block_size=512),
                                  dict(kernel=mh.MH_KERNEL_LDS | mh.MH_KERNEL_STREAM),
                                  dict(kernel=mh.MH_KERNEL_LDS | mh.MH_KERNEL_STREAM, block_size=1024),
                                  dict(kernel=mh.MH_KERNEL_MASS | mh.MH_KERNEL_STREAM, robots_per_block=1),

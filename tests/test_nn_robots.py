@@ -86,3 +86,22 @@ def test_nn_population_mixed_plan_bitexact():
     sim2.autotune(20)
     sim2.Step(200)
     assert np.array_equal(bits(sim2.collect_arrays()[0]), bits(p))
+
+
+@pytest.mark.gpu
+def test_large_nn_robots_many_chunks_bitexact():
+    """Robots of 1500-2000 masses (37-50 k springs) take several LDS chunks per
+    substep in the chunked kernel, whose next chunk is loaded while the
+    current one is worked on; the trajectory equals the mirror's."""
+    print("nn 2 large robots, chunked kernel, 100 steps", flush=True)
+    ms = mh.workload_materials()
+    els = robots.nn_population(ms, 2, mass_range=(1500, 2000), K=25, seed=3)
+    case = Case(els, ms, mh.materials.WORKLOAD_ROWS)
+    sim = gpu_sim(case, mh.MH_SOLVER_FORCE, 1e-4, WATER, kernel=mh.MH_KERNEL_STREAM)
+    info = sim.info()
+    assert info.stream_chunk > 0 and max(e.n_springs for e in els) > 4 * info.stream_chunk
+    sim.Step(100)
+    pos, vel, _ = sim.collect_arrays()
+    p, v, _ = mirror(case, mh.MH_SOLVER_FORCE, 100, 1e-4, WATER)
+    assert np.array_equal(bits(pos), bits(p))
+    assert np.array_equal(bits(vel), bits(v))
