@@ -51,6 +51,16 @@ MASS_LDS_BYTES_PER_MASS = 32.0
 LDS_NOMINAL_GBS = 256 * 128 * 2.4
 
 
+# --kernel choices: MH_KERNEL_* family bits (robots over the LDS budget may
+# always stream)
+KERNEL_FAMILIES = {"auto": 0, "mass": 4 | 2, "mass-gather": 8 | 2, "lds": 1 | 2, "stream": 2}
+
+
+def kernel_label(info):
+    return ("+".join(n for bit, n in ((8, "mass-gather"), (4, "mass"), (1, "lds"), (2, "stream"))
+                     if info.kernel & bit) + (" (chunked)" if info.stream_chunk > 0 else ""))
+
+
 def float_loop_steps(duration, dt):
     r, n = np.float32(duration), 0
     while r > np.float32(0):
@@ -98,15 +108,16 @@ def cpu_baseline(els, solver, dt, steps, threads, env_kw):
     return springs * steps / secs, secs
 
 
-def measured_traffic(workload):
+def measured_traffic(workload, kernel):
     """HBM bytes per step-kernel launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the
-    counter reads for a copy kernel of known size), if one exists."""
+    counter reads for a copy kernel of known size) for this workload and
+    kernel, if one exists."""
     path = os.path.join(REPO, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        entry = json.load(fh).get(workload)
+        entry = json.load(fh).get(f"{workload}/{kernel}")
     if not entry:
         return None
     return {"bytes_per_launch": entry["bytes_per_launch"],
@@ -128,6 +139,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--substeps", type=int, default=0, help="substeps per launch (0 = tuned)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--kernel", choices=list(KERNEL_FAMILIES), default="auto",
+                    help="restrict LDS-resident robots to one kernel family (larger robots stream)")
     ap.add_argument("--cpu-steps", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-xpbd", action="store_true", help="skip the XPBD side measurement")
@@ -178,7 +191,7 @@ def main():
 
     def make_sim(slv):
         cfg = mh.SimConfig(solver=slv, time_step=args.dt, substeps_per_launch=args.substeps,
-                           use_graph=1, **env_kw)
+                           use_graph=1, kernel=KERNEL_FAMILIES[args.kernel], **env_kw)
         sim = mh.Simulator(ms, cfg)
         t = time.perf_counter()
         sim.SetElements(els)
@@ -238,7 +251,8 @@ def main():
         per_launch = STREAM_BYTES_PER_SPRING * info.n_stream_springs * steps_per_launch
         per_unit = STREAM_BYTES_PER_SPRING
     achieved = per_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
-    traffic = measured_traffic(args.workload)
+    kernel_name = kernel_label(info)
+    traffic = measured_traffic(args.workload, kernel_name)
 
 
     out = {
@@ -253,14 +267,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded voxel materials)",
+        "data": ("synthetic (seeded voxel materials)" if args.workload == "voxel" else
+                 "synthetic (seeded NN robots: random positions, random-init material MLP)"),
         "config": {
             "workload": workload,
             "robots": args.robots, "springs_total": n_springs, "active_springs": n_active,
             "masses_total": n_masses, "substeps_per_eval": n_sub, "solver": args.solver,
-            "kernel": ("+".join(n for bit, n in ((8, "mass-gather"), (4, "mass"), (1, "lds"), (2, "stream"))
-                                if info.kernel & bit) +
-                       (" (chunked)" if info.stream_chunk > 0 else "")), "block": info.block_size,
+            "kernel": kernel_name, "block": info.block_size,
             "blocks": info.n_blocks, "lds_bytes_per_block": info.lds_bytes,
             "substeps_per_launch": info.substeps_per_launch, "hipgraph": True,
             "lds_robots": info.n_lds_robots, "stream_robots": info.n_stream_robots,

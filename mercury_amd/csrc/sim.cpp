@@ -971,34 +971,64 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
       (S && (!pop->spring_pairs || !pop->spring_mean_length || !pop->spring_material_id)))
     throw mh::Error(MH_ERR_ARG, "population arrays are null");
 
-  // validate and compact springs (air springs are no-ops, sim_kernel.cu:211)
+  // validate and compact springs (air springs are no-ops, sim_kernel.cu:211).
+  // Pass 1, robots in parallel: validation, active counts and each robot's
+  // material ids in first-appearance order.  Compact ids then follow the
+  // first appearance over the whole population, robot by robot.  Pass 2,
+  // robots in parallel: the compacted arrays at each robot's offset.
   std::vector<int> id_to_compact(256, -1);
   std::vector<uint8_t> compact_to_id;
   std::vector<uint32_t> ao(R + 1, 0), src;
   std::vector<uint16_t> ap;
   std::vector<float> alb;
   std::vector<uint8_t> amat;
-  src.reserve(S);
-  for (int r = 0; r < R; r++) {
+  std::vector<std::vector<uint8_t>> ids_seen(R);
+  mh::parallel_for(static_cast<size_t>(R), [&](size_t ri) {
+    const int r = static_cast<int>(ri);
     const uint32_t nm = mo[r + 1] - mo[r];
+    bool seen[256] = {};
+    uint32_t active = 0;
     for (uint32_t k = so[r]; k < so[r + 1]; k++) {
       const uint16_t a = pop->spring_pairs[2 * k], b = pop->spring_pairs[2 * k + 1];
       const uint8_t id = pop->spring_material_id[k];
       if (a >= nm || b >= nm) throw mh::Error(MH_ERR_ARG, "spring endpoint outside its robot");
       if (id >= s->composite_count) throw mh::Error(MH_ERR_ARG, "spring material id outside the material set");
       if (id == 0) continue;
+      active++;
+      if (!seen[id]) {
+        seen[id] = true;
+        ids_seen[r].push_back(id);
+      }
+    }
+    ao[r + 1] = active;
+  });
+  for (int r = 0; r < R; r++) {
+    ao[r + 1] += ao[r];
+    for (uint8_t id : ids_seen[r]) {
       if (id_to_compact[id] < 0) {
         id_to_compact[id] = static_cast<int>(compact_to_id.size());
         compact_to_id.push_back(id);
       }
-      src.push_back(k);
-      ap.push_back(a);
-      ap.push_back(b);
-      alb.push_back(pop->spring_mean_length[k]);
-      amat.push_back(static_cast<uint8_t>(id_to_compact[id]));
     }
-    ao[r + 1] = static_cast<uint32_t>(src.size());
   }
+  src.resize(ao[R]);
+  ap.resize(2 * static_cast<size_t>(ao[R]));
+  alb.resize(ao[R]);
+  amat.resize(ao[R]);
+  mh::parallel_for(static_cast<size_t>(R), [&](size_t ri) {
+    const int r = static_cast<int>(ri);
+    uint32_t j = ao[r];
+    for (uint32_t k = so[r]; k < so[r + 1]; k++) {
+      const uint8_t id = pop->spring_material_id[k];
+      if (id == 0) continue;
+      src[j] = k;
+      ap[2 * static_cast<size_t>(j)] = pop->spring_pairs[2 * k];
+      ap[2 * static_cast<size_t>(j) + 1] = pop->spring_pairs[2 * k + 1];
+      alb[j] = pop->spring_mean_length[k];
+      amat[j] = static_cast<uint8_t>(id_to_compact[id]);
+      j++;
+    }
+  });
   if (compact_to_id.empty()) compact_to_id.push_back(0);  // keeps tables non-empty
 
   // per robot: gather entries with both parts of every row rounded up to 4;
@@ -1040,20 +1070,22 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
     }
   });
 
-  // masses: float4 (xyz, mass) and (v, 0)
+  // masses: float4 (xyz, mass) and (v, 0); baseline COM and length per
+  // robot (SoftBody::updateBaseline at Reset)
   std::vector<float4> p4(M), v4(M);
-  for (size_t i = 0; i < M; i++) {
-    const float m = pop->mass ? pop->mass[i] : 1.0f;
-    if (s->cfg.solver == MH_SOLVER_FORCE && !(m > 0.f)) throw mh::Error(MH_ERR_ARG, "mass must be > 0");
-    p4[i] = make_float4(pop->pos[3 * i], pop->pos[3 * i + 1], pop->pos[3 * i + 2], m);
-    v4[i] = pop->vel ? make_float4(pop->vel[3 * i], pop->vel[3 * i + 1], pop->vel[3 * i + 2], 0.f)
-                     : make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-  // baseline COM and length per robot (SoftBody::updateBaseline at Reset)
   std::vector<float> bc(3 * R), bl(R);
-  for (int r = 0; r < R; r++)
+  mh::parallel_for(static_cast<size_t>(R), [&](size_t ri) {
+    const int r = static_cast<int>(ri);
+    for (size_t i = mo[r]; i < mo[r + 1]; i++) {
+      const float m = pop->mass ? pop->mass[i] : 1.0f;
+      if (s->cfg.solver == MH_SOLVER_FORCE && !(m > 0.f)) throw mh::Error(MH_ERR_ARG, "mass must be > 0");
+      p4[i] = make_float4(pop->pos[3 * i], pop->pos[3 * i + 1], pop->pos[3 * i + 2], m);
+      v4[i] = pop->vel ? make_float4(pop->vel[3 * i], pop->vel[3 * i + 1], pop->vel[3 * i + 2], 0.f)
+                       : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
     mh::baseline(pop->pos + 3 * mo[r], pop->mass_material_id + mo[r], static_cast<int>(mo[r + 1] - mo[r]),
                  &bc[3 * r], &bl[r]);
+  });
 
   mh::drop_graph(s);
   s->has_population = false;
