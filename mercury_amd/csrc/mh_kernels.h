@@ -81,6 +81,13 @@ const KernelVariant* kernel_variants(int* count);
 // masses across the LDS banks.
 __host__ __device__ inline int pos_slot(int i) { return i + (i >> 3); }
 __host__ __device__ inline int pos_slots(int n) { return n + (n >> 3) + 1; }
+// The mass-parallel kernels' slots: one unused slot after every
+// 2^MH_MASS_SKEW_SHIFT masses, and a last (ghost) slot.
+#ifndef MH_MASS_SKEW_SHIFT
+#define MH_MASS_SKEW_SHIFT 3
+#endif
+__host__ __device__ constexpr int mass_slot(int i) { return i + (i >> MH_MASS_SKEW_SHIFT); }
+__host__ __device__ constexpr int mass_slots(int block) { return block + (block >> MH_MASS_SKEW_SHIFT) + 1; }
 // Mass-parallel kernels: distinct rest lengths per pack and materials (its
 // LDS tables hold 64 of each, the last one reserved for padding records)
 constexpr int kMassLbarMax = 63;
@@ -93,7 +100,7 @@ constexpr int kMassNull = kMassTab - 1;  // table entry of padding records
 constexpr int kMassLb = 0, kMassK = 256, kMassR = 512, kMassAct = 1024, kMassZero = 1792, kMassPos = 2048;
 // step_massg: byte offset of the term slots (after one position buffer)
 // and of the term of (lane, k)
-__host__ __device__ constexpr int massg_term_base(int block) { return kMassPos + 16 * (block + block / 8 + 1); }
+__host__ __device__ constexpr int massg_term_base(int block) { return kMassPos + 16 * mass_slots(block); }
 __host__ __device__ constexpr uint32_t massg_term_offset(int block, int maxp, uint32_t lane, uint32_t k) {
   return static_cast<uint32_t>(massg_term_base(block)) + 12u * (static_cast<uint32_t>(maxp) * lane + k);
 }

@@ -560,12 +560,12 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
       const uint32_t tail = ((4u * mi) << 16) | ((4u * li) << 24);
       const uint32_t ka = fill_p[a]++;
       out[static_cast<size_t>(2 * P + ka) * nm + la] = k;
-      out[static_cast<size_t>(ka) * nm + la] = 16u * static_cast<uint32_t>(pos_slot(static_cast<int>(b))) | tail;
+      out[static_cast<size_t>(ka) * nm + la] = 16u * static_cast<uint32_t>(mass_slot(static_cast<int>(b))) | tail;
       // second endpoint: the same spring's record, or (gather kernel) the
       // LDS offset of the term its first endpoint stores
       out[static_cast<size_t>(P + fill_m[b]++) * nm + lb] =
           gather ? massg_term_offset(part.v->block, P, la, ka)
-                 : (16u * static_cast<uint32_t>(pos_slot(static_cast<int>(a))) | tail);
+                 : (16u * static_cast<uint32_t>(mass_slot(static_cast<int>(a))) | tail);
     }
     uint8_t* cnt = mcnt.data() + d.row_begin;
     for (uint32_t i = 0; i < nm; i++) {

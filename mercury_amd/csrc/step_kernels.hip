@@ -584,8 +584,8 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
 // barrier per substep.  LDS (fixed layout, so table and buffer bases are
 // instruction offsets): rest lengths float[64] | stiffness float[64] |
 // actuation float[2][64] | actuation parameters (dL0, omega, phi)
-// float[3][64] | positions float4[2][pos_slots(BLOCK)] (read one
-// buffer, write the other; skewed slots as in step_lds; the last slot of
+// float[3][64] | positions float4[2][mass_slots(BLOCK)] (read one
+// buffer, write the other; skewed slots mass_slot(i); the last slot of
 // each buffer is a far-away ghost that padding records point at).  Lanes
 // take masses in the order of p.mperm (most records first, so a wave's
 // lists have similar lengths).  Each lane keeps its records in registers:
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
 // the part-0 records, for the stress output); p.mcnt holds the two counts
 // per lane.
 template <int BLOCK>
-constexpr int mass_pos_bytes() { return 16 * (BLOCK + BLOCK / 8 + 1); }
+constexpr int mass_pos_bytes() { return 16 * mass_slots(BLOCK); }
 
 // One substep reading position buffer CUR and writing the other.
 template <int SOLVER, int BLOCK, int MAXP, bool STRESS, int CUR>
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   const PackDesc d = p.packs[blockIdx.x];
   const int nm = static_cast<int>(d.n_mass);
-  constexpr int kSlots = BLOCK + BLOCK / 8 + 1;
+  constexpr int kSlots = mass_slots(BLOCK);
   constexpr int kGhost = kSlots - 1;
   const int tid = threadIdx.x;
   const bool live = tid < nm;
@@ -701,7 +701,7 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
         st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]];
     }
   }
-  const int pslot = live ? pos_slot(mass) : kGhost;
+  const int pslot = live ? mass_slot(mass) : kGhost;
   float4* pos0 = reinterpret_cast<float4*>(smem + kMassPos);
   float4* pos1 = reinterpret_cast<float4*>(smem + kMassPos + mass_pos_bytes<BLOCK>());
   if (live) pos0[pslot] = lds_position<SOLVER>(ms);
@@ -824,7 +824,7 @@ __device__ __forceinline__ bool position_in_div_range(float4 q) {
 // actuation table is written in place too.  Barrier.  Two barriers per
 // substep, half the spring arithmetic of step_mass.  LDS: the tables of
 // step_mass with one actuation table and a zero term at kMassZero |
-// positions float4[pos_slots(BLOCK)] | terms.  First-endpoint records as in
+// positions float4[mass_slots(BLOCK)] | terms.  First-endpoint records as in
 // step_mass; a second-endpoint record is the LDS byte offset of its term
 // (padding: the zero term).  XPBD: the reciprocals of the stiffness divisors
 // sit where step_mass keeps its actuation parameters, and two flags (at
@@ -835,7 +835,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
   const PackDesc d = p.packs[blockIdx.x];
   const int nm = static_cast<int>(d.n_mass);
-  constexpr int kSlots = BLOCK + BLOCK / 8 + 1;
+  constexpr int kSlots = mass_slots(BLOCK);
   constexpr int kGhost = kSlots - 1;
   constexpr int kTerm = massg_term_base(BLOCK);
   const int tid = threadIdx.x;
@@ -865,7 +865,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
       if (STRESS && k < np) st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]];
     }
   }
-  const int pslot = live ? pos_slot(mass) : kGhost;
+  const int pslot = live ? mass_slot(mass) : kGhost;
   float4* pos = reinterpret_cast<float4*>(smem + kMassPos);
   if (live) pos[pslot] = lds_position<SOLVER>(ms);
   float* s_lb = reinterpret_cast<float*>(smem + kMassLb);
@@ -1034,7 +1034,7 @@ size_t lds_bytes_mass(int block, int n_mat) {
   // fixed tables (rest lengths, stiffness, two actuation tables, actuation
   // parameters) and two position buffers sized for the block
   (void)n_mat;
-  return static_cast<size_t>(kMassPos) + 2u * 16u * static_cast<size_t>(block + block / 8 + 1);
+  return static_cast<size_t>(kMassPos) + 2u * 16u * static_cast<size_t>(mass_slots(block));
 }
 
 size_t lds_bytes_massg(int block, int maxp) {
