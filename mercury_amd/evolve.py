@@ -297,7 +297,7 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         data_x.append(kids)
         data_y.append(fit)
         if use_surrogate and screening:
-            # with an audit sample, rank quality is judged on it alone (the
+            # with an audit sample, rank quality is measured on it alone (the
             # screened top share is range-restricted)
             rho = spearman(pred[audited], fit[audited]) if audited.sum() >= 8 else spearman(pred, fit)
             res.rho_history.append(rho)
