@@ -214,9 +214,10 @@ def age_limits(layers, age_gap):
 def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elitism=0.1,
         age_gap=5, threshold=None, max_generations=40, use_surrogate=False, screen_factor=4,
         retrain_every=5, min_rho=0.6, train_epochs=None, duration=1.0, sim=None, verbose=False,
-        audit=0.1, surrogate_kind="mlp", max_train=200000):
+        audit=0.1, surrogate_kind="mlp", max_train=200000, warm_start=False):
     """One evolution run; stops when the best fitness reaches `threshold` or
-    after max_generations.  Returns a RunResult."""
+    after max_generations.  Returns a RunResult.  warm_start: each retrain
+    continues from the previous network's weights instead of a fresh one."""
     rng = np.random.default_rng(seed)
     sim = sim or BatchSimulator(voxels=voxels, duration=duration)
     evals0, t_start = sim.evals, time.perf_counter()
@@ -249,7 +250,8 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
     for gen in range(1, max_generations + 1):
         res.generations = gen
         if use_surrogate and (sur is None or gen % retrain_every == 0):
-            sur = GenomeModel(surrogate_kind, voxels, seed=seed + gen, epochs=train_epochs)
+            if sur is None or not warm_start:
+                sur = GenomeModel(surrogate_kind, voxels, seed=seed + gen, epochs=train_epochs)
             X = np.concatenate(data_x)[-max_train:]
             Y = np.concatenate(data_y)[-max_train:]
             sur.fit(X, Y, seed=seed + gen)
@@ -445,6 +447,9 @@ def main(argv=None):
     c.add_argument("--epochs", type=int, default=None)
     c.add_argument("--surrogate", choices=("mlp", "cnn"), default="mlp")
     c.add_argument("--only", choices=("both", "plain", "surrogate"), default="both")
+    c.add_argument("--audit", type=float, default=0.1, help="share of each layer's simulated "
+                   "offspring drawn at random to measure the surrogate")
+    c.add_argument("--warm-start", action="store_true", help="retrain from the previous weights")
     rp = sub.add_parser("report", help="evaluations-to-threshold table from compare logs")
     rp.add_argument("logs", nargs="+", help="[LABEL=]path of a compare log (JSON lines); rows are "
                     "labelled LABEL:plain / LABEL:surrogate")
@@ -486,6 +491,7 @@ def main(argv=None):
         rows = compare(a.seeds, a.threshold, pop=a.pop, max_generations=a.generations,
                        voxels=a.voxels, duration=a.duration, screen_factor=a.screen_factor,
                        train_epochs=a.epochs, seed_start=a.seed_start, surrogate_kind=a.surrogate,
+                       audit=a.audit, warm_start=a.warm_start,
                        modes={"both": (False, True), "plain": (False,), "surrogate": (True,)}[a.only])
         for use in (False, True):
             e = [x["evals_to_threshold"] for x in rows if x["surrogate"] == use]

@@ -147,3 +147,21 @@ def test_short_gpu_evolution_run_cnn_surrogate():
     assert len(r.rho_history) == 2 and np.isfinite(r.best)
     with pytest.raises(ValueError):
         evolve.GenomeModel("cnn", 4)
+
+
+def test_warm_start_keeps_one_surrogate(monkeypatch):
+    made = []
+
+    class Counting(_FakeSurrogate):
+        def __init__(self, seed=0, **kw):
+            made.append(seed)
+
+    monkeypatch.setattr(evolve, "Surrogate", Counting)
+    Counting.sign = 1.0
+    evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=11, sim=_FakeSim(),
+               use_surrogate=True, warm_start=True)
+    assert len(made) == 1
+    made.clear()
+    evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=11, sim=_FakeSim(),
+               use_surrogate=True)
+    assert len(made) == 3  # generation 1, then every 5 generations
