@@ -79,8 +79,9 @@ def cpu_threads():
         return max(1, min(16, os.cpu_count() or 1))
 
 
-def cpu_baseline(els, solver, dt, steps, threads, env_kw):
-    """The oracle's FP32 mirror (plain C, -O3, no fast-math) on host cores."""
+def cpu_baseline(els, solver, dt, steps, threads, env_kw, f64=False):
+    """The oracle's FP32 mirror (plain C, -O3, no fast-math) on host cores;
+    f64: its FP64 twin instead."""
     import oracle
     from mercury_amd.simulator import pack
     a = pack(els)
@@ -97,9 +98,11 @@ def cpu_baseline(els, solver, dt, steps, threads, env_kw):
     def run(n, th):
         t = time.perf_counter()
         if solver == mh.MH_SOLVER_FORCE:
-            oracle.force_f32(pop, mt, env, 0.0, n, a["pos"], a["vel"], nthreads=th)
+            fn = oracle.force_f64 if f64 else oracle.force_f32
+            fn(pop, mt, env, 0.0, n, a["pos"], a["vel"], nthreads=th)
         else:
-            oracle.xpbd_f32(pop, mt, dt, 0.0, n, a["pos"], a["vel"], nthreads=th)
+            fn = oracle.xpbd_f64 if f64 else oracle.xpbd_f32
+            fn(pop, mt, dt, 0.0, n, a["pos"], a["vel"], nthreads=th)
         return time.perf_counter() - t
 
     run(max(1, steps // 20), threads)  # warm-up
@@ -307,12 +310,16 @@ def main():
         v_all, s_all = cpu_baseline(els, solver, args.dt, args.cpu_steps, th, env_kw)
         sub = els[:32]
         v_one, s_one = cpu_baseline(sub, solver, args.dt, max(50, args.cpu_steps // 8), 1, env_kw)
+        v64, _ = cpu_baseline(sub, solver, args.dt, max(50, args.cpu_steps // 8), th, env_kw, f64=True)
         out["cpu_baseline"] = {
             "value": v_all, "unit": "spring-updates/s", "cores": th, "kind": "port",
             "sample": f"oracle FP32 mirror, same {args.robots} robots, {args.cpu_steps} substeps "
                       f"({s_all:.1f} s wall on {th} threads)",
             "single_core_value": v_one,
             "single_core_sample": f"32 robots, {max(50, args.cpu_steps // 8)} substeps, 1 thread",
+            "fp64_oracle_value": v64,
+            "fp64_oracle_sample": f"FP64 oracle, 32 robots, {max(50, args.cpu_steps // 8)} substeps, "
+                                  f"{th} threads",
         }
         out["gpu_vs_cpu"] = value / v_all
     print(json.dumps(out))
