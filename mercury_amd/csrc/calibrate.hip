@@ -88,6 +88,8 @@ __global__ void copy_f4(const float4* __restrict__ src, float4* __restrict__ dst
 //            pseudo-random pair drawn from the pattern (|a| in [2^-79, 2^63)
 //            or 0, b in [2^-48, 2^40), random signs of a; pairs whose
 //            quotient is not a normal float are skipped)
+//        10: a reciprocal-square-root sqrt, s = x rsq(x) refined once by
+//            s + (x - s s) rsq(x) / 2, vs sqrtf (a candidate short sequence)
 __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, uint32_t count,
                                                    uint32_t* out) {
   __shared__ uint32_t part[256];
@@ -129,6 +131,12 @@ __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, ui
       a = (ab == 0.0f && x == 0.0f) || (ab >= 0x1p-126f && ab < 0x1p127f)
               ? mh::div_rn_from_rcp(x, y, mh::rcp_rn_inrange(y))
               : b;
+    } else if (which == 10) {
+      const float y = __builtin_amdgcn_rsqf(x);
+      const float sx = x * y;
+      const float r = fmaf(-sx, sx, x);
+      a = fmaf(r, 0.5f * y, sx);
+      b = sqrtf(x);
     } else if (which <= 6) {
       a = __builtin_amdgcn_sqrtf(x);
       b = sqrtf(x);
@@ -136,7 +144,7 @@ __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, ui
       a = __builtin_amdgcn_rcpf(x);
       b = 1.0f / x;
     }
-    if (which <= 4)
+    if (which <= 4 || which == 10)
       bad += (__float_as_uint(a) != __float_as_uint(b)) ? 1u : 0u;
     else if (which == 9)  // a zero quotient may differ in sign
       bad += (__float_as_uint(a) != __float_as_uint(b) && !(a == 0.0f && b == 0.0f)) ? 1u : 0u;
@@ -260,7 +268,7 @@ int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms) {
 }
 
 int mh_check_arith(int device, int which, uint32_t begin, uint32_t count, uint64_t* mismatches) {
-  if (!mismatches || which < 0 || which > 9 || count == 0 ||
+  if (!mismatches || which < 0 || which > 10 || count == 0 ||
       static_cast<uint64_t>(begin) + count > 0x100000000ull)
     return MH_ERR_ARG;
   hipError_t e = hipSetDevice(device);
