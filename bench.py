@@ -200,7 +200,7 @@ def main():
         sim.SetElements(els)
         t_up = time.perf_counter() - t
         if not args.no_autotune:
-            sim.autotune(200)
+            sim.autotune(500)
         return sim, t_up
 
     def timed(sim, k, warm):

@@ -308,8 +308,8 @@ int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s);
  * the same for the hardware reciprocal (positive inputs); 9 the kernels'
  * fast division against a / b on pseudo-random pairs (pattern i of
  * [begin, begin + count) gives one pair; |a| in [2^-79, 2^63) or 0,
- * b in [2^-48, 2^40)); 10 a reciprocal-square-root sqrt with one
- * correction against sqrtf.
+ * b in [2^-48, 2^40)); 10 the hardware sqrt with Tuckerman's rounding
+ * test (an alternative to the kernels' sqrt, 1) against sqrtf.
  * *mismatches receives the number of differing results. */
 int mh_check_arith(int device, int which, uint32_t begin, uint32_t count,
                    uint64_t* mismatches);

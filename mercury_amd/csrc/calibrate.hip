@@ -88,8 +88,8 @@ __global__ void copy_f4(const float4* __restrict__ src, float4* __restrict__ dst
 //            pseudo-random pair drawn from the pattern (|a| in [2^-79, 2^63)
 //            or 0, b in [2^-48, 2^40), random signs of a; pairs whose
 //            quotient is not a normal float are skipped)
-//        10: a reciprocal-square-root sqrt, s = x rsq(x) refined once by
-//            s + (x - s s) rsq(x) / 2, vs sqrtf (a candidate short sequence)
+//        10: mh::sqrt_rn_tuckerman_inrange (hardware sqrt with Tuckerman's
+//            rounding test) vs sqrtf
 __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, uint32_t count,
                                                    uint32_t* out) {
   __shared__ uint32_t part[256];
@@ -132,10 +132,7 @@ __global__ __launch_bounds__(256) void check_arith(int which, uint32_t begin, ui
               ? mh::div_rn_from_rcp(x, y, mh::rcp_rn_inrange(y))
               : b;
     } else if (which == 10) {
-      const float y = __builtin_amdgcn_rsqf(x);
-      const float sx = x * y;
-      const float r = fmaf(-sx, sx, x);
-      a = fmaf(r, 0.5f * y, sx);
+      a = mh::sqrt_rn_tuckerman_inrange(x);
       b = sqrtf(x);
     } else if (which <= 6) {
       a = __builtin_amdgcn_sqrtf(x);

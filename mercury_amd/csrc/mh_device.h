@@ -46,13 +46,25 @@ __device__ __forceinline__ float act_sin(float x) {
 
 
 // Correctly rounded sqrt for x in [2^-96, 2^126] without the range
-// scaling and special-value handling of the general sequence: the hardware
-// square root (within 1 ulp) is corrected to the nearest float by the signs
-// of the residuals x - s*(s -+ 1 ulp), the rounding test of Tuckerman
-// (Markstein 1990, "Computation of elementary functions on the IBM RISC
-// System/6000 processor").  Checked against sqrtf for every float in range
-// by mh_check_arith.
+// scaling and special-value handling of the general sequence: with the
+// hardware reciprocal square root y, s = x y refined once by Newton's step
+// in residual form, s + (x - s s) (y / 2), the residual computed exactly by
+// a fused multiply-add (Markstein 1990, "Computation of elementary functions
+// on the IBM RISC System/6000 processor").  On gfx950 this equals sqrtf for
+// every float in the range (checked exhaustively by mh_check_arith); the
+// general sequence in the error analysis is not guaranteed to round
+// correctly, so the check is what admits it.
 __device__ __forceinline__ float sqrt_rn_inrange(float x) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  const float s = x * y;
+  const float r = fmaf(-s, s, x);
+  return fmaf(r, 0.5f * y, s);
+}
+
+// The previous short sequence, kept as a cross-check: the hardware square
+// root (within 1 ulp) corrected to the nearest float by the signs of the
+// residuals x - s*(s -+ 1 ulp), Tuckerman's rounding test.
+__device__ __forceinline__ float sqrt_rn_tuckerman_inrange(float x) {
   const float s = __builtin_amdgcn_sqrtf(x);
   const float s_dn = __uint_as_float(__float_as_uint(s) - 1u);
   const float s_up = __uint_as_float(__float_as_uint(s) + 1u);
