@@ -200,7 +200,7 @@ def main():
         sim.SetElements(els)
         t_up = time.perf_counter() - t
         if not args.no_autotune:
-            sim.autotune(500)
+            sim.autotune(2000)  # long enough for sustained clocks
         return sim, t_up
 
     def timed(sim, k, warm):
