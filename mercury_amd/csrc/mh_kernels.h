@@ -33,7 +33,7 @@ struct StepParams {
   const uint16_t* mperm;     // per pack: phase-2 visiting order of its masses
   const uint32_t* mrec;      // mass-parallel kernel: per-mass spring records
   const uint8_t* mcnt;       // mass-parallel kernel: record counts per mass (2 each)
-  const float* lbtab;        // mass-parallel kernel: per-pack rest-length tables
+  const float* lbtab;        // mass-parallel kernels: per-pack (rest length, material) pairs
   const uint16_t* ccsr;      // chunked kernel: per chunk, entries 3 * (spring - chunk start)
   const uint16_t* crow;      // chunked kernel: per chunk, 2 n_mass + 1 row bounds (padded to quads)
   const uint32_t* ctab;      // chunked kernel: per pack (at d.pad0), entry offset of each chunk + end
@@ -88,19 +88,28 @@ __host__ __device__ inline int pos_slots(int n) { return n + (n >> 3) + 1; }
 #endif
 __host__ __device__ constexpr int mass_slot(int i) { return i + (i >> MH_MASS_SKEW_SHIFT); }
 __host__ __device__ constexpr int mass_slots(int block) { return block + (block >> MH_MASS_SKEW_SHIFT) + 1; }
-// Mass-parallel kernels: distinct rest lengths per pack and materials (its
-// LDS tables hold 64 of each, the last one reserved for padding records)
-constexpr int kMassLbarMax = 63;
-constexpr int kMassMatMax = 63;
-// Their fixed LDS layout (byte offsets): rest lengths float[64] | stiffness
-// float[64] | actuation float[2][64] | actuation parameters float[3][64]
-// (step_mass; step_massg keeps a zero term at kMassZero there) | positions
-constexpr int kMassTab = 64;
-constexpr int kMassNull = kMassTab - 1;  // table entry of padding records
-constexpr int kMassLb = 0, kMassK = 256, kMassR = 512, kMassAct = 1024, kMassZero = 1792, kMassPos = 2048;
-// step_massg: byte offset of the term slots (after one position buffer)
-// and of the term of (lane, k)
-__host__ __device__ constexpr int massg_term_base(int block) { return kMassPos + 16 * mass_slots(block); }
+// Mass-parallel kernels: distinct (rest length, material) pairs per pack;
+// the LDS pair tables hold kMassPairMax, the last one reserved for padding
+// records.  Per pair: {L, k} (float2; L the actuated rest length), {1/k,
+// rest length} (float2) and the constants {dL0, omega, phi, rest length}
+// (float4), each in its own table so that a record's byte offset
+// pair * 8 indexes the first two.
+constexpr int kMassPairMax = 512;
+constexpr int kMassPairNull = kMassPairMax - 1;
+constexpr int kMassDynBytes = 8 * kMassPairMax;
+constexpr int kMassCstBytes = 16 * kMassPairMax;
+// Their LDS layout (byte offsets): a zero term and two flags (step_massg)
+// below kMassPos | nbuf position buffers float4[mass_slots(block)] | {L, k}
+// (step_mass: two buffers; step_massg: one) | {1/k, rest length} | the
+// constants | step_massg's term slots.
+constexpr int kMassZero = 0, kMassPos = 64;
+__host__ __device__ constexpr int mass_pair_base(int block, int nbuf) {
+  return kMassPos + 16 * nbuf * mass_slots(block);
+}
+// step_massg: byte offset of the term slots and of the term of (lane, k)
+__host__ __device__ constexpr int massg_term_base(int block) {
+  return mass_pair_base(block, 1) + 2 * kMassDynBytes + kMassCstBytes;
+}
 __host__ __device__ constexpr uint32_t massg_term_offset(int block, int maxp, uint32_t lane, uint32_t k) {
   return static_cast<uint32_t>(massg_term_base(block)) + 12u * (static_cast<uint32_t>(maxp) * lane + k);
 }

@@ -126,10 +126,10 @@ struct mh_sim {
   std::vector<uint32_t> mass_off, spring_off, active_off;  // per robot prefix offsets
   std::vector<uint32_t> gather_pad;  // per robot: sum over masses of deg rounded up to 4
   std::vector<uint8_t> max_side;     // per robot: most springs one mass has on one side (capped 255)
-  std::vector<std::vector<uint32_t>> mat_set;   // per robot: compact materials of its springs (sorted;
-                                                // empty when over the mass-parallel kernel's table)
-  std::vector<std::vector<uint32_t>> lbar_set;  // per robot: distinct rest lengths (bits, sorted;
-                                                // emptied past the mass kernel's table size)
+  // per robot: distinct (rest length, material) pairs of its springs as
+  // rest-length bits << 8 | compact material, sorted; empty when there are
+  // more than the mass-parallel kernels' pair table holds
+  std::vector<std::vector<uint64_t>> pair_set;
   std::vector<uint16_t> act_pairs;   // active springs, robot-local
   std::vector<uint32_t> active_src;  // active spring -> original spring index
   std::vector<uint8_t> act_mat;      // compact material index
@@ -228,9 +228,9 @@ bool fits_lds_alone(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
 
 bool fits_mass(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
   const int m = robot_masses(s, r);
-  // a robot with springs needs its rest lengths and materials in the
-  // tables (an empty set marks one with too many distinct values)
-  const bool table_ok = robot_active(s, r) == 0 || (!s->lbar_set[r].empty() && !s->mat_set[r].empty());
+  // a robot with springs needs its pairs in the table (an empty set marks
+  // one with too many)
+  const bool table_ok = robot_active(s, r) == 0 || !s->pair_set[r].empty();
   const size_t lds = v.chunked ? lds_bytes_massg(v.block, v.nspt) : lds_bytes_mass(v.block, n_mat);
   return m <= v.block && s->max_side[r] <= v.nspt && table_ok && lds <= static_cast<size_t>(s->lds_limit);
 }
@@ -280,7 +280,7 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
   lp.v = lv;
   stp.v = sv;
   int r = 0;
-  std::vector<uint32_t> lbar_union, mat_union, merged, merged_m;
+  std::vector<uint64_t> pair_union, merged;
   while (r < s->n_robots) {
     if (!lv || !fits_resident(s, *lv, r, n_mat)) {
       const int m = robot_masses(s, r);
@@ -302,26 +302,19 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
     }
     if (lv->kind == MH_KERNEL_MASS) {
       // consecutive robots while their masses fill one workgroup and their
-      // rest lengths and materials share one table each
+      // (rest length, material) pairs share one table
       PackDesc d{};
       d.mass_begin = s->mass_off[r];
       d.spring_begin = s->active_off[r];
-      lbar_union.clear();
-      mat_union.clear();
+      pair_union.clear();
       int count = 0;
       while (r < s->n_robots && count < cap && fits_mass(s, *lv, r, n_mat)) {
         const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
         merged.clear();
-        std::set_union(lbar_union.begin(), lbar_union.end(), s->lbar_set[r].begin(), s->lbar_set[r].end(),
+        std::set_union(pair_union.begin(), pair_union.end(), s->pair_set[r].begin(), s->pair_set[r].end(),
                        std::back_inserter(merged));
-        merged_m.clear();
-        std::set_union(mat_union.begin(), mat_union.end(), s->mat_set[r].begin(), s->mat_set[r].end(),
-                       std::back_inserter(merged_m));
-        if (m > lv->block || static_cast<int>(merged.size()) > kMassLbarMax ||
-            static_cast<int>(merged_m.size()) > kMassMatMax)
-          break;
-        lbar_union.swap(merged);
-        mat_union.swap(merged_m);
+        if (m > lv->block || static_cast<int>(merged.size()) > kMassPairNull) break;
+        pair_union.swap(merged);
         d.n_mass = static_cast<uint32_t>(m);
         d.n_spring += static_cast<uint32_t>(robot_active(s, r));
         r++;
@@ -445,17 +438,16 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
   });
 }
 
-// Mass-parallel part: per pack a table of its rest lengths followed by its
-// materials' compact indices (as u32 bits; d.pad0 = offset, d.pad1 =
-// rest-length count | material count << 16), the lane order of its masses
-// (most records first, in perm), and per lane its mass's
-// first-endpoint and second-endpoint records in ascending spring index, laid
-// out [part][k][lane] (part 2: pack-local spring index of the part-0
-// records), with the two counts per lane.  A record holds LDS byte offsets:
-// the other endpoint's skewed slot * 16 | pack material index * 4 << 16 |
-// rest-length index * 4 << 24.  For the gather kernel a second-endpoint
-// record is instead the LDS offset of the term slot its first endpoint
-// fills.
+// Mass-parallel part: per pack its (rest length, material) pairs, two
+// words each (the rest length, the compact material as u32 bits; d.pad0 =
+// offset, d.pad1 = pair count), the lane order of its masses (most records
+// first, in perm), and per lane its mass's first-endpoint and
+// second-endpoint records in ascending spring index, laid out
+// [part][k][lane] (part 2: pack-local spring index of the part-0 records),
+// with the two counts per lane.  A record holds LDS byte offsets: the other
+// endpoint's skewed slot * 16 | pair index * 8 << 16.  For the gather
+// kernel a second-endpoint record is instead the LDS offset of the term
+// slot its first endpoint fills.
 void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector<uint32_t>& mrec,
                  std::vector<uint8_t>& mcnt, std::vector<float>& lbtab) {
   const int P = part.v->nspt;
@@ -465,34 +457,29 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
     return static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
                             s->mass_off.begin()) - 1;
   };
-  // pass 1 (packs in parallel): the pack's tables, unions of its robots'
+  // pass 1 (packs in parallel): the pack's pairs, the union of its robots'
   // sets, sorted
-  std::vector<std::vector<uint32_t>> tables(n_packs), mtables(n_packs);
+  std::vector<std::vector<uint64_t>> tables(n_packs);
   parallel_for(n_packs, [&](size_t pi) {
     const PackDesc& d = part.packs[pi];
-    std::vector<uint32_t>& table = tables[pi];
-    std::vector<uint32_t>& mtable = mtables[pi];
-    std::vector<uint32_t> merged;
+    std::vector<uint64_t>& table = tables[pi];
+    std::vector<uint64_t> merged;
     for (int q = first_robot(d); q < s->n_robots && s->mass_off[q] < d.mass_begin + d.n_mass; q++) {
       merged.clear();
-      std::set_union(table.begin(), table.end(), s->lbar_set[q].begin(), s->lbar_set[q].end(),
+      std::set_union(table.begin(), table.end(), s->pair_set[q].begin(), s->pair_set[q].end(),
                      std::back_inserter(merged));
       table.swap(merged);
-      merged.clear();
-      std::set_union(mtable.begin(), mtable.end(), s->mat_set[q].begin(), s->mat_set[q].end(),
-                     std::back_inserter(merged));
-      mtable.swap(merged);
     }
-    if (table.size() > static_cast<size_t>(kMassLbarMax) || mtable.size() > static_cast<size_t>(kMassMatMax))
-      throw Error(MH_ERR_INTERNAL, "mass pack tables over the kernel's capacity");
+    if (table.size() > static_cast<size_t>(kMassPairNull))
+      throw Error(MH_ERR_INTERNAL, "mass pack pair table over the kernel's capacity");
   });
-  // offsets of every pack's tables, records and counts
+  // offsets of every pack's pairs, records and counts
   size_t lb_end = lbtab.size(), rec_end = mrec.size(), cnt_end = mcnt.size();
   for (size_t pi = 0; pi < n_packs; pi++) {
     PackDesc& d = part.packs[pi];
     d.pad0 = static_cast<uint32_t>(lb_end);
-    d.pad1 = static_cast<uint32_t>(tables[pi].size()) | static_cast<uint32_t>(mtables[pi].size()) << 16;
-    lb_end += tables[pi].size() + mtables[pi].size();
+    d.pad1 = static_cast<uint32_t>(tables[pi].size());
+    lb_end += 2 * tables[pi].size();
     d.csr_begin = static_cast<uint32_t>(rec_end);
     rec_end += static_cast<size_t>(3 * P) * d.n_mass;
     d.row_begin = static_cast<uint32_t>(cnt_end);
@@ -501,16 +488,18 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
   lbtab.resize(lb_end);
   mrec.resize(rec_end, 0u);
   mcnt.resize(cnt_end);
-  // pass 2 (packs in parallel): tables, lane order and records
+  // pass 2 (packs in parallel): pairs, lane order and records
   parallel_for(n_packs, [&](size_t pi) {
     const PackDesc& d = part.packs[pi];
-    const std::vector<uint32_t>& table = tables[pi];
-    const std::vector<uint32_t>& mtable = mtables[pi];
+    const std::vector<uint64_t>& table = tables[pi];
     const uint32_t nm = d.n_mass, ns = d.n_spring;
     const int r0 = first_robot(d);
     float* tab = lbtab.data() + d.pad0;
-    for (size_t t = 0; t < table.size(); t++) std::memcpy(&tab[t], &table[t], 4);
-    for (size_t t = 0; t < mtable.size(); t++) std::memcpy(&tab[table.size() + t], &mtable[t], 4);
+    for (size_t t = 0; t < table.size(); t++) {
+      const uint32_t lb_bits = static_cast<uint32_t>(table[t] >> 8), mat = static_cast<uint32_t>(table[t] & 0xffu);
+      std::memcpy(&tab[2 * t], &lb_bits, 4);
+      std::memcpy(&tab[2 * t + 1], &mat, 4);
+    }
     uint32_t* out = mrec.data() + d.csr_begin;
     // record counts per mass, then the lane order: most first-endpoint
     // records first, then most second-endpoint records (ties keep order)
@@ -535,14 +524,15 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
       if (fill_p[i] > static_cast<uint32_t>(P) || fill_m[i] > static_cast<uint32_t>(P))
         throw Error(MH_ERR_INTERNAL, "mass record lists over the kernel's capacity");
     }
-    // table positions by value: rest lengths (bit patterns) and materials
-    auto index_of = [](const std::vector<uint32_t>& t, uint32_t v) {
-      return static_cast<uint32_t>(std::lower_bound(t.begin(), t.end(), v) - t.begin());
+    // table position of a pair by value
+    auto index_of = [&table](uint64_t v) {
+      return static_cast<uint32_t>(std::lower_bound(table.begin(), table.end(), v) - table.begin());
     };
     std::fill(fill_p.begin(), fill_p.end(), 0u);
     std::fill(fill_m.begin(), fill_m.end(), 0u);
     q = r0;
-    uint32_t last_bits = ~0u, last_li = 0;
+    uint64_t last_key = ~0ull;
+    uint32_t last_pi = 0;
     for (uint32_t k = 0; k < ns; k++) {  // ascending spring index
       const uint32_t g = d.spring_begin + k;
       while (g >= s->active_off[q + 1]) q++;
@@ -551,13 +541,12 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
       const uint32_t la = lane[a], lb = lane[b];
       uint32_t bits;
       std::memcpy(&bits, &s->act_lbar[g], 4);
-      if (bits != last_bits) {
-        last_bits = bits;
-        last_li = index_of(table, bits);
+      const uint64_t key = static_cast<uint64_t>(bits) << 8 | s->act_mat[g];
+      if (key != last_key) {
+        last_key = key;
+        last_pi = index_of(key);
       }
-      const uint32_t li = last_li;
-      const uint32_t mi = index_of(mtable, static_cast<uint32_t>(s->act_mat[g]));
-      const uint32_t tail = ((4u * mi) << 16) | ((4u * li) << 24);
+      const uint32_t tail = (8u * last_pi) << 16;
       const uint32_t ka = fill_p[a]++;
       out[static_cast<size_t>(2 * P + ka) * nm + la] = k;
       out[static_cast<size_t>(ka) * nm + la] = 16u * static_cast<uint32_t>(mass_slot(static_cast<int>(b))) | tail;
@@ -1032,34 +1021,31 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   if (compact_to_id.empty()) compact_to_id.push_back(0);  // keeps tables non-empty
 
   // per robot: gather entries with both parts of every row rounded up to 4;
-  // the most springs one mass has on one side; distinct rest lengths and
-  // materials (each set is left empty when it exceeds the mass-parallel
-  // kernel's tables)
+  // the most springs one mass has on one side; distinct (rest length,
+  // material) pairs (left empty when they exceed the mass-parallel kernels'
+  // table)
   std::vector<uint32_t> gpad(R, 0);
   std::vector<uint8_t> mside(R, 0);
-  std::vector<std::vector<uint32_t>> lsets(R), msets(R);
+  std::vector<std::vector<uint64_t>> psets(R);
   mh::parallel_for(static_cast<size_t>(R), [&](size_t ri) {
     const int r = static_cast<int>(ri);
     {
       std::vector<uint32_t> plus(mo[r + 1] - mo[r], 0), minus(mo[r + 1] - mo[r], 0);
-      std::vector<uint32_t>& ls = lsets[r];
-      std::vector<uint32_t>& mset = msets[r];
-      auto add = [](std::vector<uint32_t>& set, uint32_t v, int cap) {
-        if (set.size() <= static_cast<size_t>(cap)) {
-          auto it = std::lower_bound(set.begin(), set.end(), v);
-          if (it == set.end() || *it != v) set.insert(it, v);
-        }
-      };
+      std::vector<uint64_t>& ps = psets[r];
+      uint64_t last = ~0ull;
       for (uint32_t k = ao[r]; k < ao[r + 1]; k++) {
         plus[ap[2 * k]]++;
         minus[ap[2 * k + 1]]++;
         uint32_t bits;
         std::memcpy(&bits, &alb[k], 4);
-        add(ls, bits, mh::kMassLbarMax);
-        add(mset, amat[k], mh::kMassMatMax);
+        const uint64_t key = static_cast<uint64_t>(bits) << 8 | amat[k];
+        if (key != last && ps.size() <= static_cast<size_t>(mh::kMassPairNull)) {
+          last = key;
+          auto it = std::lower_bound(ps.begin(), ps.end(), key);
+          if (it == ps.end() || *it != key) ps.insert(it, key);
+        }
       }
-      if (ls.size() > static_cast<size_t>(mh::kMassLbarMax)) ls.clear();
-      if (mset.size() > static_cast<size_t>(mh::kMassMatMax)) mset.clear();
+      if (ps.size() > static_cast<size_t>(mh::kMassPairNull)) ps.clear();
       uint32_t g = 0, side = 0;
       for (size_t i = 0; i < plus.size(); i++) {
         g += ((plus[i] + 3u) & ~3u) + ((minus[i] + 3u) & ~3u);
@@ -1095,8 +1081,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   s->active_off = ao;
   s->gather_pad = std::move(gpad);
   s->max_side = std::move(mside);
-  s->lbar_set = std::move(lsets);
-  s->mat_set = std::move(msets);
+  s->pair_set = std::move(psets);
   s->act_pairs = std::move(ap);
   s->act_lbar = alb;
   s->fast_div_lbar = true;

@@ -121,12 +121,11 @@ __device__ __forceinline__ float4 lds_position(const MassState& s) {
   return make_float4(s.x, s.y, s.z, 0.f);
 }
 
-// Spring term of one spring: returns (cx, cy, cz); xpbd also yields lambda.
+// Spring term of one spring with actuated rest length L: returns
+// (cx, cy, cz); xpbd also yields lambda.
 template <int SOLVER>
-__device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, float R,
-                                            float kk, float& cx, float& cy, float& cz,
-                                            float& lambda) {
-  const float L = fmaf(lbar, R, lbar);
+__device__ __forceinline__ void spring_term_L(float4 pa, float4 pb, float L, float kk, float& cx, float& cy,
+                                              float& cz, float& lambda) {
   const float dx = pa.x - pb.x;
   const float dy = pa.y - pb.y;
   const float dz = pa.z - pb.z;
@@ -150,6 +149,15 @@ __device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, fl
     cy = f * dy;
     cz = f * dz;
   }
+}
+
+// The same from the rest length lbar and actuation factor R:
+// L = fma(lbar, R, lbar).
+template <int SOLVER>
+__device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, float R,
+                                            float kk, float& cx, float& cy, float& cz,
+                                            float& lambda) {
+  spring_term_L<SOLVER>(pa, pb, fmaf(lbar, R, lbar), kk, cx, cy, cz, lambda);
 }
 
 // ----------------------------------------------------------------- LDS kernel
@@ -582,34 +590,80 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
 // (the same operands in the same order, so both get the same bits), which
 // removes the spring-term buffer and gather lists from LDS and leaves one
 // barrier per substep.  LDS (fixed layout, so table and buffer bases are
-// instruction offsets): rest lengths float[64] | stiffness float[64] |
-// actuation float[2][64] | actuation parameters (dL0, omega, phi)
-// float[3][64] | positions float4[2][mass_slots(BLOCK)] (read one
-// buffer, write the other; skewed slots mass_slot(i); the last slot of
-// each buffer is a far-away ghost that padding records point at).  Lanes
-// take masses in the order of p.mperm (most records first, so a wave's
-// lists have similar lengths).  Each lane keeps its records in registers:
-// MAXP first-endpoint records then MAXP second-endpoint records, each in
+// instruction offsets): a small area below kMassPos | positions
+// float4[2][mass_slots(BLOCK)] (read one buffer, write the other; skewed
+// slots mass_slot(i); the last slot of each buffer is a far-away ghost that
+// padding records point at) | the pack's pair table, float2[2][kMassPairMax]
+// {L, k} per (rest length, material) pair, L the actuated rest length of the
+// substep that reads the buffer | float2[kMassPairMax] {1/k, rest length} |
+// the pairs' constants float4[kMassPairMax] {dL0, omega, phi, rest
+// length}.  Each substep the
+// lanes below the pair count compute the next substep's L of one pair each
+// (the actuation once per pair instead of once per spring).  Lanes take
+// masses in the order of p.mperm (most records first, so a wave's lists
+// have similar lengths).  Each lane keeps its records in registers: MAXP
+// first-endpoint records then MAXP second-endpoint records, each in
 // ascending spring index, padded to the wave's longest list with records of
-// the null material (last table entry), whose terms are signed zeros; a
-// record holds byte offsets: other endpoint's
-// slot * 16 | material * 4 << 16 | rest-length index * 4 << 24.  Records
-// live in p.mrec as [part][k][lane] (part 2: the pack-local spring index of
-// the part-0 records, for the stress output); p.mcnt holds the two counts
-// per lane.
+// the null pair (last entry: stiffness 0 for force, +inf for xpbd), whose
+// terms are signed zeros; a record holds byte offsets: the other endpoint's
+// slot * 16 | pair index * 8 << 16.  Records live in p.mrec as
+// [part][k][lane] (part 2: the pack-local spring index of the part-0
+// records, for the stress output); p.mcnt holds the two counts per lane.
 template <int BLOCK>
 constexpr int mass_pos_bytes() { return 16 * mass_slots(BLOCK); }
 
-// One substep reading position buffer CUR and writing the other.
+// Fills a pack's pair tables (d.pad0: p.lbtab offset of its pairs, two
+// words each: rest length, compact material; d.pad1: pair count) for the
+// time t: the constants, {1/k, rest length} and {L(t), k} in both dynamic
+// buffers (the second's L is overwritten before it is read).  null_k and
+// null_ky are the null pair's stiffness and reciprocal entries.
+template <int SOLVER, int BLOCK>
+__device__ __forceinline__ void mass_pairs_init(unsigned char* smem, const StepParams& p, const PackDesc& d,
+                                                int dyn0, int dyn1, int aux, int cst, float t, float null_k,
+                                                float null_ky) {
+  const int n = static_cast<int>(d.pad1);
+  for (int i = static_cast<int>(threadIdx.x); i < n; i += BLOCK) {
+    const float lb = p.lbtab[d.pad0 + 2 * i];
+    const uint32_t c = __float_as_uint(p.lbtab[d.pad0 + 2 * i + 1]);
+    const float k = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[c] : p.mat_k[c];
+    const float4 cs = make_float4(p.mat_dL0[c], p.mat_omega[c], p.mat_phi[c], lb);
+    const float L = fmaf(lb, actuation(cs.x, cs.y, cs.z, t), lb);
+    const float ky = (SOLVER == MH_SOLVER_XPBD_REF) ? 1.0f / k : 0.0f;
+    reinterpret_cast<float4*>(smem + cst)[i] = cs;
+    reinterpret_cast<float2*>(smem + aux)[i] = make_float2(ky, lb);
+    reinterpret_cast<float2*>(smem + dyn0)[i] = make_float2(L, k);
+    reinterpret_cast<float2*>(smem + dyn1)[i] = make_float2(L, k);
+  }
+  if (threadIdx.x == 0) {
+    reinterpret_cast<float4*>(smem + cst)[kMassPairNull] = make_float4(0.f, 0.f, 0.f, 1.f);
+    reinterpret_cast<float2*>(smem + aux)[kMassPairNull] = make_float2(null_ky, 1.f);
+    reinterpret_cast<float2*>(smem + dyn0)[kMassPairNull] = make_float2(1.f, null_k);
+    reinterpret_cast<float2*>(smem + dyn1)[kMassPairNull] = make_float2(1.f, null_k);
+  }
+}
+
+// The next substep's actuated rest lengths: L(t) of every pair into dyn.
+template <int BLOCK>
+__device__ __forceinline__ void mass_pairs_advance(unsigned char* smem, int n_pairs, int dyn, int cst, float t) {
+  for (int i = static_cast<int>(threadIdx.x); i < n_pairs; i += BLOCK) {
+    const float4 cs = reinterpret_cast<const float4*>(smem + cst)[i];
+    reinterpret_cast<float*>(smem + dyn)[2 * i] = fmaf(cs.w, actuation(cs.x, cs.y, cs.z, t), cs.w);
+  }
+}
+
+// One substep reading position and pair buffer CUR and writing the other.
 template <int SOLVER, int BLOCK, int MAXP, bool STRESS, int CUR>
 __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepParams& p, MassState& ms,
                                              uint32_t (&rp)[MAXP], uint32_t (&rm)[MAXP],
                                              float (&st)[MAXP], int np, int nq, int np_w, int nq_w,
-                                             bool live, int pslot, int n_pm, float t) {
+                                             bool live, int pslot, int n_pairs, float t) {
   constexpr int kP0 = kMassPos + CUR * mass_pos_bytes<BLOCK>();
   constexpr int kP1 = kMassPos + (1 - CUR) * mass_pos_bytes<BLOCK>();
-  constexpr int kR0 = kMassR + CUR * 4 * kMassTab;
-  constexpr int kR1 = kMassR + (1 - CUR) * 4 * kMassTab;
+  constexpr int kPairs = mass_pair_base(BLOCK, 2);
+  constexpr int kD0 = kPairs + CUR * kMassDynBytes;
+  constexpr int kD1 = kPairs + (1 - CUR) * kMassDynBytes;
+  constexpr int kAux = kPairs + 2 * kMassDynBytes;
+  constexpr int kCst = kPairs + 3 * kMassDynBytes;
   // idle lanes sit at the origin, 1e3*sqrt(3) from the ghost, so their
   // padding terms stay in the fast arithmetic range
   const float4 own = live ? *reinterpret_cast<const float4*>(smem + kP0 + 16 * pslot)
@@ -618,21 +672,20 @@ __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepPara
 #pragma unroll
   for (int k = 0; k < MAXP; ++k) {
     if (k < np_w) {
-      // opaque to the optimizer: decoding the record per substep costs a
-      // few VALU, hoisting every decoded field out of the step loop would
-      // cost registers
+      // opaque to the optimizer: decoding the record per substep costs two
+      // VALU, hoisting every decoded field out of the step loop would cost
+      // registers
       asm volatile("" : "+v"(rp[k]));
       const uint32_t r = rp[k];
-      const uint32_t mo = (r >> 16) & 0xffu;
-      const float lb = *reinterpret_cast<const float*>(smem + kMassLb + (r >> 24));
+      const float2 Lk = *reinterpret_cast<const float2*>(smem + kD0 + (r >> 16));
       const float4 other = *reinterpret_cast<const float4*>(smem + kP0 + (r & 0xffffu));
       float cx, cy, cz, lambda;
-      spring_term<SOLVER>(own, other, lb, *reinterpret_cast<const float*>(smem + kR0 + mo),
-                          *reinterpret_cast<const float*>(smem + kMassK + mo), cx, cy, cz, lambda);
+      spring_term_L<SOLVER>(own, other, Lk.x, Lk.y, cx, cy, cz, lambda);
       // padding records add a signed zero, which leaves a sum that
       // started at +0 unchanged
       ax = ax + cx; ay = ay + cy; az = az + cz;
-      if (STRESS && k < np) st[k] = st[k] + lambda / lb;
+      if (STRESS && k < np)
+        st[k] = st[k] + lambda / *reinterpret_cast<const float*>(smem + kAux + (r >> 16) + 4);
     }
     // one record at a time: keeps the unrolled loop's registers bounded
     __builtin_amdgcn_sched_barrier(0);
@@ -642,24 +695,17 @@ __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepPara
     if (k < nq_w) {
       asm volatile("" : "+v"(rm[k]));
       const uint32_t r = rm[k];
-      const uint32_t mo = (r >> 16) & 0xffu;
-      const float lb = *reinterpret_cast<const float*>(smem + kMassLb + (r >> 24));
+      const float2 Lk = *reinterpret_cast<const float2*>(smem + kD0 + (r >> 16));
       const float4 other = *reinterpret_cast<const float4*>(smem + kP0 + (r & 0xffffu));
       float cx, cy, cz, lambda;
-      spring_term<SOLVER>(other, own, lb, *reinterpret_cast<const float*>(smem + kR0 + mo),
-                          *reinterpret_cast<const float*>(smem + kMassK + mo), cx, cy, cz, lambda);
+      spring_term_L<SOLVER>(other, own, Lk.x, Lk.y, cx, cy, cz, lambda);
       bx = bx + cx; by = by + cy; bz = bz + cz;
     }
     __builtin_amdgcn_sched_barrier(0);
   }
   if (live)
     *reinterpret_cast<float4*>(smem + kP1 + 16 * pslot) = integrate<SOLVER>(ms, ax - bx, ay - by, az - bz, p);
-  const int tid = threadIdx.x;
-  if (tid < n_pm) {
-    const float* act = reinterpret_cast<const float*>(smem + kMassAct);
-    *reinterpret_cast<float*>(smem + kR1 + 4 * tid) =
-        actuation(act[tid], act[kMassTab + tid], act[2 * kMassTab + tid], t + p.dt);
-  }
+  mass_pairs_advance<BLOCK>(smem, n_pairs, kD1, kCst, t + p.dt);
   __syncthreads();
 }
 
@@ -672,6 +718,7 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   const int nm = static_cast<int>(d.n_mass);
   constexpr int kSlots = mass_slots(BLOCK);
   constexpr int kGhost = kSlots - 1;
+  constexpr int kPairs = mass_pair_base(BLOCK, 2);
   const int tid = threadIdx.x;
   const bool live = tid < nm;
   const uint32_t* rec = p.mrec + d.csr_begin;
@@ -680,8 +727,8 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   uint32_t rp[MAXP], rm[MAXP];
   float st[MAXP];
   int np = 0, nq = 0, mass = 0;
-  // padding record: the ghost, with the null material and rest length
-  const uint32_t pad = 16u * kGhost | (4u * kMassNull) << 16 | (4u * kMassNull) << 24;
+  // padding record: the ghost, with the null pair
+  const uint32_t pad = 16u * kGhost | (8u * kMassPairNull) << 16;
 #pragma unroll
   for (int k = 0; k < MAXP; ++k) {
     rp[k] = pad;
@@ -709,31 +756,13 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
     pos0[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
     pos1[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
   }
-  float* s_lb = reinterpret_cast<float*>(smem + kMassLb);
-  float* s_k = reinterpret_cast<float*>(smem + kMassK);
-  float* s_R = reinterpret_cast<float*>(smem + kMassR);
-  float* s_act = reinterpret_cast<float*>(smem + kMassAct);
-  // the pack's tables: d.pad1 = rest-length count | material count << 16;
-  // the compact material indices follow the rest lengths in p.lbtab
-  const int n_lb = static_cast<int>(d.pad1 & 0xffffu), n_pm = static_cast<int>(d.pad1 >> 16);
-  for (int i = tid; i < n_lb; i += BLOCK) s_lb[i] = p.lbtab[d.pad0 + i];
   float t = p.t0;
-  if (tid < n_pm) {  // n_pm < kMassTab <= BLOCK
-    const uint32_t c = __float_as_uint(p.lbtab[d.pad0 + n_lb + tid]);
-    s_k[tid] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[c] : p.mat_k[c];
-    s_act[tid] = p.mat_dL0[c];
-    s_act[kMassTab + tid] = p.mat_omega[c];
-    s_act[2 * kMassTab + tid] = p.mat_phi[c];
-    s_R[tid] = actuation(p.mat_dL0[c], p.mat_omega[c], p.mat_phi[c], t);
-  }
-  if (tid == 0) {
-    // null entries: a term with stiffness 0 (force) or compliance-divisor
-    // +inf (xpbd) is a signed zero
-    s_lb[kMassNull] = 1.0f;
-    s_k[kMassNull] = (SOLVER == MH_SOLVER_XPBD_REF) ? __builtin_inff() : 0.0f;
-    s_R[kMassNull] = 0.0f;
-    s_R[kMassTab + kMassNull] = 0.0f;
-  }
+  // null pair: a term with stiffness 0 (force) or divisor +inf (xpbd) is a
+  // signed zero
+  mass_pairs_init<SOLVER, BLOCK>(smem, p, d, kPairs, kPairs + kMassDynBytes, kPairs + 2 * kMassDynBytes,
+                                 kPairs + 3 * kMassDynBytes, t,
+                                 (SOLVER == MH_SOLVER_XPBD_REF) ? __builtin_inff() : 0.0f, 0.0f);
+  const int n_pairs = static_cast<int>(d.pad1);
   // wave-wide loop bounds
   int np_w = np, nq_w = nq;
   for (int off = 32; off > 0; off >>= 1) {
@@ -745,13 +774,13 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   // substeps in pairs, so each buffer's base is a constant offset
   int step = 0;
   for (; step + 1 < p.n_steps; step += 2) {
-    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
+    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pairs, t);
     t = t + p.dt;
-    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 1>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
+    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 1>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pairs, t);
     t = t + p.dt;
   }
   if (step < p.n_steps)
-    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pm, t);
+    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pairs, t);
 
   if (live) {
     const uint32_t gi = d.mass_begin + mass;
@@ -770,11 +799,10 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
 // (coordinates 0 or of magnitude in [2^-56, 2^20]; stiffness divisor in
 // [2^-40, 2^40], rest lengths in [2^-40, 2^20], amplitude at most 1/2); the
 // wave then takes the short sequence if every lane's squared length is also
-// in the fast sqrt range.  ky = RN(1/kk); 0 marks the null material, whose
-// term is a signed zero.  Same results as spring_term<XPBD> bit for bit.
-__device__ __forceinline__ void xpbd_term(float4 pa, float4 pb, float lbar, float R, float kk, float ky,
-                                          bool fast_ok, float& cx, float& cy, float& cz, float& lambda) {
-  const float L = fmaf(lbar, R, lbar);
+// in the fast sqrt range.  ky = RN(1/kk); 0 marks the null pair, whose
+// term is a signed zero.  Same results as spring_term_L<XPBD> bit for bit.
+__device__ __forceinline__ void xpbd_term_L(float4 pa, float4 pb, float L, float kk, float ky,
+                                            bool fast_ok, float& cx, float& cy, float& cz, float& lambda) {
   const float dx = pa.x - pb.x;
   const float dy = pa.y - pb.y;
   const float dz = pa.z - pb.z;
@@ -821,15 +849,15 @@ __device__ __forceinline__ bool position_in_div_range(float4 q) {
 // Barrier.  Phase 2: the lane adds the terms of the springs whose second
 // endpoint it is to M, reading them from their first endpoints' slots,
 // integrates with P - M and writes its position in place; the next
-// actuation table is written in place too.  Barrier.  Two barriers per
-// substep, half the spring arithmetic of step_mass.  LDS: the tables of
-// step_mass with one actuation table and a zero term at kMassZero |
-// positions float4[mass_slots(BLOCK)] | terms.  First-endpoint records as in
-// step_mass; a second-endpoint record is the LDS byte offset of its term
-// (padding: the zero term).  XPBD: the reciprocals of the stiffness divisors
-// sit where step_mass keeps its actuation parameters, and two flags (at
-// kMassZero + 16, one per substep parity) record whether any position
-// written for that substep left the fast division's range.
+// substep's actuated rest lengths are written in place too.  Barrier.  Two
+// barriers per substep, half the spring arithmetic of step_mass.  LDS: a
+// zero term at kMassZero and two flags at kMassZero + 16 | positions
+// float4[mass_slots(BLOCK)] | the pair tables {L, k}, {1/k, rest length}
+// and the constants, as in step_mass but one {L, k} buffer | terms.
+// First-endpoint records as in step_mass; a second-endpoint record is the
+// LDS byte offset of its term (padding: the zero term).  XPBD: the flags
+// (one per substep parity) record whether any position written for that
+// substep left the fast division's range.
 template <int SOLVER, int BLOCK, int MAXP, bool STRESS>
 __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -837,6 +865,9 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   const int nm = static_cast<int>(d.n_mass);
   constexpr int kSlots = mass_slots(BLOCK);
   constexpr int kGhost = kSlots - 1;
+  constexpr int kDyn = mass_pair_base(BLOCK, 1);
+  constexpr int kAux = kDyn + kMassDynBytes;
+  constexpr int kCst = kDyn + 2 * kMassDynBytes;
   constexpr int kTerm = massg_term_base(BLOCK);
   const int tid = threadIdx.x;
   const bool live = tid < nm;
@@ -846,7 +877,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   uint32_t rp[MAXP], rm[MAXP];
   float st[MAXP];
   int np = 0, nq = 0, mass = 0;
-  const uint32_t pad = 16u * kGhost | (4u * kMassNull) << 16 | (4u * kMassNull) << 24;
+  const uint32_t pad = 16u * kGhost | (8u * kMassPairNull) << 16;
 #pragma unroll
   for (int k = 0; k < MAXP; ++k) {
     rp[k] = pad;
@@ -868,30 +899,15 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   const int pslot = live ? mass_slot(mass) : kGhost;
   float4* pos = reinterpret_cast<float4*>(smem + kMassPos);
   if (live) pos[pslot] = lds_position<SOLVER>(ms);
-  float* s_lb = reinterpret_cast<float*>(smem + kMassLb);
-  float* s_k = reinterpret_cast<float*>(smem + kMassK);
-  float* s_R = reinterpret_cast<float*>(smem + kMassR);
-  const int n_lb = static_cast<int>(d.pad1 & 0xffffu), n_pm = static_cast<int>(d.pad1 >> 16);
-  for (int i = tid; i < n_lb; i += BLOCK) s_lb[i] = p.lbtab[d.pad0 + i];
-  float t = p.t0;
-  float a_dl = 0.f, a_om = 0.f, a_ph = 0.f;  // lanes < n_pm: their material's actuation
-  float* s_ky = reinterpret_cast<float*>(smem + kMassAct);
   uint32_t* s_flag = reinterpret_cast<uint32_t*>(smem + kMassZero + 16);
-  if (tid < n_pm) {  // n_pm < kMassTab <= BLOCK
-    const uint32_t c = __float_as_uint(p.lbtab[d.pad0 + n_lb + tid]);
-    s_k[tid] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[c] : p.mat_k[c];
-    if (SOLVER == MH_SOLVER_XPBD_REF) s_ky[tid] = 1.0f / p.mat_kx[c];
-    a_dl = p.mat_dL0[c];
-    a_om = p.mat_omega[c];
-    a_ph = p.mat_phi[c];
-    s_R[tid] = actuation(a_dl, a_om, a_ph, t);
-  }
+  float t = p.t0;
+  // one dynamic buffer, updated in place in phase 2; null pair: stiffness 0
+  // (force) or 1 with reciprocal 0 (xpbd), a signed zero either way
+  mass_pairs_init<SOLVER, BLOCK>(smem, p, d, kDyn, kDyn, kAux, kCst, t,
+                                 (SOLVER == MH_SOLVER_XPBD_REF) ? 1.0f : 0.0f, 0.0f);
+  const int n_pairs = static_cast<int>(d.pad1);
   if (tid == 0) {
     pos[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
-    s_lb[kMassNull] = 1.0f;
-    s_k[kMassNull] = (SOLVER == MH_SOLVER_XPBD_REF) ? 1.0f : 0.0f;
-    if (SOLVER == MH_SOLVER_XPBD_REF) s_ky[kMassNull] = 0.0f;
-    s_R[kMassNull] = 0.0f;
     float* z = reinterpret_cast<float*>(smem + kMassZero);
     z[0] = z[1] = z[2] = 0.0f;
     s_flag[0] = 0u;
@@ -921,23 +937,22 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
     for (int k = 0; k < MAXP; ++k) {
       if (k < np_w) {
         const uint32_t r = rp[k];
-        const uint32_t mo = (r >> 16) & 0xffu;
-        const float lb = *reinterpret_cast<const float*>(smem + kMassLb + (r >> 24));
         const float4 other = *reinterpret_cast<const float4*>(smem + kMassPos + (r & 0xffffu));
         float cx, cy, cz, lambda;
-        if constexpr (SOLVER == MH_SOLVER_XPBD_REF)
-          xpbd_term(own, other, lb, *reinterpret_cast<const float*>(smem + kMassR + mo),
-                    *reinterpret_cast<const float*>(smem + kMassK + mo),
-                    *reinterpret_cast<const float*>(smem + kMassAct + mo), fast_ok, cx, cy, cz, lambda);
-        else
-          spring_term<SOLVER>(own, other, lb, *reinterpret_cast<const float*>(smem + kMassR + mo),
-                              *reinterpret_cast<const float*>(smem + kMassK + mo), cx, cy, cz, lambda);
+        if constexpr (SOLVER == MH_SOLVER_XPBD_REF) {
+          const float2 e = *reinterpret_cast<const float2*>(smem + kDyn + (r >> 16));
+          const float2 a = *reinterpret_cast<const float2*>(smem + kAux + (r >> 16));
+          xpbd_term_L(own, other, e.x, e.y, a.x, fast_ok, cx, cy, cz, lambda);
+          if (STRESS && k < np) st[k] = st[k] + lambda / a.y;
+        } else {
+          const float2 Lk = *reinterpret_cast<const float2*>(smem + kDyn + (r >> 16));
+          spring_term_L<SOLVER>(own, other, Lk.x, Lk.y, cx, cy, cz, lambda);
+        }
         ax = ax + cx; ay = ay + cy; az = az + cz;
         float* slot = reinterpret_cast<float*>(mine + 12 * k);
         slot[0] = cx;
         slot[1] = cy;
         slot[2] = cz;
-        if (STRESS && k < np) st[k] = st[k] + lambda / lb;
       }
     }
     __syncthreads();
@@ -954,7 +969,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
       pos[pslot] = np4;
       if (SOLVER == MH_SOLVER_XPBD_REF && !position_in_div_range(np4)) atomicOr(&s_flag[(step + 1) & 1], 1u);
     }
-    if (tid < n_pm) s_R[tid] = actuation(a_dl, a_om, a_ph, t + p.dt);
+    mass_pairs_advance<BLOCK>(smem, n_pairs, kDyn, kCst, t + p.dt);
     t = t + p.dt;
     __syncthreads();
   }
@@ -1031,10 +1046,10 @@ size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat) {
 }
 
 size_t lds_bytes_mass(int block, int n_mat) {
-  // fixed tables (rest lengths, stiffness, two actuation tables, actuation
-  // parameters) and two position buffers sized for the block
+  // the small area, two position buffers, two {L, k} buffers, {1/k, rest
+  // length} and the pair constants
   (void)n_mat;
-  return static_cast<size_t>(kMassPos) + 2u * 16u * static_cast<size_t>(mass_slots(block));
+  return static_cast<size_t>(mass_pair_base(block, 2)) + 3u * kMassDynBytes + kMassCstBytes;
 }
 
 size_t lds_bytes_massg(int block, int maxp) {
