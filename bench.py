@@ -41,11 +41,12 @@ HBM_PEAK_GBS = 8000.0
 LDS_BYTES_PER_SPRING = 57.0
 LDS_BYTES_PER_MASS = 48.0
 # mass-parallel kernel: both endpoints evaluate a spring, each reading the
-# other endpoint's position (12 B), the rest length (4 B) and the material's
-# stiffness and actuation (8 B); per mass its own position read and write.
-# The gather flavour moves the same bytes: the first endpoint reads 24 B and
-# writes the 12 B term, the second endpoint reads the term.
-MASS_LDS_BYTES_PER_SPRING = 48.0
+# other endpoint's position (12 B) and its (rest length, material) pair's
+# actuated length and stiffness (8 B); per mass its own position read and
+# write.  The gather flavour: the first endpoint reads 20 B and writes the
+# 12 B term, the second endpoint reads the term.
+MASS_LDS_BYTES_PER_SPRING = 40.0
+MASSG_LDS_BYTES_PER_SPRING = 44.0
 MASS_LDS_BYTES_PER_MASS = 32.0
 # MI355X nominal LDS bandwidth: 256 CUs x 128 B/clk x 2.4 GHz
 LDS_NOMINAL_GBS = 256 * 128 * 2.4
@@ -235,8 +236,9 @@ def main():
     hbm_extra = None
     if info.kernel & (mh.MH_KERNEL_MASS | mh.MH_KERNEL_MASS_GATHER) and not info.kernel & mh.MH_KERNEL_STREAM:
         bound, peak, unit = "lds", lds_peak, "GB/s"
-        per_launch = (MASS_LDS_BYTES_PER_SPRING * n_springs + MASS_LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
-        per_unit = MASS_LDS_BYTES_PER_SPRING + MASS_LDS_BYTES_PER_MASS * n_masses / n_springs
+        bps = MASSG_LDS_BYTES_PER_SPRING if info.kernel & mh.MH_KERNEL_MASS_GATHER else MASS_LDS_BYTES_PER_SPRING
+        per_launch = (bps * n_springs + MASS_LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
+        per_unit = bps + MASS_LDS_BYTES_PER_MASS * n_masses / n_springs
     elif info.kernel == 1 or info.stream_chunk > 0:
         # LDS-resident kernel, or chunked streaming kernel whose terms also
         # live in LDS: LDS bytes over all springs

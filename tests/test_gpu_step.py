@@ -226,3 +226,50 @@ def test_edge_cases():
     bad2.mass = np.zeros(27, np.float32)
     with pytest.raises(mh.MhError):
         mh.Simulator(ms, mh.SimConfig(solver=FORCE)).SetElements([bad2])
+
+
+def ring_robot(ms, rng, n, k, spacing=0.01):
+    """n masses on a jittered grid, spring (i, i + j) for j = 1..k: every
+    spring has its own rest length, so the robot has n*k distinct (rest
+    length, material) pairs."""
+    from mercury_amd.robots import Element
+    side = int(np.ceil(n ** (1 / 3)))
+    g = np.stack(np.unravel_index(np.arange(n), (side, side, side)), 1).astype(np.float32) * spacing
+    pos = (g + rng.uniform(-0.2, 0.2, g.shape).astype(np.float32) * spacing + np.float32(0.002)).astype(np.float32)
+    vox = mh.voxel_build(ms, 3, 10.0, np.full(27, 2, np.uint32), mass=0.001)
+    mats = np.unique(vox.spring_material[vox.spring_material != 0])
+    pairs = np.array([(i, (i + j) % n) for i in range(n) for j in range(1, k + 1)], np.uint16)
+    lengths = mh.robots.rest_lengths(pos, pairs) * rng.uniform(0.9, 1.1, len(pairs)).astype(np.float32)
+    return Element(pos, np.full(n, vox.mass_material[0], np.uint8), pairs, lengths.astype(np.float32),
+                   rng.choice(mats, len(pairs)).astype(np.uint8), mass=np.full(n, 0.001, np.float32))
+
+
+@pytest.mark.parametrize("family", [mh.MH_KERNEL_MASS, mh.MH_KERNEL_MASS_GATHER])
+@pytest.mark.parametrize("solver", [FORCE, XPBD])
+def test_mass_pair_table_limits(family, solver):
+    """Robots of 160 distinct (rest length, material) pairs: packs of the
+    mass-parallel kernels close when the union of their pairs would pass
+    511, and a robot with 600 pairs goes to the streaming kernel."""
+    print(f"pair table limits family={family} solver={solver}", flush=True)
+    from helpers import Case
+    ms = mh.workload_materials()
+    rng = np.random.default_rng(17)
+    els = [ring_robot(ms, rng, 40, 4) for _ in range(9)]
+    els.insert(4, ring_robot(ms, rng, 150, 4))
+    case = Case(els, ms, mh.materials.WORKLOAD_ROWS)
+    dt = 1e-4 if solver == FORCE else 1e-3
+    kw = LAND if solver == FORCE else None
+    sim, pos, vel, st = run_gpu(case, solver, 200, dt, kw, stress=solver == XPBD,
+                                kernel=family | mh.MH_KERNEL_STREAM, block_size=512)
+    info = sim.info()
+    assert info.n_stream_robots == 1 and info.n_lds_robots == 9
+    # packs take consecutive robots, at most 3 of 160 pairs: {0, 1, 2}, {3}
+    # (the 600-pair robot ends the run), {5, 6, 7}, {8, 9}; plus the
+    # 600-pair robot's streaming workgroup
+    assert info.n_blocks == 4 + 1
+    p, v, s = mirror(case, solver, 200, dt, kw, stress=solver == XPBD)
+    assert np.array_equal(bits(pos), bits(p))
+    assert np.array_equal(bits(vel), bits(v))
+    if solver == XPBD:
+        assert np.array_equal(bits(st), bits(s))
+    assert np.abs(pos - case.pos0).max() > 1e-6
