@@ -294,8 +294,9 @@ int mh_calibrate_lds(int device, float* gbytes_per_s);
 int mh_calibrate_copy(int device, size_t bytes, int reps, float* avg_ms);
 /* FP32 VALU issue rate of the device in 10^9 wave64 instructions per second
  * (independent chains, 8 waves per SIMD; kind 0 v_fma_f32, kind 1
- * v_mul_f32, kind 2 v_fma_f32 with twice the chains): the peak for kernels
- * bound by vector-ALU issue. */
+ * v_mul_f32, kind 2 v_fma_f32 with twice the chains, kind 3 v_rcp_f32,
+ * kind 4 v_rcp_f32 and v_fma_f32 alternating): the peak for kernels bound
+ * by vector-ALU issue, and the cost of a transcendental against it. */
 int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s);
 
 /* No reference counterpart: checks, bit for bit, the fast arithmetic the

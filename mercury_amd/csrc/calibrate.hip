@@ -47,16 +47,27 @@ __global__ __launch_bounds__(1024) void lds_read_bw(float* out, int iters) {
 }
 
 // kind 0: v_fma_f32 chains; kind 1: v_mul_f32 chains (two sources);
-// kind 2: v_fma_f32, twice the chains (more independent work per wave)
+// kind 2: v_fma_f32, twice the chains (more independent work per wave);
+// kind 3: v_rcp_f32 chains; kind 4: v_rcp_f32 then v_fma_f32 in each chain
+// step (counted as two instructions)
 template <int KIND>
 __global__ __launch_bounds__(256) void valu_rate(float* out, int iters, float a, float b) {
   constexpr int N = KIND == 2 ? 16 : 8;
   float x[N];
 #pragma unroll
-  for (int i = 0; i < N; ++i) x[i] = static_cast<float>(threadIdx.x) + static_cast<float>(i);
+  for (int i = 0; i < N; ++i) x[i] = static_cast<float>(threadIdx.x) + static_cast<float>(i) + 1.0f;
   for (int it = 0; it < iters; ++it) {
 #pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = (KIND == 1) ? x[i] * a : fmaf(x[i], a, b);
+    for (int i = 0; i < N; ++i) {
+      if (KIND == 1)
+        x[i] = x[i] * a;
+      else if (KIND == 3)
+        x[i] = __builtin_amdgcn_rcpf(x[i]);
+      else if (KIND == 4)
+        x[i] = fmaf(__builtin_amdgcn_rcpf(x[i]), a, b);
+      else
+        x[i] = fmaf(x[i], a, b);
+    }
   }
   float s = 0.f;
 #pragma unroll
@@ -199,8 +210,12 @@ int mh_calibrate_lds(int device, float* gbytes_per_s) {
 }
 
 int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s) {
-  if (!giga_instr_per_s || kind < 0 || kind > 2) return MH_ERR_ARG;
-  auto kernel = kind == 0 ? valu_rate<0> : kind == 1 ? valu_rate<1> : valu_rate<2>;
+  if (!giga_instr_per_s || kind < 0 || kind > 4) return MH_ERR_ARG;
+  auto kernel = kind == 0   ? valu_rate<0>
+                : kind == 1 ? valu_rate<1>
+                : kind == 2 ? valu_rate<2>
+                : kind == 3 ? valu_rate<3>
+                            : valu_rate<4>;
   hipError_t e = hipSetDevice(device);
   if (e != hipSuccess) return fail(e, "hipSetDevice");
   int cus = 0;
@@ -225,7 +240,7 @@ int mh_calibrate_valu(int device, int kind, float* giga_instr_per_s) {
   (void)hipEventDestroy(b);
   (void)hipFree(out);
   if (e != hipSuccess) return fail(e, "valu calibration");
-  const double chains = kind == 2 ? 16.0 : 8.0;
+  const double chains = kind == 2 ? 16.0 : kind == 4 ? 2.0 * 8.0 : 8.0;
   const double instr = static_cast<double>(blocks) * 4.0 * iters * chains;  // wave instructions
   *giga_instr_per_s = static_cast<float>(instr / (ms * 1e-3) / 1e9);
   return MH_OK;
