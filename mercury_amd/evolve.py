@@ -15,13 +15,15 @@ Evaluator_impl.h:39-107), rebuilt for the surrogate-screened workload:
   * evaluation: every simulated robot counts one evaluation, as
     Evaluator_impl.h:103-106 counts them; fitness is the COM-x displacement
     after `duration` s on land;
-  * surrogate: scores `screen_factor` x offspring; of each layer's share
-    the best (1 - audit) by prediction and a uniform `audit` sample of the
-    rest are simulated; retrained every `retrain_every` generations on all
-    simulated pairs; when its Spearman correlation on the audit sample
-    falls below `min_rho`, the next generations are simulated without
-    screening (still scored, to watch the correlation) until it recovers
-    or a retrain brings it back.
+  * surrogate: scores `screen_factor` x offspring (8 by default); of each
+    layer's share the best (1 - audit) by prediction and a uniform `audit`
+    sample of the rest are simulated; retrained every `retrain_every`
+    generations on all simulated pairs; when its Spearman correlation on
+    the audit sample falls below `min_rho`, either the surrogate is
+    retrained at once on the grown data and screens the next generation
+    (`fallback_retrain`, the default), or the next generations are
+    simulated without screening (still scored, to watch the correlation)
+    until it recovers or the periodic retrain brings it back.
 """
 import time
 from dataclasses import dataclass, field
@@ -212,12 +214,16 @@ def age_limits(layers, age_gap):
 
 
 def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elitism=0.1,
-        age_gap=5, threshold=None, max_generations=40, use_surrogate=False, screen_factor=4,
+        age_gap=5, threshold=None, max_generations=40, use_surrogate=False, screen_factor=8,
         retrain_every=5, min_rho=0.6, train_epochs=None, duration=1.0, sim=None, verbose=False,
-        audit=0.1, surrogate_kind="mlp", max_train=200000, warm_start=False):
+        audit=0.1, surrogate_kind="mlp", max_train=200000, warm_start=False,
+        fallback_retrain=True):
     """One evolution run; stops when the best fitness reaches `threshold` or
     after max_generations.  Returns a RunResult.  warm_start: each retrain
-    continues from the previous network's weights instead of a fresh one."""
+    continues from the previous network's weights instead of a fresh one.
+    fallback_retrain: when the surrogate ranks below min_rho, retrain it
+    before the next generation (which it then screens) instead of
+    simulating unscreened generations until the periodic retrain."""
     rng = np.random.default_rng(seed)
     sim = sim or BatchSimulator(voxels=voxels, duration=duration)
     evals0, t_start = sim.evals, time.perf_counter()
@@ -249,7 +255,7 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         return res
     for gen in range(1, max_generations + 1):
         res.generations = gen
-        if use_surrogate and (sur is None or gen % retrain_every == 0):
+        if use_surrogate and (sur is None or gen % retrain_every == 0 or (fallback_retrain and not screening)):
             if sur is None or not warm_start:
                 sur = GenomeModel(surrogate_kind, voxels, seed=seed + gen, epochs=train_epochs)
             X = np.concatenate(data_x)[-max_train:]
@@ -443,13 +449,15 @@ def main(argv=None):
     c.add_argument("--seed-start", type=int, default=0)
     c.add_argument("--pop", type=int, default=4096)
     c.add_argument("--generations", type=int, default=40)
-    c.add_argument("--screen-factor", type=int, default=4)
+    c.add_argument("--screen-factor", type=int, default=8, help="offspring scored per simulated one")
     c.add_argument("--epochs", type=int, default=None)
     c.add_argument("--surrogate", choices=("mlp", "cnn"), default="mlp")
     c.add_argument("--only", choices=("both", "plain", "surrogate"), default="both")
     c.add_argument("--audit", type=float, default=0.1, help="share of each layer's simulated "
                    "offspring drawn at random to measure the surrogate")
     c.add_argument("--warm-start", action="store_true", help="retrain from the previous weights")
+    c.add_argument("--fallback-retrain", action=argparse.BooleanOptionalAction, default=True,
+                   help="retrain every generation while the surrogate ranks below the fallback rho")
     rp = sub.add_parser("report", help="evaluations-to-threshold table from compare logs")
     rp.add_argument("logs", nargs="+", help="[LABEL=]path of a compare log (JSON lines); rows are "
                     "labelled LABEL:plain / LABEL:surrogate")
@@ -491,7 +499,7 @@ def main(argv=None):
         rows = compare(a.seeds, a.threshold, pop=a.pop, max_generations=a.generations,
                        voxels=a.voxels, duration=a.duration, screen_factor=a.screen_factor,
                        train_epochs=a.epochs, seed_start=a.seed_start, surrogate_kind=a.surrogate,
-                       audit=a.audit, warm_start=a.warm_start,
+                       audit=a.audit, warm_start=a.warm_start, fallback_retrain=a.fallback_retrain,
                        modes={"both": (False, True), "plain": (False,), "surrogate": (True,)}[a.only])
         for use in (False, True):
             e = [x["evals_to_threshold"] for x in rows if x["surrogate"] == use]

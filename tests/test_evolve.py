@@ -112,7 +112,7 @@ def test_surrogate_screening_and_rho_fallback(monkeypatch):
     assert good.best >= plain.best
     _FakeSurrogate.sign = -1.0
     bad = evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=4, sim=_FakeSim(),
-                     use_surrogate=True)
+                     use_surrogate=True, fallback_retrain=False)
     # first screened generation shows rho < 0.6; the following ones are unscreened
     assert bad.rho_history[0] < 0.6 and bad.fallback_generations >= 3
 
@@ -165,3 +165,30 @@ def test_warm_start_keeps_one_surrogate(monkeypatch):
     evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=11, sim=_FakeSim(),
                use_surrogate=True)
     assert len(made) == 3  # generation 1, then every 5 generations
+
+
+def test_fallback_retrain_retrains_while_ranking_is_poor(monkeypatch):
+    """With an anti-correlated surrogate every generation falls back; the
+    fallback_retrain rule retrains it each generation (generation 1 and
+    2..6), the default every 5 generations (1 and 5)."""
+    made = []
+
+    class Counting(_FakeSurrogate):
+        def __init__(self, seed=0, **kw):
+            made.append(seed)
+
+    monkeypatch.setattr(evolve, "Surrogate", Counting)
+    Counting.sign = -1.0
+    evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=6, sim=_FakeSim(),
+               use_surrogate=True, fallback_retrain=True)
+    assert len(made) == 6
+    made.clear()
+    evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=6, sim=_FakeSim(),
+               use_surrogate=True, fallback_retrain=False)
+    assert len(made) == 2
+    # an informative surrogate never falls back, so the rule adds nothing
+    made.clear()
+    Counting.sign = 1.0
+    evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=6, sim=_FakeSim(),
+               use_surrogate=True, fallback_retrain=True)
+    assert len(made) == 2
