@@ -45,7 +45,7 @@ __device__ __forceinline__ float act_sin(float x) {
 }
 
 
-// Correctly rounded sqrt for x in [2^-96, 2^126] without the range
+// Correctly rounded sqrt for x in [2^-96, 2^128) without the range
 // scaling and special-value handling of the general sequence: with the
 // hardware reciprocal square root y, s = x y refined once by Newton's step
 // in residual form, s + (x - s s) (y / 2), the residual computed exactly by
@@ -107,15 +107,17 @@ __device__ __forceinline__ float norm3(float x, float y, float z) {
 }
 
 // d = |(x, y, z)| and inv = 1 / (d + eps), both correctly rounded.  For a
-// sum of squares in [2^-96, 2^126], d lies in [2^-48, 2^63] and d + eps
+// sum of squares in [2^-96, 2^128), d lies in [2^-48, 2^64) and d + eps
 // (eps <= 2^-40) in the fast reciprocal's range, so one wave-uniform test
-// covers both short sequences.
+// covers both short sequences.  That test is the single compare
+// s >= 2^-96: NaN fails it, and +inf passes it and gives d = inv = NaN
+// where the general sequences give d = +inf, inv = 0; the force solver's
+// only use of the pair, (k (L - d)) inv, is NaN either way.
 __device__ __forceinline__ void length_and_inverse(float x, float y, float z, float eps,
                                                    float& d, float& inv) {
   float s = x * x + y * y;
   s = s + z * z;
-  const bool fast = __float_as_uint(s) - __float_as_uint(kSqrtFastLo) <=
-                    __float_as_uint(kSqrtFastHi) - __float_as_uint(kSqrtFastLo);
+  const bool fast = s >= kSqrtFastLo;
   if (__builtin_amdgcn_ballot_w64(fast) == __builtin_amdgcn_read_exec()) {
     d = sqrt_rn_inrange(s);
     inv = rcp_rn_inrange(d + eps);

@@ -1,6 +1,7 @@
-"""The fast correctly rounded sqrt used by the step kernels, checked bit for
-bit against the compiler's correctly rounded sqrtf for EVERY float in its
-range (about 1.9e9 inputs), on the GPU."""
+"""The fast correctly rounded sqrt and reciprocal used by the step kernels,
+checked bit for bit against the compiler's correctly rounded sqrtf and
+1.0f / x for EVERY float in their ranges (about 1.9e9 and 1.1e9 inputs),
+on the GPU."""
 import ctypes
 
 import numpy as np
@@ -22,8 +23,10 @@ def mismatches(which, lo, hi):
 
 @pytest.mark.gpu
 def test_fast_sqrt_is_correctly_rounded_over_its_whole_range():
-    print("arith: sqrt_rn_inrange over [2^-96, 2^126]", flush=True)
-    lo, hi = bits(2.0 ** -96), bits(2.0 ** 126) + 1
+    # [2^-96, 2^128): every finite float from 2^-96 up (the force solver's
+    # fast path takes all of them)
+    print("arith: sqrt_rn_inrange over [2^-96, 2^128)", flush=True)
+    lo, hi = bits(2.0 ** -96), 0x7f800000
     assert mismatches(1, lo, hi) == 0
     # informational: how often the uncorrected hardware sqrt differs
     print("hardware sqrt mismatches in range:", mismatches(0, lo, hi), flush=True)
@@ -45,3 +48,13 @@ def test_fast_division_is_correctly_rounded():
     divisors [2^-48, 2^40))."""
     print("arith: fast division, 2^31 pairs", flush=True)
     assert mismatches(9, 0, 1 << 31) == 0
+
+
+@pytest.mark.gpu
+def test_fast_reciprocal_is_correctly_rounded_over_its_whole_range():
+    """mh::rcp_rn_inrange (hardware reciprocal + one fused Newton step) for
+    every float in [2^-64, 2^64], which holds d + 1e-12 for every d the
+    fast sqrt returns."""
+    print("arith: rcp_rn_inrange over [2^-64, 2^64]", flush=True)
+    lo, hi = bits(2.0 ** -64), bits(2.0 ** 64) + 1
+    assert mismatches(4, lo, hi) == 0
