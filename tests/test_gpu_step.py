@@ -273,3 +273,51 @@ def test_mass_pair_table_limits(family, solver):
     if solver == XPBD:
         assert np.array_equal(bits(st), bits(s))
     assert np.abs(pos - case.pos0).max() > 1e-6
+
+
+def grid_robot(ms, dims, spacing=0.01):
+    """A box of masses with springs to the +x, +y and +z neighbours; the rest
+    lengths are three constants (one per axis, 5 % off the spacing, so the
+    box moves), so the robot has at most 3 x (materials) pairs."""
+    from mercury_amd.robots import Element
+    nx, ny, nz = dims
+    idx = np.arange(nx * ny * nz)
+    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    pos = (np.stack([x, y, z], 1) * spacing + 0.002).astype(np.float32)
+    pairs, lengths = [], []
+    for step, ok, L in ((1, x < nx - 1, 0.95), (nx, y < ny - 1, 1.05), (nx * ny, z < nz - 1, 1.0)):
+        a = idx[ok]
+        pairs.append(np.stack([a, a + step], 1))
+        lengths.append(np.full(len(a), L * spacing, np.float32))
+    pairs = np.concatenate(pairs).astype(np.uint16)
+    vox = mh.voxel_build(ms, 3, 10.0, np.full(27, 2, np.uint32), mass=0.001)
+    mats = np.unique(vox.spring_material[vox.spring_material != 0])
+    smat = mats[np.arange(len(pairs)) % len(mats)].astype(np.uint8)
+    return Element(pos, np.full(len(pos), vox.mass_material[0], np.uint8), pairs, np.concatenate(lengths),
+                   smat, mass=np.full(len(pos), 0.001, np.float32))
+
+
+@pytest.mark.parametrize("dims,block,family,fits", [
+    ((8, 8, 12), 768, mh.MH_KERNEL_MASS, True),          # 768 masses: a full 768-lane workgroup
+    ((8, 8, 16), 1024, mh.MH_KERNEL_MASS, True),         # 1024 masses: a full 1024-lane workgroup
+    ((8, 8, 12), 768, mh.MH_KERNEL_MASS_GATHER, True),
+    ((7, 10, 11), 768, mh.MH_KERNEL_MASS, False),        # 770 masses: over the 768-lane kernel
+    ((7, 10, 11), 0, mh.MH_KERNEL_MASS, True),           # ... which the 1024-lane one takes
+])
+def test_mass_kernel_capacity_edges(dims, block, family, fits):
+    print(f"mass kernel capacity dims={dims} block={block} family={family}", flush=True)
+    from helpers import Case
+    ms = mh.workload_materials()
+    case = Case([grid_robot(ms, dims)], ms, mh.materials.WORKLOAD_ROWS)
+    if not fits:
+        with pytest.raises(mh.MhError):
+            run_gpu(case, FORCE, 10, 1e-4, LAND, kernel=family, block_size=block)
+        return
+    sim, pos, vel, _ = run_gpu(case, FORCE, 300, 1e-4, LAND, kernel=family, block_size=block)
+    info = sim.info()
+    assert info.n_lds_robots == 1 and info.n_stream_robots == 0
+    assert info.block_size == (block or 1024)
+    p, v, _ = mirror(case, FORCE, 300, 1e-4, LAND)
+    assert np.array_equal(bits(pos), bits(p))
+    assert np.array_equal(bits(vel), bits(v))
+    assert np.abs(pos - case.pos0).max() > 1e-6
