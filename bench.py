@@ -86,11 +86,7 @@ def cpu_baseline(els, solver, dt, steps, threads, env_kw, f64=False):
     import oracle
     from mercury_amd.simulator import pack
     a = pack(els)
-    if solver == mh.MH_SOLVER_FORCE:
-        base = oracle.make_base(mh.materials.WORKLOAD_ROWS)
-    else:
-        base = oracle.make_base(mh.materials.WORKLOAD_ROWS)
-    by_id, _, _ = oracle.composite_tables(base)
+    by_id, _, _ = oracle.composite_tables(oracle.make_base(mh.materials.WORKLOAD_ROWS))
     mt = oracle.MaterialTable(by_id["k"], by_id["dL0"], by_id["omega"], by_id["phi"])
     pop = oracle.Population(a["mass_offset"], a["spring_offset"], a["mass"], a["pairs"],
                             a["mean_length"], a["spring_material"])
