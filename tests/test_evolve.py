@@ -221,3 +221,29 @@ def test_mirror_augmentation_doubles_training_data(monkeypatch):
     # predictions average the two views
     want = 0.5 * (evolve.genome_features(g, 4)[:, 0] + evolve.genome_features(m, 4)[:, 0])
     assert np.allclose(model.predict(g), want)
+
+
+def test_mirrored_robots_travel_alike_in_the_oracle():
+    """The basis of mirror augmentation: a robot mirrored across the lateral
+    z axis moves its centre of mass along x about as far as the original
+    (FP32 mirror of the force step, land, 16 random 8^3 robots, 1 s)."""
+    from scipy.stats import spearmanr
+    import oracle
+    from helpers import LAND, Case, mirror
+    ms = mh.workload_materials()
+    g = evolve.random_genomes(np.random.default_rng(4), 16, 8)
+
+    def travel(gen):
+        enc = evolve.genome_to_lattice(gen, 8)
+        case = Case([mh.voxel_build(ms, 9, 10.0, e, mass=0.001) for e in enc], ms, mh.materials.WORKLOAD_ROWS)
+        p, _, _ = mirror(case, mh.MH_SOLVER_FORCE, 10000, 1e-4, LAND, nthreads=4)
+        mo, mm = case.a["mass_offset"], case.a["mass_material"]
+        out = []
+        for r in range(len(enc)):
+            air = (mm[mo[r]:mo[r + 1]] == 0).astype(np.uint8)
+            out.append(oracle.com(p[mo[r]:mo[r + 1]], air)[0] - oracle.com(case.pos0[mo[r]:mo[r + 1]], air)[0])
+        return np.array(out)
+
+    a, b = travel(g), travel(evolve.mirror_z(g, 8))
+    assert np.abs(a).max() > 1e-4                      # the robots do move
+    assert spearmanr(a, b).correlation > 0.95
