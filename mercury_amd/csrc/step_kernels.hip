@@ -1,34 +1,37 @@
 // step_kernels.hip -- fused persistent step kernels for gfx950.
 //
 // One workgroup owns one pack (a contiguous range of robots) for n_steps
-// substeps.  Every substep is
-//   phase 1 (one thread per spring): spring term from the two endpoint
-//     positions in LDS, written to a per-spring buffer;
-//   barrier;
-//   phase 2 (one thread per mass): gather of the mass's spring terms in
-//     ascending spring index (CSR list, sign bit = mass is the spring's
-//     second endpoint), integration, new position written to LDS; the next
-//     substep's actuation table is refreshed;
-//   barrier.
-// No atomics: the per-mass summation order is fixed, so results are
-// bit-reproducible and equal to the CPU mirror of the same step.
+// substeps; the state is read from HBM once per launch and written back
+// once.  No atomics: every mass receives P - M, P the sum from +0 of the
+// terms of the springs whose first endpoint it is, in ascending spring
+// index, M the same over its second-endpoint springs, so results are
+// bit-reproducible, independent of the launch plan and equal to the CPU
+// mirror of the same step.
 //
 // Step semantics (reference sim_kernel.cu:141-269 for the XPBD solver;
 // force solver per the project spec written in the header of oracle/step.c):
 //   XPBD  x~ = x + v dt;  lambda = -(d - L)/(2 + ((1/k)/dt)/dt);
 //         dp = lambda * (x~a - x~b)/(d + 1e-12);  x' = x~ + sum(+-dp);
 //         v' = (float)(0.99 * (double)(x' - x) / (double)dt)
-//   force c = ((k (L - d)) (1 / (d + 1e-12))) (xa - xb);  F = sum(+-c) - m g y^ - c_d v
-//         + ground normal/friction;  u = v + (F/m) dt;  x' = x + u dt;
+//   force c = ((k (L - d)) (1 / (d + 1e-12))) (xa - xb);  F = P - M - m g y^ - c_d v
+//         + ground normal/friction;  u = v + (F (1/m)) dt;  x' = x + u dt;
 //         v' = damping * u
 //   L = fma(Lbar, A sin(fma(omega, t, phi)), Lbar)
 //
-// LDS kernel: masses (float4), spring buffer (3 x float), CSR (u16) and row
-// pointers (u16) resident in LDS; spring records (pair, Lbar, material,
-// stiffness) resident in VGPRs for the whole launch.  Streaming kernel:
-// only masses and the actuation table in LDS; spring records, CSR and the
-// spring buffer streamed from HBM/L2 every substep (robots too large for
-// the LDS kernel).
+// Kernels:
+//   step_mass   one lane per mass, robot resident in LDS; each spring is
+//               evaluated by both endpoints from records held in VGPRs;
+//               one barrier per substep (voxel robots);
+//   step_massg  the same lanes; each spring evaluated once by its first
+//               endpoint, the term passed to the second through LDS; two
+//               barriers per substep;
+//   step_lds    one lane per spring (records in VGPRs), terms in LDS, then
+//               one lane per mass gathering them through CSR lists;
+//   step_chunk  robots too large for LDS: positions in LDS, springs
+//               streamed from HBM in LDS-sized chunks, the next chunk's
+//               inputs loaded while the current one is worked on;
+//   step_stream the fallback for robots too large for step_chunk: spring
+//               terms in HBM.
 #include <hip/hip_runtime.h>
 
 #include "mercury_hip.h"
