@@ -14,8 +14,11 @@ struct PackDesc {
   uint32_t n_mass;
   uint32_t spring_begin;  // first active spring
   uint32_t n_spring;
-  uint32_t csr_begin;     // first gather entry
-  uint32_t row_begin;     // first row pointer (n_mass + 1 entries)
+  uint32_t csr_begin;     // first gather entry (mass-parallel kernels: first record)
+  uint32_t row_begin;     // first row pointer (mass-parallel kernels: first record count)
+  // kernel-specific: chunked kernel, offset of the pack's chunk table in
+  // ctab and its chunk count; mass-parallel kernels, offset of the pack's
+  // (rest length, material) pairs in lbtab and the pair count
   uint32_t pad0, pad1;
 };
 
