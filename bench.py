@@ -50,6 +50,10 @@ MASSG_LDS_BYTES_PER_SPRING = 44.0
 MASS_LDS_BYTES_PER_MASS = 32.0
 # MI355X nominal LDS bandwidth: 256 CUs x 128 B/clk x 2.4 GHz
 LDS_NOMINAL_GBS = 256 * 128 * 2.4
+# VALU issue peak for plain (not packed) FP32 instructions: one wave64
+# instruction per CU per cycle, 256 CUs at 2.4 GHz -- half the 157.3
+# TFLOP/s FP32 vector peak, which counts packed FMA (2 lanes x 2 flops)
+VALU_PEAK_WAVE_INSTR = 256 * 2.4e9
 
 
 # --kernel choices: MH_KERNEL_* family bits (robots over the LDS budget may
@@ -71,13 +75,36 @@ def float_loop_steps(duration, dt):
 
 
 def cpu_threads():
-    n = os.environ.get("OMP_NUM_THREADS")
-    if n and n.isdigit():
-        return max(1, int(n))
+    """Host threads for the CPU baseline: every core in the process's
+    affinity mask, unless OMP_NUM_THREADS sets the host's share lower (a
+    GPU host that allots a fixed number of cores to each GPU sets it)."""
     try:
-        return max(1, min(16, len(os.sched_getaffinity(0))))
+        n = len(os.sched_getaffinity(0))
     except AttributeError:
-        return max(1, min(16, os.cpu_count() or 1))
+        n = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def host_description():
+    """nproc, the affinity mask's size, the thread cap and the CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"nproc": os.cpu_count(), "affinity_cores": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
 
 
 def cpu_baseline(els, solver, dt, steps, threads, env_kw, f64=False):
@@ -106,6 +133,17 @@ def cpu_baseline(els, solver, dt, steps, threads, env_kw, f64=False):
     secs = run(steps, threads)
     springs = len(a["mean_length"])
     return springs * steps / secs, secs
+
+
+def measured_counters(workload, kernel):
+    """Per-launch hardware counts of the step kernel from the committed
+    rocprofv3 summary (profiles/counters.json): VALU and MFMA wave
+    instructions, LDS bank-conflict cycles, for this workload and kernel."""
+    path = os.path.join(REPO, "profiles", "counters.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get(f"{workload}/{kernel}")
 
 
 def measured_traffic(workload, kernel):
@@ -254,6 +292,30 @@ def main():
     achieved = per_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
     kernel_name = kernel_label(info)
     traffic = measured_traffic(args.workload, kernel_name)
+    counters = measured_counters(args.workload, kernel_name)
+    extra = {}
+    if bound == "lds" and kernel_ms > 0:
+        # the byte model of the hot-path scope table: 57 B per spring
+        # (two endpoint positions, the 9 B record, one term write and one
+        # gather read) + 48 B per mass, whatever the kernel reads
+        survey = (LDS_BYTES_PER_SPRING * n_springs + LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
+        extra["survey_model"] = {
+            "bytes_per_spring_update": LDS_BYTES_PER_SPRING + LDS_BYTES_PER_MASS * n_masses / n_springs,
+            "achieved": survey / (kernel_ms * 1e-3) / 1e9,
+            "frac": survey / (kernel_ms * 1e-3) / 1e9 / lds_peak,
+            "frac_nominal_peak": survey / (kernel_ms * 1e-3) / 1e9 / LDS_NOMINAL_GBS}
+    if counters and kernel_ms > 0:
+        # launches in the profile ran the same substeps per launch as here
+        scale = steps_per_launch / counters["substeps_per_launch"]
+        valu = counters["valu_per_launch"] * scale / (kernel_ms * 1e-3)
+        extra["valu_issue"] = {
+            "wave_instr_per_launch": counters["valu_per_launch"] * scale,
+            "achieved": valu, "peak": VALU_PEAK_WAVE_INSTR, "unit": "wave64 instr/s",
+            "frac": valu / VALU_PEAK_WAVE_INSTR,
+            "lane_instr_per_active_spring_update": counters["valu_per_launch"] * scale * 64
+            / (n_active * steps_per_launch)}
+        extra["mfma_per_launch"] = counters.get("mfma_per_launch")
+        extra["counters_source"] = "profiles/counters.json: " + counters["summary"]
 
 
     out = {
@@ -295,6 +357,7 @@ def main():
             "launches_per_step": launches,
             "algorithmic_bytes_per_spring_update": per_unit,
             **({"streamed_part": hbm_extra} if hbm_extra else {}),
+            **extra,
         },
         "host_seconds": {"topology_build": t_build, "upload": t_upload},
     }
@@ -311,6 +374,7 @@ def main():
         v64, _ = cpu_baseline(sub, solver, args.dt, max(50, args.cpu_steps // 8), th, env_kw, f64=True)
         out["cpu_baseline"] = {
             "value": v_all, "unit": "spring-updates/s", "cores": th, "kind": "port",
+            "host": host_description(), "per_core_value": v_all / th,
             "sample": f"oracle FP32 mirror, same {args.robots} robots, {args.cpu_steps} substeps "
                       f"({s_all:.1f} s wall on {th} threads)",
             "single_core_value": v_one,

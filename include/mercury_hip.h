@@ -137,7 +137,8 @@ typedef struct {
   int kernel;          /* bit set of MH_KERNEL_* families; 0 = all */
   int block_size;      /* 0 = any; else 256, 512, 768 or 1024 threads */
   int robots_per_block;/* 0 = as many as fit; else an upper bound */
-  int substeps_per_launch; /* 0 = autotune */
+  int substeps_per_launch; /* 0 = tuned (one launch per call until
+                          mh_sim_autotune has chosen) */
   int use_graph;       /* capture each simulate's launches in a hipGraph */
 } mh_config;
 
@@ -177,6 +178,13 @@ int mh_sim_synchronize(mh_sim* sim);
 /* Caller-allocated outputs, each may be NULL: pos/vel 3*M, stress S
  * (original spring order; air springs stay 0). */
 int mh_sim_collect(mh_sim* sim, float* pos, float* vel, float* stress);
+/* No reference counterpart: sets the accumulated stress of every spring
+ * (S floats, original spring order; air springs are ignored), which
+ * mh_sim_set_population zeroes.  Devo cycles use it to keep accumulating
+ * across a population re-upload, as the reference's stress buffer does
+ * across Simulator::Devo calls (cleared only by SetElements,
+ * Simulator.cu:355). */
+int mh_sim_set_stress(mh_sim* sim, const float* stress);
 /* Per-robot fitness on the device: MH_FITNESS_REFERENCE gives
  * max((COMx - COMx0)/L, 0) with the baseline taken at set_population;
  * MH_FITNESS_DISPLACEMENT gives COMx - COMx0. */
@@ -203,12 +211,21 @@ typedef struct {
   int n_stream_springs;  /* active springs handled by the streaming kernel */
   int stream_chunk;      /* springs per LDS chunk of the streaming kernel;
                             0 = terms kept in global memory */
+  int graph_builds;      /* hipGraphs captured and instantiated so far (a
+                            simulate call replays the last one when its
+                            step count, stress flag, substeps per launch and
+                            plan are unchanged, whatever the clock) */
 } mh_sim_info;
 int mh_sim_get_info(const mh_sim* sim, mh_sim_info* info);
 /* Times the candidate launch plans (LDS variant x robots per block, then
- * streaming variant) on the current population for n_steps each and keeps
- * the fastest; results do not depend on the choice.  The choice is cached
- * per (solver, device, histogram of robot sizes) for the process. */
+ * streaming variant, then substeps per launch: all of a call in one launch,
+ * or 2500, 1000 or 250) on the current population for n_steps each and
+ * keeps the fastest; results do not depend on the choice.  The choice is
+ * cached per (solver, device, the config's kernel, block, robots-per-block,
+ * substeps and graph settings, histogram of robot sizes) for the process,
+ * and mh_sim_set_population applies a cached choice to a population of the
+ * same shape.  Waits for earlier asynchronous work; the simulated state and
+ * time are restored afterwards, also when tuning fails. */
 int mh_sim_autotune(mh_sim* sim, int n_steps);
 /* Device pointers of the state (float4 per mass: xyz + mass / xyz + 0). */
 int mh_sim_device_state(mh_sim* sim, void** pos4, void** vel4,

@@ -52,7 +52,7 @@ class SimInfo(ctypes.Structure):
         "n_robots", "n_masses", "n_springs", "n_active_springs", "kernel", "block_size",
         "n_blocks", "substeps_per_launch", "lds_bytes", "max_masses_per_block",
         "max_springs_per_block", "n_lds_robots", "n_stream_robots", "stream_block_size",
-        "robots_per_block", "n_stream_springs", "stream_chunk")]
+        "robots_per_block", "n_stream_springs", "stream_chunk", "graph_builds")]
 
 
 # name -> (restype, argtypes)
@@ -85,6 +85,7 @@ _SIGS = {
     "mh_sim_reset": (c_int, [vp]),
     "mh_sim_total_time": (c_float, [vp]),
     "mh_sim_set_total_time": (c_int, [vp, c_float]),
+    "mh_sim_set_stress": (c_int, [vp, f32p]),
     "mh_sim_get_time_step": (c_float, [vp]),
     "mh_sim_set_time_step": (c_int, [vp, c_float]),
     "mh_sim_get_info": (c_int, [vp, ctypes.POINTER(SimInfo)]),

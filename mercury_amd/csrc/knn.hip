@@ -146,11 +146,14 @@ int mh_knn_batch(int device, int n_robots, const uint32_t* mass_offset, const fl
     cleanup();
     return fail(MH_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
-  hipMemcpy(d_pos, pos, 3 * M * sizeof(float), hipMemcpyHostToDevice);
-  hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(KnnTile), hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(knn_tiles, dim3(static_cast<unsigned>(tiles.size())), dim3(kQueryBlock), 0,
-                     nullptr, d_pos, d_tiles, K, d_nbr, d_dist);
-  e = hipGetLastError();
+  e = hipMemcpy(d_pos, pos, 3 * M * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(KnnTile), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(knn_tiles, dim3(static_cast<unsigned>(tiles.size())), dim3(kQueryBlock), 0,
+                       nullptr, d_pos, d_tiles, K, d_nbr, d_dist);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(nbr, d_nbr, M * K * sizeof(uint16_t), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(dist, d_dist, M * K * sizeof(float), hipMemcpyDeviceToHost);

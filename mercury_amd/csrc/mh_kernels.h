@@ -53,7 +53,8 @@ struct StepParams {
   int csr_cap;               // LDS capacity in gather entries (multiple of 4)
   int n_steps;
   float dt;
-  float t0;
+  const float* clock;        // start time of each launch of the call (clock kernel)
+  int launch;                // this launch's index into clock
   int track_stress;
   int fast_div;              // xpbd, gather kernel: every material stiffness, rest length
                              // and amplitude lies in the fast division's range
@@ -135,6 +136,11 @@ size_t lds_bytes_massg(int block, int maxp);
 
 hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes,
                        hipStream_t stream, const StepParams& p);
+
+// Start time of each of the ceil(n_steps / K) launches of a call into
+// times[], by the reference's float recurrence t += dt from t0 (one lane;
+// the step launches that follow read it).
+hipError_t launch_clock(float* times, float t0, float dt, int n_steps, int K, hipStream_t stream);
 
 // Per-robot COM of non-air masses (SoftBody::updateCOM order).
 hipError_t launch_com(const float4* pos, const uint8_t* mass_id,

@@ -154,14 +154,18 @@ struct mh_sim {
   mh::DevBuf<float> mat_k, mat_kx, mat_dL0, mat_omega, mat_phi;
 
   mh::Plan plan;
-  int substeps = 0;
+  int substeps = 0;  // substeps per launch; 0 = every substep of a call in one launch
   float total_time = 0.f;
+  // start time of every launch of the current call, written on the device
+  // by the clock kernel ahead of the step launches (so a captured graph
+  // replays for any starting time)
+  mh::DevBuf<float> clock;
 
-  // graph cache
+  // graph cache: one instantiated graph per (steps, stress, substeps, plan)
   hipGraphExec_t graph = nullptr;
   int g_steps = -1, g_track = -1, g_sub = -1;
-  uint32_t g_t0 = 0;
   const void* g_key = nullptr;
+  int graph_builds = 0;
 
   // timing
   bool timing = false;
@@ -719,7 +723,7 @@ void choose_default_plan(mh_sim* s) {
   upload_plan(s, p);
 }
 
-StepParams make_params(mh_sim* s, const Part& part, float t0, int n_steps, int track) {
+StepParams make_params(mh_sim* s, const Part& part, int launch, int n_steps, int track) {
   StepParams p{};
   p.packs = s->packs.p + part.pack_offset;
   p.pos = s->pos.p;
@@ -749,7 +753,8 @@ StepParams make_params(mh_sim* s, const Part& part, float t0, int n_steps, int t
   p.csr_cap = part.csr_cap;
   p.n_steps = n_steps;
   p.dt = s->cfg.dt;
-  p.t0 = t0;
+  p.clock = s->clock.p;
+  p.launch = launch;
   p.track_stress = track;
   p.fast_div = s->fast_div_mats && s->fast_div_lbar;
   p.gravity = s->cfg.gravity;
@@ -761,10 +766,16 @@ StepParams make_params(mh_sim* s, const Part& part, float t0, int n_steps, int t
   return p;
 }
 
-// Issue the launches of n_steps steps starting at simulated time t0.  The
-// parts own disjoint robots: the second one runs on the aux stream, forked
-// from and joined back into the main stream.  Returns launches per part.
-int issue(mh_sim* s, float t0, int n_steps, int track) {
+int launch_count(const mh_sim* s, int n_steps) {
+  const int K = s->substeps > 0 ? s->substeps : n_steps;
+  return (n_steps + K - 1) / K;
+}
+
+// Issue the step launches of n_steps steps.  Launch i starts at
+// s->clock[i], which the clock kernel writes before them.  The parts own
+// disjoint robots: the second one runs on the aux stream, forked from and
+// joined back into the main stream.  Returns launches per part.
+int issue(mh_sim* s, int n_steps, int track) {
   const int K = s->substeps > 0 ? s->substeps : n_steps;
   const size_t np = s->plan.parts.size();
   if (np > 1) {
@@ -775,13 +786,11 @@ int issue(mh_sim* s, float t0, int n_steps, int track) {
   for (size_t i = 0; i < np; i++) {
     const Part& part = s->plan.parts[i];
     hipStream_t st = i == 0 ? s->stream : s->aux;
-    float t = t0;
     launches = 0;
     for (int done = 0; done < n_steps;) {
       const int n = std::min(K, n_steps - done);
-      StepParams p = make_params(s, part, t, n, track);
+      StepParams p = make_params(s, part, launches, n, track);
       hip_check(launch_step(*part.v, static_cast<int>(part.packs.size()), part.lds, st, p), "step launch");
-      for (int k = 0; k < n; k++) t += s->cfg.dt;  // the reference's float time recurrence
       done += n;
       launches++;
     }
@@ -799,20 +808,28 @@ void run_steps(mh_sim* s, int n_steps, int track) {
     return;
   }
   if (track && !s->stress.p) throw Error(MH_ERR_STATE, "no stress buffer");
-  const float t0 = s->total_time;
+  const int launches = launch_count(s, n_steps);
+  if (s->clock.n < static_cast<size_t>(launches)) {
+    // the table's address is baked into captured graphs
+    hip_check(hipStreamSynchronize(s->stream), "sync");
+    drop_graph(s);
+    s->clock.alloc(static_cast<size_t>(launches));
+  }
+  // launch start times from the reference's float time recurrence, on the
+  // device: the step launches (captured or not) read them
+  hip_check(launch_clock(s->clock.p, s->total_time, s->cfg.dt, n_steps,
+                         s->substeps > 0 ? s->substeps : n_steps, s->stream),
+            "clock launch");
   if (s->timing) hip_check(hipEventRecord(s->ev0, s->stream), "event");
-  int launches = 0;
   if (s->cfg.use_graph) {
-    uint32_t t0_bits;
-    std::memcpy(&t0_bits, &t0, 4);
     const bool hit = s->graph && s->g_steps == n_steps && s->g_track == track &&
-                     s->g_sub == s->substeps && s->g_t0 == t0_bits && s->g_key == &s->plan;
+                     s->g_sub == s->substeps && s->g_key == &s->plan;
     if (!hit) {
       drop_graph(s);
       hipGraph_t g = nullptr;
       hip_check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), "begin capture");
       try {
-        issue(s, t0, n_steps, track);
+        issue(s, n_steps, track);
       } catch (...) {
         (void)hipStreamEndCapture(s->stream, &g);
         if (g) (void)hipGraphDestroy(g);
@@ -825,14 +842,12 @@ void run_steps(mh_sim* s, int n_steps, int track) {
       s->g_steps = n_steps;
       s->g_track = track;
       s->g_sub = s->substeps;
-      s->g_t0 = t0_bits;
       s->g_key = &s->plan;
+      s->graph_builds++;
     }
     hip_check(hipGraphLaunch(s->graph, s->stream), "graph launch");
-    const int K = s->substeps > 0 ? s->substeps : n_steps;
-    launches = (n_steps + K - 1) / K;
   } else {
-    launches = issue(s, t0, n_steps, track);
+    issue(s, n_steps, track);
   }
   if (s->timing) {
     hip_check(hipEventRecord(s->ev1, s->stream), "event");
@@ -841,6 +856,10 @@ void run_steps(mh_sim* s, int n_steps, int track) {
   }
   for (int k = 0; k < n_steps; k++) s->total_time += s->cfg.dt;
 }
+
+// The population's tuned launch plan if the tuner has one cached for its
+// shape and this config, else the default plan (defined with the tuner).
+void choose_plan(mh_sim* s);
 
 int step_count(float duration, float dt) {
   // Simulator.cu:428-449: while (remaining > 0) { ...; remaining -= dt; }
@@ -1106,7 +1125,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   mh::hip_check(hipMemsetAsync(s->stress.p, 0, sizeof(float) * s->stress.n, s->stream), "memset");
   s->com.alloc(3 * std::max(R, 1));
   mh::upload_materials(s);
-  if (R > 0) mh::choose_default_plan(s);
+  if (R > 0) mh::choose_plan(s);
   mh::hip_check(hipStreamSynchronize(s->stream), "upload");
   s->has_population = true;
   return MH_OK;
@@ -1175,6 +1194,23 @@ int mh_sim_collect(mh_sim* s, float* pos, float* vel, float* stress) {
       mh::hip_check(hipStreamSynchronize(s->stream), "D2H");
       for (size_t k = 0; k < s->n_active; k++) stress[s->active_src[k]] = st[k];
     }
+  }
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_set_stress(mh_sim* s, const float* stress) {
+  MH_TRY
+  if (!s || !stress) throw mh::Error(MH_ERR_ARG, "null argument");
+  if (!s->has_population) throw mh::Error(MH_ERR_STATE, "no population set");
+  mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  if (s->n_active) {
+    std::vector<float> st(s->n_active);
+    for (size_t k = 0; k < s->n_active; k++) st[k] = stress[s->active_src[k]];
+    // the stream's earlier steps may still be accumulating into the buffer
+    mh::hip_check(hipMemcpyAsync(s->stress.p, st.data(), s->n_active * sizeof(float), hipMemcpyHostToDevice,
+                                 s->stream), "H2D");
+    mh::hip_check(hipStreamSynchronize(s->stream), "H2D");
   }
   return MH_OK;
   MH_CATCH
@@ -1268,6 +1304,7 @@ int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
   }
   info->robots_per_block = s->plan.rpb;
   info->substeps_per_launch = s->substeps;
+  info->graph_builds = s->graph_builds;
   return MH_OK;
   MH_CATCH
 }
@@ -1275,19 +1312,21 @@ int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
 namespace mh {
 namespace {
 
-// Tuning results by (solver, device, population shape); the chosen plan
-// never changes results, only speed.
+// Tuning results by (solver, device, what the config allows, population
+// shape); the chosen plan never changes results, only speed.
 struct TuneKey {
-  int solver, device;
+  int solver, device, kernel, block_size, robots_per_block, substeps, use_graph, lds_limit;
   std::vector<uint64_t> shape;
-  bool operator<(const TuneKey& o) const {
-    return std::tie(solver, device, shape) < std::tie(o.solver, o.device, o.shape);
+  auto tie() const {
+    return std::tie(solver, device, kernel, block_size, robots_per_block, substeps, use_graph, lds_limit, shape);
   }
+  bool operator<(const TuneKey& o) const { return tie() < o.tie(); }
 };
 struct TuneChoice {
   const KernelVariant* lds;
   const KernelVariant* stream;
   int rpb;
+  int substeps;  // 0 = every substep of a call in one launch
 };
 std::map<TuneKey, TuneChoice>& tune_cache() {
   static std::map<TuneKey, TuneChoice> c;
@@ -1299,28 +1338,64 @@ std::mutex& tune_mutex() {
 }
 
 TuneKey tune_key(const mh_sim* s) {
-  // histogram of (masses, active springs) rounded to 1/16 of a power of two
-  auto bucket = [](uint32_t v) -> uint64_t {
-    if (v < 16) return v;
-    int e = 31 - __builtin_clz(v);
-    return (static_cast<uint64_t>(e) << 4) | ((v >> (e - 4)) & 15u);
+  // the population's shape, coarsely: robot count, mean and largest robot
+  // (masses, active springs), each rounded down to a quarter of a power of
+  // two, the most springs on one side of a mass and whether every robot's
+  // (rest length, material) pairs fit the mass kernels' table -- so that
+  // populations drawn from one distribution share a choice
+  auto bucket = [](uint64_t v) -> uint64_t {
+    if (v < 8) return v;
+    const int e = 63 - __builtin_clzll(v);
+    return (static_cast<uint64_t>(e) << 2) | ((v >> (e - 2)) & 3u);
   };
-  std::map<uint64_t, uint32_t> h;
-  for (int r = 0; r < s->n_robots; r++)
-    h[(bucket(static_cast<uint32_t>(robot_masses(s, r))) << 32) |
-      bucket(static_cast<uint32_t>(robot_active(s, r)))]++;
-  TuneKey k{s->cfg.solver, s->cfg.device, {}};
-  for (auto& kv : h) {
-    k.shape.push_back(kv.first);
-    k.shape.push_back(kv.second);
+  uint64_t sm = 0, sa = 0, mm = 0, ma = 0, side = 0, tables = 1;
+  for (int r = 0; r < s->n_robots; r++) {
+    const uint64_t m = static_cast<uint64_t>(robot_masses(s, r)), a = static_cast<uint64_t>(robot_active(s, r));
+    sm += m;
+    sa += a;
+    mm = std::max(mm, m);
+    ma = std::max(ma, a);
+    side = std::max<uint64_t>(side, s->max_side[r]);
+    tables &= (a == 0 || !s->pair_set[r].empty()) ? 1u : 0u;
   }
-  return k;
+  const uint64_t n = static_cast<uint64_t>(std::max(s->n_robots, 1));
+  const mh_config& c = s->cfg;
+  return TuneKey{c.solver, c.device, c.kernel, c.block_size, c.robots_per_block, c.substeps_per_launch,
+                 c.use_graph, s->lds_limit,
+                 {bucket(n), bucket(sm / n), bucket(sa / n), bucket(mm), bucket(ma), side, tables}};
 }
 
-// Seconds for n_steps substeps with plan p: the best of three runs after a
-// warm-up (plans a few percent apart must not swap places on noise).
-double time_plan(mh_sim* s, Plan& p, int n_steps) {
+// Applies a cached choice if its variants are allowed by this simulator's
+// config and its plan fits the population; false otherwise.
+bool apply_choice(mh_sim* s, const TuneChoice& c) {
+  auto allowed = [s](const KernelVariant* v, int kinds) {
+    if (!v) return true;
+    auto cs = candidates(s, kinds);
+    return std::find(cs.begin(), cs.end(), v) != cs.end();
+  };
+  if (!allowed(c.lds, kResident) || !allowed(c.stream, MH_KERNEL_STREAM)) return false;
+  Plan p;
+  if (!make_plan(s, c.lds, c.stream, c.rpb, p)) return false;
   upload_plan(s, p);
+  s->substeps = s->cfg.substeps_per_launch > 0 ? s->cfg.substeps_per_launch : c.substeps;
+  return true;
+}
+
+// The population's tuned plan, if one is cached; else the default plan.
+void choose_plan(mh_sim* s) {
+  {
+    std::lock_guard<std::mutex> lk(tune_mutex());
+    auto it = tune_cache().find(tune_key(s));
+    if (it != tune_cache().end() && apply_choice(s, it->second)) return;
+  }
+  s->substeps = s->cfg.substeps_per_launch > 0 ? s->cfg.substeps_per_launch : 0;
+  choose_default_plan(s);
+}
+
+// Seconds for n_steps substeps with the current plan: the best of three
+// runs after a warm-up (plans a few percent apart must not swap places on
+// noise).
+double time_steps(mh_sim* s, int n_steps) {
   run_steps(s, std::max(1, n_steps / 4), 0);  // warm-up
   hip_check(hipStreamSynchronize(s->stream), "sync");
   double best = std::numeric_limits<double>::infinity();
@@ -1333,6 +1408,53 @@ double time_plan(mh_sim* s, Plan& p, int n_steps) {
   return best;
 }
 
+double time_plan(mh_sim* s, Plan& p, int n_steps) {
+  upload_plan(s, p);
+  return time_steps(s, n_steps);
+}
+
+// Saves the simulated state and the settings the tuner changes; restores
+// them when it goes out of scope, also when tuning throws.
+struct TuneScope {
+  mh_sim* s;
+  std::vector<float4> p0, v0;
+  std::vector<float> st0;
+  float t_save;
+  int use_graph, substeps;
+  bool restored = false;
+  explicit TuneScope(mh_sim* sim)
+      : s(sim), p0(sim->n_masses), v0(sim->n_masses), st0(sim->stress.n), t_save(sim->total_time),
+        use_graph(sim->cfg.use_graph), substeps(sim->substeps) {
+    copy(hipMemcpyDeviceToHost);
+  }
+  void copy(hipMemcpyKind kind) {
+    const bool d2h = kind == hipMemcpyDeviceToHost;
+    auto one = [&](void* host, void* dev, size_t bytes) {
+      if (!bytes) return;
+      hip_check(hipMemcpyAsync(d2h ? host : dev, d2h ? dev : host, bytes, kind, s->stream), "tune state copy");
+    };
+    one(p0.data(), s->pos.p, p0.size() * sizeof(float4));
+    one(v0.data(), s->vel.p, v0.size() * sizeof(float4));
+    one(st0.data(), s->stress.p, st0.size() * sizeof(float));
+    hip_check(hipStreamSynchronize(s->stream), "tune state copy");
+  }
+  void restore() {
+    restored = true;
+    s->cfg.use_graph = use_graph;
+    s->total_time = t_save;
+    (void)hipStreamSynchronize(s->stream);
+    copy(hipMemcpyHostToDevice);
+  }
+  ~TuneScope() {
+    if (restored) return;
+    s->substeps = substeps;
+    try {
+      restore();
+    } catch (...) {
+    }
+  }
+};
+
 }  // namespace
 }  // namespace mh
 
@@ -1343,29 +1465,17 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   if (s->n_robots == 0) return MH_OK;
   if (n_steps < 1) n_steps = 1;
   mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  // earlier asynchronous steps finish before the state is saved
+  mh::hip_check(hipStreamSynchronize(s->stream), "synchronize");
   const mh::TuneKey key = mh::tune_key(s);
   {
     std::lock_guard<std::mutex> lk(mh::tune_mutex());
     auto it = mh::tune_cache().find(key);
-    if (it != mh::tune_cache().end()) {
-      mh::Plan p;
-      if (mh::make_plan(s, it->second.lds, it->second.stream, it->second.rpb, p)) {
-        mh::upload_plan(s, p);
-        if (s->substeps == 0) s->substeps = 1000;
-        return MH_OK;
-      }
-    }
+    if (it != mh::tune_cache().end() && mh::apply_choice(s, it->second)) return MH_OK;
   }
   // candidates run on a copy of the state; the population is restored after
-  std::vector<float4> p0(s->n_masses), v0(s->n_masses);
-  std::vector<float> st0(s->stress.n);
-  mh::hip_check(hipMemcpy(p0.data(), s->pos.p, p0.size() * sizeof(float4), hipMemcpyDeviceToHost), "D2H");
-  mh::hip_check(hipMemcpy(v0.data(), s->vel.p, v0.size() * sizeof(float4), hipMemcpyDeviceToHost), "D2H");
-  mh::hip_check(hipMemcpy(st0.data(), s->stress.p, st0.size() * sizeof(float), hipMemcpyDeviceToHost), "D2H");
-  const float t_save = s->total_time;
-  const int use_graph = s->cfg.use_graph;
+  mh::TuneScope scope(s);
   s->cfg.use_graph = 0;
-  const int saved_sub = s->substeps;
   s->substeps = 0;
   auto lc = mh::candidates(s, mh::kResident);
   auto sc = mh::candidates(s, MH_KERNEL_STREAM);
@@ -1382,8 +1492,9 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   std::vector<int> rpbs = {s->cfg.robots_per_block};
   if (s->cfg.robots_per_block == 0) rpbs = {0, 1, 2, 4};
   double best = std::numeric_limits<double>::infinity();
-  mh::TuneChoice choice{nullptr, nullptr, 0};
-  // coordinate search: LDS variant x robots per block, then streaming variant
+  mh::TuneChoice choice{nullptr, nullptr, 0, 0};
+  // coordinate search: LDS variant x robots per block, then streaming
+  // variant, then substeps per launch
   for (const mh::KernelVariant* lv : lc) {
     for (int rpb : rpbs) {
       if (!lv && rpb != rpbs.front()) continue;
@@ -1396,7 +1507,7 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
       const double t = mh::time_plan(s, p, n_steps);
       if (t < best) {
         best = t;
-        choice = {lv, sv0, rpb};
+        choice = {lv, sv0, rpb, 0};
       }
     }
   }
@@ -1417,18 +1528,31 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   mh::Plan p;
   mh::make_plan(s, choice.lds, choice.stream, choice.rpb, p);
   mh::upload_plan(s, p);
+  // substeps per launch, timed with the caller's graph setting: one launch
+  // per call, or launches of K substeps (a caller's synchronize waits for
+  // at most one launch)
+  s->cfg.use_graph = scope.use_graph;
+  if (s->cfg.substeps_per_launch > 0) {
+    choice.substeps = s->cfg.substeps_per_launch;
+  } else {
+    double best_k = std::numeric_limits<double>::infinity();
+    for (int K : {0, 2500, 1000, 250}) {
+      if (K > 0 && 2 * K > n_steps) continue;
+      s->substeps = K;
+      const double t = mh::time_steps(s, n_steps);
+      // shorter launches must win by more than noise
+      if (t < best_k * (K == 0 ? 1.0 : 0.99)) {
+        best_k = t;
+        choice.substeps = K;
+      }
+    }
+  }
+  s->substeps = choice.substeps;
+  scope.restore();
   {
     std::lock_guard<std::mutex> lk(mh::tune_mutex());
     mh::tune_cache()[key] = choice;
   }
-  // substeps per launch: long launches amortize the launch; ~1k substeps
-  // keep a caller's synchronize responsive
-  s->substeps = saved_sub > 0 ? saved_sub : 1000;
-  s->cfg.use_graph = use_graph;
-  mh::hip_check(hipMemcpy(s->pos.p, p0.data(), p0.size() * sizeof(float4), hipMemcpyHostToDevice), "H2D");
-  mh::hip_check(hipMemcpy(s->vel.p, v0.data(), v0.size() * sizeof(float4), hipMemcpyHostToDevice), "H2D");
-  mh::hip_check(hipMemcpy(s->stress.p, st0.data(), st0.size() * sizeof(float), hipMemcpyHostToDevice), "H2D");
-  s->total_time = t_save;
   return MH_OK;
   MH_CATCH
 }

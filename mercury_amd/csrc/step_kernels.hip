@@ -238,7 +238,7 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
       s_pos[pslot[j]] = lds_position<SOLVER>(ms[j]);
     }
   }
-  float t = p.t0;
+  float t = p.clock[p.launch];
   if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], t);
   __syncthreads();
 
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
     load_mass(s, p.pos, p.vel, d.mass_begin + i, p);
     s_pos[i] = lds_position<SOLVER>(s);
   }
-  float t = p.t0;
+  float t = p.clock[p.launch];
   if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], t);
   __syncthreads();
 
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
       s_pos[pslot[j]] = lds_position<SOLVER>(ms[j]);
     }
   }
-  float t = p.t0;
+  float t = p.clock[p.launch];
   if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], t);
   __syncthreads();
 
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
     pos0[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
     pos1[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
   }
-  float t = p.t0;
+  float t = p.clock[p.launch];
   // null pair: a term with stiffness 0 (force) or divisor +inf (xpbd) is a
   // signed zero
   mass_pairs_init<SOLVER, BLOCK>(smem, p, d, kPairs, kPairs + kMassDynBytes, kPairs + 2 * kMassDynBytes,
@@ -903,7 +903,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   float4* pos = reinterpret_cast<float4*>(smem + kMassPos);
   if (live) pos[pslot] = lds_position<SOLVER>(ms);
   uint32_t* s_flag = reinterpret_cast<uint32_t*>(smem + kMassZero + 16);
-  float t = p.t0;
+  float t = p.clock[p.launch];
   // one dynamic buffer, updated in place in phase 2; null pair: stiffness 0
   // (force) or 1 with reciprocal 0 (xpbd), a signed zero either way
   mass_pairs_init<SOLVER, BLOCK>(smem, p, d, kDyn, kDyn, kAux, kCst, t,
@@ -986,6 +986,20 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
       for (int k = 0; k < MAXP; ++k)
         if (k < np) p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]] = st[k];
     }
+  }
+}
+
+// ---------------------------------------------------------------- clock
+// times[i] = start time of launch i: the float recurrence t += dt, K
+// additions per launch, from t0.
+__global__ void set_clock(float* times, float t0, float dt, int n_steps, int K) {
+  if (threadIdx.x != 0) return;
+  float t = t0;
+  int launch = 0;
+  for (int done = 0; done < n_steps; done += K) {
+    times[launch++] = t;
+    const int n = min(K, n_steps - done);
+    for (int k = 0; k < n; ++k) t = t + dt;
   }
 }
 
@@ -1080,6 +1094,11 @@ hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes, hi
   void* kargs[] = {&args};
   const void* fn = (p.track_stress && v.fn_stress) ? v.fn_stress : v.fn;
   return hipLaunchKernel(fn, dim3(n_packs), dim3(v.block), kargs, lds_bytes, stream);
+}
+
+hipError_t launch_clock(float* times, float t0, float dt, int n_steps, int K, hipStream_t stream) {
+  hipLaunchKernelGGL(set_clock, dim3(1), dim3(64), 0, stream, times, t0, dt, n_steps, K);
+  return hipGetLastError();
 }
 
 hipError_t launch_com(const float4* pos, const uint8_t* mass_id, const uint32_t* robot_mass_begin,

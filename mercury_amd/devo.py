@@ -23,7 +23,11 @@ Deliberate differences, each a defect of the reference:
     here the new material acts at once;
   - the random pairs come from numpy's PCG64 seeded by (seed, cycle, robot)
     instead of curand, so runs are reproducible on any device.
-The stress ranking uses the stress accumulated since the last upload.
+The stress ranking uses the stress accumulated since the evaluation's
+first upload: the evaluator carries it over each re-upload
+(Simulator.set_stress), so a replaced spring keeps its slot's stress, as in
+the reference, whose stress buffer only SetElements clears
+(Simulator.cu:355).
 Lengths |x_a - x_b| are evaluated as SoftBody::Reset evaluates them
 (mh_rest_lengths, Eigen's norm order); the reference's devo kernel uses
 CUDA's norm3df and sinf, which are not reproduced (host float32 sin here).

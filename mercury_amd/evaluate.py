@@ -18,7 +18,7 @@ discarded phase is not simulated.
 With devo cycles (Evaluator_impl.h:66-70), each cycle simulates devo_time
 seconds tracking stress and then replaces every robot's most stressed
 springs (mercury_amd.devo), re-uploading the population with the current
-state and the simulated time carried over.  The springs found this way are
+state, the simulated time and the accumulated stress carried over.  The springs found this way are
 what the robots keep: Update takes the collected springs and
 SoftBody::Reset recomputes their lengths from the proto positions, so the
 evaluation phase starts from the rest configuration with the new
@@ -68,6 +68,10 @@ class Evaluator:
                 new.append(nxt)
             sim.SetElements(new)
             sim.setTotalTime(t)
+            # the stress keeps accumulating over the cycles, as the
+            # reference's buffer does (cleared only by SetElements); a
+            # replaced spring keeps its slot's stress
+            sim.set_stress(np.concatenate([st.stress for st in state]) if state else np.zeros(0, np.float32))
             current = new
         return [rest_from_proto(el, src.pos) for el, src in zip(current, robots)]
 

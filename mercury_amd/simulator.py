@@ -197,6 +197,14 @@ class Simulator:
         population upload restarts it at 0)."""
         check(load().mh_sim_set_total_time(self._h, float(t)))
 
+    def set_stress(self, stress):
+        """Set every spring's accumulated stress (original spring order; no
+        reference counterpart: the population upload zeroes it)."""
+        st = np.ascontiguousarray(stress, np.float32)
+        if len(st) != self.info().n_springs:
+            raise ValueError("one stress value per spring")
+        check(load().mh_sim_set_stress(self._h, ptr(st, f32p)))
+
     # -------------------------------------------------------------- tuning
     def info(self):
         i = SimInfo()

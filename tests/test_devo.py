@@ -89,3 +89,43 @@ def test_devo_cycles_in_the_evaluator():
     assert np.array_equal(Evaluator(ms, cfg, evaluation_time=0.03).batch_evaluate(ev.last_devo), fit)
     ev2 = Evaluator(ms, cfg, evaluation_time=0.03, devo_cycles=2, devo_time=0.02, replaced_springs=16, seed=3)
     assert np.array_equal(ev2.batch_evaluate(robots), fit)
+
+
+@pytest.mark.gpu
+def test_devo_stress_accumulates_over_cycles():
+    """Two XPBD devo cycles: the second ranking uses the stress summed over
+    both cycles (the reference's buffer is cleared only by SetElements), so
+    the springs replaced equal those the oracle picks from its carried-over
+    stress."""
+    print("devo stress carry-over: 3 robots 3^3, xpbd, 2 cycles of 0.01 s", flush=True)
+    from helpers import Case
+    ms = mh.workload_materials()
+    robots = mh.voxel_population(ms, 3, voxels_per_axis=3, seed=12, mass=0.001)
+    dt, devo_time, R, seed = 1e-4, 0.01, 12, 5
+    cfg = mh.SimConfig(solver=mh.MH_SOLVER_XPBD_REF, time_step=dt)
+    ev = Evaluator(ms, cfg, evaluation_time=0.001, devo_cycles=2, devo_time=devo_time,
+                   replaced_springs=R, seed=seed)
+    ev.batch_evaluate(robots)
+    # the same cycles on the oracle's FP32 mirror
+    lookup = devo.MaterialLookup(ms)
+    current = [devo.rest_from_proto(r, r.pos) for r in robots]
+    n = len(oracle.step_times(np.float32(0), np.float32(dt), np.float32(devo_time)))
+    t, stress = np.float32(0), None
+    for c in range(2):
+        case = Case(current, ms, WORKLOAD_ROWS)
+        st = np.zeros(case.n_springs(), np.float32) if stress is None else stress
+        p, v, st = oracle.xpbd_f32(case.pop, case.mt, dt, float(t), n, case.pos0, case.vel0, st.copy())
+        p, v = p.reshape(-1, 3), v.reshape(-1, 3)
+        for _ in range(n):
+            t = np.float32(t + np.float32(dt))
+        mo, so = case.a["mass_offset"], case.a["spring_offset"]
+        new = []
+        for r, el in enumerate(current):
+            rng = np.random.default_rng([seed, 0, c, r])
+            nxt, _ = devo.devo_robot(el, st[so[r]:so[r + 1]], p[mo[r]:mo[r + 1]], float(t), R, rng, lookup)
+            nxt.pos, nxt.vel = p[mo[r]:mo[r + 1]], v[mo[r]:mo[r + 1]]
+            new.append(nxt)
+        current, stress = new, st
+    for got, want in zip(ev.last_devo, current):
+        assert np.array_equal(got.pairs, want.pairs)
+        assert np.array_equal(got.spring_material, want.spring_material)

@@ -321,3 +321,52 @@ def test_mass_kernel_capacity_edges(dims, block, family, fits):
     assert np.array_equal(bits(pos), bits(p))
     assert np.array_equal(bits(vel), bits(v))
     assert np.abs(pos - case.pos0).max() > 1e-6
+
+
+def test_graph_replays_for_any_clock():
+    """Continuing-clock calls (no Reset between them) replay one captured
+    graph: launch start times come from the device clock table, so the
+    graph does not depend on t0.  Results stay bit-exact with the mirror
+    run over the whole span, and with the eager path."""
+    print("graph replay across the clock", flush=True)
+    case = workload_case(n_robots=5, voxels=4, seed=14)
+    n = float_loop_steps(0.004, 1e-4)
+    out = []
+    for use_graph in (1, 0):
+        sim = gpu_sim(case, FORCE, 1e-4, LAND, use_graph=use_graph, substeps_per_launch=16)
+        for rep in range(3):
+            sim.Simulate(0.004)
+        sim.synchronize()
+        if use_graph:
+            assert sim.info().graph_builds == 1
+        out.append(sim.collect_arrays()[:2])
+    p, v, _ = mirror(case, FORCE, 3 * n, 1e-4, LAND)
+    for pos, vel in out:
+        assert np.array_equal(bits(pos), bits(p))
+        assert np.array_equal(bits(vel), bits(v))
+
+
+def test_tuned_substeps_and_cached_plan_respect_the_config():
+    """The tuner's choice (including substeps per launch) never changes
+    results, is applied to a later population of the same shape at upload,
+    and a simulator whose config excludes the cached kernel family tunes
+    its own plan."""
+    print("autotune cache and substeps", flush=True)
+    case = workload_case(n_robots=6, voxels=4, seed=15)
+    sim = gpu_sim(case, FORCE, 1e-4, LAND)
+    info = sim.autotune(400)
+    assert info.substeps_per_launch in (0, 250)
+    sim.Step(300)
+    p, v, _ = mirror(case, FORCE, 300, 1e-4, LAND)
+    assert np.array_equal(bits(sim.collect_arrays()[0]), bits(p))
+    # same shape, fresh simulator: the cached plan at upload
+    again = gpu_sim(case, FORCE, 1e-4, LAND)
+    i2 = again.info()
+    assert (i2.kernel, i2.block_size, i2.substeps_per_launch) == (info.kernel, info.block_size,
+                                                                 info.substeps_per_launch)
+    # a simulator restricted to streaming must not take the cached plan
+    stream = gpu_sim(case, FORCE, 1e-4, LAND, kernel=mh.MH_KERNEL_STREAM)
+    assert stream.info().kernel == mh.MH_KERNEL_STREAM
+    assert stream.autotune(200).kernel == mh.MH_KERNEL_STREAM
+    stream.Step(300)
+    assert np.array_equal(bits(stream.collect_arrays()[0]), bits(p))
