@@ -11,10 +11,21 @@
 //                          Build(); dumps masses, springs, baseline COM and
 //                          length; SoftBody::Reset() mean lengths; and the
 //                          fitness of given position sets via Update().
+//   encode <in> <out>      the same robot inputs as `voxel`; writes the
+//                          reference's VoxelRobot::Encode and
+//                          SoftBody::Encode texts of the built robot, then
+//                          each decoded by VoxelRobot::Decode /
+//                          SoftBody::Decode and encoded again (a decoder
+//                          that throws gives "exception: <what>").
+//   decode <kind> <in> <out>  decodes the text file <in> with the
+//                          reference's VoxelRobot::Decode (kind "voxel") or
+//                          SoftBody::Decode (kind "soft") and writes its
+//                          Encode text.
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <vector>
+#include <stdexcept>
 #include <string>
 #include "VoxelRobot.h"
 
@@ -134,11 +145,82 @@ int do_voxel(const char* in, const char* out) {
   return 0;
 }
 
+void put_text(FILE* f, const std::string& s) {
+  put<int32_t>(f, (int32_t)s.size());
+  fwrite(s.data(), 1, s.size(), f);
+}
+
+bool write_file(const std::string& path, const std::string& s) {
+  FILE* f = fopen(path.c_str(), "wb");
+  if (!f) return false;
+  fwrite(s.data(), 1, s.size(), f);
+  fclose(f);
+  return true;
+}
+
+std::string reencode(const std::string& kind, const std::string& path) {
+  try {
+    if (kind == "voxel") {
+      VoxelRobot V;
+      V.Decode(path);
+      return V.Encode();
+    }
+    SoftBody S;
+    S.Decode(path);
+    return S.Encode();
+  } catch (const std::exception& e) {
+    return std::string("exception: ") + e.what();
+  }
+}
+
+int do_encode(const char* in, const char* out) {
+  FILE* fi = fopen(in, "rb");
+  FILE* fo = fopen(out, "wb");
+  if (!fi || !fo) return 1;
+  float size = get<float>(fi);
+  float res = get<float>(fi);
+  int32_t nvox = get<int32_t>(fi);
+  std::vector<uint32_t> enc(nvox);
+  for (int i = 0; i < nvox; i++) enc[i] = get<uint32_t>(fi);
+  VoxelRobot R;
+  R.setResolution(res);
+  R.setUniformSize(size);
+  R.Initialize();
+  if ((int)R.getVoxels().size() != nvox) return 3;
+  for (int i = 0; i < nvox; i++) R.getVoxels()[i].mat = materials::decode(enc[i]);
+  R.Build();
+  const std::string vox = R.Encode();
+  const std::string soft = R.SoftBody::Encode();
+  put_text(fo, vox);
+  put_text(fo, soft);
+  const std::string tmp = std::string(out) + ".txt";
+  if (!write_file(tmp, vox)) return 4;
+  put_text(fo, reencode("voxel", tmp));
+  if (!write_file(tmp, soft)) return 4;
+  put_text(fo, reencode("soft", tmp));
+  remove(tmp.c_str());
+  fclose(fi);
+  fclose(fo);
+  return 0;
+}
+
+int do_decode(const char* kind, const char* in, const char* out) {
+  FILE* fo = fopen(out, "wb");
+  if (!fo) return 1;
+  const std::string s = reencode(kind, in);
+  fwrite(s.data(), 1, s.size(), fo);
+  fclose(fo);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc >= 4 && std::string(argv[1]) == "materials") return do_materials(argv[2], argv[3]);
   if (argc >= 4 && std::string(argv[1]) == "voxel") return do_voxel(argv[2], argv[3]);
-  fprintf(stderr, "usage: ref_dump materials <in> <out> | voxel <in> <out>\n");
+  if (argc >= 4 && std::string(argv[1]) == "encode") return do_encode(argv[2], argv[3]);
+  if (argc >= 5 && std::string(argv[1]) == "decode") return do_decode(argv[2], argv[3], argv[4]);
+  fprintf(stderr, "usage: ref_dump materials <in> <out> | voxel <in> <out> | encode <in> <out> | "
+                  "decode voxel|soft <in> <out>\n");
   return 1;
 }

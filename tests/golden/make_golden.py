@@ -116,6 +116,53 @@ def voxel_case(name, n, size, res, enc, rng, nsets=2):
     return nm, ns
 
 
+def format_cases(cases):
+    """VoxelRobot::Encode and SoftBody::Encode texts of built robots (base
+    material voxels), and the reference's own Decode -> Encode round trip of
+    each, into formats_reference.npz (texts as uint8 arrays)."""
+    out = {}
+    for name, n, size, res, enc in cases:
+        payload = struct.pack("<ffi", size, res, n ** 3) + enc.astype("<u4").tobytes()
+        r = Reader(run("encode", payload))
+        texts = []
+        for _ in range(4):
+            k = r.i32()
+            texts.append(r.arr("u1", k))
+        vox, soft, vox_rt, soft_rt = texts
+        out[f"{name}_n"] = np.int32(n)
+        out[f"{name}_size"] = np.float32(size)
+        out[f"{name}_res"] = np.float32(res)
+        out[f"{name}_enc"] = enc.astype("<u4")
+        out[f"{name}_voxel"] = vox
+        out[f"{name}_soft"] = soft
+        out[f"{name}_voxel_roundtrip"] = vox_rt
+        out[f"{name}_soft_roundtrip"] = soft_rt
+        print("format", name, len(vox), len(soft))
+    out["names"] = np.array([c[0] for c in cases])
+    # a soft body with faces and cells, written by this project's encoder,
+    # read by the reference's SoftBody::Decode and written again
+    sys.path.insert(0, REPO)
+    from mercury_amd import formats
+    from mercury_amd.robots import Element
+    el = Element(np.array([[0, 0, 0], [0.1, 0, 0], [0, 0.1, 0], [0, 0, 0.1]], np.float32),
+                 np.array([22, 21, 1, 0], np.uint8),
+                 np.array([[0, 1], [1, 2], [2, 3], [0, 3]], np.uint16),
+                 np.array([0.1, 0.141421363, 0.141421363, 0.1], np.float32),
+                 np.array([233, 22, 5, 0], np.uint8),
+                 mass=np.array([0.001, 0.002, 0.001, 0.0005], np.float32))
+    text = formats.encode_softbody(el, faces=[(0, 1, 2), (1, 2, 3)],
+                                   cells=[(0, 1, 2, 3, 0.000166666665, 4)], boundary_count=3)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.txt"), os.path.join(td, "out.txt")
+        with open(fin, "w") as f:
+            f.write(text)
+        subprocess.run([REF_DUMP, "decode", "soft", fin, fout], check=True)
+        with open(fout, "rb") as f:
+            out["faces_input"] = np.frombuffer(text.encode(), np.uint8)
+            out["faces_reference"] = np.frombuffer(f.read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "formats_reference.npz"), **out)
+
+
 def last_layer_air(n, enc):
     idx = np.arange(n ** 3)
     x, y, z = idx % n, (idx // n) % n, idx // (n * n)
@@ -166,6 +213,13 @@ def main():
     for c in cases:
         nm, ns = voxel_case(*c, rng=rng)
         print(c[0], nm, ns)
+    # robot text formats of base-material robots: 3^3 over the 23 base
+    # materials, and one n=5 and one n=9 case from above
+    frng = np.random.default_rng(77)
+    n = 4
+    fcases = [("n4_base", n, 0.4, 10.0, last_layer_air(n, (1 << frng.integers(0, 23, n ** 3)).astype(np.uint32)))]
+    fcases += [c for c in cases if c[0] in ("n5_random", "n9_p20_0")]
+    format_cases(fcases)
 
 
 if __name__ == "__main__":

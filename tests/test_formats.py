@@ -1,7 +1,9 @@
-"""Robot text formats (SoftBody / VoxelRobot Encode and Decode): known
-answers written out from the reference's stream semantics, exact round
-trips, and byte equality of the C++ view (include/mercury_hip_io.hpp) and
-the Python view (mercury_amd.formats).  CPU only."""
+"""Robot text formats (SoftBody / VoxelRobot Encode and Decode): texts
+written by the compiled reference's own encoders and decoders
+(tests/golden/formats_reference.npz), known answers written out from the
+reference's stream semantics, exact round trips, and byte equality of the
+C++ view (include/mercury_hip_io.hpp) and the Python view
+(mercury_amd.formats).  CPU only."""
 import os
 import subprocess
 import tempfile
@@ -124,3 +126,57 @@ def test_cpp_and_python_views_write_the_same_bytes():
     size = np.float32(n) / np.float32(10.0)
     assert formats.encode_voxel_robot(mats, (n, n, n), 10.0, formats.reference_voxel_materials(ms),
                                       sizes=(size, size, size)) == vox_cpp
+
+
+def _golden_formats():
+    return np.load(os.path.join(REPO, "tests", "golden", "formats_reference.npz"))
+
+
+def _text(z, key):
+    return z[key].tobytes().decode()
+
+
+@pytest.mark.parametrize("name", ["n4_base", "n5_random", "n9_p20_0"])
+def test_encoders_write_the_reference_encoders_bytes(name):
+    """Against texts written by the compiled reference (VoxelRobot::Encode,
+    SoftBody::Encode of the built robot; tests/golden/make_golden.py):
+    both encoders write the same bytes, and both decoders read them back."""
+    z = _golden_formats()
+    n, size, res = int(z[f"{name}_n"]), np.float32(z[f"{name}_size"]), float(z[f"{name}_res"])
+    enc = z[f"{name}_enc"]
+    soft, vox = _text(z, f"{name}_soft"), _text(z, f"{name}_voxel")
+    ms = MaterialSet.reference()
+    el = mh.voxel_build(ms, n, res, enc, mass=1.0)
+    assert formats.encode_softbody(el) == soft
+    mats = np.log2(enc).astype(np.uint8)  # one base material (or air) per lattice point
+    table = formats.reference_voxel_materials(ms)
+    assert formats.encode_voxel_robot(mats, (n, n, n), res, table, sizes=(size, size, size)) == vox
+    d, ids, faces, cells, bc = formats.decode_softbody(soft)
+    assert formats.encode_softbody(d) == soft and (faces, cells, bc) == ([], [], 0)
+    back, shape, r, _ = formats.decode_voxel_robot(vox, table)
+    assert shape == (n, n, n) and r == res and np.array_equal(back, mats)
+    # the reference's own Decode -> Encode: VoxelRobot::Decode appends the
+    # file's voxels to the default robot's instead of replacing them, so its
+    # text ends with them; SoftBody::Decode cannot read a file whose face
+    # list is empty (the key "cells=" follows "faces=" on the same line)
+    rt = _text(z, f"{name}_voxel_roundtrip").split("\n")
+    lines = vox.split("\n")
+    assert rt[:5] == lines[:5] and rt[-(n ** 3) - 1:] == lines[5:]
+    assert _text(z, f"{name}_soft_roundtrip").startswith("exception: stoi")
+
+
+def test_reference_decoder_reads_faces_and_cells_we_write():
+    """A soft body with faces, cells and a boundary count, written by this
+    encoder, read by the reference's SoftBody::Decode and written again by
+    its Encode: the same bytes."""
+    z = _golden_formats()
+    text = _text(z, "faces_input")
+    el = Element(np.array([[0, 0, 0], [0.1, 0, 0], [0, 0.1, 0], [0, 0, 0.1]], np.float32),
+                 np.array([22, 21, 1, 0], np.uint8),
+                 np.array([[0, 1], [1, 2], [2, 3], [0, 3]], np.uint16),
+                 np.array([0.1, 0.141421363, 0.141421363, 0.1], np.float32),
+                 np.array([233, 22, 5, 0], np.uint8),
+                 mass=np.array([0.001, 0.002, 0.001, 0.0005], np.float32))
+    assert formats.encode_softbody(el, faces=[(0, 1, 2), (1, 2, 3)],
+                                   cells=[(0, 1, 2, 3, 0.000166666665, 4)], boundary_count=3) == text
+    assert _text(z, "faces_reference") == text
