@@ -176,14 +176,15 @@ int mh_sim_simulate(mh_sim* sim, float duration, int track_stresses);
 int mh_sim_step(mh_sim* sim, int n_steps, int track_stresses);
 int mh_sim_synchronize(mh_sim* sim);
 /* Caller-allocated outputs, each may be NULL: pos/vel 3*M, stress S
- * (original spring order; air springs stay 0). */
+ * (original spring order; air springs report what mh_sim_set_stress gave
+ * them, else 0). */
 int mh_sim_collect(mh_sim* sim, float* pos, float* vel, float* stress);
 /* No reference counterpart: sets the accumulated stress of every spring
- * (S floats, original spring order; air springs are ignored), which
- * mh_sim_set_population zeroes.  Devo cycles use it to keep accumulating
- * across a population re-upload, as the reference's stress buffer does
- * across Simulator::Devo calls (cleared only by SetElements,
- * Simulator.cu:355). */
+ * (S floats, original spring order), which mh_sim_set_population zeroes.
+ * Air springs are not stepped, so they keep the value given here.  Devo
+ * cycles use it to keep accumulating across a population re-upload, as the
+ * reference's stress buffer does across Simulator::Devo calls (cleared only
+ * by SetElements, Simulator.cu:355). */
 int mh_sim_set_stress(mh_sim* sim, const float* stress);
 /* Per-robot fitness on the device: MH_FITNESS_REFERENCE gives
  * max((COMx - COMx0)/L, 0) with the baseline taken at set_population;

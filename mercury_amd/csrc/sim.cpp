@@ -132,6 +132,9 @@ struct mh_sim {
   std::vector<std::vector<uint64_t>> pair_set;
   std::vector<uint16_t> act_pairs;   // active springs, robot-local
   std::vector<uint32_t> active_src;  // active spring -> original spring index
+  // stress held by air springs (original order; empty = all zero): the step
+  // never touches them, so they keep what mh_sim_set_stress gave them
+  std::vector<float> held_stress;
   std::vector<uint8_t> act_mat;      // compact material index
   std::vector<float> act_lbar;       // mean length per active spring
   std::vector<uint8_t> compact_to_id;
@@ -1113,6 +1116,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   s->n_masses = M;
   s->n_springs = S;
   s->n_active = s->active_src.size();
+  s->held_stress.clear();
   s->total_time = 0.f;
 
   s->pos.upload(p4.data(), M, s->stream);
@@ -1186,7 +1190,10 @@ int mh_sim_collect(mh_sim* s, float* pos, float* vel, float* stress) {
     }
   }
   if (stress) {
-    std::fill(stress, stress + s->n_springs, 0.f);
+    if (s->held_stress.empty())
+      std::fill(stress, stress + s->n_springs, 0.f);
+    else
+      std::copy(s->held_stress.begin(), s->held_stress.end(), stress);
     if (s->n_active) {
       std::vector<float> st(s->n_active);
       mh::hip_check(hipMemcpyAsync(st.data(), s->stress.p, s->n_active * sizeof(float), hipMemcpyDeviceToHost,
@@ -1204,6 +1211,10 @@ int mh_sim_set_stress(mh_sim* s, const float* stress) {
   if (!s || !stress) throw mh::Error(MH_ERR_ARG, "null argument");
   if (!s->has_population) throw mh::Error(MH_ERR_STATE, "no population set");
   mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  // air springs keep their value, as slots of the reference's buffer do
+  // while solveDistance skips them (sim_kernel.cu:211)
+  s->held_stress.assign(stress, stress + s->n_springs);
+  for (uint32_t k : s->active_src) s->held_stress[k] = 0.f;
   if (s->n_active) {
     std::vector<float> st(s->n_active);
     for (size_t k = 0; k < s->n_active; k++) st[k] = stress[s->active_src[k]];

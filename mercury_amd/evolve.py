@@ -237,14 +237,18 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         age_gap=5, threshold=None, max_generations=40, use_surrogate=False, screen_factor=8,
         retrain_every=5, min_rho=0.6, train_epochs=None, duration=1.0, sim=None, verbose=False,
         audit=0.1, surrogate_kind="mlp", max_train=200000, warm_start=False,
-        fallback_retrain=True, mirror=False):
+        fallback_retrain=True, mirror=False, log=None):
     """One evolution run; stops when the best fitness reaches `threshold` or
     after max_generations.  Returns a RunResult.  warm_start: each retrain
     continues from the previous network's weights instead of a fresh one.
     fallback_retrain: when the surrogate ranks below min_rho, retrain it
     before the next generation (which it then screens) instead of
     simulating unscreened generations until the periodic retrain.
-    mirror: train the surrogate on every genome and its z-mirror."""
+    mirror: train the surrogate on every genome and its z-mirror.
+    log: a list that receives one dict per generation: whether the
+    surrogate screened it, how many offspring were scored and simulated,
+    and for a screened generation the scored genomes, their predictions
+    and layers, the indices simulated and which of them are audits."""
     rng = np.random.default_rng(seed)
     sim = sim or BatchSimulator(voxels=voxels, duration=duration)
     evals0, t_start = sim.evals, time.perf_counter()
@@ -302,8 +306,10 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
         kids = np.concatenate(kids)
         kid_age = np.concatenate(kid_age)
         layer_of = np.repeat(np.arange(layers), n_child // layers)
+        entry = dict(generation=gen, screened=bool(use_surrogate and screening), scored=0)
         if use_surrogate and screening:
             pred = sur.predict(kids)
+            entry.update(scored=len(kids), genomes=kids, pred=pred, layer_of=layer_of)
             keep, audited = [], []
             n_audit = int(round(audit * per))
             for i in range(layers):
@@ -319,17 +325,22 @@ def run(seed=0, pop=4096, voxels=8, layers=8, crossover=0.5, mutation=0.05, elit
                 audited.insert(-1, np.zeros(len(top), bool))
             keep = np.concatenate(keep)
             audited = np.concatenate(audited)
+            entry.update(keep=keep, audited=audited)
             kids, kid_age, layer_of, pred = kids[keep], kid_age[keep], layer_of[keep], pred[keep]
         else:
             res.fallback_generations += int(use_surrogate)
         fit = sim.fitness(kids)
         data_x.append(kids)
         data_y.append(fit)
+        entry["simulated"] = len(kids)
+        if log is not None:
+            log.append(entry)
         if use_surrogate and screening:
             # with an audit sample, rank quality is measured on it alone (the
             # screened top share is range-restricted)
             rho = spearman(pred[audited], fit[audited]) if audited.sum() >= 8 else spearman(pred, fit)
             res.rho_history.append(rho)
+            entry["rho"] = rho
             if rho < min_rho:
                 screening = False
         elif use_surrogate:

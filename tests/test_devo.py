@@ -129,3 +129,25 @@ def test_devo_stress_accumulates_over_cycles():
     for got, want in zip(ev.last_devo, current):
         assert np.array_equal(got.pairs, want.pairs)
         assert np.array_equal(got.spring_material, want.spring_material)
+
+
+@pytest.mark.gpu
+def test_air_springs_keep_the_stress_they_are_given():
+    """Air springs are not stepped: collect reports the stress set_stress
+    gave them, as the reference's per-slot buffer keeps it while
+    solveDistance skips them (sim_kernel.cu:211)."""
+    print("air-spring stress is held", flush=True)
+    ms = mh.workload_materials()
+    robots = mh.voxel_population(ms, 2, voxels_per_axis=3, seed=4, mass=0.001)
+    sim = mh.Simulator(ms, mh.SimConfig(solver=mh.MH_SOLVER_XPBD_REF, time_step=1e-4))
+    sim.SetElements(robots)
+    air = np.concatenate([r.spring_material == 0 for r in robots])
+    assert air.any() and (~air).any()
+    given = np.arange(1, len(air) + 1, dtype=np.float32)
+    sim.set_stress(given)
+    sim.Step(10, trackStresses=True)
+    st = sim.collect_arrays()[2]
+    assert np.array_equal(st[air], given[air])
+    assert not np.array_equal(st[~air], given[~air])  # stepped springs accumulate
+    sim.SetElements(robots)  # a new upload clears every slot
+    assert not sim.collect_arrays()[2].any()

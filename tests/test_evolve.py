@@ -247,3 +247,73 @@ def test_mirrored_robots_travel_alike_in_the_oracle():
     a, b = travel(g), travel(evolve.mirror_z(g, 8))
     assert np.abs(a).max() > 1e-4                      # the robots do move
     assert spearmanr(a, b).correlation > 0.95
+
+
+class _RecordingSim(evolve.BatchSimulator):
+    """The simulator of the evolution loop, remembering every batch it scored."""
+
+    def __init__(self, **kw):
+        super().__init__(**kw)
+        self.batches = []
+
+    def fitness(self, genomes):
+        self.batches.append(np.array(genomes, copy=True))
+        return super().fitness(genomes)
+
+
+def _check_screen(entry, per, n_audit, layers):
+    """The simulated offspring of a screened generation are, per layer, the
+    best (per - n_audit) by prediction and n_audit audits drawn from the rest."""
+    pred, lay, keep, aud = entry["pred"], entry["layer_of"], entry["keep"], entry["audited"]
+    assert len(keep) == per * layers and aud.sum() == n_audit * layers
+    for i in range(layers):
+        mine = np.flatnonzero(lay == i)
+        k = keep[lay[keep] == i]
+        a = aud[lay[keep] == i]
+        top = mine[np.argsort(-pred[mine], kind="stable")[:per - n_audit]]
+        assert np.array_equal(np.sort(k[~a]), np.sort(top))
+        rest = np.setdiff1d(mine, top)
+        assert len(np.unique(k[a])) == n_audit and np.isin(k[a], rest).all()
+
+
+@pytest.mark.gpu
+def test_surrogate_screened_generation_full_size(monkeypatch):
+    """One generation of the screened protocol at full size: population
+    4096 robots of 8^3 voxels, ALPS with 8 layers, elitism 0.1, swap
+    crossover 0.5, mutation 0.05; the MLP surrogate scores 4 x 4096 =
+    16384 offspring and the best of each layer plus a 10 % audit are
+    simulated for 1 s -- 4096 evaluations per generation, counted one per
+    simulated robot as Evaluator_impl.h:103-106 counts them.  Then the same
+    with a sign-flipped surrogate: its audit ranking falls below 0.6, so
+    the next generation is simulated without screening."""
+    print("screened generation: pop 4096, 8^3, 1 s, 16384 scored", flush=True)
+    pop, layers = 4096, 8
+    per, n_audit = pop // layers, int(round(0.1 * (pop // layers)))
+    sim = _RecordingSim(voxels=8, duration=1.0)
+    log = []
+    kw = dict(pop=pop, voxels=8, layers=layers, crossover=0.5, mutation=0.05, elitism=0.1,
+              use_surrogate=True, screen_factor=4, fallback_retrain=False, surrogate_kind="mlp",
+              duration=1.0, sim=sim)
+    r = evolve.run(seed=0, max_generations=1, log=log, **kw)
+    assert [len(b) for b in sim.batches] == [pop, pop] and sim.evals == 2 * pop
+    assert [h[0] for h in r.history] == [pop, 2 * pop]
+    (e,) = log
+    assert e["screened"] and e["scored"] == 4 * pop and e["simulated"] == pop
+    _check_screen(e, per, n_audit, layers)
+    assert np.array_equal(sim.batches[1], e["genomes"][e["keep"]])
+    print(f"audit rho {e['rho']:.3f}", flush=True)
+
+    class Flipped(evolve.GenomeModel):
+        def predict(self, genomes):
+            return -super().predict(genomes)
+
+    monkeypatch.setattr(evolve, "GenomeModel", Flipped)
+    sim.batches.clear()
+    log2 = []
+    r2 = evolve.run(seed=1, max_generations=2, log=log2, **kw)
+    first, second = log2
+    assert first["screened"] and first["rho"] < 0.6
+    _check_screen(first, per, n_audit, layers)
+    assert not second["screened"] and second["simulated"] == pop
+    assert r2.fallback_generations == 1 and len(r2.rho_history) == 2
+    assert [len(b) for b in sim.batches] == [pop, pop, pop]

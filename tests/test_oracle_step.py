@@ -325,3 +325,37 @@ def test_force_mirror_matches_independent_restatement(drag):
     ep, ev = numpy_force_step_f32(pos, vel, mass, pairs, lbar, mat, k, R, env)
     assert np.array_equal(np.asarray(p).reshape(-1, 3).view(np.uint32), ep.view(np.uint32))
     assert np.array_equal(np.asarray(v).reshape(-1, 3).view(np.uint32), ev.view(np.uint32))
+
+
+def test_fp64_one_second_is_sensitive_to_one_ulp():
+    """Why FP32 and FP64 displacement rankings over a 1 s land evaluation
+    cannot be identical: the FP64 step itself, with one coordinate of one
+    mass per robot moved by one float ulp (about 3e-9 m), ends up to
+    0.46 mm away and orders robot pairs differently (64 robots of 8^3
+    voxels, 10000 steps, measured 2 pairs)."""
+    import os
+    from helpers import LAND, workload_case
+    case = workload_case(n_robots=64, voxels=8, p_empty=0.2, seed=0)
+    env = oracle.make_env(1e-4, **LAND)
+    mo, mm = case.a["mass_offset"], case.a["mass_material"]
+    th = max(1, min(16, len(os.sched_getaffinity(0))))
+
+    def disp(p):
+        p = p.reshape(-1, 3)
+        out = []
+        for r in range(len(mo) - 1):
+            sl = slice(mo[r], mo[r + 1])
+            air = mm[sl] == 0
+            out.append(p[sl][~air].mean(0)[0] - case.pos0[sl][~air].astype(np.float64).mean(0)[0])
+        return np.array(out)
+
+    pos = case.pos0.copy()
+    rng = np.random.default_rng(0)
+    for r in range(len(mo) - 1):
+        i = mo[r] + rng.integers(0, mo[r + 1] - mo[r])
+        pos[i, 0] = np.nextafter(pos[i, 0], np.float32(1))
+    a = disp(oracle.force_f64(case.pop, case.mt, env, 0.0, 10000, case.pos0, case.vel0, nthreads=th)[0])
+    b = disp(oracle.force_f64(case.pop, case.mt, env, 0.0, 10000, pos, case.vel0, nthreads=th)[0])
+    i, j = np.triu_indices(len(a), 1)
+    assert np.abs(a - b).max() > 1e-4
+    assert (np.sign(a[i] - a[j]) != np.sign(b[i] - b[j])).any()
