@@ -276,6 +276,22 @@ int mh_mlp_get_params(const mh_mlp* net, float* out);
 int mh_mlp_set_params(mh_mlp* net, const float* in);
 /* y[batch x out] = net(x[batch x in]); host buffers. */
 int mh_mlp_forward(mh_mlp* net, const float* x, int batch, float* y);
+/* No reference counterpart; for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops that score offspring every
+ * generation): mh_mlp_forward on device buffers x [batch x in] and
+ * y [batch x out], enqueued on `stream` (a hipStream_t; NULL = the
+ * network's own stream) without waiting, with the same results.  The hidden
+ * activations live in a fixed device array of MH_MLP_SCRATCH_FLOATS floats
+ * (16 MiB on each device the library runs on, held until the process
+ * ends), used in two halves: batch x (widest hidden layer) over
+ * MH_MLP_SCRATCH_FLOATS / 2 gives MH_ERR_RANGE and launches nothing.  All
+ * calls on one device share the array, so calls that overlap in time there
+ * (other streams, other threads) race; training the same network
+ * concurrently races too.  Calls no hipMalloc, hipFree or allocating
+ * routine: the array's address is resolved by mh_mlp_create. */
+#define MH_MLP_SCRATCH_FLOATS (4u << 20)
+int mh_mlp_forward_noalloc(mh_mlp* net, const float* x, int batch, float* y,
+                           void* stream);
 /* Loss = mean 0.5 (y - t)^2 and its gradient w.r.t. the parameters (same
  * layout as get_params), evaluated with the bf16 weights. */
 int mh_mlp_loss_grad(mh_mlp* net, const float* x, const float* target,

@@ -104,6 +104,7 @@ _SIGS = {
     "mh_mlp_get_params": (c_int, [vp, f32p]),
     "mh_mlp_set_params": (c_int, [vp, f32p]),
     "mh_mlp_forward": (c_int, [vp, f32p, c_int, f32p]),
+    "mh_mlp_forward_noalloc": (c_int, [vp, vp, c_int, vp, vp]),
     "mh_mlp_loss_grad": (c_int, [vp, f32p, f32p, c_int, f32p, f32p]),
     "mh_mlp_train": (c_int, [vp, f32p, f32p, c_int, c_int, c_int, c_float, c_uint64, f32p]),
     "mh_cnn_create": (c_int, [c_int, ctypes.c_uint64, ctypes.POINTER(vp)]),

@@ -47,7 +47,8 @@ __device__ __forceinline__ float ld<__hip_bfloat16>(const __hip_bfloat16* p, siz
 // and op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n].  Epilogue EPI:
 // 0 = store; 1 = + bias[n], store Z and gelu(Z) (hidden layer forward);
 // 2 = + bias[n], store (output layer forward); 3 = multiply by
-// gelu'(Zprev[m][n]) (hidden layer backward).
+// gelu'(Zprev[m][n]) (hidden layer backward); 4 = + bias[n], store gelu
+// (hidden layer forward without keeping Z: the same values as 1 stores in C).
 template <typename TA, typename TB, bool TRA, bool TRB, int EPI>
 __global__ __launch_bounds__(256) void gemm(const TA* __restrict__ A, int lda, const TB* __restrict__ B,
                                             int ldb, float* __restrict__ C, int ldc, int M, int N, int K,
@@ -102,6 +103,8 @@ __global__ __launch_bounds__(256) void gemm(const TA* __restrict__ A, int lda, c
         v += bias[n];
       } else if (EPI == 3) {
         v *= gelu_grad(Zprev[o]);
+      } else if (EPI == 4) {
+        v = gelu_f(v + bias[n]);
       }
       C[o] = v;
     }

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -26,8 +27,16 @@
 #include "mh_internal.h"
 #include "mh_nn_kernels.h"
 
+// Scratch of mh_mlp_forward_noalloc: the hidden activations of one call, in
+// two halves that alternate between layers.  Every device holds its own
+// copy (MH_MLP_SCRATCH_FLOATS floats, 16 MiB) until the process ends.  In
+// host code this name stands for a host-side shadow, not a device address:
+// kernels take the address hipGetSymbolAddress gives on each device.
+static __device__ __align__(256) float scratch[MH_MLP_SCRATCH_FLOATS];
+
 struct mh_mlp {
   int device = 0;
+  float* scratch = nullptr;   // device address of ::scratch on `device`
   std::vector<int> sizes;     // n_layers + 1
   std::vector<size_t> w_off;  // offset of W_l in the parameter vector; b_l follows W_l
   size_t n_params = 0;
@@ -135,6 +144,22 @@ int fail(const mh::Error& e) {
   return e.code;
 }
 
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), resolved once per device.
+float* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) throw mh::Error(MH_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    hipcheck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch[device] = static_cast<float*>(p);
+  }
+  return g_scratch[device];
+}
+
 }  // namespace
 
 extern "C" {
@@ -145,6 +170,7 @@ int mh_mlp_create(int device, int n_layers, const int* sizes, uint64_t seed, mh_
     auto net = std::make_unique<mh_mlp>();
     net->device = device;
     hipcheck(hipSetDevice(device), "hipSetDevice");
+    net->scratch = scratch_address(device);
     hipcheck(hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking), "stream");
     net->sizes.assign(sizes, sizes + n_layers + 1);
     for (int l = 0; l <= n_layers; l++)
@@ -237,6 +263,43 @@ int mh_mlp_forward(mh_mlp* net, const float* x, int batch, float* y) {
                               net->stream), "D2H");
     }
     hipcheck(hipStreamSynchronize(net->stream), "sync");
+    return MH_OK;
+  } catch (const mh::Error& e) {
+    return fail(e);
+  }
+}
+
+int mh_mlp_forward_noalloc(mh_mlp* net, const float* x, int batch, float* y, void* stream) {
+  try {
+    if (!net || batch < 0 || (batch > 0 && (!x || !y))) throw mh::Error(MH_ERR_ARG, "null argument");
+    if (batch == 0) return MH_OK;
+    const int L = net->L();
+    size_t widest = 0;
+    for (int l = 1; l < L; l++) widest = std::max<size_t>(widest, net->sizes[l]);
+    const size_t half = MH_MLP_SCRATCH_FLOATS / 2;
+    if (static_cast<size_t>(batch) * widest > half)
+      throw mh::Error(MH_ERR_RANGE, "batch x widest hidden layer (" + std::to_string(batch) + " x " +
+                                        std::to_string(widest) + ") over the scratch array's half (" +
+                                        std::to_string(half) + " floats)");
+    hipcheck(hipSetDevice(net->device), "hipSetDevice");
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : net->stream;
+    const float* prev = x;
+    for (int l = 0; l < L; l++) {
+      const int ni = net->sizes[l], no = net->sizes[l + 1];
+      const __hip_bfloat16* W = net->w16.p + net->w_off[l];
+      const float* b = net->params.p + net->b_off(l);
+      float* out = l == L - 1 ? y : net->scratch + (l % 2) * half;
+      if (l < L - 1)
+        hipLaunchKernelGGL((gemm<float, __hip_bfloat16, false, true, 4>), grid_for(batch, no), dim3(256), 0, s,
+                           prev, ni, W, ni, out, no, batch, no, ni, b, static_cast<float*>(nullptr),
+                           static_cast<const float*>(nullptr));
+      else
+        hipLaunchKernelGGL((gemm<float, __hip_bfloat16, false, true, 2>), grid_for(batch, no), dim3(256), 0, s,
+                           prev, ni, W, ni, out, no, batch, no, ni, b, static_cast<float*>(nullptr),
+                           static_cast<const float*>(nullptr));
+      hipcheck(hipGetLastError(), "mlp forward (no-alloc)");
+      prev = out;
+    }
     return MH_OK;
   } catch (const mh::Error& e) {
     return fail(e);
