@@ -99,6 +99,16 @@ int mh_voxel_build_batch(const mh_material_set* set, int n_robots, int nx, int n
                          float* mass_pos, uint8_t* mass_material_id, uint32_t* spring_offset,
                          uint16_t* spring_pairs, float* spring_length,
                          uint8_t* spring_material_id, int64_t spring_capacity);
+/* No reference counterpart: what mh_voxel_build gives every robot of an
+ * nx*ny*nz lattice whatever its materials (VoxelRobot::Build's DFS visits
+ * every lattice point, VoxelRobot.cpp:310-331): mass positions (3 per
+ * point, shifted), the springs in DFS order and their lengths, and per
+ * spring the up to 4 lattice points (0xffff = none) whose voxels decide
+ * its material: decode(OR of their non-air encoding bits).  spring_voxels
+ * holds 4 entries per spring. */
+int mh_voxel_template(int nx, int ny, int nz, float resolution, float* mass_pos,
+                      uint16_t* spring_pairs, float* spring_length,
+                      uint16_t* spring_voxels, int max_springs, int* n_springs);
 /* Mean lengths |p[m0] - p[m1]| as SoftBody::Reset computes them. */
 int mh_rest_lengths(const float* pos, const uint16_t* pairs, int n_springs,
                     float* out);
@@ -169,6 +179,28 @@ int mh_sim_destroy(mh_sim* sim);
  * per-block packing, CSR gather lists and material tables, zeroes stresses
  * and the simulated time. */
 int mh_sim_set_population(mh_sim* sim, const mh_population* pop);
+/* No reference counterpart: Simulator::SetElements for voxel robots of one
+ * nx*ny*nz lattice (at most 1024 points), built on the device from their
+ * voxel encodings (VoxelRobot::Build per robot, VoxelRobot.cpp:30-331, with
+ * the rules mh_voxel_build restates; BatchBuild, VoxelRobot.cpp:291-308).
+ * voxel_encoding holds n_robots * nx*ny*nz encodings, host or device
+ * memory; nothing else crosses to the device.  Every robot gets the masses
+ * and springs mh_voxel_build gives it (positions after ShiftX/ShiftY,
+ * `mass` per mass, zero velocity; springs in DFS order, the builder's
+ * lengths as mean lengths): simulate, collect (stress per spring in each
+ * robot's DFS order, air springs included), fitness and com give the
+ * results of mh_voxel_build_batch followed by mh_sim_set_population,
+ * bit for bit.  Runs on the mass-parallel kernel, one robot per workgroup
+ * (MH_ERR_ARG when the config excludes it); MH_ERR_RANGE when a robot has
+ * more than 511 distinct (rest length, material) pairs. */
+int mh_sim_set_voxel_population(mh_sim* sim, int n_robots, int nx, int ny, int nz,
+                                float resolution, float mass,
+                                const uint32_t* voxel_encoding);
+/* No reference counterpart: the materials of a device-built voxel
+ * population (each may be NULL): mass ids (n_robots * points), spring ids
+ * (n_robots * springs, DFS order), voxel encodings after Strip. */
+int mh_sim_voxel_materials(mh_sim* sim, uint8_t* mass_material_id,
+                           uint8_t* spring_material_id, uint32_t* stripped_encoding);
 /* Runs the reference's float time loop: while (remaining > 0) { step(t);
  * t += dt; remaining -= dt; }.  Asynchronous: returns once launched. */
 int mh_sim_simulate(mh_sim* sim, float duration, int track_stresses);

@@ -68,6 +68,7 @@ _SIGS = {
     "mh_voxel_max_springs": (c_int, [c_int, c_int, c_int]),
     "mh_voxel_build": (c_int, [vp, c_int, c_int, c_int, c_float, u32p, f32p, u8p, u16p, f32p,
                                u8p, u32p, c_int, ctypes.POINTER(c_int), u32p]),
+    "mh_voxel_template": (c_int, [c_int, c_int, c_int, c_float, f32p, u16p, f32p, u16p, c_int, i32p]),
     "mh_voxel_build_batch": (c_int, [vp, c_int, c_int, c_int, c_int, c_float, u32p, c_int, f32p,
                                      u8p, u32p, u16p, f32p, u8p, ctypes.c_int64]),
     "mh_rest_lengths": (c_int, [f32p, u16p, c_int, f32p]),
@@ -85,6 +86,8 @@ _SIGS = {
     "mh_sim_reset": (c_int, [vp]),
     "mh_sim_total_time": (c_float, [vp]),
     "mh_sim_set_total_time": (c_int, [vp, c_float]),
+    "mh_sim_set_voxel_population": (c_int, [vp, c_int, c_int, c_int, c_int, c_float, c_float, u32p]),
+    "mh_sim_voxel_materials": (c_int, [vp, u8p, u8p, u32p]),
     "mh_sim_set_stress": (c_int, [vp, f32p]),
     "mh_sim_get_time_step": (c_float, [vp]),
     "mh_sim_set_time_step": (c_int, [vp, c_float]),

@@ -139,6 +139,24 @@ int mh_voxel_build(const mh_material_set* set, int nx, int ny, int nz, float res
   MH_CATCH
 }
 
+int mh_voxel_template(int nx, int ny, int nz, float resolution, float* mass_pos, uint16_t* spring_pairs,
+                      float* spring_length, uint16_t* spring_voxels, int max_springs, int* n_springs) {
+  MH_TRY
+  if (!mass_pos || !spring_pairs || !spring_length || !spring_voxels || !n_springs)
+    throw mh::Error(MH_ERR_ARG, "null argument");
+  if (!(resolution > 0.f)) throw mh::Error(MH_ERR_ARG, "resolution must be > 0");
+  mh::VoxelTemplate t;
+  mh::voxel_template(nx, ny, nz, resolution, t);
+  *n_springs = t.n_springs;
+  if (t.n_springs > max_springs) throw mh::Error(MH_ERR_RANGE, "max_springs too small");
+  std::memcpy(mass_pos, t.pos.data(), sizeof(float) * 3 * t.n);
+  std::memcpy(spring_pairs, t.pairs.data(), sizeof(uint16_t) * 2 * t.n_springs);
+  std::memcpy(spring_length, t.length.data(), sizeof(float) * t.n_springs);
+  std::memcpy(spring_voxels, t.voxels.data(), sizeof(uint16_t) * 4 * t.n_springs);
+  return MH_OK;
+  MH_CATCH
+}
+
 int mh_voxel_build_batch(const mh_material_set* set, int n_robots, int nx, int ny, int nz,
                          float resolution, const uint32_t* voxel_encoding, int threads,
                          float* mass_pos, uint8_t* mass_material_id, uint32_t* spring_offset,

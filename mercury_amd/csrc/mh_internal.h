@@ -58,6 +58,18 @@ struct VoxelOut {
   std::vector<uint32_t> spring_enc;
   std::vector<uint32_t> stripped_enc;
 };
+// What every voxel robot of one lattice shape shares: mass positions
+// (after ShiftX / ShiftY), springs in the DFS order with their lengths,
+// and per spring the up to 4 voxels (lattice point indices, 0xffff = none)
+// whose non-air encoding bits, OR-ed and decoded, give its material.
+struct VoxelTemplate {
+  int nx = 0, ny = 0, nz = 0, n = 0, n_springs = 0;
+  std::vector<float> pos;          // 3n
+  std::vector<uint16_t> pairs;     // 2 n_springs
+  std::vector<float> length;       // n_springs
+  std::vector<uint16_t> voxels;    // 4 n_springs
+};
+void voxel_template(int nx, int ny, int nz, float resolution, VoxelTemplate& t);
 void voxel_build(const mh_material_set& set, int nx, int ny, int nz,
                  float resolution, const uint32_t* voxel_enc, VoxelOut& out);
 

@@ -142,6 +142,47 @@ hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes,
 // the step launches that follow read it).
 hipError_t launch_clock(float* times, float t0, float dt, int n_steps, int K, hipStream_t stream);
 
+// Voxel populations built on the device (voxel_kernels.hip): one
+// workgroup per robot of an nx*ny*nz lattice (at most kVoxMaxPoints
+// points) writes the state and the mass-parallel kernel's packing.
+constexpr int kVoxMaxPoints = 1024;
+constexpr int kVoxMaxSprings = 13 * kVoxMaxPoints;
+constexpr int kVoxPairWords = 512;  // (length index, material id) bits: 64 lengths x 256 ids
+struct VoxelBuildParams {
+  int nx, ny, nz, n, n_springs;
+  int n_lengths;                 // distinct spring lengths of the shape (at most kVoxPairWords / 8)
+  int maxp;                      // record slots per side of the mass kernel
+  uint32_t valid_bits;           // encoding bits of the material set
+  float mass;                    // every mass
+  const uint32_t* enc;           // robot-major voxel encodings, n per robot
+  const float* tpos;             // the shape: 3n positions
+  const uint16_t* pairs;         // 2 per spring, DFS order
+  const uint16_t* voxels;        // 4 per spring: points deciding its material (0xffff none)
+  const uint8_t* length_index;   // per spring: index into lengths
+  const float* lengths;          // distinct lengths
+  const uint16_t* first_off;     // n + 1: springs whose first endpoint is the point
+  const uint16_t* first_list;    //   (ascending spring index)
+  const uint16_t* second_off;    // n + 1: springs whose second endpoint is the point
+  const uint16_t* second_list;
+  const uint8_t* code_id;        // decode table by encoded index: composite id,
+  const uint32_t* code_enc;      //   the material's encoding,
+  const uint8_t* code_solid;     //   1 unless every parameter is 0 (Strip's air test)
+  // outputs, robot r at r * n (masses), r * n_springs (springs)
+  float4* pos;
+  float4* vel;
+  uint8_t* mass_id;
+  uint32_t* stripped;            // voxel encodings after Strip
+  uint8_t* spring_id;
+  uint32_t* mrec;                // 3 maxp n per robot
+  uint8_t* mcnt;                 // 2n per robot
+  uint16_t* mperm;               // n per robot
+  float* lbtab;                  // 2 kMassPairMax per robot
+  PackDesc* packs;               // one per robot
+  uint32_t* active;              // non-air springs per robot
+  int* error;                    // first error code, 0 if none
+};
+hipError_t launch_voxel_population(const VoxelBuildParams& p, int n_robots, hipStream_t stream);
+
 // Per-robot COM of non-air masses (SoftBody::updateCOM order).
 hipError_t launch_com(const float4* pos, const uint8_t* mass_id,
                       const uint32_t* robot_mass_begin, int n_robots,

@@ -156,6 +156,25 @@ struct mh_sim {
   mh::DevBuf<mh::PackDesc> packs;
   mh::DevBuf<float> mat_k, mat_kx, mat_dL0, mat_omega, mat_phi;
 
+  // decode table by encoded index, for populations built on the device:
+  // composite id, the material's encoding, 1 unless every parameter is 0
+  std::vector<uint8_t> code_id, code_solid;
+  std::vector<uint32_t> code_enc;
+  uint32_t valid_bits = 0;  // encoding bits of the material set
+  // a voxel population built on the device (mh_sim_set_voxel_population):
+  // stress is kept per spring in the robots' own order, air springs
+  // included; the shape template of the last lattice shape stays resident
+  bool voxel_device = false;
+  int vt_nx = 0, vt_ny = 0, vt_nz = 0, vt_n = 0, vt_springs = 0, vt_lengths = 0;
+  uint32_t vt_res_bits = 0;
+  std::vector<float> vt_pos;  // host copy of the shape's positions (baselines)
+  mh::DevBuf<float> vt_dpos, vt_dlen;
+  mh::DevBuf<uint16_t> vt_pairs, vt_vox, vt_foff, vt_flist, vt_soff, vt_slist;
+  mh::DevBuf<uint8_t> vt_lidx, vt_code_id, vt_code_solid;
+  mh::DevBuf<uint32_t> vt_code_enc, vox_enc, vox_stripped, vox_active;
+  mh::DevBuf<uint8_t> vox_spring_id;
+  mh::DevBuf<int> vox_error;
+
   mh::Plan plan;
   int substeps = 0;  // substeps per launch; 0 = every substep of a call in one launch
   float total_time = 0.f;
@@ -934,6 +953,12 @@ int mh_sim_create(const mh_config* cfg, const mh_material_set* set, mh_sim** out
   s->cfg = *cfg;
   s->mats_by_id = set->by_id;
   s->composite_count = set->composite_count;
+  for (const mh::Material& m : set->by_enc) {
+    s->code_id.push_back(static_cast<uint8_t>(m.id));
+    s->code_enc.push_back(m.encoding);
+    s->code_solid.push_back(!(m.k == 0.f && m.dL0 == 0.f && m.omega == 0.f && m.phi == 0.f));
+  }
+  s->valid_bits = set->base.size() >= 32 ? 0xffffffffu : ((1u << set->base.size()) - 1u);
   s->substeps = cfg->substeps_per_launch > 0 ? cfg->substeps_per_launch : 0;
   int lds = 0;
   if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, cfg->device) == hipSuccess &&
@@ -1117,6 +1142,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   s->n_springs = S;
   s->n_active = s->active_src.size();
   s->held_stress.clear();
+  s->voxel_device = false;
   s->total_time = 0.f;
 
   s->pos.upload(p4.data(), M, s->stream);
@@ -1132,6 +1158,280 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   if (R > 0) mh::choose_plan(s);
   mh::hip_check(hipStreamSynchronize(s->stream), "upload");
   s->has_population = true;
+  return MH_OK;
+  MH_CATCH
+}
+
+namespace mh {
+namespace {
+
+// The mass-parallel variant that takes one robot of n masses per workgroup:
+// the smallest block that holds it, within what the config allows.
+const KernelVariant* voxel_variant(const mh_sim* s, int n) {
+  int count = 0;
+  const KernelVariant* all = kernel_variants(&count);
+  const KernelVariant* best = nullptr;
+  for (int i = 0; i < count; i++) {
+    const KernelVariant& v = all[i];
+    if (v.solver != s->cfg.solver || v.kind != MH_KERNEL_MASS || v.chunked) continue;
+    if (s->cfg.kernel && !(s->cfg.kernel & MH_KERNEL_MASS)) continue;
+    if (s->cfg.block_size && v.block != s->cfg.block_size) continue;
+    if (v.block < n || v.nspt < 13) continue;
+    if (lds_bytes_mass(v.block, s->composite_count) > static_cast<size_t>(s->lds_limit)) continue;
+    if (!best || v.block < best->block) best = &v;
+  }
+  return best;
+}
+
+// The shape template of an nx*ny*nz lattice on the device, kept while
+// populations of the same shape follow.
+void prepare_voxel_shape(mh_sim* s, int nx, int ny, int nz, float resolution) {
+  uint32_t res_bits;
+  std::memcpy(&res_bits, &resolution, 4);
+  if (s->vt_n && s->vt_nx == nx && s->vt_ny == ny && s->vt_nz == nz && s->vt_res_bits == res_bits) return;
+  s->vt_n = 0;
+  VoxelTemplate t;
+  voxel_template(nx, ny, nz, resolution, t);
+  const int n = t.n, S = t.n_springs;
+  // distinct lengths (bit patterns), each spring's index among them
+  std::vector<float> lengths(t.length);
+  std::sort(lengths.begin(), lengths.end(), [](float a, float b) {
+    uint32_t x, y;
+    std::memcpy(&x, &a, 4);
+    std::memcpy(&y, &b, 4);
+    return x < y;
+  });
+  lengths.erase(std::unique(lengths.begin(), lengths.end(),
+                            [](float a, float b) { return std::memcmp(&a, &b, 4) == 0; }),
+                lengths.end());
+  if (static_cast<int>(lengths.size()) > kVoxPairWords / 8)
+    throw Error(MH_ERR_RANGE, "lattice has more distinct spring lengths than the device build's pair index holds");
+  std::vector<uint8_t> lidx(S);
+  for (int k = 0; k < S; k++) {
+    for (size_t j = 0; j < lengths.size(); j++)
+      if (std::memcmp(&lengths[j], &t.length[k], 4) == 0) lidx[k] = static_cast<uint8_t>(j);
+  }
+  // per point: springs whose first (second) endpoint it is, ascending
+  std::vector<uint16_t> foff(n + 1, 0), soff(n + 1, 0), flist(S), slist(S);
+  for (int k = 0; k < S; k++) {
+    foff[t.pairs[2 * k] + 1]++;
+    soff[t.pairs[2 * k + 1] + 1]++;
+  }
+  for (int i = 0; i < n; i++) {
+    foff[i + 1] = static_cast<uint16_t>(foff[i + 1] + foff[i]);
+    soff[i + 1] = static_cast<uint16_t>(soff[i + 1] + soff[i]);
+  }
+  std::vector<uint16_t> fill_f(foff.begin(), foff.end() - 1), fill_s(soff.begin(), soff.end() - 1);
+  for (int k = 0; k < S; k++) {
+    flist[fill_f[t.pairs[2 * k]]++] = static_cast<uint16_t>(k);
+    slist[fill_s[t.pairs[2 * k + 1]]++] = static_cast<uint16_t>(k);
+  }
+  for (int i = 0; i < n; i++)
+    if (foff[i + 1] - foff[i] > 13 || soff[i + 1] - soff[i] > 13)
+      throw Error(MH_ERR_INTERNAL, "lattice point with more than 13 springs on one side");
+  s->vt_dpos.upload(t.pos.data(), t.pos.size(), s->stream);
+  s->vt_pairs.upload(t.pairs.data(), t.pairs.size(), s->stream);
+  s->vt_vox.upload(t.voxels.data(), t.voxels.size(), s->stream);
+  s->vt_lidx.upload(lidx.data(), lidx.size(), s->stream);
+  s->vt_dlen.upload(lengths.data(), lengths.size(), s->stream);
+  s->vt_foff.upload(foff.data(), foff.size(), s->stream);
+  s->vt_flist.upload(flist.data(), flist.size(), s->stream);
+  s->vt_soff.upload(soff.data(), soff.size(), s->stream);
+  s->vt_slist.upload(slist.data(), slist.size(), s->stream);
+  s->vt_code_id.upload(s->code_id.data(), s->code_id.size(), s->stream);
+  s->vt_code_enc.upload(s->code_enc.data(), s->code_enc.size(), s->stream);
+  s->vt_code_solid.upload(s->code_solid.data(), s->code_solid.size(), s->stream);
+  hip_check(hipStreamSynchronize(s->stream), "voxel shape upload");
+  s->vt_pos = t.pos;
+  s->vt_nx = nx;
+  s->vt_ny = ny;
+  s->vt_nz = nz;
+  s->vt_res_bits = res_bits;
+  s->vt_springs = S;
+  s->vt_lengths = static_cast<int>(lengths.size());
+  s->vt_n = n;
+}
+
+}  // namespace
+}  // namespace mh
+
+int mh_sim_set_voxel_population(mh_sim* s, int n_robots, int nx, int ny, int nz, float resolution, float mass,
+                                const uint32_t* voxel_encoding) {
+  MH_TRY
+  if (!s) throw mh::Error(MH_ERR_ARG, "null sim");
+  if (n_robots < 0 || (n_robots > 0 && !voxel_encoding)) throw mh::Error(MH_ERR_ARG, "invalid robot list");
+  if (!(resolution > 0.f)) throw mh::Error(MH_ERR_ARG, "resolution must be > 0");
+  if (!(mass > 0.f) || !std::isfinite(mass)) throw mh::Error(MH_ERR_ARG, "mass must be > 0");
+  if (nx < 1 || ny < 1 || nz < 1 || static_cast<long>(nx) * ny * nz > mh::kVoxMaxPoints)
+    throw mh::Error(MH_ERR_RANGE, "device-built voxel robots have 1..1024 lattice points");
+  mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  const int n = nx * ny * nz;
+  const mh::KernelVariant* v = mh::voxel_variant(s, n);
+  if (!v)
+    throw mh::Error(MH_ERR_ARG, "no mass-parallel step kernel holds this lattice within the config "
+                                "(kernel family, block size) and the LDS budget");
+  mh::hip_check(hipStreamSynchronize(s->stream), "synchronize");
+  mh::drop_graph(s);
+  s->has_population = false;
+  mh::prepare_voxel_shape(s, nx, ny, nz, resolution);
+  const int R = n_robots, S = s->vt_springs, P = v->nspt;
+  const size_t M = static_cast<size_t>(R) * n, ST = static_cast<size_t>(R) * S;
+
+  s->pos.alloc(M);
+  s->vel.alloc(M);
+  s->mass_id.alloc(M);
+  s->stress.alloc(std::max<size_t>(ST, 1));
+  mh::hip_check(hipMemsetAsync(s->stress.p, 0, sizeof(float) * s->stress.n, s->stream), "memset");
+  s->com.alloc(3 * std::max(R, 1));
+  s->vox_enc.alloc(M);
+  if (M)
+    mh::hip_check(hipMemcpyAsync(s->vox_enc.p, voxel_encoding, M * sizeof(uint32_t), hipMemcpyDefault, s->stream),
+                  "encodings copy");
+  s->vox_stripped.alloc(M);
+  s->vox_spring_id.alloc(ST);
+  s->mrec.alloc(static_cast<size_t>(R) * 3 * P * n);
+  s->mcnt.alloc(static_cast<size_t>(R) * 2 * n);
+  s->mperm.alloc(M);
+  s->lbtab.alloc(static_cast<size_t>(R) * 2 * mh::kMassPairMax);
+  s->packs.alloc(R);
+  s->vox_active.alloc(R);
+  s->vox_error.alloc(1);
+  mh::hip_check(hipMemsetAsync(s->vox_error.p, 0, sizeof(int), s->stream), "memset");
+  mh::VoxelBuildParams p{};
+  p.nx = nx;
+  p.ny = ny;
+  p.nz = nz;
+  p.n = n;
+  p.n_springs = S;
+  p.n_lengths = s->vt_lengths;
+  p.maxp = P;
+  p.valid_bits = s->valid_bits;
+  p.mass = mass;
+  p.enc = s->vox_enc.p;
+  p.tpos = s->vt_dpos.p;
+  p.pairs = s->vt_pairs.p;
+  p.voxels = s->vt_vox.p;
+  p.length_index = s->vt_lidx.p;
+  p.lengths = s->vt_dlen.p;
+  p.first_off = s->vt_foff.p;
+  p.first_list = s->vt_flist.p;
+  p.second_off = s->vt_soff.p;
+  p.second_list = s->vt_slist.p;
+  p.code_id = s->vt_code_id.p;
+  p.code_enc = s->vt_code_enc.p;
+  p.code_solid = s->vt_code_solid.p;
+  p.pos = s->pos.p;
+  p.vel = s->vel.p;
+  p.mass_id = s->mass_id.p;
+  p.stripped = s->vox_stripped.p;
+  p.spring_id = s->vox_spring_id.p;
+  p.mrec = s->mrec.p;
+  p.mcnt = s->mcnt.p;
+  p.mperm = s->mperm.p;
+  p.lbtab = s->lbtab.p;
+  p.packs = s->packs.p;
+  p.active = s->vox_active.p;
+  p.error = s->vox_error.p;
+  mh::hip_check(mh::launch_voxel_population(p, R, s->stream), "voxel build launch");
+  int err = 0;
+  std::vector<uint32_t> active(R);
+  std::vector<uint8_t> mid(M);
+  mh::hip_check(hipMemcpyAsync(&err, s->vox_error.p, sizeof(int), hipMemcpyDeviceToHost, s->stream), "D2H");
+  if (R)
+    mh::hip_check(hipMemcpyAsync(active.data(), s->vox_active.p, R * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 s->stream), "D2H");
+  if (M) mh::hip_check(hipMemcpyAsync(mid.data(), s->mass_id.p, M, hipMemcpyDeviceToHost, s->stream), "D2H");
+  mh::hip_check(hipStreamSynchronize(s->stream), "voxel build");
+  if (err == MH_ERR_ARG) throw mh::Error(MH_ERR_ARG, "voxel encoding uses a bit outside the material set");
+  if (err)
+    throw mh::Error(MH_ERR_RANGE, "a robot has more distinct (rest length, material) pairs than the "
+                                  "mass-parallel kernel's table holds; use mh_sim_set_population");
+
+  // host bookkeeping: offsets, baselines (SoftBody::updateBaseline at Reset)
+  s->n_robots = R;
+  s->mass_off.assign(R + 1, 0);
+  s->spring_off.assign(R + 1, 0);
+  s->active_off.assign(R + 1, 0);
+  for (int r = 0; r < R; r++) {
+    s->mass_off[r + 1] = static_cast<uint32_t>((r + 1) * n);
+    s->spring_off[r + 1] = static_cast<uint32_t>((r + 1) * S);
+    s->active_off[r + 1] = s->active_off[r] + active[r];
+  }
+  s->base_com.assign(3 * static_cast<size_t>(R), 0.f);
+  s->base_len.assign(R, 0.f);
+  mh::parallel_for(static_cast<size_t>(R), [&](size_t r) {
+    mh::baseline(s->vt_pos.data(), mid.data() + r * n, n, &s->base_com[3 * r], &s->base_len[r]);
+  });
+  s->gather_pad.clear();
+  s->max_side.clear();
+  s->pair_set.clear();
+  s->act_pairs.clear();
+  s->active_src.clear();
+  s->act_mat.clear();
+  s->act_lbar.clear();
+  s->held_stress.clear();
+  s->compact_to_id.resize(s->composite_count);
+  std::iota(s->compact_to_id.begin(), s->compact_to_id.end(), 0);
+  s->n_masses = M;
+  s->n_springs = ST;
+  s->n_active = s->active_off[R];
+  s->total_time = 0.f;
+  s->fast_div_lbar = false;
+  std::vector<uint32_t> rb(R + 1);
+  for (int r = 0; r <= R; r++) rb[r] = static_cast<uint32_t>(r * n);
+  s->robot_begin.upload(rb.data(), R + 1, s->stream);
+  mh::upload_materials(s);
+
+  // the plan: one part, one robot per workgroup
+  mh::Plan plan;
+  if (R > 0) {
+    mh::Part part;
+    part.v = v;
+    part.packs.resize(R);
+    for (int r = 0; r < R; r++) {
+      mh::PackDesc& d = part.packs[r];
+      d.mass_begin = static_cast<uint32_t>(r * n);
+      d.n_mass = static_cast<uint32_t>(n);
+      d.spring_begin = static_cast<uint32_t>(r) * S;
+      d.n_spring = active[r];
+      d.csr_begin = static_cast<uint32_t>(r) * 3 * P * n;
+      d.row_begin = static_cast<uint32_t>(r) * 2 * n;
+      d.pad0 = static_cast<uint32_t>(r) * 2 * mh::kMassPairMax;
+    }
+    part.mass_cap = n;
+    part.spring_cap = static_cast<int>(*std::max_element(active.begin(), active.end()));
+    part.lds = mh::lds_bytes_mass(v->block, s->composite_count);
+    part.n_robots = R;
+    if (part.lds > 64 * 1024)
+      for (const void* fn : {v->fn, v->fn_stress})
+        if (fn) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(part.lds));
+    plan.parts.push_back(std::move(part));
+  }
+  s->plan = std::move(plan);
+  if (s->cfg.substeps_per_launch > 0) s->substeps = s->cfg.substeps_per_launch;
+  s->voxel_device = true;
+  mh::hip_check(hipStreamSynchronize(s->stream), "upload");
+  s->has_population = true;
+  return MH_OK;
+  MH_CATCH
+}
+
+int mh_sim_voxel_materials(mh_sim* s, uint8_t* mass_material_id, uint8_t* spring_material_id,
+                           uint32_t* stripped_encoding) {
+  MH_TRY
+  if (!s) throw mh::Error(MH_ERR_ARG, "null sim");
+  if (!s->has_population || !s->voxel_device) throw mh::Error(MH_ERR_STATE, "no device-built voxel population");
+  mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  if (mass_material_id && s->n_masses)
+    mh::hip_check(hipMemcpyAsync(mass_material_id, s->mass_id.p, s->n_masses, hipMemcpyDeviceToHost, s->stream),
+                  "D2H");
+  if (spring_material_id && s->n_springs)
+    mh::hip_check(hipMemcpyAsync(spring_material_id, s->vox_spring_id.p, s->n_springs, hipMemcpyDeviceToHost,
+                                 s->stream), "D2H");
+  if (stripped_encoding && s->n_masses)
+    mh::hip_check(hipMemcpyAsync(stripped_encoding, s->vox_stripped.p, s->n_masses * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, s->stream), "D2H");
+  mh::hip_check(hipStreamSynchronize(s->stream), "D2H");
   return MH_OK;
   MH_CATCH
 }
@@ -1189,7 +1489,12 @@ int mh_sim_collect(mh_sim* s, float* pos, float* vel, float* stress) {
       vel[3 * i + 2] = tmp[i].z;
     }
   }
-  if (stress) {
+  if (stress && s->voxel_device) {
+    if (s->n_springs)
+      mh::hip_check(hipMemcpyAsync(stress, s->stress.p, s->n_springs * sizeof(float), hipMemcpyDeviceToHost,
+                                   s->stream), "D2H");
+    mh::hip_check(hipStreamSynchronize(s->stream), "D2H");
+  } else if (stress) {
     if (s->held_stress.empty())
       std::fill(stress, stress + s->n_springs, 0.f);
     else
@@ -1211,6 +1516,15 @@ int mh_sim_set_stress(mh_sim* s, const float* stress) {
   if (!s || !stress) throw mh::Error(MH_ERR_ARG, "null argument");
   if (!s->has_population) throw mh::Error(MH_ERR_STATE, "no population set");
   mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
+  if (s->voxel_device) {
+    // one slot per spring in the robots' own order; air springs are never stepped
+    if (s->n_springs) {
+      mh::hip_check(hipMemcpyAsync(s->stress.p, stress, s->n_springs * sizeof(float), hipMemcpyHostToDevice,
+                                   s->stream), "H2D");
+      mh::hip_check(hipStreamSynchronize(s->stream), "H2D");
+    }
+    return MH_OK;
+  }
   // air springs keep their value, as slots of the reference's buffer do
   // while solveDistance skips them (sim_kernel.cu:211)
   s->held_stress.assign(stress, stress + s->n_springs);
@@ -1478,6 +1792,26 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   mh::hip_check(hipSetDevice(s->cfg.device), "hipSetDevice");
   // earlier asynchronous steps finish before the state is saved
   mh::hip_check(hipStreamSynchronize(s->stream), "synchronize");
+  if (s->voxel_device) {
+    // the device-built packing is fixed: only substeps per launch are timed
+    if (s->cfg.substeps_per_launch > 0) return MH_OK;
+    mh::TuneScope scope(s);
+    double best_k = std::numeric_limits<double>::infinity();
+    int choice = 0;
+    for (int K : {0, 2500, 1000, 250}) {
+      if (K > 0 && 2 * K > n_steps) continue;
+      s->substeps = K;
+      const double t = mh::time_steps(s, n_steps);
+      if (t < best_k * (K == 0 ? 1.0 : 0.99)) {
+        best_k = t;
+        choice = K;
+      }
+    }
+    s->substeps = choice;
+    scope.substeps = choice;
+    scope.restore();
+    return MH_OK;
+  }
   const mh::TuneKey key = mh::tune_key(s);
   {
     std::lock_guard<std::mutex> lk(mh::tune_mutex());

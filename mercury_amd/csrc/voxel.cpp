@@ -83,7 +83,126 @@ Material spring_material(const mh_material_set& set, const Lattice& L,
   return set.decode(enc);
 }
 
+// The voxels whose non-air bits decide the material of the spring from
+// (sx, sy, sz) along offset o (0xffff = none): as spring_material, the
+// material is decode(OR of their non-air bits); decode(0) is air, and a
+// single voxel's own non-air bits decode to its own material.
+void spring_voxels(const Lattice& L, int sx, int sy, int sz, const int* o, uint16_t out[4]) {
+  for (int k = 0; k < 4; k++) out[k] = 0xffff;
+  const int l1 = std::abs(o[0]) + std::abs(o[1]) + std::abs(o[2]);
+  if (l1 == 1) {
+    const int ax = o[0] ? 0 : (o[1] ? 1 : 2);
+    const int u = ax == 0 ? 1 : 0, w = ax == 2 ? 1 : 2;
+    int k = 0;
+    for (int du = 0; du <= 1; du++)
+      for (int dw = 0; dw <= 1; dw++) {
+        int c[3] = {sx, sy, sz};
+        c[u] -= du;
+        c[w] -= dw;
+        if (L.valid(c[0], c[1], c[2])) out[k++] = static_cast<uint16_t>(L.index(c[0], c[1], c[2]));
+      }
+    return;
+  }
+  const int m[3] = {sx - (o[0] == -1), sy - (o[1] == -1), sz - (o[2] == -1)};
+  out[0] = static_cast<uint16_t>(L.index(m[0], m[1], m[2]));
+  if (l1 == 3) return;
+  const int nb[3] = {m[0] - (o[0] == 0), m[1] - (o[1] == 0), m[2] - (o[2] == 0)};
+  if (L.valid(nb[0], nb[1], nb[2])) out[1] = static_cast<uint16_t>(L.index(nb[0], nb[1], nb[2]));
+}
+
+// The reference's DFS from (0,0,0) over the 13 forward offsets, with an
+// explicit stack of (node, next offset) frames: emit(src, dst, sx, sy, sz,
+// offset) for every (node, offset) with a valid target, before the
+// target's visited check (BuildSpringsRecurse, VoxelRobot.cpp:60-193).
+template <typename Emit>
+void dfs_springs(const Lattice& L, Emit&& emit) {
+  const int n = L.nx * L.ny * L.nz;
+  std::vector<uint8_t> visited(n, 0);
+  std::vector<std::pair<int, int>> frames;
+  visited[0] = 1;
+  frames.emplace_back(0, 0);
+  while (!frames.empty()) {
+    auto& f = frames.back();
+    if (f.second == 13) {
+      frames.pop_back();
+      continue;
+    }
+    const int src = f.first;
+    const int* o = kOffsets[f.second++];
+    int sx, sy, sz;
+    L.coords(src, sx, sy, sz);
+    const int tx = sx + o[0], ty = sy + o[1], tz = sz + o[2];
+    if (!L.valid(tx, ty, tz)) continue;
+    const int dst = L.index(tx, ty, tz);
+    emit(src, dst, sx, sy, sz, o);
+    if (!visited[dst]) {
+      visited[dst] = 1;
+      frames.emplace_back(dst, 0);  // invalidates f; not used below
+    }
+  }
+}
+
+// Lattice positions (1/res * index), then ShiftX (max x -> 0) and ShiftY
+// (min y -> 0) over all masses, the translation added in float.
+void lattice_positions(const Lattice& L, float resolution, std::vector<float>& base, std::vector<float>& pos) {
+  const int n = L.nx * L.ny * L.nz;
+  const float inv = 1.0f / resolution;
+  base.assign(3 * static_cast<size_t>(n), 0.f);
+  for (int i = 0; i < n; i++) {
+    int x, y, z;
+    L.coords(i, x, y, z);
+    base[3 * i] = inv * static_cast<float>(x);
+    base[3 * i + 1] = inv * static_cast<float>(y);
+    base[3 * i + 2] = inv * static_cast<float>(z);
+  }
+  float max_x = base[0];
+  for (int i = 1; i < n; i++) max_x = base[3 * i] > max_x ? base[3 * i] : max_x;
+  const float tx = -max_x;
+  float min_y = base[1];
+  for (int i = 1; i < n; i++) min_y = base[3 * i + 1] < min_y ? base[3 * i + 1] : min_y;
+  const float ty = -min_y;
+  pos.assign(3 * static_cast<size_t>(n), 0.f);
+  for (int i = 0; i < n; i++) {
+    pos[3 * i] = base[3 * i] + tx;
+    pos[3 * i + 1] = base[3 * i + 1] + ty;
+    pos[3 * i + 2] = base[3 * i + 2];
+  }
+}
+
+// Spring length as Eigen's Vector3f::norm(): sqrtf(x*x + (y*y + z*z)).
+float spring_length(const std::vector<float>& base, int src, int dst) {
+  const float dx = base[3 * src] - base[3 * dst];
+  const float dy = base[3 * src + 1] - base[3 * dst + 1];
+  const float dz = base[3 * src + 2] - base[3 * dst + 2];
+  const float yz = dy * dy + dz * dz;
+  return std::sqrt(dx * dx + yz);
+}
+
 }  // namespace
+
+void voxel_template(int nx, int ny, int nz, float resolution, VoxelTemplate& t) {
+  if (nx < 1 || ny < 1 || nz < 1 || static_cast<long>(nx) * ny * nz > 65536)
+    throw Error(MH_ERR_RANGE, "voxel lattice must have 1..65536 points (u16 mass indices)");
+  const Lattice L{nx, ny, nz};
+  t.nx = nx;
+  t.ny = ny;
+  t.nz = nz;
+  t.n = nx * ny * nz;
+  std::vector<float> base;
+  lattice_positions(L, resolution, base, t.pos);
+  t.pairs.clear();
+  t.length.clear();
+  t.voxels.clear();
+  dfs_springs(L, [&](int src, int dst, int sx, int sy, int sz, const int* o) {
+    t.pairs.push_back(static_cast<uint16_t>(src));
+    t.pairs.push_back(static_cast<uint16_t>(dst));
+    t.length.push_back(spring_length(base, src, dst));
+    uint16_t v[4];
+    spring_voxels(L, sx, sy, sz, o, v);
+    t.voxels.insert(t.voxels.end(), v, v + 4);
+  });
+  t.n_springs = static_cast<int>(t.length.size());
+}
 
 void voxel_build(const mh_material_set& set, int nx, int ny, int nz, float resolution,
                  const uint32_t* voxel_enc, VoxelOut& out) {
@@ -135,72 +254,28 @@ void voxel_build(const mh_material_set& set, int nx, int ny, int nz, float resol
     if (best < 0 || comp[i] != best) vox[i] = set.by_enc[0];
 
   // ---- masses
-  const float inv = 1.0f / resolution;
-  out.pos.assign(3 * static_cast<size_t>(n), 0.f);
+  std::vector<float> base;
+  lattice_positions(L, resolution, base, out.pos);
   out.mass_id.assign(n, 0);
   out.stripped_enc.assign(n, 0);
-  std::vector<float> base(3 * static_cast<size_t>(n));
   for (int i = 0; i < n; i++) {
-    int x, y, z;
-    L.coords(i, x, y, z);
-    base[3 * i] = inv * static_cast<float>(x);
-    base[3 * i + 1] = inv * static_cast<float>(y);
-    base[3 * i + 2] = inv * static_cast<float>(z);
     out.mass_id[i] = static_cast<uint8_t>(vox[i].id);
     out.stripped_enc[i] = vox[i].encoding;
   }
 
-  // ---- springs: iterative DFS, frames (node, next offset)
+  // ---- springs in the DFS emission order
   out.pairs.clear();
   out.length.clear();
   out.spring_id.clear();
   out.spring_enc.clear();
-  std::vector<uint8_t> visited(n, 0);
-  std::vector<std::pair<int, int>> frames;
-  visited[0] = 1;
-  frames.emplace_back(0, 0);
-  while (!frames.empty()) {
-    auto& f = frames.back();
-    if (f.second == 13) {
-      frames.pop_back();
-      continue;
-    }
-    const int src = f.first;
-    const int* o = kOffsets[f.second++];
-    int sx, sy, sz;
-    L.coords(src, sx, sy, sz);
-    const int tx = sx + o[0], ty = sy + o[1], tz = sz + o[2];
-    if (!L.valid(tx, ty, tz)) continue;
-    const int dst = L.index(tx, ty, tz);
-    const float dx = base[3 * src] - base[3 * dst];
-    const float dy = base[3 * src + 1] - base[3 * dst + 1];
-    const float dz = base[3 * src + 2] - base[3 * dst + 2];
-    const float yz = dy * dy + dz * dz;
-    const float len = std::sqrt(dx * dx + yz);
+  dfs_springs(L, [&](int src, int dst, int sx, int sy, int sz, const int* o) {
     const Material m = spring_material(set, L, vox, sx, sy, sz, o);
     out.pairs.push_back(static_cast<uint16_t>(src));
     out.pairs.push_back(static_cast<uint16_t>(dst));
-    out.length.push_back(len);
+    out.length.push_back(spring_length(base, src, dst));
     out.spring_id.push_back(static_cast<uint8_t>(m.id));
     out.spring_enc.push_back(m.encoding);
-    if (!visited[dst]) {
-      visited[dst] = 1;
-      frames.emplace_back(dst, 0);  // invalidates f; not used below
-    }
-  }
-
-  // ---- ShiftX then ShiftY over all masses (translation adds in float)
-  float max_x = base[0];
-  for (int i = 1; i < n; i++) max_x = base[3 * i] > max_x ? base[3 * i] : max_x;
-  const float tx = -max_x;
-  float min_y = base[1];
-  for (int i = 1; i < n; i++) min_y = base[3 * i + 1] < min_y ? base[3 * i + 1] : min_y;
-  const float ty = -min_y;
-  for (int i = 0; i < n; i++) {
-    out.pos[3 * i] = base[3 * i] + tx;
-    out.pos[3 * i + 1] = base[3 * i + 1] + ty;
-    out.pos[3 * i + 2] = base[3 * i + 2];
-  }
+  });
 }
 
 void baseline(const float* pos, const uint8_t* mass_id, int n, float com[3], float* length) {

@@ -131,12 +131,17 @@ def genome_features(g, voxels, rows=WORKLOAD_ROWS):
 
 
 class BatchSimulator:
-    """Builds and simulates genome batches; one evaluation per robot."""
+    """Builds and simulates genome batches; one evaluation per robot.
+    device_build: the robots are built on the device from their voxel
+    encodings (Simulator.set_voxel_population; only the encodings cross
+    PCIe), else on host threads and uploaded (voxel_build_batch +
+    set_population); both give the same fitness."""
 
     def __init__(self, voxels=8, duration=1.0, dt=1e-4, spacing=0.1, mass=0.001, device=0,
-                 build_threads=0):
+                 build_threads=0, device_build=True):
         self.voxels, self.duration = voxels, duration
         self.build_threads = build_threads
+        self.device_build = device_build
         self.ms = workload_materials()
         self.res = 1.0 / spacing
         self.mass = mass
@@ -148,10 +153,14 @@ class BatchSimulator:
 
     def fitness(self, genomes):
         enc = genome_to_lattice(genomes, self.voxels)
-        pop = voxel_build_batch(self.ms, self.voxels + 1, self.res, enc, mass=self.mass,
-                                threads=self.build_threads)
         t0 = time.perf_counter()
-        self.sim.set_population(pop)
+        if self.device_build:
+            self.sim.set_voxel_population(enc, self.voxels + 1, self.res, mass=self.mass)
+        else:
+            pop = voxel_build_batch(self.ms, self.voxels + 1, self.res, enc, mass=self.mass,
+                                    threads=self.build_threads)
+            t0 = time.perf_counter()
+            self.sim.set_population(pop)
         if not self.tuned:
             self.sim.autotune(100)
             self.tuned = True

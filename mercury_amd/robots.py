@@ -72,6 +72,24 @@ def voxel_build(material_set, n, resolution, voxel_encoding, mass=1.0):
     return e
 
 
+def voxel_template(n, resolution):
+    """What every n^3 voxel robot shares (mh_voxel_template): mass positions
+    (n^3, 3), springs (S, 2) in DFS order, their lengths, and per spring the
+    4 lattice points (0xffff = none) whose voxels decide its material."""
+    L = load()
+    nv = n ** 3
+    maxs = L.mh_voxel_max_springs(n, n, n)
+    pos = np.zeros((nv, 3), np.float32)
+    pairs = np.zeros((maxs, 2), np.uint16)
+    ln = np.zeros(maxs, np.float32)
+    vox = np.zeros((maxs, 4), np.uint16)
+    ns = ctypes.c_int()
+    check(L.mh_voxel_template(n, n, n, float(resolution), ptr(pos, f32p), ptr(pairs, u16p), ptr(ln, f32p),
+                              ptr(vox, u16p), maxs, ctypes.byref(ns)))
+    s = ns.value
+    return pos, pairs[:s].copy(), ln[:s].copy(), vox[:s].copy()
+
+
 def voxel_build_batch(material_set, n, resolution, voxel_encodings, mass=1.0, threads=0):
     """VoxelRobot::BatchBuild: many n^3 lattices built by native threads into
     one concatenated population (dict of arrays, as Simulator.set_population

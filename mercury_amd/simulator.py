@@ -133,6 +133,27 @@ class Simulator:
         self._n_robots = n
         self._elements = []
 
+    def set_voxel_population(self, voxel_encodings, n, resolution, mass=1.0):
+        """Voxel robots of one n^3 lattice built on the device from their
+        encodings (n_robots x n^3; mh_sim_set_voxel_population): the same
+        results as voxel_build_batch followed by set_population."""
+        enc = np.ascontiguousarray(voxel_encodings, np.uint32).reshape(-1, n ** 3)
+        check(load().mh_sim_set_voxel_population(self._h, len(enc), n, n, n, float(resolution), float(mass),
+                                                 ptr(enc, u32p)))
+        self._arrays = None
+        self._n_robots = len(enc)
+        self._elements = []
+
+    def voxel_materials(self):
+        """(mass ids, spring ids, stripped encodings) of a device-built voxel
+        population, robot-major."""
+        i = self.info()
+        mid = np.zeros(i.n_masses, np.uint8)
+        sid = np.zeros(i.n_springs, np.uint8)
+        st = np.zeros(i.n_masses, np.uint32)
+        check(load().mh_sim_voxel_materials(self._h, ptr(mid, u8p), ptr(sid, u8p), ptr(st, u32p)))
+        return mid, sid, st
+
     # ------------------------------------------------------------- stepping
     def Simulate(self, sim_duration, trackStresses=False):
         """Simulator::Simulate (Simulator.cu:369-458): float time loop."""

@@ -126,3 +126,36 @@ def test_voxel_build_batch_matches_single_builds():
         got = mh.robots.voxel_build_batch(ms, 5, 10.0, encs, mass=0.002, threads=threads)
         for k in ref:
             assert np.array_equal(ref[k], got[k]), (threads, k)
+
+
+@pytest.mark.parametrize("reference_set", [False, True])
+def test_voxel_template_gives_every_build(reference_set):
+    """The shape template (positions, DFS springs, lengths) is what
+    mh_voxel_build gives any robot of the lattice, and decoding the OR of
+    the template's voxels' non-air bits reproduces every spring material
+    after Strip, for base and two-material voxel encodings."""
+    from mercury_amd.robots import voxel_template
+    ms = mh.MaterialSet.reference() if reference_set else mh.workload_materials()
+    nb = 22 if reference_set else 4
+    rng = np.random.default_rng(8)
+    for n in (2, 4, 6):
+        pos, pairs, ln, vox = voxel_template(n, 10.0)
+        for trial in range(6):
+            mats = rng.integers(1, nb + 1, (n ** 3, 2))
+            enc = (np.uint32(1) << mats[:, 0].astype(np.uint32))
+            if trial % 2:
+                enc |= np.uint32(1) << mats[:, 1].astype(np.uint32)
+            enc[rng.random(n ** 3) < (0.2 if trial < 4 else 0.6)] = 1
+            enc = mh.robots.last_layer_air(n, enc)
+            el = mh.voxel_build(ms, n, 10.0, enc)
+            assert np.array_equal(el.pos.view(np.uint32), pos.view(np.uint32))
+            assert np.array_equal(el.pairs, pairs) and np.array_equal(el.mean_length.view(np.uint32), ln.view(np.uint32))
+            # non-air bits of each voxel's decoded material after Strip
+            nab = np.array([0 if e == 1 else int(ms.by_id(int(ms.decode(int(e))[0]))[1]) for e in el.stripped_encoding],
+                           np.uint32)
+            ored = np.zeros(len(pairs), np.uint32)
+            for k in range(4):
+                ok = vox[:, k] != 0xffff
+                ored[ok] |= nab[vox[ok, k]]
+            want = np.array([ms.decode(int(e))[0] for e in ored], np.uint8)
+            assert np.array_equal(want, el.spring_material)
