@@ -34,7 +34,7 @@ from . import _lib
 from .materials import WORKLOAD_ROWS, workload_materials
 from .robots import last_layer_air, voxel_build, voxel_build_batch
 from .simulator import SimConfig, Simulator
-from .surrogate import GenomeSurrogate, Surrogate, spearman
+from .surrogate import GenomeSurrogate, Surrogate, layered_features, spearman
 
 LAND = dict(gravity=9.81, ground_k=1.0e5, mu_s=1.0, mu_k=0.8, drag=0.0, damping=0.99)
 
@@ -75,59 +75,9 @@ def genome_to_lattice(g, voxels):
 
 
 def genome_features(g, voxels, rows=WORKLOAD_ROWS):
-    """64 features per genome, vectorized over the population: material
-    fractions, per-material centroids and spreads, solid-voxel moments and
-    bounding box, actuation-phase moments, ground contact, front/back
-    balance and the offset between the two actuation phases."""
-    nb = len(rows)
-    N = len(g)
-    v = voxels
-    idx = np.arange(v ** 3)
-    c = np.stack([idx % v, (idx // v) % v, idx // (v * v)], 1).astype(np.float32) / max(v - 1, 1)
-    feats = []
-    onehot = [(g == m).astype(np.float32) for m in range(nb + 1)]
-    cnt = [o.sum(1) for o in onehot]
-    feats += [x / g.shape[1] for x in cnt]                                  # nb+1
-    for m in range(1, nb + 1):
-        w = onehot[m]
-        s = np.maximum(cnt[m], 1)[:, None]
-        mean = (w @ c) / s
-        var = (w @ (c * c)) / s - mean * mean
-        feats += list(mean.T) + list(np.sqrt(np.maximum(var, 0)).T)           # 6 nb
-    solid = (g > 0).astype(np.float32)
-    ss = np.maximum(solid.sum(1), 1)[:, None]
-    mean = (solid @ c) / ss
-    var = (solid @ (c * c)) / ss - mean * mean
-    big = np.where(solid[:, :, None] > 0, c[None], np.inf)
-    lo = np.where(np.isfinite(big.min(1)), big.min(1), 0)
-    hi = np.where(solid[:, :, None] > 0, c[None], -np.inf).max(1)
-    hi = np.where(np.isfinite(hi), hi, 0)
-    feats += list(mean.T) + list(np.sqrt(np.maximum(var, 0)).T) + list(lo.T) + list(hi.T)  # 12
-    amp = np.array([0.0] + [r[1] for r in rows], np.float32)
-    ph = np.array([0.0] + [r[3] for r in rows], np.float32)
-    a = amp[g]
-    wc, ws = a * np.cos(ph[g]), a * np.sin(ph[g])
-    n_s = ss[:, 0]
-    for k in range(3):
-        feats += [(wc @ c[:, k]) / n_s, (ws @ c[:, k]) / n_s]
-    feats += [wc.sum(1) / n_s, ws.sum(1) / n_s]                               # 8
-    bottom = (c[:, 1] == 0).astype(np.float32)
-    nbot = max(bottom.sum(), 1)
-    for m in range(1, nb + 1):
-        feats.append((onehot[m] @ bottom) / nbot)
-    feats.append((solid @ bottom) / nbot)                                   # nb+1
-    front = (c[:, 0] >= 0.5).astype(np.float32)
-    for m in range(1, nb + 1):
-        feats.append((onehot[m] @ front) / np.maximum(cnt[m], 1))           # nb
-    act = [m for m in range(1, nb + 1) if amp[m] > 0][:2]
-    if len(act) == 2:
-        m0 = (onehot[act[0]] @ c) / np.maximum(cnt[act[0]], 1)[:, None]
-        m1 = (onehot[act[1]] @ c) / np.maximum(cnt[act[1]], 1)[:, None]
-        feats += list((m0 - m1).T)                                          # 3
-    out = np.stack([np.asarray(f, np.float32) for f in feats], 1)
-    if out.shape[1] < 64:
-        out = np.concatenate([out, np.zeros((N, 64 - out.shape[1]), np.float32)], 1)
-    return out[:, :64]
+    """The surrogate's 64 inputs of genomes g[robot, voxel]
+    (surrogate.layered_features of the genome's own materials)."""
+    return layered_features(g, voxels, len(rows))
 
 
 class BatchSimulator:

@@ -1,8 +1,9 @@
 """Surrogate fitness model for voxel genomes.
 
-Features (64 per robot) are computed from the robot as built (after Strip):
-material fractions, per-material spatial moments, actuation-phase moments,
-ground contact, front/back balance and spring statistics.  The network is
+Features (64 per robot): per height layer and base material, the share of
+the layer's voxels holding the material and their front-to-back balance
+(layered_features), computed from the robot as built (after Strip) or from
+its genome.  The network is
 libmercury_hip's MLP (mh_mlp_*): bf16 weights, fp32 accumulation, trained
 on the GPU by hand-written forward/backward/Adam kernels.
 """
@@ -16,64 +17,55 @@ from ._lib import check, f32p, i32p, load, ptr, u8p
 N_FEATURES = 64
 
 
+def layer_bins(voxels_per_axis, n_base):
+    """Height bins of the layered features: one per voxel layer while
+    2 * n_base * bins fits N_FEATURES, else layers grouped evenly."""
+    return max(1, min(voxels_per_axis, N_FEATURES // (2 * max(n_base, 1))))
+
+
+def layered_features(mat, voxels_per_axis, n_base):
+    """The surrogate's 64 inputs from voxel materials mat[robot, voxel]
+    (0 = empty, 1..n_base; voxel (x, y, z) at x + v (y + v z), y up).  Per
+    height bin and base material: the share of the bin's voxels holding it
+    and their front-to-back balance, the mean of x scaled to [-1, 1] over
+    the bin's voxels (empty ones count 0).  Which materials touch the ground
+    and how each layer's actuators sit front to back decide how far and in
+    which direction a robot travels along x; the balance is odd under the
+    x mirror, as the displacement is.  Zero-padded to N_FEATURES."""
+    v = voxels_per_axis
+    mat = np.asarray(mat).reshape(len(mat), v, v, v)  # [robot, z, y, x]
+    bins = layer_bins(v, n_base)
+    ybin = (np.arange(v) * bins) // v
+    half = (v - 1) / 2.0
+    xc = (np.arange(v, dtype=np.float64) - half) / (half if v > 1 else 1.0)
+    per_bin = np.bincount(ybin, minlength=bins).astype(np.float64) * v * v
+    out = np.zeros((len(mat), N_FEATURES), np.float32)
+    f = 0
+    for b in range(bins):
+        sl = mat[:, :, ybin == b, :]
+        for m in range(1, n_base + 1):
+            hit = (sl == m)
+            out[:, f] = hit.sum(axis=(1, 2, 3)) / per_bin[b]
+            out[:, f + 1] = (hit * xc).sum(axis=(1, 2, 3)) / per_bin[b]
+            f += 2
+    return out
+
+
 def genome_features(elements, voxels_per_axis, rows):
-    """64 features per voxel robot.  rows: the material set's base rows
-    (k, dL0, omega, phi); actuated bases are those with dL0 > 0."""
+    """layered_features of built voxel robots (their voxels after Strip;
+    every voxel holds one base material).  rows: the material set's base
+    rows (k, dL0, omega, phi)."""
     nv = voxels_per_axis
     n = nv + 1
-    nb = len(rows)
-    amp = np.array([r[1] for r in rows], np.float64)
-    phase = np.array([r[3] for r in rows], np.float64)
     idx = np.arange(n ** 3)
     gx, gy, gz = idx % n, (idx // n) % n, idx // (n * n)
     inside = (gx < nv) & (gy < nv) & (gz < nv)
-    cx, cy, cz = (gx[inside] / max(nv - 1, 1), gy[inside] / max(nv - 1, 1), gz[inside] / max(nv - 1, 1))
-    coords = np.stack([cx, cy, cz], 1)
-    out = np.zeros((len(elements), N_FEATURES), np.float32)
+    mat = np.zeros((len(elements), nv ** 3), np.int64)
     for r, e in enumerate(elements):
         enc = e.stripped_encoding[inside]
-        mat = np.zeros(len(enc), np.int64)
         nz = enc > 1
-        mat[nz] = np.log2(enc[nz]).astype(np.int64)  # base index 1..nb (single-bit voxels)
-        f = []
-        counts = np.bincount(mat, minlength=nb + 1).astype(np.float64)
-        f += list(counts / len(mat))                                   # 1 + nb
-        solid = mat > 0
-        for m in range(1, nb + 1):
-            sel = mat == m
-            c = coords[sel]
-            f += list(c.mean(0)) if len(c) else [0.0] * 3               # 3 nb
-            f += list(c.std(0)) if len(c) else [0.0] * 3                # 3 nb
-        c = coords[solid]
-        if len(c):
-            f += list(c.mean(0)) + list(c.std(0)) + list(c.min(0)) + list(c.max(0))
-        else:
-            f += [0.0] * 12                                             # 12
-        a = amp[mat - 1] * solid if nb else np.zeros(len(mat))
-        ph = phase[np.maximum(mat - 1, 0)]
-        wc, ws = a * np.cos(ph), a * np.sin(ph)
-        cnt = max(solid.sum(), 1)
-        for k in range(3):
-            f += [float((wc * coords[:, k]).sum() / cnt), float((ws * coords[:, k]).sum() / cnt)]
-        f += [float(wc.sum() / cnt), float(ws.sum() / cnt)]               # 8
-        bottom = coords[:, 1] == 0
-        for m in range(1, nb + 1):
-            f.append(float(((mat == m) & bottom).sum()) / max(bottom.sum(), 1))
-        f.append(float((solid & bottom).sum()) / max(bottom.sum(), 1))    # nb + 1
-        front = coords[:, 0] >= 0.5
-        for m in range(1, nb + 1):
-            sel = mat == m
-            f.append(float((sel & front).sum()) / max(sel.sum(), 1))      # nb
-        active = e.spring_material != 0
-        ln = e.mean_length[active]
-        f += [float(active.mean()) if len(active) else 0.0,
-              float(ln.mean()) if len(ln) else 0.0, float(ln.std()) if len(ln) else 0.0]  # 3
-        act = [m for m in range(1, nb + 1) if amp[m - 1] > 0]
-        cm = [coords[mat == m].mean(0) if (mat == m).any() else np.zeros(3) for m in act[:2]]
-        f += list(cm[0] - cm[1]) if len(cm) == 2 else [0.0] * 3           # 3
-        f = f[:N_FEATURES] + [0.0] * max(0, N_FEATURES - len(f))
-        out[r] = np.asarray(f, np.float32)
-    return out
+        mat[r, nz] = np.log2(enc[nz]).astype(np.int64)  # base index 1..nb (single-bit voxels)
+    return layered_features(mat, nv, len(rows))
 
 
 def spearman(a, b):
@@ -176,15 +168,38 @@ class Surrogate:
         self.net = MLP([N_FEATURES, *hidden, 1], seed=seed, device=device)
         self.mu = None
 
-    def fit(self, X, y, epochs=30, batch=512, lr=1e-3, seed=0):
+    def fit(self, X, y, epochs=30, batch=512, lr=1e-3, seed=0, holdout=0.1, patience=3):
+        """Adam on standardized inputs and targets.  With holdout > 0 a
+        seeded share of the rows is kept out of training; after every epoch
+        the rank correlation on it is measured, and the weights of the best
+        epoch are kept (training stops after `patience` epochs without a
+        better one).  Returns the last epoch's training loss."""
         X = np.asarray(X, np.float32)
         y = np.asarray(y, np.float32)
         self.mu = X.mean(0)
         self.sd = X.std(0) + 1e-6
         self.ymu = float(y.mean())
         self.ysd = float(y.std() + 1e-12)
-        return self.net.train((X - self.mu) / self.sd, (y - self.ymu) / self.ysd, epochs=epochs,
-                              batch=batch, lr=lr, seed=seed)
+        Xs, ys = (X - self.mu) / self.sd, (y - self.ymu) / self.ysd
+        n_val = int(len(X) * holdout) if holdout > 0 and len(X) >= 200 else 0
+        if n_val == 0:
+            return self.net.train(Xs, ys, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        perm = np.random.default_rng(seed).permutation(len(X))
+        val, tr = perm[:n_val], perm[n_val:]
+        Xt, yt = np.ascontiguousarray(Xs[tr]), np.ascontiguousarray(ys[tr])
+        best, best_p, since, loss = -np.inf, None, 0, 0.0
+        for ep in range(epochs):
+            loss = self.net.train(Xt, yt, epochs=1, batch=batch, lr=lr, seed=seed + ep)
+            rho = spearman(self.net.forward(Xs[val])[:, 0], ys[val])
+            if rho > best:
+                best, best_p, since = rho, self.net.get_params(), 0
+            else:
+                since += 1
+                if since >= patience:
+                    break
+        self.net.set_params(best_p)
+        self.val_rho = best
+        return loss
 
     def predict(self, X):
         X = np.asarray(X, np.float32)

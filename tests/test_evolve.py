@@ -56,8 +56,7 @@ def test_vectorized_features_agree_with_element_features():
     el = mh.robots.voxel_build(ms, 4, 10.0, enc[0], mass=0.001)
     fa = evolve.genome_features(g, 3)[0]
     fb = surrogate.genome_features([el], 3, mh.materials.WORKLOAD_ROWS)[0]
-    # material fractions, per-material centroids/spreads, solid moments, bbox
-    np.testing.assert_allclose(fa[:5 + 24 + 12], fb[:5 + 24 + 12], atol=1e-5)
+    np.testing.assert_array_equal(fa, fb)
 
 
 class _FakeSim:
@@ -99,7 +98,10 @@ class _FakeSurrogate:
         return 0.0
 
     def predict(self, X):
-        return self.sign * (X[:, 3] - X[:, 0])  # share of material 3 minus empty share
+        # share of material 3 minus the empty share, from the per-layer
+        # shares (4 layers x 4 materials x (share, balance) at 4^3 voxels)
+        share = X[:, :32:2].reshape(len(X), 4, 4).mean(1)
+        return self.sign * (share[:, 2] - (1.0 - share.sum(1)))
 
 
 def test_surrogate_screening_and_rho_fallback(monkeypatch):

@@ -25,8 +25,29 @@ def test_genome_features_shape_and_determinism():
     f1 = surrogate.genome_features(els, 4, mh.materials.WORKLOAD_ROWS)
     f2 = surrogate.genome_features(els, 4, mh.materials.WORKLOAD_ROWS)
     assert f1.shape == (5, 64) and np.array_equal(f1, f2) and np.isfinite(f1).all()
-    assert np.allclose(f1[:, :5].sum(1), 1.0)  # material fractions
+    # 4 layers x 4 materials x (share, front-to-back balance), zero-padded
+    assert surrogate.layer_bins(4, 4) == 4 and not f1[:, 32:].any()
+    share = f1[:, :32:2].reshape(5, 4, 4)
+    assert (share.sum(2) <= 1.0 + 1e-6).all() and (np.abs(f1[:, 1:32:2]) <= share.reshape(5, 16) + 1e-6).all()
     assert len(np.unique(f1, axis=0)) == 5
+
+
+def test_layered_features_known_answer_and_mirror():
+    """One 2^3 genome by hand: material 3 in voxel (x=1, y=0, z=0) and
+    material 1 in (0, 1, 1); mirroring along x negates every balance and
+    keeps every share."""
+    g = np.zeros((1, 8), np.uint8)
+    g[0, 1] = 3                    # x=1, y=0, z=0
+    g[0, 0 + 2 * (1 + 2 * 1)] = 1  # x=0, y=1, z=1
+    f = surrogate.layered_features(g, 2, 4)[0]
+    want = np.zeros(64, np.float32)
+    # bin 0 (y = 0): material 3 share 1/4, balance +1/4; bin 1 (y = 1): material 1 share 1/4, balance -1/4
+    want[2 * 2], want[2 * 2 + 1] = 0.25, 0.25
+    want[8 + 0], want[8 + 1] = 0.25, -0.25
+    assert np.array_equal(f, want)
+    m = g.reshape(1, 2, 2, 2)[:, :, :, ::-1].reshape(1, 8)
+    fm = surrogate.layered_features(m, 2, 4)[0]
+    assert np.array_equal(fm[0::2], f[0::2]) and np.array_equal(fm[1::2], -f[1::2])
 
 
 def bf16_round(w):
