@@ -53,7 +53,8 @@ LDS_NOMINAL_GBS = 256 * 128 * 2.4
 # VALU issue peak for plain (not packed) FP32 instructions: one wave64
 # instruction per CU per cycle, 256 CUs at 2.4 GHz -- half the 157.3
 # TFLOP/s FP32 vector peak, which counts packed FMA (2 lanes x 2 flops)
-VALU_PEAK_WAVE_INSTR = 256 * 2.4e9
+MI355X_CUS = 256
+VALU_PEAK_WAVE_INSTR = MI355X_CUS * 2.4e9
 
 
 # --kernel choices: MH_KERNEL_* family bits (robots over the LDS budget may
@@ -162,6 +163,28 @@ def measured_traffic(workload, kernel):
             "source": "profiles/traffic.json: " + entry["summary"]}
 
 
+def roofline_head(bound, achieved, peak, unit, extra):
+    """bound / achieved / peak / unit / frac of the dominant kernel.  With
+    hardware counters for it, the bound is VALU issue when the kernel
+    issues at least 0.9 wave64 VALU instructions per CU-cycle (independent
+    FMA chains sustain 0.92 on MI355X): then the kernel is at the issue
+    limit, and the byte model's fraction is reported beside it; frac_survey
+    is the fraction on the algorithmic LDS model of 57 B per spring plus
+    48 B per mass per substep."""
+    v = extra.get("valu_issue")
+    if v and v["per_cu_cycle"] >= 0.9:
+        out = {"bound": "valu", "achieved": v["achieved"], "peak": v["peak"], "unit": v["unit"],
+               "frac": v["frac"], "byte_model_bound": bound, "byte_model_achieved": achieved,
+               "byte_model_peak": peak, "byte_model_unit": unit,
+               "byte_model_frac": (achieved / peak) if achieved else None}
+    else:
+        out = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+               "frac": (achieved / peak) if achieved else None}
+    if "survey_model" in extra:
+        out["frac_survey"] = extra["survey_model"]["frac"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +207,8 @@ def main():
     ap.add_argument("--no-xpbd", action="store_true", help="skip the XPBD side measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the known-size copy kernel (counter calibration)")
+    ap.add_argument("--calibrate-valu", action="store_true",
+                    help="only run the VALU issue-rate kernels (FMA, MUL and reciprocal chains)")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -192,6 +217,16 @@ def main():
         mh._lib.check(mh.load().mh_calibrate_copy(0, args.calibrate_copy, 5, ctypes.byref(ms)))
         print(json.dumps({"copy_bytes": args.calibrate_copy, "avg_ms": ms.value,
                           "GB/s": 2 * args.calibrate_copy / (ms.value * 1e-3) / 1e9}))
+        return
+
+    if args.calibrate_valu:
+        import ctypes
+        rates = {}
+        for kind, name in enumerate(("fma", "mul", "fma_16_chains", "rcp", "rcp_fma")):
+            g = ctypes.c_float()
+            mh._lib.check(mh.load().mh_calibrate_valu(0, kind, ctypes.byref(g)))
+            rates[name] = g.value * 1e9
+        print(json.dumps({"valu_wave_instr_per_s": rates}))
         return
 
     if args.gpus != 1:
@@ -295,9 +330,9 @@ def main():
     counters = measured_counters(args.workload, kernel_name)
     extra = {}
     if bound == "lds" and kernel_ms > 0:
-        # the byte model of the hot-path scope table: 57 B per spring
-        # (two endpoint positions, the 9 B record, one term write and one
-        # gather read) + 48 B per mass, whatever the kernel reads
+        # the algorithmic LDS model: 57 B per spring (two endpoint
+        # positions, the 9 B record, one term write and one gather read)
+        # + 48 B per mass, whatever the kernel reads
         survey = (LDS_BYTES_PER_SPRING * n_springs + LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
         extra["survey_model"] = {
             "bytes_per_spring_update": LDS_BYTES_PER_SPRING + LDS_BYTES_PER_MASS * n_masses / n_springs,
@@ -305,18 +340,24 @@ def main():
             "frac": survey / (kernel_ms * 1e-3) / 1e9 / lds_peak,
             "frac_nominal_peak": survey / (kernel_ms * 1e-3) / 1e9 / LDS_NOMINAL_GBS}
     if counters and kernel_ms > 0:
-        # launches in the profile ran the same substeps per launch as here
+        # counters of a profiled launch (profiles/counters.json), scaled to
+        # this launch's substeps; gpu_cycles_per_launch is GRBM_GUI_ACTIVE
+        # per XCD, so instructions per CU-cycle do not depend on the clock
         scale = steps_per_launch / counters["substeps_per_launch"]
-        valu = counters["valu_per_launch"] * scale / (kernel_ms * 1e-3)
+        valu_launch = counters["valu_per_launch"] * scale
+        valu = valu_launch / (kernel_ms * 1e-3)
+        per_cu_cycle = counters["valu_per_launch"] / (MI355X_CUS * counters["gpu_cycles_per_launch"])
         extra["valu_issue"] = {
-            "wave_instr_per_launch": counters["valu_per_launch"] * scale,
+            "wave_instr_per_launch": valu_launch,
             "achieved": valu, "peak": VALU_PEAK_WAVE_INSTR, "unit": "wave64 instr/s",
             "frac": valu / VALU_PEAK_WAVE_INSTR,
-            "lane_instr_per_active_spring_update": counters["valu_per_launch"] * scale * 64
-            / (n_active * steps_per_launch)}
+            "per_cu_cycle": per_cu_cycle,
+            "fma_chains_per_cu_cycle": counters.get("fma_chains_per_cu_cycle"),
+            "transcendental_share": counters.get("trans_per_launch", 0) / counters["valu_per_launch"],
+            "lane_instr_per_active_spring_update": valu_launch * 64 / (n_active * steps_per_launch)}
         extra["mfma_per_launch"] = counters.get("mfma_per_launch")
+        extra["mfma_busy_cycles_per_launch"] = counters.get("mfma_busy_cycles_per_launch")
         extra["counters_source"] = "profiles/counters.json: " + counters["summary"]
-
 
     out = {
         "metric": metric,
@@ -344,11 +385,7 @@ def main():
         },
         "active_spring_updates_per_s": n_active * n_sub * args.steps / secs,
         "roofline": {
-            "bound": bound,
-            "achieved": achieved,
-            "peak": peak,
-            "unit": unit,
-            "frac": (achieved / peak) if achieved else None,
+            **roofline_head(bound, achieved, peak, unit, extra),
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             **({"traffic_source": traffic["source"]} if traffic else {}),
             "peak_kind": ("measured ds_read_b128 bandwidth (mh_calibrate_lds); nominal "

@@ -128,9 +128,9 @@ size_t lds_bytes_mass(int block, int n_mat);
 // Chunked streaming kernel: per lane and chunk at most kChunkSpl springs,
 // kChunkEpl quads of gather entries and kChunkRpl quads of row bounds are in
 // flight; a robot has at most kChunkMax chunks.  Row bounds per chunk:
-// 2 n_mass + 1 entries padded to whole quads.
+// n_mass + 1 entries padded to whole quads.
 constexpr int kChunkSpl = 8, kChunkEpl = 8, kChunkRpl = 2, kChunkMax = 256;
-__host__ __device__ constexpr int chunk_row_quads(int n_mass) { return (2 * n_mass + 1 + 3) / 4; }
+__host__ __device__ constexpr int chunk_row_quads(int n_mass) { return (n_mass + 1 + 3) / 4; }
 size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat);
 size_t lds_bytes_massg(int block, int maxp);
 
@@ -173,7 +173,7 @@ struct VoxelBuildParams {
   uint8_t* mass_id;
   uint32_t* stripped;            // voxel encodings after Strip
   uint8_t* spring_id;
-  uint32_t* mrec;                // 3 maxp n per robot
+  uint32_t* mrec;                // (4 maxp + 1) n per robot
   uint8_t* mcnt;                 // 2n per robot
   uint16_t* mperm;               // n per robot
   float* lbtab;                  // 2 kMassPairMax per robot

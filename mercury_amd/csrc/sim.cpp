@@ -124,7 +124,7 @@ struct mh_sim {
   // population (host copies used for packing and collect)
   int n_robots = 0;
   std::vector<uint32_t> mass_off, spring_off, active_off;  // per robot prefix offsets
-  std::vector<uint32_t> gather_pad;  // per robot: sum over masses of deg rounded up to 4
+  std::vector<uint32_t> gather_pad;  // per robot: sum over masses of the degree rounded up to 4
   std::vector<uint8_t> max_side;     // per robot: most springs one mass has on one side (capped 255)
   // per robot: distinct (rest length, material) pairs of its springs as
   // rest-length bits << 8 | compact material, sorted; empty when there are
@@ -248,7 +248,7 @@ int robot_active(const mh_sim* s, int r) { return static_cast<int>(s->active_off
 
 bool fits_lds_alone(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
   const int m = robot_masses(s, r), sp = robot_active(s, r), g = static_cast<int>(s->gather_pad[r]);
-  return m <= v.block * v.mpt && sp <= v.block * v.nspt && 3 * (sp + 1) <= 65535 && g <= 65535 &&
+  return m <= v.block * v.mpt && sp <= v.block * v.nspt && 6 * (sp + 1) + 1 <= 65535 && g <= 65535 &&
          lds_bytes_lds(std::max(m, 1), std::max(sp, 1), g, n_mat) <= static_cast<size_t>(s->lds_limit);
 }
 
@@ -274,8 +274,8 @@ bool fits_resident(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
 }
 
 // Gather-entry capacity of a chunk of C springs for a robot of m masses: 2C
-// entries plus padding of up to 3 per row part, in whole quads.
-int chunk_csr_cap(int C, int m) { return (2 * C + 6 * m + 3) & ~3; }
+// entries plus padding of up to 3 per row, in whole quads.
+int chunk_csr_cap(int C, int m) { return (2 * C + 3 * m + 3) & ~3; }
 
 // Springs per chunk for the chunked kernel and a robot of m masses: the
 // largest candidate whose LDS layout (chunk terms, skewed positions, one
@@ -361,8 +361,9 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
       const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
       const int sp = static_cast<int>(d.n_spring) + robot_active(s, r);
       const int g = gathers + static_cast<int>(s->gather_pad[r]);
-      // gather entries hold 3 * slot in 16 bits: slots up to spring_cap + 1
-      const bool fits = m <= mcap && sp <= scap && 3 * (sp + 1) <= 65535 && g <= 65535 &&
+      // gather entries hold 2 * (3 * slot) + 1 in 16 bits: slots up to
+      // spring_cap + 1
+      const bool fits = m <= mcap && sp <= scap && 6 * (sp + 1) + 1 <= 65535 && g <= 65535 &&
                         lds_bytes_lds(std::max(lp.mass_cap, m), std::max(lp.spring_cap, sp),
                                       std::max(lp.csr_cap, g), n_mat) <= static_cast<size_t>(s->lds_limit);
       if (!fits) break;
@@ -408,9 +409,9 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
 void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::vector<uint16_t>& ccsr,
                     std::vector<uint16_t>& crow, std::vector<uint32_t>& ctab) {
   const uint32_t C = static_cast<uint32_t>(part.spring_cap);
-  const uint16_t zero_entry = static_cast<uint16_t>(3u * (C + 1));
+  const uint16_t zero_entry = static_cast<uint16_t>(6u * (C + 1));
   auto pad = [](uint32_t n) { return (n + 3u) & ~3u; };
-  std::vector<uint32_t> lpair, count, fill_plus, fill_minus;
+  std::vector<uint32_t> lpair, count, fill_plus;
   for (PackDesc& d : part.packs) {
     d.csr_begin = static_cast<uint32_t>(ccsr.size());
     d.row_begin = static_cast<uint32_t>(crow.size());
@@ -430,32 +431,28 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
     for (uint32_t c = 0; c < n_chunks; c++) {
       ctab.push_back(static_cast<uint32_t>(ccsr.size()) - d.csr_begin);
       const uint32_t s0 = c * C, s1 = std::min(ns, s0 + C);
-      count.assign(2 * nm + 1, 0);
+      // one row per mass: the chunk's springs of the mass in ascending
+      // spring index, entry 2 * (3 * (spring - s0)), + 1 for the second
+      // endpoint, padded to a quad with the zero slot
+      count.assign(nm + 1, 0);
       for (uint32_t k = s0; k < s1; k++) {
-        count[2 * lpair[2 * k] + 1]++;      // first-endpoint part size
-        count[2 * lpair[2 * k + 1] + 2]++;  // second-endpoint part size
+        count[lpair[2 * k] + 1]++;
+        count[lpair[2 * k + 1] + 1]++;
       }
-      for (uint32_t i = 0; i < nm; i++) {
-        count[2 * i + 1] = count[2 * i] + pad(count[2 * i + 1]);
-        count[2 * i + 2] = count[2 * i + 1] + pad(count[2 * i + 2]);
-      }
-      if (count[2 * nm] > static_cast<uint32_t>(part.csr_cap))
+      for (uint32_t i = 0; i < nm; i++) count[i + 1] = count[i] + pad(count[i + 1]);
+      if (count[nm] > static_cast<uint32_t>(part.csr_cap))
         throw Error(MH_ERR_INTERNAL, "chunk gather list over its LDS capacity");
       const size_t base = ccsr.size();
-      ccsr.resize(base + count[2 * nm], zero_entry);
-      fill_plus.resize(nm);
-      fill_minus.resize(nm);
-      for (uint32_t i = 0; i < nm; i++) {
-        fill_plus[i] = count[2 * i];
-        fill_minus[i] = count[2 * i + 1];
-      }
+      ccsr.resize(base + count[nm], zero_entry);
+      fill_plus.assign(count.begin(), count.end() - 1);
       for (uint32_t k = s0; k < s1; k++) {
-        ccsr[base + fill_plus[lpair[2 * k]]++] = static_cast<uint16_t>(3u * (k - s0));
-        ccsr[base + fill_minus[lpair[2 * k + 1]]++] = static_cast<uint16_t>(3u * (k - s0));
+        const uint16_t e = static_cast<uint16_t>(6u * (k - s0));
+        ccsr[base + fill_plus[lpair[2 * k]]++] = e;
+        ccsr[base + fill_plus[lpair[2 * k + 1]]++] = static_cast<uint16_t>(e + 1u);
       }
-      for (uint32_t i = 0; i <= 2 * nm; i++) crow.push_back(static_cast<uint16_t>(count[i]));
-      for (uint32_t i = 2 * nm + 1; i < 4u * static_cast<uint32_t>(chunk_row_quads(static_cast<int>(nm))); i++)
-        crow.push_back(static_cast<uint16_t>(count[2 * nm]));
+      for (uint32_t i = 0; i <= nm; i++) crow.push_back(static_cast<uint16_t>(count[i]));
+      for (uint32_t i = nm + 1; i < 4u * static_cast<uint32_t>(chunk_row_quads(static_cast<int>(nm))); i++)
+        crow.push_back(static_cast<uint16_t>(count[nm]));
     }
     ctab.push_back(static_cast<uint32_t>(ccsr.size()) - d.csr_begin);
   }
@@ -478,6 +475,10 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
                  std::vector<uint8_t>& mcnt, std::vector<float>& lbtab) {
   const int P = part.v->nspt;
   const bool gather = part.v->chunked != 0;
+  // record words per lane: step_mass 2P records, the first-endpoint mask
+  // and 2P spring indices; step_massg P records, 2P entries and P spring
+  // indices
+  const int parts = gather ? 4 * P : 4 * P + 1;
   const size_t n_packs = part.packs.size();
   auto first_robot = [s](const PackDesc& d) {
     return static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
@@ -507,7 +508,7 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
     d.pad1 = static_cast<uint32_t>(tables[pi].size());
     lb_end += 2 * tables[pi].size();
     d.csr_begin = static_cast<uint32_t>(rec_end);
-    rec_end += static_cast<size_t>(3 * P) * d.n_mass;
+    rec_end += static_cast<size_t>(parts) * d.n_mass;
     d.row_begin = static_cast<uint32_t>(cnt_end);
     cnt_end += 2u * d.n_mass;
   }
@@ -527,9 +528,10 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
       std::memcpy(&tab[2 * t + 1], &mat, 4);
     }
     uint32_t* out = mrec.data() + d.csr_begin;
-    // record counts per mass, then the lane order: most first-endpoint
-    // records first, then most second-endpoint records (ties keep order)
-    std::vector<uint32_t> fill_p(nm, 0), fill_m(nm, 0), order(nm), lane(nm);
+    // record counts per mass, then the lane order: most records first
+    // (step_massg: most first-endpoint records, then most entries; ties
+    // keep order)
+    std::vector<uint32_t> fill_p(nm, 0), fill_m(nm, 0), order(nm), lane(nm), first(nm, 0);
     int q = r0;
     for (uint32_t k = 0; k < ns; k++) {
       const uint32_t g = d.spring_begin + k;
@@ -538,24 +540,26 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
       fill_p[s->act_pairs[2 * g] + off]++;
       fill_m[s->act_pairs[2 * g + 1] + off]++;
     }
+    for (uint32_t i = 0; i < nm; i++) {
+      if (fill_p[i] > static_cast<uint32_t>(P) || fill_m[i] > static_cast<uint32_t>(P))
+        throw Error(MH_ERR_INTERNAL, "mass record lists over the kernel's capacity");
+    }
     std::iota(order.begin(), order.end(), 0u);
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-      return fill_p[x] != fill_p[y] ? fill_p[x] > fill_p[y] : fill_m[x] > fill_m[y];
+      const uint32_t tx = fill_p[x] + fill_m[x], ty = fill_p[y] + fill_m[y];
+      if (gather && fill_p[x] != fill_p[y]) return fill_p[x] > fill_p[y];
+      return tx > ty;
     });
     for (uint32_t i = 0; i < nm; i++) {
       perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
       lane[order[i]] = i;
     }
-    for (uint32_t i = 0; i < nm; i++) {
-      if (fill_p[i] > static_cast<uint32_t>(P) || fill_m[i] > static_cast<uint32_t>(P))
-        throw Error(MH_ERR_INTERNAL, "mass record lists over the kernel's capacity");
-    }
     // table position of a pair by value
     auto index_of = [&table](uint64_t v) {
       return static_cast<uint32_t>(std::lower_bound(table.begin(), table.end(), v) - table.begin());
     };
+    std::vector<uint32_t> fill(nm, 0);  // entries (records, for step_mass) so far per mass
     std::fill(fill_p.begin(), fill_p.end(), 0u);
-    std::fill(fill_m.begin(), fill_m.end(), 0u);
     q = r0;
     uint64_t last_key = ~0ull;
     uint32_t last_pi = 0;
@@ -573,19 +577,34 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
         last_pi = index_of(key);
       }
       const uint32_t tail = (8u * last_pi) << 16;
-      const uint32_t ka = fill_p[a]++;
-      out[static_cast<size_t>(2 * P + ka) * nm + la] = k;
-      out[static_cast<size_t>(ka) * nm + la] = 16u * static_cast<uint32_t>(mass_slot(static_cast<int>(b))) | tail;
-      // second endpoint: the same spring's record, or (gather kernel) the
-      // LDS offset of the term its first endpoint stores
-      out[static_cast<size_t>(P + fill_m[b]++) * nm + lb] =
-          gather ? massg_term_offset(part.v->block, P, la, ka)
-                 : (16u * static_cast<uint32_t>(mass_slot(static_cast<int>(a))) | tail);
+      const uint32_t rec_a = 16u * static_cast<uint32_t>(mass_slot(static_cast<int>(b))) | tail;
+      if (!gather) {
+        // both endpoints hold a record, in ascending spring index; the
+        // first endpoint's carries the stress
+        const uint32_t ka = fill[a]++;
+        out[static_cast<size_t>(ka) * nm + la] = rec_a;
+        out[static_cast<size_t>(2 * P + 1 + ka) * nm + la] = k;
+        first[a] |= 1u << ka;
+        const uint32_t kb = fill[b]++;
+        out[static_cast<size_t>(kb) * nm + lb] = 16u * static_cast<uint32_t>(mass_slot(static_cast<int>(a))) | tail;
+        out[static_cast<size_t>(2 * P + 1 + kb) * nm + lb] = k;
+      } else {
+        // the first endpoint computes the term into its slot ka; both
+        // endpoints list the slot, the second with bit 0 set
+        const uint32_t ka = fill_p[a]++;
+        out[static_cast<size_t>(ka) * nm + la] = rec_a;
+        out[static_cast<size_t>(3 * P + ka) * nm + la] = k;
+        const uint32_t slot = massg_term_offset(part.v->block, P, la, ka);
+        out[static_cast<size_t>(P + fill[a]++) * nm + la] = slot;
+        out[static_cast<size_t>(P + fill[b]++) * nm + lb] = slot | 1u;
+      }
     }
+    if (!gather)
+      for (uint32_t i = 0; i < nm; i++) out[static_cast<size_t>(2 * P) * nm + lane[i]] = first[i];
     uint8_t* cnt = mcnt.data() + d.row_begin;
     for (uint32_t i = 0; i < nm; i++) {
-      cnt[2 * i] = static_cast<uint8_t>(fill_p[order[i]]);
-      cnt[2 * i + 1] = static_cast<uint8_t>(fill_m[order[i]]);
+      cnt[2 * i] = static_cast<uint8_t>(gather ? fill_p[order[i]] : fill[order[i]]);
+      cnt[2 * i + 1] = static_cast<uint8_t>(gather ? fill[order[i]] : 0u);
     }
   });
   std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
@@ -604,7 +623,7 @@ void upload_plan(mh_sim* s, Plan& plan) {
   std::vector<uint32_t> spair(na), csr, row;
   csr.reserve(2 * na + 6 * s->n_masses);
   std::vector<uint16_t> perm(s->n_masses);
-  std::vector<uint32_t> order, count, nplus, nminus, lpair;
+  std::vector<uint32_t> order, count, nplus, lpair;
   std::vector<PackDesc> all;
   row.reserve(2 * s->n_masses + 64);
   bool need_scratch = false;
@@ -621,15 +640,16 @@ void upload_plan(mh_sim* s, Plan& plan) {
     } else if (part.v->chunked) {
       upload_chunked(s, part, spair, ccsr, crow, ctab);
     } else {
-      const uint32_t zero_entry = 3u * static_cast<uint32_t>(part.spring_cap + 1);
-      const uint32_t scale = padded ? 3u : 1u;
+      // entries: 2 * (3 * slot) (LDS parts) or 2 * slot (streaming parts),
+      // + 1 where the mass is the spring's second endpoint
+      const uint32_t zero_entry = 6u * static_cast<uint32_t>(part.spring_cap + 1);
+      const uint32_t scale = padded ? 6u : 2u;
       auto pad = [padded](uint32_t n) { return padded ? (n + 3u) & ~3u : n; };
       for (PackDesc& d : part.packs) {
         d.csr_begin = static_cast<uint32_t>(csr.size());
         d.row_begin = static_cast<uint32_t>(row.size());
         const uint32_t nm = d.n_mass;
         nplus.assign(nm, 0);
-        nminus.assign(nm, 0);
         lpair.resize(2 * static_cast<size_t>(d.n_spring));
         // pack-local pairs: robot-local index + robot's offset inside the pack
         int r = static_cast<int>(std::upper_bound(s->mass_off.begin(), s->mass_off.end(), d.mass_begin) -
@@ -646,32 +666,25 @@ void upload_plan(mh_sim* s, Plan& plan) {
           lpair[2 * k] = a;
           lpair[2 * k + 1] = b;
           nplus[a]++;
-          nminus[b]++;
+          nplus[b]++;
         }
-        // row bounds: [2i] first-endpoint part, [2i+1] second-endpoint part
-        count.assign(2 * nm + 1, 0);
-        for (uint32_t i = 0; i < nm; i++) {
-          count[2 * i + 1] = count[2 * i] + pad(nplus[i]);
-          count[2 * i + 2] = count[2 * i + 1] + pad(nminus[i]);
-        }
+        // row bounds: mass i's springs at [row[i], row[i+1])
+        count.assign(nm + 1, 0);
+        for (uint32_t i = 0; i < nm; i++) count[i + 1] = count[i] + pad(nplus[i]);
         // phase-2 visiting order: longest rows first (ties keep order)
         order.resize(nm);
         std::iota(order.begin(), order.end(), 0u);
         std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-          return count[2 * x + 2] - count[2 * x] > count[2 * y + 2] - count[2 * y];
+          return count[x + 1] - count[x] > count[y + 1] - count[y];
         });
         for (uint32_t i = 0; i < nm; i++) perm[d.mass_begin + i] = static_cast<uint16_t>(order[i]);
-        for (uint32_t i = 0; i <= 2 * nm; i++) row.push_back(count[i]);
-        csr.resize(d.csr_begin + count[2 * nm], zero_entry);
-        std::vector<uint32_t> fill_plus(nm), fill_minus(nm);
-        for (uint32_t i = 0; i < nm; i++) {
-          fill_plus[i] = count[2 * i];
-          fill_minus[i] = count[2 * i + 1];
-        }
+        for (uint32_t i = 0; i <= nm; i++) row.push_back(count[i]);
+        csr.resize(d.csr_begin + count[nm], zero_entry);
+        std::vector<uint32_t> fill(count.begin(), count.end() - 1);
         for (uint32_t k = 0; k < d.n_spring; k++) {
           const uint32_t a = lpair[2 * k], b = lpair[2 * k + 1];
-          csr[d.csr_begin + fill_plus[a]++] = scale * k;
-          csr[d.csr_begin + fill_minus[b]++] = scale * k;
+          csr[d.csr_begin + fill[a]++] = scale * k;
+          csr[d.csr_begin + fill[b]++] = scale * k + 1u;
         }
       }
       // launch order: most work first, so uneven packs balance across CUs
@@ -1067,7 +1080,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   });
   if (compact_to_id.empty()) compact_to_id.push_back(0);  // keeps tables non-empty
 
-  // per robot: gather entries with both parts of every row rounded up to 4;
+  // per robot: gather entries with every mass's row rounded up to 4;
   // the most springs one mass has on one side; distinct (rest length,
   // material) pairs (left empty when they exceed the mass-parallel kernels'
   // table)
@@ -1095,7 +1108,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
       if (ps.size() > static_cast<size_t>(mh::kMassPairNull)) ps.clear();
       uint32_t g = 0, side = 0;
       for (size_t i = 0; i < plus.size(); i++) {
-        g += ((plus[i] + 3u) & ~3u) + ((minus[i] + 3u) & ~3u);
+        g += (plus[i] + minus[i] + 3u) & ~3u;
         side = std::max(side, std::max(plus[i], minus[i]));
       }
       gpad[r] = g;
@@ -1289,7 +1302,7 @@ int mh_sim_set_voxel_population(mh_sim* s, int n_robots, int nx, int ny, int nz,
                   "encodings copy");
   s->vox_stripped.alloc(M);
   s->vox_spring_id.alloc(ST);
-  s->mrec.alloc(static_cast<size_t>(R) * 3 * P * n);
+  s->mrec.alloc(static_cast<size_t>(R) * (4 * P + 1) * n);
   s->mcnt.alloc(static_cast<size_t>(R) * 2 * n);
   s->mperm.alloc(M);
   s->lbtab.alloc(static_cast<size_t>(R) * 2 * mh::kMassPairMax);
@@ -1394,7 +1407,7 @@ int mh_sim_set_voxel_population(mh_sim* s, int n_robots, int nx, int ny, int nz,
       d.n_mass = static_cast<uint32_t>(n);
       d.spring_begin = static_cast<uint32_t>(r) * S;
       d.n_spring = active[r];
-      d.csr_begin = static_cast<uint32_t>(r) * 3 * P * n;
+      d.csr_begin = static_cast<uint32_t>(r) * (4 * P + 1) * n;
       d.row_begin = static_cast<uint32_t>(r) * 2 * n;
       d.pad0 = static_cast<uint32_t>(r) * 2 * mh::kMassPairMax;
     }

@@ -2,9 +2,9 @@
 //
 // One workgroup owns one pack (a contiguous range of robots) for n_steps
 // substeps; the state is read from HBM once per launch and written back
-// once.  No atomics: every mass receives P - M, P the sum from +0 of the
-// terms of the springs whose first endpoint it is, in ascending spring
-// index, M the same over its second-endpoint springs, so results are
+// once.  No atomics: every mass receives one sum from +0 over its springs
+// in ascending spring index, each spring's term added where the mass is
+// the first endpoint and subtracted where it is the second, so results are
 // bit-reproducible, independent of the launch plan and equal to the CPU
 // mirror of the same step.
 //
@@ -13,7 +13,7 @@
 //   XPBD  x~ = x + v dt;  lambda = -(d - L)/(2 + ((1/k)/dt)/dt);
 //         dp = lambda * (x~a - x~b)/(d + 1e-12);  x' = x~ + sum(+-dp);
 //         v' = (float)(0.99 * (double)(x' - x) / (double)dt)
-//   force c = ((k (L - d)) (1 / (d + 1e-12))) (xa - xb);  F = P - M - m g y^ - c_d v
+//   force c = ((k (L - d)) (1 / (d + 1e-12))) (xa - xb);  F = sum(+-c) - m g y^ - c_d v
 //         + ground normal/friction;  u = v + (F (1/m)) dt;  x' = x + u dt;
 //         v' = damping * u
 //   L = fma(Lbar, A sin(fma(omega, t, phi)), Lbar)
@@ -166,15 +166,14 @@ __device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, fl
 // ----------------------------------------------------------------- LDS kernel
 // LDS layout (from LDS address 0): spring terms float[3][spring_cap + 2]
 // (12 B per slot, rounded up to 16 B) | positions float4[pos_slots(mass_cap)] |
-// gather entries u16[csr_cap] | row bounds u16[2 mass_cap + 2] |
-// actuation float[n_mat].  Slot spring_cap is the sink of lanes without a
-// spring; slot spring_cap + 1 holds zeros.  A gather entry is the word
-// offset 3 * slot of a term.  Mass i's row has two parts, [row[2i],
-// row[2i+1]) with the springs whose first endpoint it is and [row[2i+1],
-// row[2i+2]) with those whose second endpoint it is, each in ascending
-// spring index and padded to a multiple of 4 entries with the zero slot
-// (adding +0 leaves a sum that started at +0 unchanged); the mass receives
-// P - M of the two part sums.
+// gather entries u16[csr_cap] | row bounds u16[mass_cap + 1] (the layout
+// reserves 2 mass_cap + 2) | actuation float[n_mat].  Slot spring_cap is
+// the sink of lanes without a spring; slot spring_cap + 1 holds zeros.  A
+// gather entry is 2 * (3 * slot) of a term, + 1 where the mass is the
+// spring's second endpoint.  Mass i's row [row[i], row[i+1]) lists its
+// springs in ascending spring index, padded to a multiple of 4 with the
+// zero slot (adding +0 leaves a sum that started at +0 unchanged); the
+// mass receives the canonical sum.
 // Springs are dealt to waves in contiguous runs of whole 64-lane
 // iterations, so every wave runs ceil(springs / BLOCK) iterations.
 // Phase-2 lanes visit masses in the order of p.mperm (longest rows first).
@@ -199,9 +198,9 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
   const int iters = (ns + 64 * kWaves - 1) / (64 * kWaves);  // <= NSPT
   const int wave_first = wave * iters * 64;
 
-  const int n_csr = static_cast<int>(p.row[d.row_begin + 2 * nm]);
+  const int n_csr = static_cast<int>(p.row[d.row_begin + nm]);
   for (int i = tid; i < n_csr; i += BLOCK) s_csr[i] = static_cast<uint16_t>(p.csr[d.csr_begin + i]);
-  for (int i = tid; i <= 2 * nm; i += BLOCK) s_row[i] = static_cast<uint16_t>(p.row[d.row_begin + i]);
+  for (int i = tid; i <= nm; i += BLOCK) s_row[i] = static_cast<uint16_t>(p.row[d.row_begin + i]);
   if (tid < 3) s_term[3 * zero + tid] = 0.f;
 
   // spring records -> registers; lanes past the pack's springs compute a
@@ -262,38 +261,30 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
     for (int j = 0; j < MPT; ++j) {
       if (mid[j] >= 0) {
         const int i = mid[j];
-        const int e1 = s_row[2 * i + 1], e2 = s_row[2 * i + 2];
-        float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
-        // P (first-endpoint terms) and M (second-endpoint terms), each
-        // summed in ascending spring order, four gathers in flight; the
-        // mass receives P - M
-        for (int e = s_row[2 * i]; e < e1; e += 4) {
+        const int e2 = s_row[i + 1];
+        float ax = 0.0f, ay = 0.0f, az = 0.0f;
+        // the mass's springs in ascending spring order, four gathers in
+        // flight; an entry is 2 * (3 * slot) + 1 where the mass is the
+        // spring's second endpoint: fma(term, -1, sum) is sum - term
+        // rounded once
+        for (int e = s_row[i]; e < e2; e += 4) {
           const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
-          const float* q0 = s_term + (w.x & 0xffffu);
-          const float* q1 = s_term + (w.x >> 16);
-          const float* q2 = s_term + (w.y & 0xffffu);
-          const float* q3 = s_term + (w.y >> 16);
+          const float* q0 = s_term + ((w.x & 0xffffu) >> 1);
+          const float* q1 = s_term + (w.x >> 17);
+          const float* q2 = s_term + ((w.y & 0xffffu) >> 1);
+          const float* q3 = s_term + (w.y >> 17);
+          const float g0 = __uint_as_float(0x3f800000u | (w.x << 31));
+          const float g1 = __uint_as_float(0x3f800000u | ((w.x & 0x10000u) << 15));
+          const float g2 = __uint_as_float(0x3f800000u | (w.y << 31));
+          const float g3 = __uint_as_float(0x3f800000u | ((w.y & 0x10000u) << 15));
           const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
           const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
-          ax = ax + x0; ay = ay + y0; az = az + z0;
-          ax = ax + x1; ay = ay + y1; az = az + z1;
-          ax = ax + x2; ay = ay + y2; az = az + z2;
-          ax = ax + x3; ay = ay + y3; az = az + z3;
+          ax = fmaf(x0, g0, ax); ay = fmaf(y0, g0, ay); az = fmaf(z0, g0, az);
+          ax = fmaf(x1, g1, ax); ay = fmaf(y1, g1, ay); az = fmaf(z1, g1, az);
+          ax = fmaf(x2, g2, ax); ay = fmaf(y2, g2, ay); az = fmaf(z2, g2, az);
+          ax = fmaf(x3, g3, ax); ay = fmaf(y3, g3, ay); az = fmaf(z3, g3, az);
         }
-        for (int e = e1; e < e2; e += 4) {
-          const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
-          const float* q0 = s_term + (w.x & 0xffffu);
-          const float* q1 = s_term + (w.x >> 16);
-          const float* q2 = s_term + (w.y & 0xffffu);
-          const float* q3 = s_term + (w.y >> 16);
-          const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
-          const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
-          bx = bx + x0; by = by + y0; bz = bz + z0;
-          bx = bx + x1; by = by + y1; bz = bz + z1;
-          bx = bx + x2; by = by + y2; bz = bz + z2;
-          bx = bx + x3; by = by + y3; bz = bz + z3;
-        }
-        s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax - bx, ay - by, az - bz, p);
+        s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax, ay, az, p);
       }
     }
     if (tid < p.n_mat) s_R[tid] = actuation(p.mat_dL0[tid], p.mat_omega[tid], p.mat_phi[tid], tn);
@@ -373,21 +364,19 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
       s.mg = s.m * p.gravity;
       const float4 lp = s_pos[i];
       s.tx = lp.x; s.ty = lp.y; s.tz = lp.z;
-      const int e1 = static_cast<int>(row[2 * i + 1]), e2 = static_cast<int>(row[2 * i + 2]);
-      float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
-      for (int e = static_cast<int>(row[2 * i]); e < e1; ++e) {
-        const float4 q = sc[csr[e]];
-        ax = ax + q.x;
-        ay = ay + q.y;
-        az = az + q.z;
+      // the mass's springs in ascending spring order; an entry is
+      // 2 * slot, + 1 where the mass is the spring's second endpoint
+      const int e2 = static_cast<int>(row[i + 1]);
+      float ax = 0.0f, ay = 0.0f, az = 0.0f;
+      for (int e = static_cast<int>(row[i]); e < e2; ++e) {
+        const uint32_t c = csr[e];
+        const float4 q = sc[c >> 1];
+        const float g = __uint_as_float(0x3f800000u | (c << 31));
+        ax = fmaf(q.x, g, ax);
+        ay = fmaf(q.y, g, ay);
+        az = fmaf(q.z, g, az);
       }
-      for (int e = e1; e < e2; ++e) {
-        const float4 q = sc[csr[e]];
-        bx = bx + q.x;
-        by = by + q.y;
-        bz = bz + q.z;
-      }
-      s_pos[i] = integrate<SOLVER>(s, ax - bx, ay - by, az - bz, p);
+      s_pos[i] = integrate<SOLVER>(s, ax, ay, az, p);
       p.pos[d.mass_begin + i] = make_float4(s.x, s.y, s.z, s.m);
       p.vel[d.mass_begin + i] = make_float4(s.vx, s.vy, s.vz, 0.f);
     }
@@ -406,10 +395,11 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
 // are loaded into registers one chunk ahead, so the global-memory latency
 // of chunk c + 1 overlaps chunk c's work.  Phase 1 copies the chunk's gather
 // lists and row bounds into LDS and writes its spring terms to LDS; after a
-// barrier, phase 2 adds the chunk's first-endpoint terms to P and
-// second-endpoint terms to M of each lane's masses, in ascending spring
-// order; a barrier ends the chunk.  After the last chunk the masses
-// integrate with P - M.  LDS: terms | positions | gather entries
+// barrier, phase 2 adds the chunk's terms to the running sum of each
+// lane's masses in ascending spring order (+ where the mass is the first
+// endpoint, - where it is the second); a barrier ends the chunk.  Chunks
+// run in ascending spring order, so after the last chunk each mass holds
+// its canonical sum and integrates.  LDS: terms | positions | gather entries
 // u16[csr_cap] | row bounds u16[4 chunk_row_quads(mass_cap)] | actuation
 // float[n_mat] | stiffness float[n_mat] | chunk entry offsets
 // u32[kChunkMax + 1] (from p.ctab at d.pad0).
@@ -488,12 +478,9 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
   fetch(0);
 
   for (int step = 0; step < p.n_steps; ++step) {
-    float P[MPT][3], M[MPT][3];
+    float A[MPT][3];
 #pragma unroll
-    for (int j = 0; j < MPT; ++j) {
-      P[j][0] = P[j][1] = P[j][2] = 0.0f;
-      M[j][0] = M[j][1] = M[j][2] = 0.0f;
-    }
+    for (int j = 0; j < MPT; ++j) A[j][0] = A[j][1] = A[j][2] = 0.0f;
     const float tn = t + p.dt;
     for (int c = 0; c < n_chunks; ++c) {
       const int s0 = c * C;
@@ -536,38 +523,30 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
       for (int j = 0; j < MPT; ++j) {
         const int i = tid + j * BLOCK;
         if (i < nm) {
-          const int e1 = s_row[2 * i + 1], e2 = s_row[2 * i + 2];
-          float ax = P[j][0], ay = P[j][1], az = P[j][2];
-          for (int e = s_row[2 * i]; e < e1; e += 4) {
+          // the chunk's springs of the mass in ascending spring order; an
+          // entry is 2 * (3 * (spring - chunk start)), + 1 where the mass
+          // is the second endpoint: fma(term, -1, sum) is sum - term
+          const int e2 = s_row[i + 1];
+          float ax = A[j][0], ay = A[j][1], az = A[j][2];
+          for (int e = s_row[i]; e < e2; e += 4) {
             const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
-            const float* q0 = s_term + (w.x & 0xffffu);
-            const float* q1 = s_term + (w.x >> 16);
-            const float* q2 = s_term + (w.y & 0xffffu);
-            const float* q3 = s_term + (w.y >> 16);
+            const float* q0 = s_term + ((w.x & 0xffffu) >> 1);
+            const float* q1 = s_term + (w.x >> 17);
+            const float* q2 = s_term + ((w.y & 0xffffu) >> 1);
+            const float* q3 = s_term + (w.y >> 17);
+            const float g0 = __uint_as_float(0x3f800000u | (w.x << 31));
+            const float g1 = __uint_as_float(0x3f800000u | ((w.x & 0x10000u) << 15));
+            const float g2 = __uint_as_float(0x3f800000u | (w.y << 31));
+            const float g3 = __uint_as_float(0x3f800000u | ((w.y & 0x10000u) << 15));
             const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
             const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
-            ax = ax + x0; ay = ay + y0; az = az + z0;
-            ax = ax + x1; ay = ay + y1; az = az + z1;
-            ax = ax + x2; ay = ay + y2; az = az + z2;
-            ax = ax + x3; ay = ay + y3; az = az + z3;
+            ax = fmaf(x0, g0, ax); ay = fmaf(y0, g0, ay); az = fmaf(z0, g0, az);
+            ax = fmaf(x1, g1, ax); ay = fmaf(y1, g1, ay); az = fmaf(z1, g1, az);
+            ax = fmaf(x2, g2, ax); ay = fmaf(y2, g2, ay); az = fmaf(z2, g2, az);
+            ax = fmaf(x3, g3, ax); ay = fmaf(y3, g3, ay); az = fmaf(z3, g3, az);
           }
-          P[j][0] = ax; P[j][1] = ay; P[j][2] = az;
-          float bx = M[j][0], by = M[j][1], bz = M[j][2];
-          for (int e = e1; e < e2; e += 4) {
-            const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
-            const float* q0 = s_term + (w.x & 0xffffu);
-            const float* q1 = s_term + (w.x >> 16);
-            const float* q2 = s_term + (w.y & 0xffffu);
-            const float* q3 = s_term + (w.y >> 16);
-            const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
-            const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
-            bx = bx + x0; by = by + y0; bz = bz + z0;
-            bx = bx + x1; by = by + y1; bz = bz + z1;
-            bx = bx + x2; by = by + y2; bz = bz + z2;
-            bx = bx + x3; by = by + y3; bz = bz + z3;
-          }
-          M[j][0] = bx; M[j][1] = by; M[j][2] = bz;
-          if (last) s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax - bx, ay - by, az - bz, p);
+          A[j][0] = ax; A[j][1] = ay; A[j][2] = az;
+          if (last) s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax, ay, az, p);
         }
       }
       if (last && tid < p.n_mat)
@@ -604,14 +583,19 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
 // lanes below the pair count compute the next substep's L of one pair each
 // (the actuation once per pair instead of once per spring).  Lanes take
 // masses in the order of p.mperm (most records first, so a wave's lists
-// have similar lengths).  Each lane keeps its records in registers: MAXP
-// first-endpoint records then MAXP second-endpoint records, each in
-// ascending spring index, padded to the wave's longest list with records of
-// the null pair (last entry: stiffness 0 for force, +inf for xpbd), whose
-// terms are signed zeros; a record holds byte offsets: the other endpoint's
-// slot * 16 | pair index * 8 << 16.  Records live in p.mrec as
-// [part][k][lane] (part 2: the pack-local spring index of the part-0
-// records, for the stress output); p.mcnt holds the two counts per lane.
+// have similar lengths).  Each lane keeps its records in registers: one
+// record per spring of its mass, in ascending spring index (a spring from
+// the mass to itself twice), padded to the wave's longest list with
+// records of the null pair (last entry: stiffness 0 for force, +inf for
+// xpbd), whose terms are signed zeros; a record holds byte offsets: the
+// other endpoint's slot * 16 | pair index * 8 << 16.  The lane computes
+// every term from its own side, which for a spring whose second endpoint
+// it is gives the exact negation of the first endpoint's term, and adds
+// them in list order: the canonical sum.  Records live in p.mrec as
+// [part][lane]: parts 0..2 MAXP - 1 the records, part 2 MAXP a bit mask of
+// the records whose first endpoint the mass is (they carry the stress),
+// parts 2 MAXP + 1 + k the pack-local spring index of record k; p.mcnt
+// holds two bytes per lane, the record count first.
 template <int BLOCK>
 constexpr int mass_pos_bytes() { return 16 * mass_slots(BLOCK); }
 
@@ -657,9 +641,9 @@ __device__ __forceinline__ void mass_pairs_advance(unsigned char* smem, int n_pa
 // One substep reading position and pair buffer CUR and writing the other.
 template <int SOLVER, int BLOCK, int MAXP, bool STRESS, int CUR>
 __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepParams& p, MassState& ms,
-                                             uint32_t (&rp)[MAXP], uint32_t (&rm)[MAXP],
-                                             float (&st)[MAXP], int np, int nq, int np_w, int nq_w,
-                                             bool live, int pslot, int n_pairs, float t) {
+                                             uint32_t (&rr)[2 * MAXP], float (&st)[STRESS ? 2 * MAXP : 1],
+                                             uint32_t first, int nr, int nr_w, bool live, int pslot, int n_pairs,
+                                             float t) {
   constexpr int kP0 = kMassPos + CUR * mass_pos_bytes<BLOCK>();
   constexpr int kP1 = kMassPos + (1 - CUR) * mass_pos_bytes<BLOCK>();
   constexpr int kPairs = mass_pair_base(BLOCK, 2);
@@ -671,15 +655,15 @@ __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepPara
   // padding terms stay in the fast arithmetic range
   const float4 own = live ? *reinterpret_cast<const float4*>(smem + kP0 + 16 * pslot)
                           : make_float4(0.f, 0.f, 0.f, 0.f);
-  float ax = 0.0f, ay = 0.0f, az = 0.0f, bx = 0.0f, by = 0.0f, bz = 0.0f;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
 #pragma unroll
-  for (int k = 0; k < MAXP; ++k) {
-    if (k < np_w) {
+  for (int k = 0; k < 2 * MAXP; ++k) {
+    if (k < nr_w) {
       // opaque to the optimizer: decoding the record per substep costs two
       // VALU, hoisting every decoded field out of the step loop would cost
       // registers
-      asm volatile("" : "+v"(rp[k]));
-      const uint32_t r = rp[k];
+      asm volatile("" : "+v"(rr[k]));
+      const uint32_t r = rr[k];
       const float2 Lk = *reinterpret_cast<const float2*>(smem + kD0 + (r >> 16));
       const float4 other = *reinterpret_cast<const float4*>(smem + kP0 + (r & 0xffffu));
       float cx, cy, cz, lambda;
@@ -687,33 +671,20 @@ __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepPara
       // padding records add a signed zero, which leaves a sum that
       // started at +0 unchanged
       ax = ax + cx; ay = ay + cy; az = az + cz;
-      if (STRESS && k < np)
-        st[k] = st[k] + lambda / *reinterpret_cast<const float*>(smem + kAux + (r >> 16) + 4);
+      if constexpr (STRESS)
+        if (k < nr && ((first >> k) & 1u))
+          st[k] = st[k] + lambda / *reinterpret_cast<const float*>(smem + kAux + (r >> 16) + 4);
     }
     // one record at a time: keeps the unrolled loop's registers bounded
     __builtin_amdgcn_sched_barrier(0);
   }
-#pragma unroll
-  for (int k = 0; k < MAXP; ++k) {
-    if (k < nq_w) {
-      asm volatile("" : "+v"(rm[k]));
-      const uint32_t r = rm[k];
-      const float2 Lk = *reinterpret_cast<const float2*>(smem + kD0 + (r >> 16));
-      const float4 other = *reinterpret_cast<const float4*>(smem + kP0 + (r & 0xffffu));
-      float cx, cy, cz, lambda;
-      spring_term_L<SOLVER>(other, own, Lk.x, Lk.y, cx, cy, cz, lambda);
-      bx = bx + cx; by = by + cy; bz = bz + cz;
-    }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  if (live)
-    *reinterpret_cast<float4*>(smem + kP1 + 16 * pslot) = integrate<SOLVER>(ms, ax - bx, ay - by, az - bz, p);
+  if (live) *reinterpret_cast<float4*>(smem + kP1 + 16 * pslot) = integrate<SOLVER>(ms, ax, ay, az, p);
   mass_pairs_advance<BLOCK>(smem, n_pairs, kD1, kCst, t + p.dt);
   __syncthreads();
 }
 
 // STRESS (xpbd only) accumulates per-spring stress; the variant without it
-// frees MAXP registers per lane.
+// frees 2 MAXP registers per lane.
 template <int SOLVER, int BLOCK, int MAXP, bool STRESS>
 __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -727,28 +698,29 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
   const uint32_t* rec = p.mrec + d.csr_begin;
 
   MassState ms;
-  uint32_t rp[MAXP], rm[MAXP];
-  float st[MAXP];
-  int np = 0, nq = 0, mass = 0;
+  uint32_t rr[2 * MAXP];
+  float st[STRESS ? 2 * MAXP : 1];
+  uint32_t first = 0u;
+  int nr = 0, mass = 0;
   // padding record: the ghost, with the null pair
   const uint32_t pad = 16u * kGhost | (8u * kMassPairNull) << 16;
 #pragma unroll
-  for (int k = 0; k < MAXP; ++k) {
-    rp[k] = pad;
-    rm[k] = pad;
-    st[k] = 0.f;
-  }
+  for (int k = 0; k < 2 * MAXP; ++k) rr[k] = pad;
+#pragma unroll
+  for (int k = 0; k < (STRESS ? 2 * MAXP : 1); ++k) st[k] = 0.f;
   if (live) {
     mass = p.mperm[d.mass_begin + tid];
     load_mass(ms, p.pos, p.vel, d.mass_begin + mass, p);
-    np = p.mcnt[d.row_begin + 2 * tid];
-    nq = p.mcnt[d.row_begin + 2 * tid + 1];
+    nr = p.mcnt[d.row_begin + 2 * tid];
 #pragma unroll
-    for (int k = 0; k < MAXP; ++k) {
-      if (k < np) rp[k] = rec[static_cast<size_t>(k) * nm + tid];
-      if (k < nq) rm[k] = rec[static_cast<size_t>(MAXP + k) * nm + tid];
-      if (STRESS && k < np)
-        st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]];
+    for (int k = 0; k < 2 * MAXP; ++k)
+      if (k < nr) rr[k] = rec[static_cast<size_t>(k) * nm + tid];
+    if constexpr (STRESS) {
+      first = rec[static_cast<size_t>(2 * MAXP) * nm + tid];
+#pragma unroll
+      for (int k = 0; k < 2 * MAXP; ++k)
+        if (k < nr && ((first >> k) & 1u))
+          st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + 1 + k) * nm + tid]];
     }
   }
   const int pslot = live ? mass_slot(mass) : kGhost;
@@ -766,33 +738,31 @@ __global__ __launch_bounds__(BLOCK) void step_mass(StepParams p) {
                                  kPairs + 3 * kMassDynBytes, t,
                                  (SOLVER == MH_SOLVER_XPBD_REF) ? __builtin_inff() : 0.0f, 0.0f);
   const int n_pairs = static_cast<int>(d.pad1);
-  // wave-wide loop bounds
-  int np_w = np, nq_w = nq;
-  for (int off = 32; off > 0; off >>= 1) {
-    np_w = max(np_w, __shfl_xor(np_w, off));
-    nq_w = max(nq_w, __shfl_xor(nq_w, off));
-  }
+  // wave-wide loop bound
+  int nr_w = nr;
+  for (int off = 32; off > 0; off >>= 1) nr_w = max(nr_w, __shfl_xor(nr_w, off));
   __syncthreads();
 
   // substeps in pairs, so each buffer's base is a constant offset
   int step = 0;
   for (; step + 1 < p.n_steps; step += 2) {
-    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pairs, t);
+    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rr, st, first, nr, nr_w, live, pslot, n_pairs, t);
     t = t + p.dt;
-    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 1>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pairs, t);
+    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 1>(smem, p, ms, rr, st, first, nr, nr_w, live, pslot, n_pairs, t);
     t = t + p.dt;
   }
   if (step < p.n_steps)
-    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rp, rm, st, np, nq, np_w, nq_w, live, pslot, n_pairs, t);
+    mass_substep<SOLVER, BLOCK, MAXP, STRESS, 0>(smem, p, ms, rr, st, first, nr, nr_w, live, pslot, n_pairs, t);
 
   if (live) {
     const uint32_t gi = d.mass_begin + mass;
     p.pos[gi] = make_float4(ms.x, ms.y, ms.z, ms.m);
     p.vel[gi] = make_float4(ms.vx, ms.vy, ms.vz, 0.f);
-    if (STRESS) {
+    if constexpr (STRESS) {
 #pragma unroll
-      for (int k = 0; k < MAXP; ++k)
-        if (k < np) p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]] = st[k];
+      for (int k = 0; k < 2 * MAXP; ++k)
+        if (k < nr && ((first >> k) & 1u))
+          p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + 1 + k) * nm + tid]] = st[k];
     }
   }
 }
@@ -847,20 +817,25 @@ __device__ __forceinline__ bool position_in_div_range(float4 q) {
 
 // ------------------------------------------------ mass-parallel gather kernel
 // As step_mass, but each spring is evaluated once, by its first endpoint.
-// Phase 1: a lane computes its first-endpoint terms, adds them to P and
-// stores each in its own LDS term slots (12 B per (lane, k), lane-major).
-// Barrier.  Phase 2: the lane adds the terms of the springs whose second
-// endpoint it is to M, reading them from their first endpoints' slots,
-// integrates with P - M and writes its position in place; the next
-// substep's actuated rest lengths are written in place too.  Barrier.  Two
-// barriers per substep, half the spring arithmetic of step_mass.  LDS: a
-// zero term at kMassZero and two flags at kMassZero + 16 | positions
-// float4[mass_slots(BLOCK)] | the pair tables {L, k}, {1/k, rest length}
-// and the constants, as in step_mass but one {L, k} buffer | terms.
-// First-endpoint records as in step_mass; a second-endpoint record is the
-// LDS byte offset of its term (padding: the zero term).  XPBD: the flags
-// (one per substep parity) record whether any position written for that
-// substep left the fast division's range.
+// Phase 1: a lane computes the terms of the springs whose first endpoint it
+// is and stores each in its own LDS term slots (12 B per (lane, k),
+// lane-major).  Barrier.  Phase 2: the lane walks one list of entries, one
+// per spring of its mass in ascending spring index, each the LDS byte
+// offset of the spring's term (its own slot or the first endpoint's) with
+// bit 0 set where the mass is the second endpoint; it adds
+// fma(term, +-1, sum) (the product by +-1 is exact, so this is the sum plus
+// or minus the term, rounded once: the canonical sum), integrates and
+// writes its position in place; the next substep's actuated rest lengths
+// are written in place too.  Barrier.  Two barriers per substep, half the
+// spring arithmetic of step_mass.  LDS: a zero term at kMassZero and two
+// flags at kMassZero + 16 | positions float4[mass_slots(BLOCK)] | the pair
+// tables {L, k}, {1/k, rest length} and the constants, as in step_mass but
+// one {L, k} buffer | terms.  p.mrec parts: 0..MAXP - 1 the first-endpoint
+// records (as step_mass's), MAXP..3 MAXP - 1 the entries (padding: the
+// zero term), 3 MAXP + k the pack-local spring index of first-endpoint
+// record k; p.mcnt: two counts per lane (records, entries).  XPBD: the
+// flags (one per substep parity) record whether any position written for
+// that substep left the fast division's range.
 template <int SOLVER, int BLOCK, int MAXP, bool STRESS>
 __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -877,27 +852,30 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   const uint32_t* rec = p.mrec + d.csr_begin;
 
   MassState ms;
-  uint32_t rp[MAXP], rm[MAXP];
+  uint32_t rp[MAXP], re[2 * MAXP];
   float st[MAXP];
-  int np = 0, nq = 0, mass = 0;
+  int np = 0, ne = 0, mass = 0;
   const uint32_t pad = 16u * kGhost | (8u * kMassPairNull) << 16;
 #pragma unroll
   for (int k = 0; k < MAXP; ++k) {
     rp[k] = pad;
-    rm[k] = kMassZero;
     st[k] = 0.f;
   }
+#pragma unroll
+  for (int k = 0; k < 2 * MAXP; ++k) re[k] = kMassZero;
   if (live) {
     mass = p.mperm[d.mass_begin + tid];
     load_mass(ms, p.pos, p.vel, d.mass_begin + mass, p);
     np = p.mcnt[d.row_begin + 2 * tid];
-    nq = p.mcnt[d.row_begin + 2 * tid + 1];
+    ne = p.mcnt[d.row_begin + 2 * tid + 1];
 #pragma unroll
     for (int k = 0; k < MAXP; ++k) {
       if (k < np) rp[k] = rec[static_cast<size_t>(k) * nm + tid];
-      if (k < nq) rm[k] = rec[static_cast<size_t>(MAXP + k) * nm + tid];
-      if (STRESS && k < np) st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]];
+      if (STRESS && k < np) st[k] = p.sstress[d.spring_begin + rec[static_cast<size_t>(3 * MAXP + k) * nm + tid]];
     }
+#pragma unroll
+    for (int k = 0; k < 2 * MAXP; ++k)
+      if (k < ne) re[k] = rec[static_cast<size_t>(MAXP + k) * nm + tid];
   }
   const int pslot = live ? mass_slot(mass) : kGhost;
   float4* pos = reinterpret_cast<float4*>(smem + kMassPos);
@@ -916,10 +894,10 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
     s_flag[0] = 0u;
     s_flag[1] = 0u;
   }
-  int np_w = np, nq_w = nq;
+  int np_w = np, ne_w = ne;
   for (int off = 32; off > 0; off >>= 1) {
     np_w = max(np_w, __shfl_xor(np_w, off));
-    nq_w = max(nq_w, __shfl_xor(nq_w, off));
+    ne_w = max(ne_w, __shfl_xor(ne_w, off));
   }
   unsigned char* mine = smem + kTerm + 12 * MAXP * tid;
   if (SOLVER == MH_SOLVER_XPBD_REF) {
@@ -935,7 +913,6 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
       fast_ok = p.fast_div && s_flag[step & 1] == 0u;
       if (tid == 0) s_flag[(step + 1) & 1] = 0u;  // read last in the previous substep
     }
-    float ax = 0.0f, ay = 0.0f, az = 0.0f;
 #pragma unroll
     for (int k = 0; k < MAXP; ++k) {
       if (k < np_w) {
@@ -951,7 +928,6 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
           const float2 Lk = *reinterpret_cast<const float2*>(smem + kDyn + (r >> 16));
           spring_term_L<SOLVER>(own, other, Lk.x, Lk.y, cx, cy, cz, lambda);
         }
-        ax = ax + cx; ay = ay + cy; az = az + cz;
         float* slot = reinterpret_cast<float*>(mine + 12 * k);
         slot[0] = cx;
         slot[1] = cy;
@@ -959,16 +935,20 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
       }
     }
     __syncthreads();
-    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
 #pragma unroll
-    for (int k = 0; k < MAXP; ++k) {
-      if (k < nq_w) {
-        const float* q = reinterpret_cast<const float*>(smem + rm[k]);
-        bx = bx + q[0]; by = by + q[1]; bz = bz + q[2];
+    for (int k = 0; k < 2 * MAXP; ++k) {
+      if (k < ne_w) {
+        const uint32_t e = re[k];
+        const float* q = reinterpret_cast<const float*>(smem + (e & ~1u));
+        const float sg = __uint_as_float(0x3f800000u | (e << 31));
+        ax = fmaf(q[0], sg, ax);
+        ay = fmaf(q[1], sg, ay);
+        az = fmaf(q[2], sg, az);
       }
     }
     if (live) {
-      const float4 np4 = integrate<SOLVER>(ms, ax - bx, ay - by, az - bz, p);
+      const float4 np4 = integrate<SOLVER>(ms, ax, ay, az, p);
       pos[pslot] = np4;
       if (SOLVER == MH_SOLVER_XPBD_REF && !position_in_div_range(np4)) atomicOr(&s_flag[(step + 1) & 1], 1u);
     }
@@ -984,7 +964,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
     if (STRESS) {
 #pragma unroll
       for (int k = 0; k < MAXP; ++k)
-        if (k < np) p.sstress[d.spring_begin + rec[static_cast<size_t>(2 * MAXP + k) * nm + tid]] = st[k];
+        if (k < np) p.sstress[d.spring_begin + rec[static_cast<size_t>(3 * MAXP + k) * nm + tid]] = st[k];
     }
   }
 }

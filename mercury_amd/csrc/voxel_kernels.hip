@@ -18,11 +18,11 @@
 //      voxels the shape template lists per spring);
 //   5. the robot's (rest length, material) pair table: a presence bit per
 //      (length index, material id), ranked by a prefix popcount;
-//   6. lane order: masses by non-air first-endpoint springs, then
-//      second-endpoint springs, both descending, ties by mass index;
-//   7. per lane its records in ascending spring index (first-endpoint list,
-//      second-endpoint list, and the spring indices of the first list for
-//      the stress output), as the host packing writes them.
+//   6. lane order: masses by non-air springs, descending, ties by mass
+//      index;
+//   7. per lane one record per non-air spring in ascending spring index,
+//      the mask of the records whose first endpoint the mass is, and each
+//      record's spring index, as the host packing writes them.
 // Integer atomics only (LDS counts, bit sets, the error flag), so every
 // output is deterministic.
 #include <hip/hip_runtime.h>
@@ -175,13 +175,13 @@ __global__ __launch_bounds__(kVoxThreads) void voxel_population(VoxelBuildParams
     return below[kk >> 5] + __popc(w & ((1u << (kk & 31)) - 1u));
   };
 
-  // 6. lane order
+  // 6. lane order: most springs first (ties by mass index)
   int nq = 0, npf = 0;
   if (tid < n) {
     for (int j = p.first_off[tid]; j < p.first_off[tid + 1]; j++) npf += sid[p.first_list[j]] != 0;
     for (int j = p.second_off[tid]; j < p.second_off[tid + 1]; j++) nq += sid[p.second_list[j]] != 0;
     if (npf > p.maxp || nq > p.maxp) flag_error(p.error, MH_ERR_RANGE);
-    key[tid] = static_cast<uint16_t>(npf * 64 + nq);
+    key[tid] = static_cast<uint16_t>(npf + nq);
     atomicAdd(&n_active, npf);
   }
   __syncthreads();
@@ -197,33 +197,34 @@ __global__ __launch_bounds__(kVoxThreads) void voxel_population(VoxelBuildParams
   }
   __syncthreads();
 
-  // 7. records
+  // 7. records: one per non-air spring of the mass in ascending spring
+  // index (merging its first-endpoint and second-endpoint lists), the mask
+  // of the first-endpoint ones and each record's spring index
   const int P = p.maxp;
-  uint32_t* rec = p.mrec + static_cast<size_t>(r) * 3 * P * n;
+  uint32_t* rec = p.mrec + static_cast<size_t>(r) * (4 * P + 1) * n;
   if (tid < n && np_r <= kMassPairNull) {
     const int lane = lane_of[tid];
+    int jf = p.first_off[tid], jf_end = p.first_off[tid + 1];
+    int js = p.second_off[tid], js_end = p.second_off[tid + 1];
+    uint32_t mask = 0u;
     int k = 0;
-    for (int j = p.first_off[tid]; j < p.first_off[tid + 1] && k < P; j++) {
-      const int s = p.first_list[j];
+    while ((jf < jf_end || js < js_end) && k < 2 * P) {
+      // the lower spring index next; a spring from the mass to itself
+      // (none in a lattice) would come first as its first endpoint
+      const bool take_first = js >= js_end || (jf < jf_end && p.first_list[jf] <= p.second_list[js]);
+      const int s = take_first ? p.first_list[jf++] : p.second_list[js++];
       if (!sid[s]) continue;
-      const uint32_t other = p.pairs[2 * s + 1];
+      const uint32_t other = take_first ? p.pairs[2 * s + 1] : p.pairs[2 * s];
       rec[static_cast<size_t>(k) * n + lane] =
           16u * static_cast<uint32_t>(mass_slot(static_cast<int>(other))) | (8u * pair_rank(s)) << 16;
-      rec[static_cast<size_t>(2 * P + k) * n + lane] = static_cast<uint32_t>(s);
+      rec[static_cast<size_t>(2 * P + 1 + k) * n + lane] = static_cast<uint32_t>(s);
+      if (take_first) mask |= 1u << k;
       k++;
     }
-    k = 0;
-    for (int j = p.second_off[tid]; j < p.second_off[tid + 1] && k < P; j++) {
-      const int s = p.second_list[j];
-      if (!sid[s]) continue;
-      const uint32_t other = p.pairs[2 * s];
-      rec[static_cast<size_t>(P + k) * n + lane] =
-          16u * static_cast<uint32_t>(mass_slot(static_cast<int>(other))) | (8u * pair_rank(s)) << 16;
-      k++;
-    }
+    rec[static_cast<size_t>(2 * P) * n + lane] = mask;
     uint8_t* c = p.mcnt + static_cast<size_t>(r) * 2 * n;
-    c[2 * lane] = static_cast<uint8_t>(npf);
-    c[2 * lane + 1] = static_cast<uint8_t>(nq);
+    c[2 * lane] = static_cast<uint8_t>(k);
+    c[2 * lane + 1] = 0;
   }
   if (tid == 0) {
     PackDesc d;
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(kVoxThreads) void voxel_population(VoxelBuildParams
     d.n_mass = static_cast<uint32_t>(n);
     d.spring_begin = static_cast<uint32_t>(r) * S;
     d.n_spring = static_cast<uint32_t>(n_active);
-    d.csr_begin = static_cast<uint32_t>(r) * 3 * P * n;
+    d.csr_begin = static_cast<uint32_t>(r) * (4 * P + 1) * n;
     d.row_begin = static_cast<uint32_t>(r) * 2 * n;
     d.pad0 = static_cast<uint32_t>(r) * 2 * kMassPairMax;
     d.pad1 = static_cast<uint32_t>(np_r);

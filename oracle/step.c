@@ -19,10 +19,14 @@
  *   (common/simulator/Simulator.cu:428-449).
  * The reference sums corrections with shared-memory float atomics (order
  * undefined).  The canonical order here, and in the HIP kernels, is: per
- * mass, P = sum from +0.0f of the terms of the springs whose first
- * endpoint it is (ascending spring index), M = the same over the springs
- * whose second endpoint it is, and the mass receives P - M.  |.| is sqrtf((x*x + y*y) + z*z) and sin is
- * orc_sinf (CUDA's norm3df/sinf are outside the reference tree).
+ * mass, one sum from +0.0f over every spring it belongs to, in ascending
+ * spring index, adding the spring's term where the mass is the first
+ * endpoint and subtracting it where it is the second (first before second
+ * for a spring joining a mass to itself).  The term seen from the second
+ * endpoint, computed from its own side, is the exact IEEE negation, since
+ * a - b == -(b - a) and the length is symmetric.  |.| is
+ * sqrtf((x*x + y*y) + z*z) and sin is orc_sinf (CUDA's norm3df/sinf are
+ * outside the reference tree).
  *
  * force (project spec; the reference snapshot has no force solver -- paper
  * p.2 Alg. 1 and the land/water environment of paper p.2 Sec. II.E):
@@ -136,29 +140,25 @@ static int run_jobs(job_t proto, int nthreads) {
   return 0;
 }
 
-/* Canonical summation order of the FP32 steps: per mass, P = the sum from
- * +0.0f of the terms of the springs whose first endpoint it is, in
- * ascending spring index; M = the same for the springs whose second
- * endpoint it is; the mass receives P - M.  Air springs contribute nothing.
- * acc must hold zeros on entry. */
+/* Canonical summation order of the FP32 steps: per mass, one sum from
+ * +0.0f over its springs in ascending spring index, + the term where it is
+ * the first endpoint, - the term where it is the second (first, then
+ * second, for a spring from a mass to itself).  Air springs contribute
+ * nothing.  acc must hold zeros on entry. */
 static void gather_terms(const orc_topology* tp, uint32_t sb, uint32_t ns, uint32_t nm,
                          const float* term, float* acc, float* accm) {
+  (void)nm;
+  (void)accm;
   for (uint32_t s = 0; s < ns; s++) {
     if (tp->mat[sb + s] == 0) continue;
-    uint32_t a = tp->pairs[2 * (sb + s)];
+    uint32_t a = tp->pairs[2 * (sb + s)], b = tp->pairs[2 * (sb + s) + 1];
     acc[3 * a] = acc[3 * a] + term[3 * s];
     acc[3 * a + 1] = acc[3 * a + 1] + term[3 * s + 1];
     acc[3 * a + 2] = acc[3 * a + 2] + term[3 * s + 2];
+    acc[3 * b] = acc[3 * b] - term[3 * s];
+    acc[3 * b + 1] = acc[3 * b + 1] - term[3 * s + 1];
+    acc[3 * b + 2] = acc[3 * b + 2] - term[3 * s + 2];
   }
-  for (uint32_t i = 0; i < 3 * nm; i++) accm[i] = 0.0f;
-  for (uint32_t s = 0; s < ns; s++) {
-    if (tp->mat[sb + s] == 0) continue;
-    uint32_t b = tp->pairs[2 * (sb + s) + 1];
-    accm[3 * b] = accm[3 * b] + term[3 * s];
-    accm[3 * b + 1] = accm[3 * b + 1] + term[3 * s + 1];
-    accm[3 * b + 2] = accm[3 * b + 2] + term[3 * s + 2];
-  }
-  for (uint32_t i = 0; i < 3 * nm; i++) acc[i] = acc[i] - accm[i];
 }
 
 /* ------------------------------------------------------------ xpbd fp32 */

@@ -185,8 +185,8 @@ def fma32(a, b, c):
 
 def numpy_xpbd_step_f32(pos, vel, pairs, lbar, mat, K, R, dt):
     """Independent float32 restatement of one XPBD step with an explicit
-    per-mass list gather (the kernel's order) instead of the oracle's
-    spring-order scatter."""
+    per-mass gather over its springs in ascending index (the kernels'
+    order) instead of the oracle's spring-order scatter."""
     f = np.float32
     xt = (pos + vel * f(dt)).astype(np.float32)
     n = len(pos)
@@ -204,15 +204,12 @@ def numpy_xpbd_step_f32(pos, vel, pairs, lbar, mat, K, R, dt):
         terms[s] = (lam * nrm).astype(np.float32)
     out_p, out_v = pos.copy(), vel.copy()
     for i in range(n):
-        P = np.zeros(3, np.float32)
-        M = np.zeros(3, np.float32)
-        for s, (a, b) in enumerate(pairs):      # first endpoint
-            if s in terms and a == i:
-                P = (P + terms[s]).astype(np.float32)
-        for s, (a, b) in enumerate(pairs):      # second endpoint
-            if s in terms and b == i:
-                M = (M + terms[s]).astype(np.float32)
-        acc = (P - M).astype(np.float32)
+        acc = np.zeros(3, np.float32)
+        for s, (a, b) in enumerate(pairs):      # ascending spring index
+            if s in terms and a == i:           # first endpoint: + term
+                acc = (acc + terms[s]).astype(np.float32)
+            if s in terms and b == i:           # second endpoint: - term
+                acc = (acc - terms[s]).astype(np.float32)
         xn = (xt[i] + acc).astype(np.float32)
         out_v[i] = ((0.99 * (xn - pos[i]).astype(np.float64)) / np.float64(f(dt))).astype(np.float32)
         out_p[i] = xn
@@ -270,15 +267,12 @@ def numpy_force_step_f32(pos, vel, mass, pairs, lbar, mat, k, R, env):
         terms[s] = (fs * d3).astype(np.float32)
     out_p, out_v = pos.copy(), vel.copy()
     for i in range(len(pos)):
-        P = np.zeros(3, np.float32)
-        M = np.zeros(3, np.float32)
-        for s, (a, b) in enumerate(pairs):      # first endpoint
-            if s in terms and a == i:
-                P = (P + terms[s]).astype(np.float32)
-        for s, (a, b) in enumerate(pairs):      # second endpoint
-            if s in terms and b == i:
-                M = (M + terms[s]).astype(np.float32)
-        F = (P - M).astype(np.float32)
+        F = np.zeros(3, np.float32)
+        for s, (a, b) in enumerate(pairs):      # ascending spring index
+            if s in terms and a == i:           # first endpoint: + term
+                F = (F + terms[s]).astype(np.float32)
+            if s in terms and b == i:           # second endpoint: - term
+                F = (F - terms[s]).astype(np.float32)
         v = vel[i].copy()
         F[1] = f(F[1] - f(mass[i] * g))
         if c != 0:
