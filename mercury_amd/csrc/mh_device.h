@@ -89,14 +89,19 @@ __device__ __forceinline__ float rcp_rn_inrange(float y) {
 constexpr float kRcpFastLo = 0x1p-64f;
 constexpr float kRcpFastHi = 0x1p+64f;
 
-// |(x, y, z)| as sqrtf((x*x + y*y) + z*z), correctly rounded sqrt.  When
-// every lane of the wave has its sum of squares in the fast range (the
-// usual case: spring lengths are far from 2^-48 and 2^63) the short
-// sequence runs; otherwise the whole wave takes the general one.  The
-// branch is wave-uniform, so no lane pays for both.
+// Sum of squares x*x + y*y + z*z as fmaf(z, z, fmaf(y, y, x * x)): three
+// operations, two of them fused.
+__device__ __forceinline__ float sum_sq3(float x, float y, float z) {
+  return fmaf(z, z, fmaf(y, y, x * x));
+}
+
+// |(x, y, z)| as sqrtf(sum_sq3), correctly rounded sqrt.  When every lane
+// of the wave has its sum of squares in the fast range (the usual case:
+// spring lengths are far from 2^-48 and 2^63) the short sequence runs;
+// otherwise the whole wave takes the general one.  The branch is
+// wave-uniform, so no lane pays for both.
 __device__ __forceinline__ float norm3(float x, float y, float z) {
-  float s = x * x + y * y;
-  s = s + z * z;
+  const float s = sum_sq3(x, y, z);
   // s is +0, positive or NaN: one unsigned compare of the bit pattern
   // tests kSqrtFastLo <= s <= kSqrtFastHi
   const bool fast = __float_as_uint(s) - __float_as_uint(kSqrtFastLo) <=
@@ -106,21 +111,26 @@ __device__ __forceinline__ float norm3(float x, float y, float z) {
   return sqrtf(s);
 }
 
-// d = |(x, y, z)| and inv = 1 / (d + eps), both correctly rounded.  For a
-// sum of squares in [2^-96, 2^128), d lies in [2^-48, 2^64) and d + eps
-// (eps <= 2^-40) in the fast reciprocal's range, so one wave-uniform test
-// covers both short sequences.  That test is the single compare
-// s >= 2^-96: NaN fails it, and +inf passes it and gives d = inv = NaN
-// where the general sequences give d = +inf, inv = 0; the force solver's
-// only use of the pair, (k (L - d)) inv, is NaN either way.
+// Sums of squares from which a spring takes the short sequences below:
+// s >= 2^-30 gives d = RN(sqrt(s)) >= 2^-15, whose half ulp is at least
+// 2^-39, so adding any eps below 2^-39 (the step's 1e-12 is 0x1.19799ap-40)
+// rounds back to d and the sum need not be formed.
+constexpr float kLenFastLo = 0x1p-30f;
+
+// d = |(x, y, z)| and inv = 1 / (d + eps) for 0 <= eps < 2^-39, both
+// correctly rounded.  For a sum of squares in [2^-30, 2^128), d lies in
+// [2^-15, 2^64), d + eps == d, and d is in the fast reciprocal's range, so
+// one wave-uniform test covers both short sequences.  That test is the
+// single compare s >= 2^-30: NaN fails it, and +inf passes it and gives
+// d = inv = NaN where the general sequences give d = +inf, inv = 0; the
+// force solver's only use of the pair, (k (L - d)) inv, is NaN either way.
 __device__ __forceinline__ void length_and_inverse(float x, float y, float z, float eps,
                                                    float& d, float& inv) {
-  float s = x * x + y * y;
-  s = s + z * z;
-  const bool fast = s >= kSqrtFastLo;
+  const float s = sum_sq3(x, y, z);
+  const bool fast = s >= kLenFastLo;
   if (__builtin_amdgcn_ballot_w64(fast) == __builtin_amdgcn_read_exec()) {
     d = sqrt_rn_inrange(s);
-    inv = rcp_rn_inrange(d + eps);
+    inv = rcp_rn_inrange(d);
   } else {
     d = sqrtf(s);
     inv = 1.0f / (d + eps);

@@ -779,18 +779,17 @@ __device__ __forceinline__ void xpbd_term_L(float4 pa, float4 pb, float L, float
   const float dx = pa.x - pb.x;
   const float dy = pa.y - pb.y;
   const float dz = pa.z - pb.z;
-  float s = dx * dx + dy * dy;
-  s = s + dz * dz;
-  const bool fast = fast_ok && __float_as_uint(s) - __float_as_uint(kSqrtFastLo) <=
-                                   __float_as_uint(kSqrtFastHi) - __float_as_uint(kSqrtFastLo);
+  const float s = sum_sq3(dx, dy, dz);
+  // in [2^-30, 2^126]: d + kEps == d (see kLenFastLo)
+  const bool fast = fast_ok && __float_as_uint(s) - __float_as_uint(kLenFastLo) <=
+                                   __float_as_uint(kSqrtFastHi) - __float_as_uint(kLenFastLo);
   float nx, ny, nz;
   if (__builtin_amdgcn_ballot_w64(fast) == __builtin_amdgcn_read_exec()) {
     const float d = sqrt_rn_inrange(s);
-    const float den = d + kEps;
-    const float y = rcp_rn_inrange(den);
-    nx = div_rn_from_rcp(dx, den, y);
-    ny = div_rn_from_rcp(dy, den, y);
-    nz = div_rn_from_rcp(dz, den, y);
+    const float y = rcp_rn_inrange(d);
+    nx = div_rn_from_rcp(dx, d, y);
+    ny = div_rn_from_rcp(dy, d, y);
+    nz = div_rn_from_rcp(dz, d, y);
     lambda = div_rn_from_rcp(-(d - L), kk, ky);
   } else {
     const float d = sqrtf(s);

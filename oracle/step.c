@@ -25,8 +25,8 @@
  * for a spring joining a mass to itself).  The term seen from the second
  * endpoint, computed from its own side, is the exact IEEE negation, since
  * a - b == -(b - a) and the length is symmetric.  |.| is
- * sqrtf((x*x + y*y) + z*z) and sin is orc_sinf (CUDA's norm3df/sinf are
- * outside the reference tree).
+ * sqrtf(fmaf(z, z, fmaf(y, y, x*x))) and sin is orc_sinf (CUDA's
+ * norm3df/sinf are outside the reference tree).
  *
  * force (project spec; the reference snapshot has no force solver -- paper
  * p.2 Alg. 1 and the land/water environment of paper p.2 Sec. II.E):
@@ -77,9 +77,7 @@ int orc_step_times(float t0, float dt, float duration, float* times,
 }
 
 static float nrm3f(float x, float y, float z) {
-  float s = x * x + y * y;
-  s = s + z * z;
-  return sqrtf(s);
+  return sqrtf(fmaf(z, z, fmaf(y, y, x * x)));
 }
 
 /* ------------------------------------------------------------ threading */

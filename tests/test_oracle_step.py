@@ -196,8 +196,8 @@ def numpy_xpbd_step_f32(pos, vel, pairs, lbar, mat, K, R, dt):
             continue
         L = fma32(lbar[s], R[mat[s]], lbar[s])
         d3 = (xt[a] - xt[b]).astype(np.float32)
-        ss = f(f(d3[0] * d3[0]) + f(d3[1] * d3[1]))
-        d = f(np.sqrt(f(ss + f(d3[2] * d3[2]))))
+        ss = fma32(d3[1], d3[1], f(d3[0] * d3[0]))
+        d = f(np.sqrt(fma32(d3[2], d3[2], ss)))
         den = f(d + f(1e-12))
         nrm = (d3 / den).astype(np.float32)
         lam = f(f(-(d - L)) / K[mat[s]])
@@ -261,8 +261,8 @@ def numpy_force_step_f32(pos, vel, mass, pairs, lbar, mat, k, R, env):
             continue
         L = fma32(lbar[s], R[mat[s]], lbar[s])
         d3 = (pos[a] - pos[b]).astype(np.float32)
-        ss = f(f(d3[0] * d3[0]) + f(d3[1] * d3[1]))
-        d = f(np.sqrt(f(ss + f(d3[2] * d3[2]))))
+        ss = fma32(d3[1], d3[1], f(d3[0] * d3[0]))
+        d = f(np.sqrt(fma32(d3[2], d3[2], ss)))
         fs = f(f(k[mat[s]] * f(L - d)) * f(f(1) / f(d + f(1e-12))))
         terms[s] = (fs * d3).astype(np.float32)
     out_p, out_v = pos.copy(), vel.copy()
