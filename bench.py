@@ -185,6 +185,49 @@ def roofline_head(bound, achieved, peak, unit, extra):
     return out
 
 
+BF16_DENSE_PEAK = 2.5e15  # MI355X dense bf16 matrix FLOP/s
+
+
+def surrogate_epoch():
+    """One training epoch of each surrogate on synthetic data: the MLP
+    64-256-256-1 on 200 000 samples (batch 512) and the convolutional
+    network on 20 000 genomes of 8^3 voxels (batch 256).  GEMM FLOPs per
+    sample: forward 2 n_in n_out per layer, the weight gradient the same,
+    the input gradient for every layer but the first."""
+    from mercury_amd import evolve, surrogate
+    rng = np.random.default_rng(0)
+    out = {}
+    sizes = [64, 256, 256, 1]
+    mlp = surrogate.MLP(sizes, seed=1)
+    n, batch = 200000, 512
+    X = rng.normal(size=(n, 64)).astype(np.float32)
+    y = rng.normal(size=(n, 1)).astype(np.float32)
+    mlp.train(X[:4096], y[:4096], epochs=1, batch=batch)  # warm-up
+    t = time.perf_counter()
+    mlp.train(X, y, epochs=1, batch=batch)
+    sec = time.perf_counter() - t
+    layer = [2 * a * b for a, b in zip(sizes[:-1], sizes[1:])]
+    flop = (n // batch) * batch * (2 * sum(layer) + sum(layer[1:]))
+    out["mlp"] = {"samples": n, "batch": batch, "seconds": sec, "gemm_flop": flop,
+                  "tflops_wall": flop / sec / 1e12, "frac_bf16_peak_wall": flop / sec / BF16_DENSE_PEAK}
+    cnn = surrogate.CNN(seed=1)
+    n, batch = 20000, 256
+    G = evolve.random_genomes(rng, n, 8)
+    t_c = rng.normal(size=n).astype(np.float32)
+    cnn.train(G[:512], t_c[:512], epochs=1, batch=batch)
+    t = time.perf_counter()
+    cnn.train(G, t_c, epochs=1, batch=batch)
+    sec = time.perf_counter() - t
+    conv = [2 * 512 * 135 * 32, 2 * 512 * 864 * 32, 2 * 64 * 864 * 64]
+    dense = [2 * 64 * 256, 2 * 256 * 256, 2 * 256 * 1]
+    per = 2 * sum(conv) + sum(conv[1:]) + 2 * sum(dense) + sum(dense)
+    flop = (n // batch) * batch * per
+    out["cnn"] = {"genomes": n, "batch": batch, "seconds": sec, "gemm_flop": flop,
+                  "tflops_wall": flop / sec / 1e12, "frac_bf16_peak_wall": flop / sec / BF16_DENSE_PEAK}
+    print(json.dumps({"surrogate_epoch": out, "dtype": "bf16 operands, f32 accumulate",
+                      "peak_flops": BF16_DENSE_PEAK}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +252,8 @@ def main():
                     help="only run the known-size copy kernel (counter calibration)")
     ap.add_argument("--calibrate-valu", action="store_true",
                     help="only run the VALU issue-rate kernels (FMA, MUL and reciprocal chains)")
+    ap.add_argument("--surrogate-epoch", action="store_true",
+                    help="only time one training epoch of each surrogate (matrix-core GEMMs)")
     args = ap.parse_args()
 
     if args.calibrate_copy:
@@ -227,6 +272,10 @@ def main():
             mh._lib.check(mh.load().mh_calibrate_valu(0, kind, ctypes.byref(g)))
             rates[name] = g.value * 1e9
         print(json.dumps({"valu_wave_instr_per_s": rates}))
+        return
+
+    if args.surrogate_epoch:
+        surrogate_epoch()
         return
 
     if args.gpus != 1:
@@ -382,6 +431,7 @@ def main():
             "substeps_per_launch": info.substeps_per_launch, "hipgraph": True,
             "lds_robots": info.n_lds_robots, "stream_robots": info.n_stream_robots,
             "robots_per_block": info.robots_per_block,
+            "stream_slices": info.stream_slices,
         },
         "active_spring_updates_per_s": n_active * n_sub * args.steps / secs,
         "roofline": {

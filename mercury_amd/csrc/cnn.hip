@@ -14,8 +14,9 @@
 // backward pass gathers column gradients back to positions in a fixed
 // order (col2im), weight gradients reduce over the batch with split-K
 // partials summed in a fixed order, so training is deterministic.  Weights
-// are bf16 with an fp32 master copy for Adam; activations and
-// accumulation are fp32.  Loss = mean 0.5 (y - target)^2.
+// are bf16 with an fp32 master copy for Adam; activations are fp32 and
+// enter every GEMM rounded to bf16 (weights, im2col columns, gradients),
+// with fp32 accumulation on the matrix cores.  Loss = mean 0.5 (y - target)^2.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
@@ -142,49 +143,29 @@ __global__ void gap_bwd_gelu(const float* df, const float* Z, int B, float* dZ) 
   dZ[t] = df[n * kC3 + c] * (1.0f / kP2) * gelu_grad(Z[t]);
 }
 
-// Weight gradient with split-K: part[s][i][j] = sum over rows r in split s
-// (ascending) of D[r][i] * X[r][j]; D is [M x Ni], X is [M x Nj].
+// Weight gradient with split-K: part[s][i][j] = sum over rows r of split s
+// of D[r][i] * X[r][j] on the matrix cores (bf16 operands, fp32 sums in a
+// fixed order); D is [M x Ni], X is [M x Nj].
 __global__ __launch_bounds__(256) void wgrad_split(const float* __restrict__ Dm, int Ni,
                                                    const float* __restrict__ X, int Nj, int M, int rows,
                                                    float* __restrict__ part) {
-  __shared__ float sd[kTk][kTile + 1];
-  __shared__ float sx[kTk][kTile + 1];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  __shared__ __align__(16) uint16_t sd[kTile][kMRow];
+  __shared__ __align__(16) uint16_t sx[kTile][kMRow];
   const int i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
   const int r_begin = blockIdx.z * rows, r_end = min(M, r_begin + rows);
-  float acc[4][4] = {};
-  for (int r0 = r_begin; r0 < r_end; r0 += kTk) {
-    for (int e = threadIdx.x; e < kTile * kTk; e += 256) {
-      const int ii = e % kTile, rr = e / kTile;
-      const int r = r0 + rr;
-      const int i = i0 + ii, j = j0 + ii;
-      sd[rr][ii] = (r < r_end && i < Ni) ? Dm[static_cast<size_t>(r) * Ni + i] : 0.f;
-      sx[rr][ii] = (r < r_end && j < Nj) ? X[static_cast<size_t>(r) * Nj + j] : 0.f;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int rr = 0; rr < kTk; ++rr) {
-      float a[4], b[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        a[q] = sd[rr][ty * 4 + q];
-        b[q] = sx[rr][tx * 4 + q];
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int w = 0; w < 4; ++w) acc[q][w] = fmaf(a[q], b[w], acc[q][w]);
-    }
-    __syncthreads();
-  }
+  f32x4_t acc[2][2];
+  // op(A)[i][r] = D[r][i], op(B)[r][j] = X[r][j]
+  mfma_tile<float, float, true, false>(Dm, Ni, X, Nj, Ni, Nj, r_begin, r_end, i0, j0, sd, sx, acc);
   float* out = part + static_cast<size_t>(blockIdx.z) * Ni * Nj;
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int w = 0; w < 4; ++w) {
-      const int i = i0 + ty * 4 + q, j = j0 + tx * 4 + w;
-      if (i < Ni && j < Nj) out[static_cast<size_t>(i) * Nj + j] = acc[q][w];
-    }
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + mfma_row(a, q), j = j0 + mfma_col(b);
+        if (i < Ni && j < Nj) out[static_cast<size_t>(i) * Nj + j] = acc[a][b][q];
+      }
 }
 
 // out[e] = sum over splits s (ascending) of part[s][e]

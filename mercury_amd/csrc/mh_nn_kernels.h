@@ -1,6 +1,6 @@
 // mh_nn_kernels.h -- device kernels shared by the surrogate networks
-// (mlp.hip, cnn.hip): a tiled FP32-accumulate GEMM with bias/GELU
-// epilogues, the squared-error loss gradient, fixed-order column sums (bias
+// (mlp.hip, cnn.hip): a matrix-core GEMM (bf16 operands, FP32 accumulate)
+// with bias/GELU epilogues, the squared-error loss gradient, fixed-order column sums (bias
 // gradients), Adam, and small helpers.  Every reduction runs in a fixed
 // order, so training is deterministic.
 //
@@ -20,7 +20,6 @@
 namespace {
 
 constexpr int kTile = 64;
-constexpr int kTk = 16;
 constexpr float kGeluC = 0.7978845608028654f;
 
 __device__ __forceinline__ float gelu_f(float x) {
@@ -43,10 +42,91 @@ __device__ __forceinline__ float ld<__hip_bfloat16>(const __hip_bfloat16* p, siz
   return __bfloat162float(p[i]);
 }
 
-// C[M x N] = op(A) op(B) with op(A)[m][k] = TA ? A[k*lda + m] : A[m*lda + k]
-// and op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n].  Epilogue EPI:
-// 0 = store; 1 = + bias[n], store Z and gelu(Z) (hidden layer forward);
-// 2 = + bias[n], store (output layer forward); 3 = multiply by
+// bf16 operands for the matrix cores: round to nearest even, as
+// __float2bfloat16 does (finite inputs).
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+template <typename T>
+__device__ __forceinline__ uint16_t ld_bf16(const T* p, size_t i);
+template <>
+__device__ __forceinline__ uint16_t ld_bf16<float>(const float* p, size_t i) { return bf16_bits(p[i]); }
+template <>
+__device__ __forceinline__ uint16_t ld_bf16<__hip_bfloat16>(const __hip_bfloat16* p, size_t i) {
+  return __builtin_bit_cast(uint16_t, p[i]);
+}
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef uint16_t u16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+// One 64 x 64 tile of op(A) op(B) over k in [k_begin, k_end) on the matrix
+// cores (v_mfma_f32_16x16x32_bf16: bf16 operands, fp32 accumulate), 256
+// threads.  op(A)[m][k] = TRA ? A[k*lda + m] : A[m*lda + k], op(B)[k][n] =
+// TRB ? B[n*ldb + k] : B[k*ldb + n]; elements outside M, N or the k range
+// are zeros.  Each step stages a 64 x 32 slice of both operands in LDS,
+// rounded to bf16 and k-contiguous per row of op(A) and per column of
+// op(B); wave w owns the 32 x 32 quarter (32 (w / 2), 32 (w % 2)) as 2 x 2
+// blocks of 16 x 16.  Lane l feeds row (or column) l % 16 of a block with
+// the 8 k values 8 (l / 16) .. 8 (l / 16) + 7 and receives rows
+// 4 (l / 16) .. 4 (l / 16) + 3 of column l % 16.  Every output sums its k
+// in the same order on every run.
+constexpr int kMk = 32;                  // k per step
+constexpr int kMRow = kMk + 8;           // LDS row pitch in bf16 (80 B, 16 B aligned)
+template <typename TA, typename TB, bool TRA, bool TRB>
+__device__ __forceinline__ void mfma_tile(const TA* __restrict__ A, int lda, const TB* __restrict__ B, int ldb,
+                                          int M, int N, int k_begin, int k_end, int m0, int n0,
+                                          uint16_t (*sa)[kMRow], uint16_t (*sb)[kMRow], f32x4_t (&acc)[2][2]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = k_begin; k0 < k_end; k0 += kMk) {
+    // 64 x 32 elements of each operand, 8 per thread, read along the
+    // operand's contiguous dimension
+    for (int e = threadIdx.x; e < kTile * kMk; e += 256) {
+      int mm, kk;
+      if (TRA) { mm = e % kTile; kk = e / kTile; } else { kk = e % kMk; mm = e / kMk; }
+      const int m = m0 + mm, k = k0 + kk;
+      uint16_t va = 0;
+      if (m < M && k < k_end)
+        va = TRA ? ld_bf16(A, static_cast<size_t>(k) * lda + m) : ld_bf16(A, static_cast<size_t>(m) * lda + k);
+      sa[mm][kk] = va;
+      int nn, kb;
+      if (TRB) { kb = e % kMk; nn = e / kMk; } else { nn = e % kTile; kb = e / kTile; }
+      const int n = n0 + nn, kq = k0 + kb;
+      uint16_t vb = 0;
+      if (n < N && kq < k_end)
+        vb = TRB ? ld_bf16(B, static_cast<size_t>(n) * ldb + kq) : ld_bf16(B, static_cast<size_t>(kq) * ldb + n);
+      sb[nn][kb] = vb;
+    }
+    __syncthreads();
+    bf16x8_t fa[2], fb[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      fa[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&sa[wm + 16 * i + (lane & 15)][8 * (lane >> 4)]));
+      fb[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&sb[wn + 16 * i + (lane & 15)][8 * (lane >> 4)]));
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+}
+
+// Output position of accumulator (i, j, r) of this lane in the tile.
+__device__ __forceinline__ int mfma_row(int i, int r) {
+  return ((threadIdx.x >> 6) >> 1) * 32 + 16 * i + 4 * ((threadIdx.x & 63) >> 4) + r;
+}
+__device__ __forceinline__ int mfma_col(int j) { return ((threadIdx.x >> 6) & 1) * 32 + 16 * j + (threadIdx.x & 15); }
+
+// C[M x N] = op(A) op(B) (mfma_tile's operands, bf16 products, fp32 sums).
+// Epilogue EPI: 0 = store; 1 = + bias[n], store Z and gelu(Z) (hidden layer
+// forward); 2 = + bias[n], store (output layer forward); 3 = multiply by
 // gelu'(Zprev[m][n]) (hidden layer backward); 4 = + bias[n], store gelu
 // (hidden layer forward without keeping Z: the same values as 1 stores in C).
 template <typename TA, typename TB, bool TRA, bool TRB, int EPI>
@@ -54,60 +134,34 @@ __global__ __launch_bounds__(256) void gemm(const TA* __restrict__ A, int lda, c
                                             int ldb, float* __restrict__ C, int ldc, int M, int N, int K,
                                             const float* __restrict__ bias, float* __restrict__ Z,
                                             const float* __restrict__ Zprev) {
-  __shared__ float sa[kTk][kTile + 1];
-  __shared__ float sb[kTk][kTile + 1];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  __shared__ __align__(16) uint16_t sa[kTile][kMRow];
+  __shared__ __align__(16) uint16_t sb[kTile][kMRow];
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
-  float acc[4][4] = {};
-  for (int k0 = 0; k0 < K; k0 += kTk) {
-    for (int e = threadIdx.x; e < kTile * kTk; e += 256) {
-      const int mm = e % kTile, kk = e / kTile;
-      const int m = m0 + mm, k = k0 + kk;
-      float va = 0.f;
-      if (m < M && k < K) va = TRA ? ld(A, static_cast<size_t>(k) * lda + m) : ld(A, static_cast<size_t>(m) * lda + k);
-      sa[kk][mm] = va;
-      const int n = n0 + mm;
-      float vb = 0.f;
-      if (n < N && k < K) vb = TRB ? ld(B, static_cast<size_t>(n) * ldb + k) : ld(B, static_cast<size_t>(k) * ldb + n);
-      sb[kk][mm] = vb;
-    }
-    __syncthreads();
+  f32x4_t acc[2][2];
+  mfma_tile<TA, TB, TRA, TRB>(A, lda, B, ldb, M, N, 0, K, m0, n0, sa, sb, acc);
 #pragma unroll
-    for (int kk = 0; kk < kTk; ++kk) {
-      float a[4], b[4];
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        a[i] = sa[kk][ty * 4 + i];
-        b[i] = sb[kk][tx * 4 + i];
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + mfma_row(i, r), n = n0 + mfma_col(j);
+        if (m >= M || n >= N) continue;
+        const size_t o = static_cast<size_t>(m) * ldc + n;
+        float v = acc[i][j][r];
+        if (EPI == 1) {
+          v += bias[n];
+          Z[o] = v;
+          v = gelu_f(v);
+        } else if (EPI == 2) {
+          v += bias[n];
+        } else if (EPI == 3) {
+          v *= gelu_grad(Zprev[o]);
+        } else if (EPI == 4) {
+          v = gelu_f(v + bias[n]);
+        }
+        C[o] = v;
       }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int m = m0 + ty * 4 + i, n = n0 + tx * 4 + j;
-      if (m >= M || n >= N) continue;
-      const size_t o = static_cast<size_t>(m) * ldc + n;
-      float v = acc[i][j];
-      if (EPI == 1) {
-        v += bias[n];
-        Z[o] = v;
-        v = gelu_f(v);
-      } else if (EPI == 2) {
-        v += bias[n];
-      } else if (EPI == 3) {
-        v *= gelu_grad(Zprev[o]);
-      } else if (EPI == 4) {
-        v = gelu_f(v + bias[n]);
-      }
-      C[o] = v;
-    }
 }
 
 // dY = (Y - T) / B, per-row loss terms 0.5 (Y - T)^2

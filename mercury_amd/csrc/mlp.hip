@@ -1,17 +1,19 @@
 // mlp.hip -- the surrogate network: fully connected layers with bf16
-// weights (fp32 master copy for Adam), fp32 activations and fp32
-// accumulation, trained on the device by hand-written forward, backward and
-// Adam kernels.
+// weights (fp32 master copy for Adam) and fp32 activations, trained on the
+// device by hand-written forward, backward and Adam kernels.  Every GEMM
+// runs on the matrix cores with bf16 operands (activations and gradients
+// rounded to nearest even as they enter it) and fp32 accumulation.
 //
 // Layer l: Z_l = A_{l-1} W_l^T + b_l, A_l = gelu(Z_l) on hidden layers,
 // A_L = Z_L.  gelu(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
 // Loss = mean over the batch of 0.5 (A_L - target)^2.  Adam follows Kingma
 // & Ba (2015), Algorithm 1 (bias-corrected moments).
 //
-// GEMMs: 64x64 output tiles, 256 threads, 4x4 outputs per thread, K in
-// steps of 16 staged through LDS.  Column sums (bias gradients) run one
-// lane per column over the rows, so every reduction has a fixed order and
-// training is deterministic.
+// GEMMs: 64x64 output tiles, 256 threads (four waves of 2x2 16x16 MFMA
+// blocks), K in steps of 32 staged through LDS as bf16.  Column sums (bias
+// gradients, from the fp32 gradients) run one lane per column over the
+// rows, so every reduction has a fixed order and training is
+// deterministic.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 

@@ -263,17 +263,19 @@ def _mlp_sig(L):
         return
     dpp = ctypes.POINTER(f64p)
     L.orc_mlp_forward.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), dpp, dpp,
-                                  ctypes.c_int, f64p, f64p]
+                                  ctypes.c_int, f64p, f64p, ctypes.c_int]
     L.orc_mlp_loss_grad.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), dpp, dpp,
-                                    ctypes.c_int, f64p, f64p, dpp, dpp]
+                                    ctypes.c_int, f64p, f64p, dpp, dpp, ctypes.c_int]
     L.orc_mlp_loss_grad.restype = ctypes.c_double
     L.orc_adam.argtypes = [ctypes.c_int, f64p, f64p, f64p, f64p, ctypes.c_double,
                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
     L._mlp_sig = True
 
 
-def mlp_forward(W, b, x):
-    """W: list of [out x in] arrays, b: list of [out]; GELU hidden layers."""
+def mlp_forward(W, b, x, bf16_operands=False):
+    """W: list of [out x in] arrays, b: list of [out]; GELU hidden layers.
+    bf16_operands rounds the activations entering each product to bf16 (via
+    float32, nearest even), as the device's matrix-core GEMMs do."""
     L = lib()
     _mlp_sig(L)
     W = [np.ascontiguousarray(w, np.float64) for w in W]
@@ -282,11 +284,13 @@ def mlp_forward(W, b, x):
     x = np.ascontiguousarray(x, np.float64)
     y = np.zeros((x.shape[0], W[-1].shape[0]), np.float64)
     L.orc_mlp_forward(len(W), sizes, _dpp(W), _dpp(b), x.shape[0], x.ctypes.data_as(f64p),
-                      y.ctypes.data_as(f64p))
+                      y.ctypes.data_as(f64p), int(bool(bf16_operands)))
     return y
 
 
-def mlp_loss_grad(W, b, x, target):
+def mlp_loss_grad(W, b, x, target, bf16_operands=False):
+    """Loss and gradients; bf16_operands also rounds the back-propagated
+    gradients entering each product (bias gradients sum unrounded ones)."""
     L = lib()
     _mlp_sig(L)
     W = [np.ascontiguousarray(w, np.float64) for w in W]
@@ -297,7 +301,8 @@ def mlp_loss_grad(W, b, x, target):
     gW = [np.zeros_like(w) for w in W]
     gb = [np.zeros_like(v) for v in b]
     loss = L.orc_mlp_loss_grad(len(W), sizes, _dpp(W), _dpp(b), x.shape[0],
-                               x.ctypes.data_as(f64p), t.ctypes.data_as(f64p), _dpp(gW), _dpp(gb))
+                               x.ctypes.data_as(f64p), t.ctypes.data_as(f64p), _dpp(gW), _dpp(gb),
+                               int(bool(bf16_operands)))
     return loss, gW, gb
 
 

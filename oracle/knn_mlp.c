@@ -15,10 +15,17 @@
  * MLP: y = W_L g(... g(W_1 x + b_1) ...) + b_L with g = GELU, tanh form
  * g(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).  Loss is
  * mean_b 0.5 (y_b - target_b)^2.  Adam is Kingma & Ba (2015) Algorithm 1
- * with bias correction.
+ * with bias correction.  With bf16_operands set, every product operand
+ * that is an activation or a back-propagated gradient is rounded as the
+ * device's matrix-core GEMMs round it (to float, then to bf16, nearest
+ * even): the layer inputs in W a, both factors of the weight gradient
+ * delta a^T, and delta in W^T delta; bias gradients sum the unrounded
+ * deltas.  The weights are taken as given (the caller passes bf16 values).
  * Pinned by: central finite differences of the loss, a one-layer closed
  * form (linear regression gradient), and Adam's first step equal to
- * -lr * sign(g) (tests/test_oracle_knn_mlp.py).
+ * -lr * sign(g) (tests/test_oracle_knn_mlp.py); the bf16_operands mode by
+ * a numpy restatement with its own rounding, which differs from the
+ * unrounded network by far more than the two disagree.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -71,36 +78,56 @@ static double gelu_d(double x) {
          0.5 * x * (1.0 - th * th) * kGeluC * (1.0 + 3.0 * 0.044715 * x * x);
 }
 
+/* v rounded to float, then to bf16 (round to nearest even), as a double */
+static double bf16r(double v) {
+  float f = (float)v;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u = ((u + 0x7fffu + ((u >> 16) & 1u)) >> 16) << 16;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+/* a copy of n values, rounded by bf16r when rb is set */
+static double* operand(const double* v, size_t n, int rb) {
+  double* o = (double*)malloc(sizeof(double) * (n ? n : 1));
+  for (size_t i = 0; i < n; i++) o[i] = rb ? bf16r(v[i]) : v[i];
+  return o;
+}
+
 /* activations buffer: pre-activations z_l and outputs a_l for every layer */
 static void forward_all(int nl, const int* sz, const double* const* W,
                         const double* const* b, int B, const double* x,
-                        double** z, double** a) {
+                        double** z, double** a, int rb) {
   const double* in = x;
   for (int l = 0; l < nl; l++) {
     int ni = sz[l], no = sz[l + 1];
+    double* op = operand(in, (size_t)B * ni, rb);
     for (int r = 0; r < B; r++)
       for (int o = 0; o < no; o++) {
         double s = b[l][o];
         const double* w = W[l] + (size_t)o * ni;
-        const double* xi = in + (size_t)r * ni;
+        const double* xi = op + (size_t)r * ni;
         for (int i = 0; i < ni; i++) s += w[i] * xi[i];
         z[l][(size_t)r * no + o] = s;
         a[l][(size_t)r * no + o] = (l < nl - 1) ? gelu(s) : s;
       }
+    free(op);
     in = a[l];
   }
 }
 
 int orc_mlp_forward(int n_layers, const int* sizes, const double* const* W,
                     const double* const* b, int batch, const double* x,
-                    double* y) {
+                    double* y, int bf16_operands) {
+  if (n_layers < 1 || batch < 0) return -1;
   double** z = (double**)malloc(sizeof(double*) * (size_t)n_layers);
   double** a = (double**)malloc(sizeof(double*) * (size_t)n_layers);
   for (int l = 0; l < n_layers; l++) {
     z[l] = (double*)malloc(sizeof(double) * (size_t)batch * sizes[l + 1]);
     a[l] = (double*)malloc(sizeof(double) * (size_t)batch * sizes[l + 1]);
   }
-  forward_all(n_layers, sizes, W, b, batch, x, z, a);
+  forward_all(n_layers, sizes, W, b, batch, x, z, a, bf16_operands);
   memcpy(y, a[n_layers - 1], sizeof(double) * (size_t)batch * sizes[n_layers]);
   for (int l = 0; l < n_layers; l++) { free(z[l]); free(a[l]); }
   free(z); free(a);
@@ -110,7 +137,7 @@ int orc_mlp_forward(int n_layers, const int* sizes, const double* const* W,
 double orc_mlp_loss_grad(int n_layers, const int* sizes,
                          const double* const* W, const double* const* b,
                          int batch, const double* x, const double* target,
-                         double* const* gW, double* const* gb) {
+                         double* const* gW, double* const* gb, int bf16_operands) {
   int nl = n_layers;
   double** z = (double**)malloc(sizeof(double*) * (size_t)nl);
   double** a = (double**)malloc(sizeof(double*) * (size_t)nl);
@@ -118,7 +145,7 @@ double orc_mlp_loss_grad(int n_layers, const int* sizes,
     z[l] = (double*)malloc(sizeof(double) * (size_t)batch * sizes[l + 1]);
     a[l] = (double*)malloc(sizeof(double) * (size_t)batch * sizes[l + 1]);
   }
-  forward_all(nl, sizes, W, b, batch, x, z, a);
+  forward_all(nl, sizes, W, b, batch, x, z, a, bf16_operands);
   int nout = sizes[nl];
   double loss = 0.0;
   /* delta of the last layer: dL/dz = (y - t) / B */
@@ -132,26 +159,29 @@ double orc_mlp_loss_grad(int n_layers, const int* sizes,
   loss /= batch;
   for (int l = nl - 1; l >= 0; l--) {
     int ni = sizes[l], no = sizes[l + 1];
-    const double* in = (l == 0) ? x : a[l - 1];
+    double* in = operand((l == 0) ? x : a[l - 1], (size_t)batch * ni, bf16_operands);
+    double* dr = operand(delta, (size_t)batch * no, bf16_operands);
     memset(gW[l], 0, sizeof(double) * (size_t)ni * no);
     memset(gb[l], 0, sizeof(double) * (size_t)no);
     for (int r = 0; r < batch; r++)
       for (int o = 0; o < no; o++) {
-        double d = delta[(size_t)r * no + o];
-        gb[l][o] += d;
+        gb[l][o] += delta[(size_t)r * no + o];
+        double d = dr[(size_t)r * no + o];
         for (int i = 0; i < ni; i++) gW[l][(size_t)o * ni + i] += d * in[(size_t)r * ni + i];
       }
+    free(in);
     if (l > 0) {
       double* nd = (double*)malloc(sizeof(double) * (size_t)batch * ni);
       for (int r = 0; r < batch; r++)
         for (int i = 0; i < ni; i++) {
           double s = 0.0;
-          for (int o = 0; o < no; o++) s += W[l][(size_t)o * ni + i] * delta[(size_t)r * no + o];
+          for (int o = 0; o < no; o++) s += W[l][(size_t)o * ni + i] * dr[(size_t)r * no + o];
           nd[(size_t)r * ni + i] = s * gelu_d(z[l - 1][(size_t)r * ni + i]);
         }
       free(delta);
       delta = nd;
     }
+    free(dr);
   }
   free(delta);
   for (int l = 0; l < nl; l++) { free(z[l]); free(a[l]); }

@@ -130,15 +130,17 @@ int orc_knn(const float* pos, int n, int K, uint16_t* nbr, float* dist);
 /* ------------------------------------------------------------- surrogate */
 /* Dense MLP, row-major weights W_l [out x in], biases b_l [out].
  * act: 0 = GELU (tanh form) on hidden layers, linear output.
- * x [batch x in0] -> y [batch x out_last].  Double precision. */
+ * x [batch x in0] -> y [batch x out_last].  Double precision; with
+ * bf16_operands the activations and gradients entering each product are
+ * rounded to bf16 (via float, nearest even), as the device GEMMs do. */
 int orc_mlp_forward(int n_layers, const int* sizes, const double* const* W,
                     const double* const* b, int batch, const double* x,
-                    double* y);
+                    double* y, int bf16_operands);
 /* Loss = mean over batch of 0.5*(y - target)^2; gradients written to gW/gb. */
 double orc_mlp_loss_grad(int n_layers, const int* sizes,
                          const double* const* W, const double* const* b,
                          int batch, const double* x, const double* target,
-                         double* const* gW, double* const* gb);
+                         double* const* gW, double* const* gb, int bf16_operands);
 /* One Adam step (Kingma & Ba 2015, Algorithm 1) on n parameters. */
 void orc_adam(int n, double* p, const double* g, double* m, double* v,
               double lr, double beta1, double beta2, double eps, int t);

@@ -90,6 +90,54 @@ def test_mlp_one_layer_is_linear_regression():
     assert np.allclose(gb[0], e.mean(0))
 
 
+def _bf16(v):
+    """float64 -> float32 -> bf16 (nearest even), back to float64."""
+    u = np.ascontiguousarray(v, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def test_mlp_bf16_operands_mode_matches_numpy_restatement():
+    """bf16_operands rounds the activations entering W a, both factors of
+    the weight gradient and the delta entering W^T delta; bias gradients sum
+    unrounded deltas.  Restated here with numpy matrix products; the
+    rounding changes the results by far more than the two disagree, so a
+    dropped or misplaced rounding fails."""
+    gelu = lambda z: 0.5 * z * (1 + np.tanh(math.sqrt(2 / math.pi) * (z + 0.044715 * z ** 3)))
+
+    def dgelu(z):
+        th = np.tanh(math.sqrt(2 / math.pi) * (z + 0.044715 * z ** 3))
+        return 0.5 * (1 + th) + 0.5 * z * (1 - th * th) * math.sqrt(2 / math.pi) * (1 + 3 * 0.044715 * z * z)
+
+    rng = np.random.default_rng(8)
+    W, b = make_net(rng, [12, 20, 16, 1])
+    W = [_bf16(w) for w in W]
+    x = rng.normal(size=(30, 12))
+    t = rng.normal(size=(30, 1))
+    a, Z, A = x, [], [x]
+    for l in range(3):
+        z = _bf16(a) @ W[l].T + b[l]
+        Z.append(z)
+        a = gelu(z) if l < 2 else z
+        A.append(a)
+    y = oracle.mlp_forward(W, b, x, bf16_operands=True)
+    assert np.allclose(y, A[-1], rtol=1e-12, atol=1e-12)
+    assert np.abs(y - oracle.mlp_forward(W, b, x)).max() > 1e-4
+    d = (A[-1] - t) / len(x)
+    want = []
+    for l in (2, 1, 0):
+        want = [(_bf16(d).T @ _bf16(A[l])), d.sum(0)] + want
+        if l > 0:
+            d = (_bf16(d) @ W[l]) * dgelu(Z[l - 1])
+    loss, gW, gb = oracle.mlp_loss_grad(W, b, x, t, bf16_operands=True)
+    assert loss == pytest.approx(0.5 * np.mean((A[-1] - t) ** 2), rel=1e-12)
+    for l in range(3):
+        assert np.allclose(gW[l], want[2 * l], rtol=1e-10, atol=1e-14)
+        assert np.allclose(gb[l], want[2 * l + 1], rtol=1e-10, atol=1e-14)
+    _, gW0, _ = oracle.mlp_loss_grad(W, b, x, t)
+    assert max(np.abs(gW0[l] - gW[l]).max() for l in range(3)) > 1e-6
+
+
 def test_gelu_hidden_layer_values():
     W = [np.array([[1.0]]), np.array([[1.0]])]
     b = [np.zeros(1), np.zeros(1)]

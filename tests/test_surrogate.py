@@ -66,6 +66,15 @@ def oracle_layers(net, p):
 
 @pytest.mark.gpu
 def test_mlp_forward_and_gradient_match_fp64_oracle():
+    """The matrix-core GEMMs take bf16 operands: the oracle rounds the same
+    activations and gradients (bf16_operands) and computes in float64.  What
+    remains is fp32 against fp64 arithmetic, plus the rare operand whose
+    fp32 and fp64 values round to neighbouring bf16 values.  In a numpy
+    emulation of this network (seeds 0-5) the 90th percentile of the
+    relative error was at most 6e-8 (forward) and 6e-7 (gradient), the
+    maximum 4.4e-4 and 1.6e-4.  Against an oracle that skips the rounding
+    the 90th percentiles are at least 1.6e-3 and 3e-5.  So the bounds are
+    p90 <= 1e-6 / 5e-6, max <= 2e-3 / 1e-3."""
     print("mlp fwd/bwd vs oracle", flush=True)
     rng = np.random.default_rng(1)
     net = surrogate.MLP([64, 256, 256, 1], seed=7)
@@ -73,14 +82,17 @@ def test_mlp_forward_and_gradient_match_fp64_oracle():
     t = rng.normal(size=(300, 1)).astype(np.float32)
     W, b = oracle_layers(net, net.get_params())
     y = net.forward(x)
-    yo = oracle.mlp_forward(W, b, x.astype(np.float64))
-    assert np.abs(y - yo).max() <= 1e-4 * max(1.0, np.abs(yo).max())
+    yo = oracle.mlp_forward(W, b, x.astype(np.float64), bf16_operands=True)
+    e = np.abs(y - yo) / max(1.0, np.abs(yo).max())
+    print("forward relative error p90 %.2e max %.2e" % (np.percentile(e, 90), e.max()), flush=True)
+    assert np.percentile(e, 90) <= 1e-6 and e.max() <= 2e-3
     loss, g = net.loss_grad(x, t)
-    lo, gW, gb = oracle.mlp_loss_grad(W, b, x.astype(np.float64), t.astype(np.float64))
+    lo, gW, gb = oracle.mlp_loss_grad(W, b, x.astype(np.float64), t.astype(np.float64), bf16_operands=True)
     assert loss == pytest.approx(lo, rel=1e-4)
     go = np.concatenate([np.concatenate([w.reshape(-1), v]) for w, v in zip(gW, gb)])
-    scale = np.abs(go).max()
-    assert np.abs(g - go).max() <= 1e-4 * scale
+    e = np.abs(g - go) / np.abs(go).max()
+    print("gradient relative error p90 %.2e max %.2e" % (np.percentile(e, 90), e.max()), flush=True)
+    assert np.percentile(e, 90) <= 5e-6 and e.max() <= 1e-3
     # a dropped GELU derivative or a transposed operand would be O(scale) off
     assert np.corrcoef(g, go)[0, 1] > 0.99999
 
