@@ -150,9 +150,6 @@ typedef struct {
   int substeps_per_launch; /* 0 = tuned (one launch per call until
                           mh_sim_autotune has chosen) */
   int use_graph;       /* capture each simulate's launches in a hipGraph */
-  int stream_slices;   /* 0 = tuned (1 until mh_sim_autotune has chosen);
-                          else the streaming robots run in this many
-                          slices, one after the other */
 } mh_config;
 
 /* Reference defaults: XPBD, dt = 1e-4 (Simulator.h:55), the Land
@@ -251,14 +248,10 @@ typedef struct {
                             simulate call replays the last one when its
                             step count, stress flag, substeps per launch and
                             plan are unchanged, whatever the clock) */
-  int stream_slices;     /* the streaming robots run in this many slices,
-                            one after the other, each for every substep
-                            of a call (1 = all at once) */
 } mh_sim_info;
 int mh_sim_get_info(const mh_sim* sim, mh_sim_info* info);
 /* Times the candidate launch plans (LDS variant x robots per block, then
- * streaming variant, then slices of the streaming robots: 1, 2, 4 or 8, then
- * substeps per launch: all of a call in one launch,
+ * streaming variant, then substeps per launch: all of a call in one launch,
  * or 2500, 1000 or 250) on the current population for n_steps each and
  * keeps the fastest; results do not depend on the choice.  The choice is
  * cached per (solver, device, the config's kernel, block, robots-per-block,

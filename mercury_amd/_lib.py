@@ -37,7 +37,7 @@ class Config(ctypes.Structure):
                 ("ground_k", c_float), ("mu_s", c_float), ("mu_k", c_float),
                 ("drag", c_float), ("damping", c_float), ("device", c_int),
                 ("kernel", c_int), ("block_size", c_int), ("robots_per_block", c_int),
-                ("substeps_per_launch", c_int), ("use_graph", c_int), ("stream_slices", c_int)]
+                ("substeps_per_launch", c_int), ("use_graph", c_int)]
 
 
 class Population(ctypes.Structure):
@@ -52,7 +52,7 @@ class SimInfo(ctypes.Structure):
         "n_robots", "n_masses", "n_springs", "n_active_springs", "kernel", "block_size",
         "n_blocks", "substeps_per_launch", "lds_bytes", "max_masses_per_block",
         "max_springs_per_block", "n_lds_robots", "n_stream_robots", "stream_block_size",
-        "robots_per_block", "n_stream_springs", "stream_chunk", "graph_builds", "stream_slices")]
+        "robots_per_block", "n_stream_springs", "stream_chunk", "graph_builds")]
 
 
 # name -> (restype, argtypes)

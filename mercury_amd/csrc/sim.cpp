@@ -98,12 +98,6 @@ struct Part {
   size_t lds = 0;
   uint32_t pack_offset = 0;         // first descriptor in the device array
   int n_robots = 0;
-  // The part's packs run in slices, one after the other, each for every
-  // substep of a call: a slice's spring data can stay in the last-level
-  // cache (256 MB Infinity Cache) across substeps where the whole part's
-  // would not.  Robots are independent, so results do not depend on it.
-  int slices = 1;
-  std::vector<uint32_t> slice_begin;  // slices + 1 bounds into packs
 };
 
 // Robots that fit the LDS kernel go to the LDS part, the rest to the
@@ -303,8 +297,7 @@ int chunk_springs(const mh_sim* s, const KernelVariant& v, int m, int n_mat) {
 // LDS variant (at most rpb robots each); every other robot gets a
 // workgroup of the streaming variant.  Returns false if a robot fits
 // neither kernel.
-bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv, int rpb, Plan& plan,
-               int slices = 1) {
+bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv, int rpb, Plan& plan) {
   plan = Plan{};
   plan.rpb = rpb;
   const int n_mat = static_cast<int>(s->compact_to_id.size());
@@ -403,7 +396,6 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
     } else {
       stp.lds = lds_bytes_stream(std::max(stp.mass_cap, 1), n_mat);
     }
-    stp.slices = std::max(1, std::min(slices, static_cast<int>(stp.packs.size())));
     plan.parts.push_back(std::move(stp));
   }
   return true;
@@ -620,20 +612,6 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
   });
 }
 
-// Deals a part's packs (most work first) round-robin into its slices, so
-// every slice has a similar mix of sizes and stays in most-work-first order.
-void slice_packs(Part& part) {
-  const size_t n = part.packs.size(), S = static_cast<size_t>(std::max(1, part.slices));
-  std::vector<PackDesc> dealt;
-  dealt.reserve(n);
-  part.slice_begin.assign(1, 0u);
-  for (size_t sl = 0; sl < S; sl++) {
-    for (size_t k = sl; k < n; k += S) dealt.push_back(part.packs[k]);
-    part.slice_begin.push_back(static_cast<uint32_t>(dealt.size()));
-  }
-  part.packs.swap(dealt);
-}
-
 // Device arrays that depend on the packing: pack-local spring pairs, the
 // gather lists, row bounds and phase-2 order.  Mass i's row is two parts:
 // the springs whose first endpoint it is, then those whose second endpoint
@@ -714,7 +692,6 @@ void upload_plan(mh_sim* s, Plan& plan) {
         return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
       });
     }
-    slice_packs(part);
     part.pack_offset = static_cast<uint32_t>(all.size());
     all.insert(all.end(), part.packs.begin(), part.packs.end());
     if (part.lds > 64 * 1024) {
@@ -775,9 +752,8 @@ void choose_default_plan(mh_sim* s) {
     }
   }
   Plan p;
-  const int sl = std::max(1, s->cfg.stream_slices);
-  bool ok = make_plan(s, best, sv, s->cfg.robots_per_block, p, sl);
-  for (size_t i = 1; !ok && i < sc.size(); i++) ok = make_plan(s, best, sc[i], s->cfg.robots_per_block, p, sl);
+  bool ok = make_plan(s, best, sv, s->cfg.robots_per_block, p);
+  for (size_t i = 1; !ok && i < sc.size(); i++) ok = make_plan(s, best, sc[i], s->cfg.robots_per_block, p);
   if (!ok) throw Error(MH_ERR_RANGE, "no step kernel fits this population (robot too large for LDS)");
   upload_plan(s, p);
 }
@@ -845,17 +821,13 @@ int issue(mh_sim* s, int n_steps, int track) {
   for (size_t i = 0; i < np; i++) {
     const Part& part = s->plan.parts[i];
     hipStream_t st = i == 0 ? s->stream : s->aux;
-    for (size_t sl = 0; sl + 1 < part.slice_begin.size(); sl++) {
-      const uint32_t b0 = part.slice_begin[sl], b1 = part.slice_begin[sl + 1];
-      launches = 0;
-      for (int done = 0; done < n_steps;) {
-        const int n = std::min(K, n_steps - done);
-        StepParams p = make_params(s, part, launches, n, track);
-        p.packs += b0;
-        hip_check(launch_step(*part.v, static_cast<int>(b1 - b0), part.lds, st, p), "step launch");
-        done += n;
-        launches++;
-      }
+    launches = 0;
+    for (int done = 0; done < n_steps;) {
+      const int n = std::min(K, n_steps - done);
+      StepParams p = make_params(s, part, launches, n, track);
+      hip_check(launch_step(*part.v, static_cast<int>(part.packs.size()), part.lds, st, p), "step launch");
+      done += n;
+      launches++;
     }
   }
   if (np > 1) {
@@ -973,7 +945,6 @@ void mh_config_default(mh_config* c) {
   c->robots_per_block = 0;
   c->substeps_per_launch = 0;
   c->use_graph = 1;
-  c->stream_slices = 0;
 }
 
 int mh_sim_create(const mh_config* cfg, const mh_material_set* set, mh_sim** out) {
@@ -987,7 +958,6 @@ int mh_sim_create(const mh_config* cfg, const mh_material_set* set, mh_sim** out
     throw mh::Error(MH_ERR_ARG, "block_size must be 0, 256, 512, 768 or 1024");
   if (cfg->kernel < MH_KERNEL_AUTO || cfg->kernel > mh::kAllFamilies)
     throw mh::Error(MH_ERR_ARG, "unknown kernel kind");
-  if (cfg->stream_slices < 0) throw mh::Error(MH_ERR_ARG, "stream_slices must be >= 0");
   int ndev = 0;
   mh::hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
   if (cfg->device < 0 || cfg->device >= ndev) throw mh::Error(MH_ERR_HIP, "no such HIP device");
@@ -1445,7 +1415,6 @@ int mh_sim_set_voxel_population(mh_sim* s, int n_robots, int nx, int ny, int nz,
     part.spring_cap = static_cast<int>(*std::max_element(active.begin(), active.end()));
     part.lds = mh::lds_bytes_mass(v->block, s->composite_count);
     part.n_robots = R;
-    part.slice_begin = {0u, static_cast<uint32_t>(R)};
     if (part.lds > 64 * 1024)
       for (const void* fn : {v->fn, v->fn_stress})
         if (fn) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(part.lds));
@@ -1669,7 +1638,6 @@ int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
       info->n_stream_robots = part.n_robots;
       for (const mh::PackDesc& d : part.packs) info->n_stream_springs += static_cast<int>(d.n_spring);
       info->stream_chunk = part.v->chunked ? part.spring_cap : 0;
-      info->stream_slices = part.slices;
     }
   }
   info->robots_per_block = s->plan.rpb;
@@ -1685,11 +1653,10 @@ namespace {
 // Tuning results by (solver, device, what the config allows, population
 // shape); the chosen plan never changes results, only speed.
 struct TuneKey {
-  int solver, device, kernel, block_size, robots_per_block, substeps, use_graph, lds_limit, slices;
+  int solver, device, kernel, block_size, robots_per_block, substeps, use_graph, lds_limit;
   std::vector<uint64_t> shape;
   auto tie() const {
-    return std::tie(solver, device, kernel, block_size, robots_per_block, substeps, use_graph, lds_limit, slices,
-                    shape);
+    return std::tie(solver, device, kernel, block_size, robots_per_block, substeps, use_graph, lds_limit, shape);
   }
   bool operator<(const TuneKey& o) const { return tie() < o.tie(); }
 };
@@ -1698,7 +1665,6 @@ struct TuneChoice {
   const KernelVariant* stream;
   int rpb;
   int substeps;  // 0 = every substep of a call in one launch
-  int slices;    // of the streaming part
 };
 std::map<TuneKey, TuneChoice>& tune_cache() {
   static std::map<TuneKey, TuneChoice> c;
@@ -1733,7 +1699,7 @@ TuneKey tune_key(const mh_sim* s) {
   const uint64_t n = static_cast<uint64_t>(std::max(s->n_robots, 1));
   const mh_config& c = s->cfg;
   return TuneKey{c.solver, c.device, c.kernel, c.block_size, c.robots_per_block, c.substeps_per_launch,
-                 c.use_graph, s->lds_limit, c.stream_slices,
+                 c.use_graph, s->lds_limit,
                  {bucket(n), bucket(sm / n), bucket(sa / n), bucket(mm), bucket(ma), side, tables}};
 }
 
@@ -1747,8 +1713,7 @@ bool apply_choice(mh_sim* s, const TuneChoice& c) {
   };
   if (!allowed(c.lds, kResident) || !allowed(c.stream, MH_KERNEL_STREAM)) return false;
   Plan p;
-  if (!make_plan(s, c.lds, c.stream, c.rpb, p, s->cfg.stream_slices > 0 ? s->cfg.stream_slices : c.slices))
-    return false;
+  if (!make_plan(s, c.lds, c.stream, c.rpb, p)) return false;
   upload_plan(s, p);
   s->substeps = s->cfg.substeps_per_launch > 0 ? s->cfg.substeps_per_launch : c.substeps;
   return true;
@@ -1885,7 +1850,7 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   std::vector<int> rpbs = {s->cfg.robots_per_block};
   if (s->cfg.robots_per_block == 0) rpbs = {0, 1, 2, 4};
   double best = std::numeric_limits<double>::infinity();
-  mh::TuneChoice choice{nullptr, nullptr, 0, 0, 1};
+  mh::TuneChoice choice{nullptr, nullptr, 0, 0};
   // coordinate search: LDS variant x robots per block, then streaming
   // variant, then substeps per launch
   for (const mh::KernelVariant* lv : lc) {
@@ -1900,7 +1865,7 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
       const double t = mh::time_plan(s, p, n_steps);
       if (t < best) {
         best = t;
-        choice = {lv, sv0, rpb, 0, 1};
+        choice = {lv, sv0, rpb, 0};
       }
     }
   }
@@ -1918,29 +1883,8 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
     }
   }
   if (!std::isfinite(best)) throw mh::Error(MH_ERR_RANGE, "no step kernel fits this population");
-  // slices of the streaming part (a plan without one has nothing to slice)
-  {
-    mh::Plan p;
-    mh::make_plan(s, choice.lds, choice.stream, choice.rpb, p);
-    size_t stream_packs = 0;
-    for (auto& part : p.parts)
-      if (part.v->kind == MH_KERNEL_STREAM) stream_packs = part.packs.size();
-    for (int sl : {2, 4, 8}) {
-      if (s->cfg.stream_slices > 0) {
-        choice.slices = s->cfg.stream_slices;
-        break;
-      }
-      if (stream_packs < static_cast<size_t>(64 * sl)) break;
-      if (!mh::make_plan(s, choice.lds, choice.stream, choice.rpb, p, sl)) break;
-      const double t = mh::time_plan(s, p, n_steps);
-      if (t < best * 0.99) {
-        best = t;
-        choice.slices = sl;
-      }
-    }
-  }
   mh::Plan p;
-  mh::make_plan(s, choice.lds, choice.stream, choice.rpb, p, choice.slices);
+  mh::make_plan(s, choice.lds, choice.stream, choice.rpb, p);
   mh::upload_plan(s, p);
   // substeps per launch, timed with the caller's graph setting: one launch
   // per call, or launches of K substeps (a caller's synchronize waits for

@@ -33,7 +33,6 @@ class SimConfig:
     robots_per_block: int = 0
     substeps_per_launch: int = 0
     use_graph: int = 1
-    stream_slices: int = 0
 
     def to_c(self):
         c = Config()
@@ -42,7 +41,7 @@ class SimConfig:
         c.mu_s, c.mu_k, c.drag, c.damping = self.mu_s, self.mu_k, self.drag, self.damping
         c.device, c.kernel, c.block_size = self.device, self.kernel, self.block_size
         c.robots_per_block, c.substeps_per_launch = self.robots_per_block, self.substeps_per_launch
-        c.use_graph, c.stream_slices = self.use_graph, self.stream_slices
+        c.use_graph = self.use_graph
         return c
 
 

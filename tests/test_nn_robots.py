@@ -86,12 +86,6 @@ def test_nn_population_mixed_plan_bitexact():
     sim2.autotune(20)
     sim2.Step(200)
     assert np.array_equal(bits(sim2.collect_arrays()[0]), bits(p))
-    # so do the streamed robots run in slices, one after the other
-    sim3 = gpu_sim(case, mh.MH_SOLVER_FORCE, 1e-4, WATER, stream_slices=2)
-    assert sim3.info().stream_slices == 2
-    sim3.Step(200)
-    assert np.array_equal(bits(sim3.collect_arrays()[0]), bits(p))
-    assert np.array_equal(bits(sim3.collect_arrays()[1]), bits(v))
 
 
 @pytest.mark.gpu
