@@ -10,10 +10,12 @@
 //   mean over the 4^3 positions -> 64 features
 //   dense  64 -> 256 GELU -> 256 GELU -> 1
 // Convolutions run as GEMMs over im2col rows (one row per output position,
-// column k*C + c for kernel offset k = (dz+1)*9 + (dy+1)*3 + (dx+1)); the
-// backward pass gathers column gradients back to positions in a fixed
-// order (col2im), weight gradients reduce over the batch with split-K
-// partials summed in a fixed order, so training is deterministic.  Weights
+// column k*C + c for kernel offset k = (dz+1)*9 + (dy+1)*3 + (dx+1)) whose
+// operands the matrix-core tile gathers as it loads them, so no im2col
+// matrix is written; the input gradient is a GEMM over (k, output channel)
+// of the shifted output gradients; weight gradients reduce over the batch
+// with split-K partials summed in a fixed order, so training is
+// deterministic.  Weights
 // are bf16 with an fp32 master copy for Adam; activations are fp32 and
 // enter every GEMM rounded to bf16 (weights, im2col columns, gradients),
 // with fp32 accumulation on the matrix cores.  Loss = mean 0.5 (y - target)^2.
@@ -43,58 +45,79 @@ __device__ __forceinline__ bool inside(int z, int y, int x, int D) {
   return z >= 0 && z < D && y >= 0 && y < D && x >= 0 && x < D;
 }
 
-// col[r][k*5 + c] = 1 if the voxel at position(r) + offset(k) lies in the
-// grid and holds material c; r = n * 512 + pos.  One thread per (r, k).
-__global__ void im2col_onehot(const uint8_t* g, int B, float* col) {
-  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (t >= static_cast<size_t>(B) * kP1 * 27) return;
-  const size_t r = t / 27;
-  const int k = static_cast<int>(t % 27);
-  const int n = static_cast<int>(r / kP1), pos = static_cast<int>(r % kP1);
-  const int x = pos % kG, y = (pos / kG) % kG, z = pos / (kG * kG);
-  const int zz = z + k / 9 - 1, yy = y + (k / 3) % 3 - 1, xx = x + k % 3 - 1;
-  int m = -1;
-  if (inside(zz, yy, xx, kG)) m = g[static_cast<size_t>(n) * kP1 + (zz * kG + yy) * kG + xx];
-  float* out = col + r * kK1 + k * kC0;
-#pragma unroll
-  for (int c = 0; c < kC0; ++c) out[c] = (m == c) ? 1.f : 0.f;
+// Convolutions run as GEMMs whose operands are gathered as the tile loads
+// them (no im2col matrix in memory).  Row r = n * D^3 + pos of an
+// activation [B * D^3][C]; column k*C + c of its im2col matrix is channel c
+// at pos + offset(k), offset(k) = (k / 9 - 1, (k / 3) % 3 - 1, k % 3 - 1) in
+// (z, y, x), zero outside the grid.
+template <int D, int C>
+__device__ __forceinline__ uint16_t conv_col(const float* act, int r, int kk) {
+  constexpr int P = D * D * D;
+  const int k = kk / C, c = kk - k * C;
+  const int n = r / P, pos = r - n * P;
+  const int x = pos % D + k % 3 - 1, y = (pos / D) % D + (k / 3) % 3 - 1, z = pos / (D * D) + k / 9 - 1;
+  if (!inside(z, y, x, D)) return 0;
+  return bf16_bits(act[(static_cast<size_t>(n) * P + (z * D + y) * D + x) * C + c]);
 }
-
-// col[r][k*C + c] = act[n, pos(r) + offset(k), c] (0 outside the D^3 grid);
-// one thread per output element.
-__global__ void im2col_f(const float* act, int B, int D, int C, float* col) {
-  const int P = D * D * D;
-  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (t >= static_cast<size_t>(B) * P * 27 * C) return;
-  const int c = static_cast<int>(t % C);
-  const size_t rk = t / C;
-  const int k = static_cast<int>(rk % 27);
-  const size_t r = rk / 27;
-  const int n = static_cast<int>(r / P), pos = static_cast<int>(r % P);
-  const int x = pos % D, y = (pos / D) % D, z = pos / (D * D);
-  const int zz = z + k / 9 - 1, yy = y + (k / 3) % 3 - 1, xx = x + k % 3 - 1;
-  col[t] = inside(zz, yy, xx, D) ? act[(static_cast<size_t>(n) * P + (zz * D + yy) * D + xx) * C + c] : 0.f;
+// The same for the genome itself: channel c of a voxel is 1 where it holds
+// material c (0 = empty, 1..4).
+__device__ __forceinline__ uint16_t onehot_col(const uint8_t* g, int r, int kk) {
+  const int k = kk / kC0, c = kk - k * kC0;
+  const int n = r / kP1, pos = r - n * kP1;
+  const int x = pos % kG + k % 3 - 1, y = (pos / kG) % kG + (k / 3) % 3 - 1, z = pos / (kG * kG) + k / 9 - 1;
+  if (!inside(z, y, x, kG)) return 0;
+  return g[static_cast<size_t>(n) * kP1 + (z * kG + y) * kG + x] == c ? uint16_t(0x3f80) : uint16_t(0);
 }
-
-// d[n, pos, c] = sum over k of dcol[n, pos - offset(k), k*C + c] (k
-// ascending, positions inside the grid), times gelu'(Z[n, pos, c]) when Z
-// is given.
-__global__ void col2im_f(const float* dcol, const float* Z, int B, int D, int C, float* d) {
-  const int P = D * D * D;
-  const size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
-  if (t >= static_cast<size_t>(B) * P * C) return;
-  const int c = static_cast<int>(t % C);
-  const size_t r = t / C;
-  const int n = static_cast<int>(r / P), pos = static_cast<int>(r % P);
-  const int x = pos % D, y = (pos / D) % D, z = pos / (D * D);
-  float s = 0.f;
-  for (int k = 0; k < 27; ++k) {
-    const int zz = z - (k / 9 - 1), yy = y - ((k / 3) % 3 - 1), xx = x - (k % 3 - 1);
-    if (inside(zz, yy, xx, D))
-      s += dcol[(static_cast<size_t>(n) * P + (zz * D + yy) * D + xx) * (27 * C) + k * C + c];
+// im2col matrices as the forward GEMM's A operand (rows r, columns k*C + c)
+// and as the weight gradient's X operand (the same matrix)
+template <int D, int C>
+struct ConvColsA {
+  const float* act;
+  static constexpr bool kContigK = true;
+  __device__ uint16_t operator()(int m, int k) const { return conv_col<D, C>(act, m, k); }
+};
+template <int D, int C>
+struct ConvColsB {
+  const float* act;
+  static constexpr bool kContigK = false;
+  __device__ uint16_t operator()(int k, int n) const { return conv_col<D, C>(act, k, n); }
+};
+struct OneHotA {
+  const uint8_t* g;
+  static constexpr bool kContigK = true;
+  __device__ uint16_t operator()(int m, int k) const { return onehot_col(g, m, k); }
+};
+struct OneHotB {
+  const uint8_t* g;
+  static constexpr bool kContigK = false;
+  __device__ uint16_t operator()(int k, int n) const { return onehot_col(g, k, n); }
+};
+// Input gradient of a convolution, d[n, pos, c] = sum over k, o of
+// dZ[n, pos - offset(k), o] W[o][k*C + c] (positions inside the grid), as a
+// GEMM over k*O + o: A gathers the shifted output gradients, B reads the
+// weights [O x 27 C] re-indexed.
+template <int D, int O>
+struct ShiftedGrad {
+  const float* dz;
+  static constexpr bool kContigK = true;
+  __device__ uint16_t operator()(int m, int kk) const {
+    constexpr int P = D * D * D;
+    const int k = kk / O, o = kk - k * O;
+    const int n = m / P, pos = m - n * P;
+    const int x = pos % D - (k % 3 - 1), y = (pos / D) % D - ((k / 3) % 3 - 1), z = pos / (D * D) - (k / 9 - 1);
+    if (!inside(z, y, x, D)) return 0;
+    return bf16_bits(dz[(static_cast<size_t>(n) * P + (z * D + y) * D + x) * O + o]);
   }
-  d[t] = Z ? s * gelu_grad(Z[t]) : s;
-}
+};
+template <int C, int O>
+struct ConvWeightT {
+  const __hip_bfloat16* w;
+  static constexpr bool kContigK = false;
+  __device__ uint16_t operator()(int kk, int c) const {
+    const int k = kk / O, o = kk - k * O;
+    return ld_bf16(w, static_cast<size_t>(o) * (27 * C) + k * C + c);
+  }
+};
 
 // 2x2x2 average pooling 8^3 -> 4^3: the 8 inputs summed z, y, x ascending
 __global__ void pool2_fwd(const float* a, int B, float* p) {
@@ -143,50 +166,6 @@ __global__ void gap_bwd_gelu(const float* df, const float* Z, int B, float* dZ) 
   dZ[t] = df[n * kC3 + c] * (1.0f / kP2) * gelu_grad(Z[t]);
 }
 
-// Weight gradient with split-K: part[s][i][j] = sum over rows r of split s
-// of D[r][i] * X[r][j] on the matrix cores (bf16 operands, fp32 sums in a
-// fixed order); D is [M x Ni], X is [M x Nj].
-__global__ __launch_bounds__(256) void wgrad_split(const float* __restrict__ Dm, int Ni,
-                                                   const float* __restrict__ X, int Nj, int M, int rows,
-                                                   float* __restrict__ part) {
-  __shared__ __align__(16) uint16_t sd[kTile][kMRow];
-  __shared__ __align__(16) uint16_t sx[kTile][kMRow];
-  const int i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
-  const int r_begin = blockIdx.z * rows, r_end = min(M, r_begin + rows);
-  f32x4_t acc[2][2];
-  // op(A)[i][r] = D[r][i], op(B)[r][j] = X[r][j]
-  mfma_tile<float, float, true, false>(Dm, Ni, X, Nj, Ni, Nj, r_begin, r_end, i0, j0, sd, sx, acc);
-  float* out = part + static_cast<size_t>(blockIdx.z) * Ni * Nj;
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int i = i0 + mfma_row(a, q), j = j0 + mfma_col(b);
-        if (i < Ni && j < Nj) out[static_cast<size_t>(i) * Nj + j] = acc[a][b][q];
-      }
-}
-
-// out[e] = sum over splits s (ascending) of part[s][e]
-__global__ void sum_splits(const float* part, int n, int splits, float* out) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
-  float s = 0.f;
-  for (int k = 0; k < splits; ++k) s += part[static_cast<size_t>(k) * n + e];
-  out[e] = s;
-}
-
-// column sums with split rows: part[s][c] = sum over rows of split s
-__global__ void colsum_split(const float* X, int M, int N, int rows, float* part) {
-  const int c = threadIdx.x + blockIdx.x * blockDim.x;
-  if (c >= N) return;
-  const int r_begin = blockIdx.y * rows, r_end = min(M, r_begin + rows);
-  float s = 0.f;
-  for (int r = r_begin; r < r_end; ++r) s += X[static_cast<size_t>(r) * N + c];
-  part[static_cast<size_t>(blockIdx.y) * N + c] = s;
-}
-
 __global__ void gather_genomes(const uint8_t* src, const int* idx, int n, uint8_t* dst) {
   const int r = blockIdx.x;
   if (r >= n) return;
@@ -219,9 +198,9 @@ struct mh_cnn {
   int cap = 0;
   // activations (B rows of genomes)
   Dev<uint8_t> g;
-  Dev<float> col1, z1, a1, col2, z2, a2, p2, col3, z3, a3, feat, zh1, ah1, zh2, ah2, y, t;
+  Dev<float> z1, a1, z2, a2, p2, z3, a3, feat, zh1, ah1, zh2, ah2, y, t;
   // gradients
-  Dev<float> dy, terms, dzh2, dzh1, dfeat, dz3, dcol3, dp2, dz2, dcol2, dz1;
+  Dev<float> dy, terms, dzh2, dzh1, dfeat, dz3, dp2, dz2, dz1;
   Dev<uint8_t> data_g;
   Dev<float> data_t;
   Dev<int> idx;
@@ -239,14 +218,11 @@ void ensure(mh_cnn* net, int B) {
   if (B <= net->cap) return;
   const size_t R1 = static_cast<size_t>(B) * kP1, R2 = static_cast<size_t>(B) * kP2;
   net->g.ensure(static_cast<size_t>(B) * kP1);
-  net->col1.ensure(R1 * kK1);
   net->z1.ensure(R1 * kC1);
   net->a1.ensure(R1 * kC1);
-  net->col2.ensure(R1 * kK2);
   net->z2.ensure(R1 * kC2);
   net->a2.ensure(R1 * kC2);
   net->p2.ensure(R2 * kC2);
-  net->col3.ensure(R2 * kK3);
   net->z3.ensure(R2 * kC3);
   net->a3.ensure(R2 * kC3);
   net->feat.ensure(static_cast<size_t>(B) * kC3);
@@ -262,10 +238,8 @@ void ensure(mh_cnn* net, int B) {
   net->dzh1.ensure(static_cast<size_t>(B) * kH);
   net->dfeat.ensure(static_cast<size_t>(B) * kC3);
   net->dz3.ensure(R2 * kC3);
-  net->dcol3.ensure(R2 * kK3);
   net->dp2.ensure(R2 * kC2);
   net->dz2.ensure(R1 * kC2);
-  net->dcol2.ensure(R1 * kK2);
   net->dz1.ensure(R1 * kC1);
   net->cap = B;
 }
@@ -278,16 +252,22 @@ void dense(mh_cnn* net, int l, const float* in, int M, float* out, float* Z) {
                      in, K, net->W(l), K, out, N, M, N, K, net->b(l), Z, static_cast<const float*>(nullptr));
 }
 
+// convolution l as a GEMM over its gathered operand la: EPI 1 stores Z and gelu
+template <typename LA>
+void conv(mh_cnn* net, int l, LA la, int M, float* out, float* Z) {
+  const int N = kLayers[l].out, K = kLayers[l].in;
+  hipLaunchKernelGGL((gemm_l<LA, StridedB<__hip_bfloat16, true>, 1>), grid_for(M, N), dim3(256), 0, net->stream, la,
+                     StridedB<__hip_bfloat16, true>{net->W(l), K}, out, N, M, N, K, net->b(l), Z,
+                     static_cast<const float*>(nullptr));
+}
+
 void forward(mh_cnn* net, int B) {
   const size_t R1 = static_cast<size_t>(B) * kP1, R2 = static_cast<size_t>(B) * kP2;
   hipStream_t s = net->stream;
-  hipLaunchKernelGGL(im2col_onehot, dim3(blocks_for(R1 * 27)), dim3(256), 0, s, net->g.p, B, net->col1.p);
-  dense<1>(net, 0, net->col1.p, static_cast<int>(R1), net->a1.p, net->z1.p);
-  hipLaunchKernelGGL(im2col_f, dim3(blocks_for(R1 * kK2)), dim3(256), 0, s, net->a1.p, B, kG, kC1, net->col2.p);
-  dense<1>(net, 1, net->col2.p, static_cast<int>(R1), net->a2.p, net->z2.p);
+  conv(net, 0, OneHotA{net->g.p}, static_cast<int>(R1), net->a1.p, net->z1.p);
+  conv(net, 1, ConvColsA<kG, kC1>{net->a1.p}, static_cast<int>(R1), net->a2.p, net->z2.p);
   hipLaunchKernelGGL(pool2_fwd, dim3(blocks_for(R2 * kC2)), dim3(256), 0, s, net->a2.p, B, net->p2.p);
-  hipLaunchKernelGGL(im2col_f, dim3(blocks_for(R2 * kK3)), dim3(256), 0, s, net->p2.p, B, kG2, kC2, net->col3.p);
-  dense<1>(net, 2, net->col3.p, static_cast<int>(R2), net->a3.p, net->z3.p);
+  conv(net, 2, ConvColsA<kG2, kC2>{net->p2.p}, static_cast<int>(R2), net->a3.p, net->z3.p);
   hipLaunchKernelGGL(gap_fwd, dim3(blocks_for(static_cast<size_t>(B) * kC3)), dim3(256), 0, s, net->a3.p, B,
                      net->feat.p);
   dense<1>(net, 3, net->feat.p, B, net->ah1.p, net->zh1.p);
@@ -297,19 +277,30 @@ void forward(mh_cnn* net, int B) {
 }
 
 // gradient of layer l's weights and bias from dZ [M x out] and its input
-// X [M x in], reduced over rows in kSplit fixed-order partials
-void layer_grads(mh_cnn* net, int l, const float* dZ, const float* X, int M) {
+// (loader lx of the [M x in] matrix), reduced over rows in kSplit
+// fixed-order partials
+template <typename LX>
+void layer_grads(mh_cnn* net, int l, const float* dZ, LX lx, int M) {
   const int No = kLayers[l].out, Ni = kLayers[l].in;
   const int rows = (M + kSplit - 1) / kSplit;
   hipStream_t s = net->stream;
-  hipLaunchKernelGGL(wgrad_split, dim3((Ni + kTile - 1) / kTile, (No + kTile - 1) / kTile, kSplit), dim3(256),
-                     0, s, dZ, No, X, Ni, M, rows, net->part.p);
+  hipLaunchKernelGGL(wgrad_l<LX>, dim3((Ni + kTile - 1) / kTile, (No + kTile - 1) / kTile, kSplit), dim3(256), 0, s,
+                     dZ, No, lx, Ni, M, rows, net->part.p);
   hipLaunchKernelGGL(sum_splits, dim3(blocks_for(static_cast<size_t>(No) * Ni)), dim3(256), 0, s, net->part.p,
                      No * Ni, kSplit, net->grads.p + net->w_off[l]);
-  hipLaunchKernelGGL(colsum_split, dim3((No + 255) / 256, kSplit), dim3(256), 0, s, dZ, M, No, rows,
-                     net->part.p);
+  hipLaunchKernelGGL(col_partial, dim3((No + kColTile - 1) / kColTile, kSplit), dim3(kColThreads), 0, s, dZ, M, No,
+                     rows, net->part.p);
   hipLaunchKernelGGL(sum_splits, dim3(blocks_for(No)), dim3(256), 0, s, net->part.p, No, kSplit,
                      net->grads.p + net->b_off[l]);
+}
+
+// input gradient of convolution l (input grid D, C channels, O outputs)
+// from dZ [M x O]: EPI 3 multiplies by gelu'(Zprev), EPI 0 plain
+template <int D, int C, int O, int EPI>
+void conv_back(mh_cnn* net, int l, const float* dZ, int M, float* dIn, const float* Zprev) {
+  hipLaunchKernelGGL((gemm_l<ShiftedGrad<D, O>, ConvWeightT<C, O>, EPI>), grid_for(M, C), dim3(256), 0, net->stream,
+                     ShiftedGrad<D, O>{dZ}, ConvWeightT<C, O>{net->W(l)}, dIn, C, M, C, 27 * O,
+                     static_cast<const float*>(nullptr), static_cast<float*>(nullptr), Zprev);
 }
 
 // dIn [M x in] = dZ [M x out] W (EPI 3: times gelu'(Zprev), EPI 0 plain)
@@ -324,25 +315,22 @@ void dense_back(mh_cnn* net, int l, const float* dZ, int M, float* dIn, const fl
 void backward(mh_cnn* net, int B) {
   const size_t R1 = static_cast<size_t>(B) * kP1, R2 = static_cast<size_t>(B) * kP2;
   hipStream_t s = net->stream;
-  layer_grads(net, 5, net->dy.p, net->ah2.p, B);
+  using Rows = StridedB<float, false>;
+  layer_grads(net, 5, net->dy.p, Rows{net->ah2.p, kH}, B);
   dense_back<3>(net, 5, net->dy.p, B, net->dzh2.p, net->zh2.p);
-  layer_grads(net, 4, net->dzh2.p, net->ah1.p, B);
+  layer_grads(net, 4, net->dzh2.p, Rows{net->ah1.p, kH}, B);
   dense_back<3>(net, 4, net->dzh2.p, B, net->dzh1.p, net->zh1.p);
-  layer_grads(net, 3, net->dzh1.p, net->feat.p, B);
+  layer_grads(net, 3, net->dzh1.p, Rows{net->feat.p, kC3}, B);
   dense_back<0>(net, 3, net->dzh1.p, B, net->dfeat.p, nullptr);
   hipLaunchKernelGGL(gap_bwd_gelu, dim3(blocks_for(R2 * kC3)), dim3(256), 0, s, net->dfeat.p, net->z3.p, B,
                      net->dz3.p);
-  layer_grads(net, 2, net->dz3.p, net->col3.p, static_cast<int>(R2));
-  dense_back<0>(net, 2, net->dz3.p, static_cast<int>(R2), net->dcol3.p, nullptr);
-  hipLaunchKernelGGL(col2im_f, dim3(blocks_for(R2 * kC2)), dim3(256), 0, s, net->dcol3.p,
-                     static_cast<const float*>(nullptr), B, kG2, kC2, net->dp2.p);
+  layer_grads(net, 2, net->dz3.p, ConvColsB<kG2, kC2>{net->p2.p}, static_cast<int>(R2));
+  conv_back<kG2, kC2, kC3, 0>(net, 2, net->dz3.p, static_cast<int>(R2), net->dp2.p, nullptr);
   hipLaunchKernelGGL(pool2_bwd_gelu, dim3(blocks_for(R1 * kC2)), dim3(256), 0, s, net->dp2.p, net->z2.p, B,
                      net->dz2.p);
-  layer_grads(net, 1, net->dz2.p, net->col2.p, static_cast<int>(R1));
-  dense_back<0>(net, 1, net->dz2.p, static_cast<int>(R1), net->dcol2.p, nullptr);
-  hipLaunchKernelGGL(col2im_f, dim3(blocks_for(R1 * kC1)), dim3(256), 0, s, net->dcol2.p, net->z1.p, B, kG, kC1,
-                     net->dz1.p);
-  layer_grads(net, 0, net->dz1.p, net->col1.p, static_cast<int>(R1));
+  layer_grads(net, 1, net->dz2.p, ConvColsB<kG, kC1>{net->a1.p}, static_cast<int>(R1));
+  conv_back<kG, kC1, kC2, 3>(net, 1, net->dz2.p, static_cast<int>(R1), net->dz1.p, net->z1.p);
+  layer_grads(net, 0, net->dz1.p, OneHotB{net->g.p}, static_cast<int>(R1));
   hipcheck(hipGetLastError(), "cnn backward");
 }
 

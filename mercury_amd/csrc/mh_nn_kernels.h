@@ -1,7 +1,7 @@
 // mh_nn_kernels.h -- device kernels shared by the surrogate networks
 // (mlp.hip, cnn.hip): a matrix-core GEMM (bf16 operands, FP32 accumulate)
 // with bias/GELU epilogues, the squared-error loss gradient, fixed-order column sums (bias
-// gradients), Adam, and small helpers.  Every reduction runs in a fixed
+// gradients), split-K weight gradients, Adam, and small helpers.  Every reduction runs in a fixed
 // order, so training is deterministic.
 //
 // gelu(x) = 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).  Adam follows
@@ -61,23 +61,46 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef uint16_t u16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
+// Operand loaders of the matrix-core tile: A(m, k) and B(k, n) return the
+// bf16 bits of op(A)[m][k] and op(B)[k][n] (callers stay within M, N and
+// the k range).  kContigK says which index is contiguous in memory (k, or
+// m / n), so a tile's elements are read along it.
+//   StridedA: op(A)[m][k] = TR ? p[k*ld + m] : p[m*ld + k]
+//   StridedB: op(B)[k][n] = TR ? p[n*ld + k] : p[k*ld + n]
+template <typename T, bool TR>
+struct StridedA {
+  const T* p;
+  int ld;
+  static constexpr bool kContigK = !TR;
+  __device__ uint16_t operator()(int m, int k) const {
+    return TR ? ld_bf16(p, static_cast<size_t>(k) * ld + m) : ld_bf16(p, static_cast<size_t>(m) * ld + k);
+  }
+};
+template <typename T, bool TR>
+struct StridedB {
+  const T* p;
+  int ld;
+  static constexpr bool kContigK = TR;
+  __device__ uint16_t operator()(int k, int n) const {
+    return TR ? ld_bf16(p, static_cast<size_t>(n) * ld + k) : ld_bf16(p, static_cast<size_t>(k) * ld + n);
+  }
+};
+
 // One 64 x 64 tile of op(A) op(B) over k in [k_begin, k_end) on the matrix
 // cores (v_mfma_f32_16x16x32_bf16: bf16 operands, fp32 accumulate), 256
-// threads.  op(A)[m][k] = TRA ? A[k*lda + m] : A[m*lda + k], op(B)[k][n] =
-// TRB ? B[n*ldb + k] : B[k*ldb + n]; elements outside M, N or the k range
-// are zeros.  Each step stages a 64 x 32 slice of both operands in LDS,
-// rounded to bf16 and k-contiguous per row of op(A) and per column of
-// op(B); wave w owns the 32 x 32 quarter (32 (w / 2), 32 (w % 2)) as 2 x 2
-// blocks of 16 x 16.  Lane l feeds row (or column) l % 16 of a block with
-// the 8 k values 8 (l / 16) .. 8 (l / 16) + 7 and receives rows
-// 4 (l / 16) .. 4 (l / 16) + 3 of column l % 16.  Every output sums its k
-// in the same order on every run.
+// threads.  Elements outside M, N or the k range are zeros.  Each step
+// stages a 64 x 32 slice of both operands in LDS as bf16, k-contiguous per
+// row of op(A) and per column of op(B); wave w owns the 32 x 32 quarter
+// (32 (w / 2), 32 (w % 2)) as 2 x 2 blocks of 16 x 16.  Lane l feeds row
+// (or column) l % 16 of a block with the 8 k values 8 (l / 16) ..
+// 8 (l / 16) + 7 and receives rows 4 (l / 16) .. 4 (l / 16) + 3 of column
+// l % 16.  Every output sums its k in the same order on every run.
 constexpr int kMk = 32;                  // k per step
 constexpr int kMRow = kMk + 8;           // LDS row pitch in bf16 (80 B, 16 B aligned)
-template <typename TA, typename TB, bool TRA, bool TRB>
-__device__ __forceinline__ void mfma_tile(const TA* __restrict__ A, int lda, const TB* __restrict__ B, int ldb,
-                                          int M, int N, int k_begin, int k_end, int m0, int n0,
-                                          uint16_t (*sa)[kMRow], uint16_t (*sb)[kMRow], f32x4_t (&acc)[2][2]) {
+template <typename LA, typename LB>
+__device__ __forceinline__ void mfma_tile(const LA& la, const LB& lb, int M, int N, int k_begin, int k_end, int m0,
+                                          int n0, uint16_t (*sa)[kMRow], uint16_t (*sb)[kMRow],
+                                          f32x4_t (&acc)[2][2]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
 #pragma unroll
@@ -89,19 +112,13 @@ __device__ __forceinline__ void mfma_tile(const TA* __restrict__ A, int lda, con
     // operand's contiguous dimension
     for (int e = threadIdx.x; e < kTile * kMk; e += 256) {
       int mm, kk;
-      if (TRA) { mm = e % kTile; kk = e / kTile; } else { kk = e % kMk; mm = e / kMk; }
+      if (LA::kContigK) { kk = e % kMk; mm = e / kMk; } else { mm = e % kTile; kk = e / kTile; }
       const int m = m0 + mm, k = k0 + kk;
-      uint16_t va = 0;
-      if (m < M && k < k_end)
-        va = TRA ? ld_bf16(A, static_cast<size_t>(k) * lda + m) : ld_bf16(A, static_cast<size_t>(m) * lda + k);
-      sa[mm][kk] = va;
+      sa[mm][kk] = (m < M && k < k_end) ? la(m, k) : uint16_t(0);
       int nn, kb;
-      if (TRB) { kb = e % kMk; nn = e / kMk; } else { nn = e % kTile; kb = e / kTile; }
+      if (LB::kContigK) { kb = e % kMk; nn = e / kMk; } else { nn = e % kTile; kb = e / kTile; }
       const int n = n0 + nn, kq = k0 + kb;
-      uint16_t vb = 0;
-      if (n < N && kq < k_end)
-        vb = TRB ? ld_bf16(B, static_cast<size_t>(n) * ldb + kq) : ld_bf16(B, static_cast<size_t>(kq) * ldb + n);
-      sb[nn][kb] = vb;
+      sb[nn][kb] = (n < N && kq < k_end) ? lb(kq, n) : uint16_t(0);
     }
     __syncthreads();
     bf16x8_t fa[2], fb[2];
@@ -124,21 +141,15 @@ __device__ __forceinline__ int mfma_row(int i, int r) {
 }
 __device__ __forceinline__ int mfma_col(int j) { return ((threadIdx.x >> 6) & 1) * 32 + 16 * j + (threadIdx.x & 15); }
 
-// C[M x N] = op(A) op(B) (mfma_tile's operands, bf16 products, fp32 sums).
-// Epilogue EPI: 0 = store; 1 = + bias[n], store Z and gelu(Z) (hidden layer
-// forward); 2 = + bias[n], store (output layer forward); 3 = multiply by
-// gelu'(Zprev[m][n]) (hidden layer backward); 4 = + bias[n], store gelu
-// (hidden layer forward without keeping Z: the same values as 1 stores in C).
-template <typename TA, typename TB, bool TRA, bool TRB, int EPI>
-__global__ __launch_bounds__(256) void gemm(const TA* __restrict__ A, int lda, const TB* __restrict__ B,
-                                            int ldb, float* __restrict__ C, int ldc, int M, int N, int K,
-                                            const float* __restrict__ bias, float* __restrict__ Z,
-                                            const float* __restrict__ Zprev) {
-  __shared__ __align__(16) uint16_t sa[kTile][kMRow];
-  __shared__ __align__(16) uint16_t sb[kTile][kMRow];
-  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
-  f32x4_t acc[2][2];
-  mfma_tile<TA, TB, TRA, TRB>(A, lda, B, ldb, M, N, 0, K, m0, n0, sa, sb, acc);
+// Writes a tile's accumulators to C with epilogue EPI: 0 = store; 1 =
+// + bias[n], store Z and gelu(Z) (hidden layer forward); 2 = + bias[n],
+// store (output layer forward); 3 = multiply by gelu'(Zprev[m][n]) (hidden
+// layer backward); 4 = + bias[n], store gelu (hidden layer forward without
+// keeping Z: the same values as 1 stores in C).
+template <int EPI>
+__device__ __forceinline__ void mfma_store(const f32x4_t (&acc)[2][2], float* __restrict__ C, int ldc, int M, int N,
+                                           int m0, int n0, const float* __restrict__ bias, float* __restrict__ Z,
+                                           const float* __restrict__ Zprev) {
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -164,6 +175,35 @@ __global__ __launch_bounds__(256) void gemm(const TA* __restrict__ A, int lda, c
       }
 }
 
+// C[M x N] = op(A) op(B) (bf16 products, fp32 sums) with loaders la, lb,
+// then epilogue EPI.
+template <typename LA, typename LB, int EPI>
+__global__ __launch_bounds__(256) void gemm_l(LA la, LB lb, float* __restrict__ C, int ldc, int M, int N, int K,
+                                              const float* __restrict__ bias, float* __restrict__ Z,
+                                              const float* __restrict__ Zprev) {
+  __shared__ __align__(16) uint16_t sa[kTile][kMRow];
+  __shared__ __align__(16) uint16_t sb[kTile][kMRow];
+  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  f32x4_t acc[2][2];
+  mfma_tile(la, lb, M, N, 0, K, m0, n0, sa, sb, acc);
+  mfma_store<EPI>(acc, C, ldc, M, N, m0, n0, bias, Z, Zprev);
+}
+
+// The same for strided operands: op(A)[m][k] = TRA ? A[k*lda + m] :
+// A[m*lda + k], op(B)[k][n] = TRB ? B[n*ldb + k] : B[k*ldb + n].
+template <typename TA, typename TB, bool TRA, bool TRB, int EPI>
+__global__ __launch_bounds__(256) void gemm(const TA* __restrict__ A, int lda, const TB* __restrict__ B,
+                                            int ldb, float* __restrict__ C, int ldc, int M, int N, int K,
+                                            const float* __restrict__ bias, float* __restrict__ Z,
+                                            const float* __restrict__ Zprev) {
+  __shared__ __align__(16) uint16_t sa[kTile][kMRow];
+  __shared__ __align__(16) uint16_t sb[kTile][kMRow];
+  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  f32x4_t acc[2][2];
+  mfma_tile(StridedA<TA, TRA>{A, lda}, StridedB<TB, TRB>{B, ldb}, M, N, 0, K, m0, n0, sa, sb, acc);
+  mfma_store<EPI>(acc, C, ldc, M, N, m0, n0, bias, Z, Zprev);
+}
+
 // dY = (Y - T) / B, per-row loss terms 0.5 (Y - T)^2
 __global__ void loss_grad(const float* Y, const float* T, int n, float inv_b, float* dY, float* terms) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -173,13 +213,64 @@ __global__ void loss_grad(const float* Y, const float* T, int n, float inv_b, fl
   terms[i] = 0.5f * e * e;
 }
 
-// out[n] = sum over rows of X[m][n] (row order fixed)
-__global__ void col_sum(const float* X, int M, int N, float* out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// Column sums in a fixed order: part[s][n] = sum of X[r][n] over the rows r
+// of split s ([s rows, (s + 1) rows) within M).  A block of kColThreads
+// takes kColTile columns: thread (g, c) adds rows g, g + kColGroups, ... of
+// its split in ascending order, then one thread per column adds the
+// kColGroups partials in ascending g.
+constexpr int kColTile = 64, kColGroups = 16, kColThreads = kColTile * kColGroups;
+__global__ __launch_bounds__(kColThreads) void col_partial(const float* __restrict__ X, int M, int N, int rows,
+                                                           float* __restrict__ part) {
+  __shared__ float acc[kColGroups][kColTile];
+  const int c = threadIdx.x % kColTile, g = threadIdx.x / kColTile;
+  const int n = blockIdx.x * kColTile + c;
+  const int r0 = blockIdx.y * rows, r1 = min(M, r0 + rows);
+  float sum = 0.f;
+  if (n < N)
+    for (int r = r0 + g; r < r1; r += kColGroups) sum += X[static_cast<size_t>(r) * N + n];
+  acc[g][c] = sum;
+  __syncthreads();
+  if (g == 0 && n < N) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < kColGroups; ++k) t += acc[k][c];
+    part[static_cast<size_t>(blockIdx.y) * N + n] = t;
+  }
+}
+
+// Weight gradient with split-K: part[s][i][j] = sum over rows r of split s
+// of D[r][i] * X(r, j) on the matrix cores (bf16 operands, fp32 sums in a
+// fixed order); D is [M x Ni] row-major, X a B-operand loader giving row r,
+// column j of an [M x Nj] matrix.
+template <typename LX>
+__global__ __launch_bounds__(256) void wgrad_l(const float* __restrict__ Dm, int Ni, LX lx, int Nj, int M, int rows,
+                                               float* __restrict__ part) {
+  __shared__ __align__(16) uint16_t sd[kTile][kMRow];
+  __shared__ __align__(16) uint16_t sx[kTile][kMRow];
+  const int i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
+  const int r_begin = blockIdx.z * rows, r_end = min(M, r_begin + rows);
+  f32x4_t acc[2][2];
+  // op(A)[i][r] = D[r][i]
+  mfma_tile(StridedA<float, true>{Dm, Ni}, lx, Ni, Nj, r_begin, r_end, i0, j0, sd, sx, acc);
+  float* out = part + static_cast<size_t>(blockIdx.z) * Ni * Nj;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = i0 + mfma_row(a, q), j = j0 + mfma_col(b);
+        if (i < Ni && j < Nj) out[static_cast<size_t>(i) * Nj + j] = acc[a][b][q];
+      }
+}
+
+// out[e] = sum over splits s (ascending) of part[s][e]
+__global__ void sum_splits(const float* part, int n, int splits, float* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
   float s = 0.f;
-  for (int m = 0; m < M; ++m) s += X[static_cast<size_t>(m) * N + n];
-  out[n] = s;
+  for (int k = 0; k < splits; ++k) s += part[static_cast<size_t>(k) * n + e];
+  out[e] = s;
 }
 
 __global__ void adam_step(float* p, const float* g, float* m, float* v, __hip_bfloat16* w16, int n,

@@ -10,10 +10,10 @@
 // & Ba (2015), Algorithm 1 (bias-corrected moments).
 //
 // GEMMs: 64x64 output tiles, 256 threads (four waves of 2x2 16x16 MFMA
-// blocks), K in steps of 32 staged through LDS as bf16.  Column sums (bias
-// gradients, from the fp32 gradients) run one lane per column over the
-// rows, so every reduction has a fixed order and training is
-// deterministic.
+// blocks), K in steps of 32 staged through LDS as bf16.  Weight gradients
+// reduce over the batch in row splits whose partials are summed in order;
+// bias gradients (column sums of the fp32 gradients) likewise, so every
+// reduction has a fixed order and training is deterministic.
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
@@ -46,6 +46,7 @@ struct mh_mlp {
   Dev<__hip_bfloat16> w16;    // bf16 copy of the whole parameter vector (weights used)
   std::vector<Dev<float>> Z, A, dA;  // per layer activations / gradients
   Dev<float> x, t, dY, terms, data_x, data_t;
+  Dev<float> part;            // split-K partials of the weight and bias gradients
   Dev<int> idx;
   int batch_cap = 0;
   int adam_t = 0;
@@ -58,6 +59,10 @@ struct mh_mlp {
 };
 
 namespace {
+
+// weight and bias gradients reduce over the batch in up to this many row
+// splits (at least 64 rows each), summed in order
+constexpr int kMlpSplit = 8;
 
 void ensure_batch(mh_mlp* net, int B) {
   if (B <= net->batch_cap) return;
@@ -107,11 +112,18 @@ void backward(mh_mlp* net, const float* in, int B) {
     const float* a_prev = l == 0 ? in : net->A[l - 1].p;
     float* gW = net->grads.p + net->w_off[l];
     float* gb = net->grads.p + net->b_off(l);
-    // gW[no x ni] = delta^T[no x B] a_prev[B x ni]
-    hipLaunchKernelGGL((gemm<float, float, true, false, 0>), grid_for(no, ni), dim3(256), 0, net->stream,
-                       delta, no, a_prev, ni, gW, ni, no, ni, B, static_cast<const float*>(nullptr),
-                       static_cast<float*>(nullptr), static_cast<const float*>(nullptr));
-    hipLaunchKernelGGL(col_sum, dim3((no + 255) / 256), dim3(256), 0, net->stream, delta, B, no, gb);
+    // gW[no x ni] = delta^T[no x B] a_prev[B x ni] and gb = column sums of
+    // delta, each as S row-split partials summed in order
+    const int S = std::max(1, std::min(kMlpSplit, B / 64));
+    const int rows = (B + S - 1) / S;
+    hipLaunchKernelGGL((wgrad_l<StridedB<float, false>>), dim3((ni + kTile - 1) / kTile, (no + kTile - 1) / kTile, S),
+                       dim3(256), 0, net->stream, delta, no, StridedB<float, false>{a_prev, ni}, ni, B, rows,
+                       net->part.p);
+    hipLaunchKernelGGL(sum_splits, dim3((no * ni + 255) / 256), dim3(256), 0, net->stream, net->part.p, no * ni, S,
+                       gW);
+    hipLaunchKernelGGL(col_partial, dim3((no + kColTile - 1) / kColTile, S), dim3(kColThreads), 0, net->stream,
+                       delta, B, no, rows, net->part.p);
+    hipLaunchKernelGGL(sum_splits, dim3((no + 255) / 256), dim3(256), 0, net->stream, net->part.p, no, S, gb);
     if (l > 0) {
       // delta_prev[B x ni] = (delta[B x no] W[no x ni]) * gelu'(Z_{l-1})
       const __hip_bfloat16* W = net->w16.p + net->w_off[l];
@@ -183,6 +195,9 @@ int mh_mlp_create(int device, int n_layers, const int* sizes, uint64_t seed, mh_
       off += static_cast<size_t>(sizes[l + 1]) * sizes[l] + sizes[l + 1];
     }
     net->n_params = off;
+    size_t widest = 0;
+    for (int l = 0; l < n_layers; l++) widest = std::max<size_t>(widest, static_cast<size_t>(sizes[l]) * sizes[l + 1]);
+    net->part.ensure(static_cast<size_t>(kMlpSplit) * widest);
     // He-uniform weights (bound sqrt(6 / fan_in)), zero biases
     std::mt19937_64 rng(seed);
     std::vector<float> p(off, 0.f);
