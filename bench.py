@@ -34,6 +34,12 @@ STREAM_BYTES_PER_SPRING = 65.0
 # chunked streaming kernel (terms in LDS chunks), HBM bytes per
 # spring-update: record 9 B + two u16 gather entries 4 B
 CHUNK_HBM_BYTES_PER_SPRING = 13.0
+# mass-parallel streaming kernel: each endpoint reads its 8 B record of the
+# spring (other endpoint and material, rest length) every substep
+MSTREAM_HBM_BYTES_PER_SPRING = 16.0
+# the spring record alone (two u16 endpoints, f32 rest length, u8 material):
+# the least any kernel that streams the springs reads per spring-update
+SPRING_RECORD_BYTES = 9.0
 HBM_PEAK_GBS = 8000.0
 # LDS bytes per spring-update of the LDS-resident kernel, per robot size:
 # 57 B per spring (two endpoint positions 24 B, spring record 9 B, result
@@ -59,11 +65,13 @@ VALU_PEAK_WAVE_INSTR = MI355X_CUS * 2.4e9
 
 # --kernel choices: MH_KERNEL_* family bits (robots over the LDS budget may
 # always stream)
-KERNEL_FAMILIES = {"auto": 0, "mass": 4 | 2, "mass-gather": 8 | 2, "lds": 1 | 2, "stream": 2}
+KERNEL_FAMILIES = {"auto": 0, "mass": 4 | 2 | 16, "mass-gather": 8 | 2 | 16, "lds": 1 | 2 | 16, "stream": 2,
+                   "mass-stream": 16}
 
 
 def kernel_label(info):
-    return ("+".join(n for bit, n in ((8, "mass-gather"), (4, "mass"), (1, "lds"), (2, "stream"))
+    return ("+".join(n for bit, n in ((8, "mass-gather"), (4, "mass"), (1, "lds"), (16, "mass-stream"),
+                                      (2, "stream"))
                      if info.kernel & bit) + (" (chunked)" if info.stream_chunk > 0 else ""))
 
 
@@ -352,7 +360,16 @@ def main():
     lds_peak = float(lds_peak.value)
     steps_per_launch = n_sub / max(launches, 1)
     hbm_extra = None
-    if info.kernel & (mh.MH_KERNEL_MASS | mh.MH_KERNEL_MASS_GATHER) and not info.kernel & mh.MH_KERNEL_STREAM:
+    if info.kernel & mh.MH_KERNEL_MASS_STREAM and info.n_stream_springs > 0:
+        # the streamed robots carry the work: HBM bytes of their records
+        bound, peak, unit = "hbm", HBM_PEAK_GBS, "GB/s"
+        per_launch = MSTREAM_HBM_BYTES_PER_SPRING * info.n_stream_springs * steps_per_launch
+        per_unit = MSTREAM_HBM_BYTES_PER_SPRING
+        if kernel_ms > 0:
+            rec = SPRING_RECORD_BYTES * info.n_stream_springs * steps_per_launch / (kernel_ms * 1e-3) / 1e9
+            hbm_extra = {"record_model": {"bytes_per_spring_update": SPRING_RECORD_BYTES,
+                                          "achieved": rec, "frac": rec / HBM_PEAK_GBS}}
+    elif info.kernel & (mh.MH_KERNEL_MASS | mh.MH_KERNEL_MASS_GATHER) and not info.kernel & mh.MH_KERNEL_STREAM:
         bound, peak, unit = "lds", lds_peak, "GB/s"
         bps = MASSG_LDS_BYTES_PER_SPRING if info.kernel & mh.MH_KERNEL_MASS_GATHER else MASS_LDS_BYTES_PER_SPRING
         per_launch = (bps * n_springs + MASS_LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
@@ -431,7 +448,6 @@ def main():
             "substeps_per_launch": info.substeps_per_launch, "hipgraph": True,
             "lds_robots": info.n_lds_robots, "stream_robots": info.n_stream_robots,
             "robots_per_block": info.robots_per_block,
-            "stream_slices": info.stream_slices,
         },
         "active_spring_updates_per_s": n_active * n_sub * args.steps / secs,
         "roofline": {

@@ -125,14 +125,18 @@ enum { MH_SOLVER_XPBD_REF = 0, MH_SOLVER_FORCE = 1 };
  *           its springs (both endpoints evaluate a spring);
  *   MASS_GATHER  robots resident in LDS, one lane per mass; a spring is
  *           evaluated by its first endpoint and its second endpoint reads
- *           the stored term.
+ *           the stored term;
+ *   MASS_STREAM  robots over the LDS budget (force solver, up to 2048
+ *           masses of at most 255 springs each), one lane per mass
+ *           evaluating each of its springs, spring records streamed per
+ *           substep.
  * A robot can use the MASS kinds when its masses fit one workgroup (up to
  * 1024), no mass is the first (or the second) endpoint of more than 13 to
  * 16 springs (by workgroup size), and its springs have at most 511
  * distinct (rest length, material) pairs; other robots go to LDS or
  * STREAM.  The kinds allowed never change results, only speed. */
 enum { MH_KERNEL_AUTO = 0, MH_KERNEL_LDS = 1, MH_KERNEL_STREAM = 2, MH_KERNEL_MASS = 4,
-       MH_KERNEL_MASS_GATHER = 8 };
+       MH_KERNEL_MASS_GATHER = 8, MH_KERNEL_MASS_STREAM = 16 };
 enum { MH_FITNESS_REFERENCE = 0, MH_FITNESS_DISPLACEMENT = 1 };
 
 typedef struct {

@@ -19,6 +19,7 @@ f32p = ctypes.POINTER(ctypes.c_float)
 
 MH_SOLVER_XPBD_REF, MH_SOLVER_FORCE = 0, 1
 MH_KERNEL_AUTO, MH_KERNEL_LDS, MH_KERNEL_STREAM, MH_KERNEL_MASS, MH_KERNEL_MASS_GATHER = 0, 1, 2, 4, 8
+MH_KERNEL_MASS_STREAM = 16
 MH_FITNESS_REFERENCE, MH_FITNESS_DISPLACEMENT = 0, 1
 
 

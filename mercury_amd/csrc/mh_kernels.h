@@ -65,7 +65,8 @@ struct StepParams {
 struct KernelVariant {
   int solver;     // MH_SOLVER_*
   int kind;       // MH_KERNEL_LDS / MH_KERNEL_STREAM / MH_KERNEL_MASS
-  int chunked;    // STREAM kind: 1 = spring terms in LDS chunks, 0 = in global memory;
+  int chunked;    // STREAM kind: 1 = spring terms in LDS chunks, 0 = in global memory,
+                  // 2 = mass-parallel with streamed records (step_mstream);
                   // MASS kind: 1 = terms computed once and gathered (step_massg)
   int block;      // threads per workgroup
   int nspt;       // springs per thread held in registers (LDS kind); records
@@ -133,6 +134,10 @@ constexpr int kChunkSpl = 8, kChunkEpl = 8, kChunkRpl = 2, kChunkMax = 256;
 __host__ __device__ constexpr int chunk_row_quads(int n_mass) { return (n_mass + 1 + 3) / 4; }
 size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat);
 size_t lds_bytes_massg(int block, int maxp);
+// Mass-parallel streaming kernel (step_mstream): its per-material table
+// holds kMStreamMats entries, the compact materials and a null one.
+constexpr int kMStreamMats = 256;
+size_t lds_bytes_mstream(int block, int mpt);
 
 hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes,
                        hipStream_t stream, const StepParams& p);

@@ -126,6 +126,7 @@ struct mh_sim {
   std::vector<uint32_t> mass_off, spring_off, active_off;  // per robot prefix offsets
   std::vector<uint32_t> gather_pad;  // per robot: sum over masses of the degree rounded up to 4
   std::vector<uint8_t> max_side;     // per robot: most springs one mass has on one side (capped 255)
+  std::vector<uint16_t> max_deg;     // per robot: most springs of one mass (capped 65535)
   // per robot: distinct (rest length, material) pairs of its springs as
   // rest-length bits << 8 | compact material, sorted; empty when there are
   // more than the mass-parallel kernels' pair table holds
@@ -264,13 +265,24 @@ bool fits_mass(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
 // MH_KERNEL_* family bit of a variant (the gather flavour of the
 // mass-parallel kernel is a family of its own).
 int family(const KernelVariant& v) {
-  return v.kind == MH_KERNEL_MASS && v.chunked ? MH_KERNEL_MASS_GATHER : v.kind;
+  if (v.kind == MH_KERNEL_MASS && v.chunked) return MH_KERNEL_MASS_GATHER;
+  if (v.kind == MH_KERNEL_STREAM && v.chunked == 2) return MH_KERNEL_MASS_STREAM;
+  return v.kind;
 }
 constexpr int kResident = MH_KERNEL_LDS | MH_KERNEL_MASS | MH_KERNEL_MASS_GATHER;
-constexpr int kAllFamilies = kResident | MH_KERNEL_STREAM;
+constexpr int kStreaming = MH_KERNEL_STREAM | MH_KERNEL_MASS_STREAM;
+constexpr int kAllFamilies = kResident | kStreaming;
 
 bool fits_resident(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
   return v.kind == MH_KERNEL_MASS ? fits_mass(s, v, r, n_mat) : fits_lds_alone(s, v, r, n_mat);
+}
+
+// The mass-parallel streaming kernel: the robot's masses fill at most
+// block * mpt virtual lanes, no mass has more than 255 springs (u8
+// counts), the materials and the null one fit its table.
+bool fits_mstream(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
+  return robot_masses(s, r) <= v.block * v.mpt && s->max_deg[r] <= 255 && n_mat + 1 <= kMStreamMats &&
+         lds_bytes_mstream(v.block, v.mpt) <= static_cast<size_t>(s->lds_limit);
 }
 
 // Gather-entry capacity of a chunk of C springs for a robot of m masses: 2C
@@ -311,8 +323,9 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
     if (!lv || !fits_resident(s, *lv, r, n_mat)) {
       const int m = robot_masses(s, r);
       if (!sv) return false;
-      if (sv->chunked ? chunk_springs(s, *sv, m, n_mat) == 0
-                      : lds_bytes_stream(std::max(m, 1), n_mat) > static_cast<size_t>(s->lds_limit))
+      if (sv->chunked == 2 ? !fits_mstream(s, *sv, r, n_mat)
+          : sv->chunked    ? chunk_springs(s, *sv, m, n_mat) == 0
+                           : lds_bytes_stream(std::max(m, 1), n_mat) > static_cast<size_t>(s->lds_limit))
         return false;
       PackDesc d{};
       d.mass_begin = s->mass_off[r];
@@ -386,7 +399,9 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
     plan.parts.push_back(std::move(lp));
   }
   if (!stp.packs.empty()) {
-    if (sv->chunked) {
+    if (sv->chunked == 2) {
+      stp.lds = lds_bytes_mstream(sv->block, sv->mpt);
+    } else if (sv->chunked) {
       // one chunk size for the part: the one that fits its largest robot
       stp.spring_cap = chunk_springs(s, *sv, std::max(stp.mass_cap, 1), n_mat);
       stp.csr_cap = chunk_csr_cap(stp.spring_cap, std::max(stp.mass_cap, 1));
@@ -455,6 +470,77 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
         crow.push_back(static_cast<uint16_t>(count[nm]));
     }
     ctab.push_back(static_cast<uint32_t>(ccsr.size()) - d.csr_begin);
+  }
+  std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
+    return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
+  });
+}
+
+// Mass-parallel streaming part (one robot per pack): its masses in
+// virtual-lane order, most springs first (ties by index), into perm; per
+// virtual lane its spring count (u8, mcnt at d.row_begin); per group of 64
+// virtual lanes the offset of its records, in records from d.csr_begin
+// (u32, row at d.pad0); the records of group g as [k][64] for k below the
+// group's longest list: per spring of the mass in ascending spring index
+// {16 * slot of the other endpoint | 8 * compact material << 16, rest
+// length bits}, padded with {16 * ghost | 8 * n_mat << 16, 1.0f}.  mrec
+// holds two words per record; d.csr_begin counts records.
+void upload_mstream(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector<uint32_t>& mrec,
+                    std::vector<uint8_t>& mcnt, std::vector<uint32_t>& row) {
+  const int n_mat = static_cast<int>(s->compact_to_id.size());
+  const uint32_t ghost = static_cast<uint32_t>(mass_slots(part.v->block * part.v->mpt) - 1);
+  const uint32_t pad_w0 = 16u * ghost | (8u * static_cast<uint32_t>(n_mat)) << 16;
+  std::vector<uint32_t> deg, order, lane, fill, start;
+  if (mrec.size() & 1u) mrec.push_back(0u);  // records are 8-byte aligned
+  for (PackDesc& d : part.packs) {
+    const uint32_t nm = d.n_mass, ns = d.n_spring;  // one robot: pack-local = robot-local
+    deg.assign(nm, 0);
+    for (uint32_t k = 0; k < ns; k++) {
+      const uint32_t g = d.spring_begin + k;
+      deg[s->act_pairs[2 * g]]++;
+      deg[s->act_pairs[2 * g + 1]]++;
+    }
+    order.resize(nm);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return deg[x] > deg[y]; });
+    lane.assign(nm, 0);
+    for (uint32_t v = 0; v < nm; v++) {
+      lane[order[v]] = v;
+      perm[d.mass_begin + v] = static_cast<uint16_t>(order[v]);
+    }
+    d.row_begin = static_cast<uint32_t>(mcnt.size());
+    for (uint32_t v = 0; v < nm; v++) mcnt.push_back(static_cast<uint8_t>(deg[order[v]]));
+    d.csr_begin = static_cast<uint32_t>(mrec.size() / 2);
+    d.pad0 = static_cast<uint32_t>(row.size());
+    const uint32_t groups = (nm + 63) / 64;
+    start.assign(groups + 1, 0);
+    for (uint32_t gi = 0; gi < groups; gi++) {
+      uint32_t longest = 0;
+      for (uint32_t v = 64 * gi; v < std::min(nm, 64 * gi + 64); v++) longest = std::max(longest, deg[order[v]]);
+      row.push_back(start[gi]);
+      start[gi + 1] = start[gi] + 64 * longest;
+    }
+    const size_t base = mrec.size();
+    mrec.resize(base + 2 * static_cast<size_t>(start[groups]));
+    for (size_t i = base; i < mrec.size(); i += 2) {
+      mrec[i] = pad_w0;
+      mrec[i + 1] = 0x3f800000u;
+    }
+    fill.assign(nm, 0);
+    for (uint32_t k = 0; k < ns; k++) {  // ascending spring index
+      const uint32_t g = d.spring_begin + k;
+      const uint32_t a = s->act_pairs[2 * g], b = s->act_pairs[2 * g + 1];
+      uint32_t lb;
+      std::memcpy(&lb, &s->act_lbar[g], 4);
+      const uint32_t m8 = (8u * s->act_mat[g]) << 16;
+      for (int e = 0; e < 2; e++) {
+        const uint32_t own = e ? b : a, other = e ? a : b;
+        const uint32_t v = lane[own];
+        const size_t at = base + 2 * (static_cast<size_t>(start[v / 64]) + 64u * fill[own]++ + (v & 63u));
+        mrec[at] = 16u * static_cast<uint32_t>(mass_slot(static_cast<int>(other))) | m8;
+        mrec[at + 1] = lb;
+      }
+    }
   }
   std::stable_sort(part.packs.begin(), part.packs.end(), [](const PackDesc& x, const PackDesc& y) {
     return static_cast<uint64_t>(x.n_spring) + x.n_mass > static_cast<uint64_t>(y.n_spring) + y.n_mass;
@@ -635,9 +721,11 @@ void upload_plan(mh_sim* s, Plan& plan) {
   for (Part& part : plan.parts) {
     const bool padded = part.v->kind == MH_KERNEL_LDS;
     need_scratch |= part.v->kind == MH_KERNEL_STREAM && !part.v->chunked;
-    if (part.v->kind == MH_KERNEL_MASS) {
+    if (part.v->kind == MH_KERNEL_STREAM && part.v->chunked == 2) {
+      upload_mstream(s, part, perm, mrec, mcnt, row);
+    } else if (part.v->kind == MH_KERNEL_MASS) {
       upload_mass(s, part, perm, mrec, mcnt, lbtab);
-    } else if (part.v->chunked) {
+    } else if (part.v->chunked == 1) {
       upload_chunked(s, part, spair, ccsr, crow, ctab);
     } else {
       // entries: 2 * (3 * slot) (LDS parts) or 2 * slot (streaming parts),
@@ -738,7 +826,7 @@ std::vector<const KernelVariant*> candidates(const mh_sim* s, int kinds) {
 // (chunked ones first) that fits them.
 void choose_default_plan(mh_sim* s) {
   auto lc = candidates(s, kResident);
-  auto sc = candidates(s, MH_KERNEL_STREAM);
+  auto sc = candidates(s, kStreaming);
   const KernelVariant* sv = sc.empty() ? nullptr : sc.front();
   const int n_mat = static_cast<int>(s->compact_to_id.size());
   const KernelVariant* best = nullptr;
@@ -1086,6 +1174,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   // table)
   std::vector<uint32_t> gpad(R, 0);
   std::vector<uint8_t> mside(R, 0);
+  std::vector<uint16_t> mdeg(R, 0);
   std::vector<std::vector<uint64_t>> psets(R);
   mh::parallel_for(static_cast<size_t>(R), [&](size_t ri) {
     const int r = static_cast<int>(ri);
@@ -1106,13 +1195,15 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
         }
       }
       if (ps.size() > static_cast<size_t>(mh::kMassPairNull)) ps.clear();
-      uint32_t g = 0, side = 0;
+      uint32_t g = 0, side = 0, deg = 0;
       for (size_t i = 0; i < plus.size(); i++) {
         g += (plus[i] + minus[i] + 3u) & ~3u;
         side = std::max(side, std::max(plus[i], minus[i]));
+        deg = std::max(deg, plus[i] + minus[i]);
       }
       gpad[r] = g;
       mside[r] = static_cast<uint8_t>(std::min<uint32_t>(side, 255));
+      mdeg[r] = static_cast<uint16_t>(std::min<uint32_t>(deg, 65535));
     }
   });
 
@@ -1141,6 +1232,7 @@ int mh_sim_set_population(mh_sim* s, const mh_population* pop) {
   s->active_off = ao;
   s->gather_pad = std::move(gpad);
   s->max_side = std::move(mside);
+  s->max_deg = std::move(mdeg);
   s->pair_set = std::move(psets);
   s->act_pairs = std::move(ap);
   s->act_lbar = alb;
@@ -1377,6 +1469,7 @@ int mh_sim_set_voxel_population(mh_sim* s, int n_robots, int nx, int ny, int nz,
   });
   s->gather_pad.clear();
   s->max_side.clear();
+  s->max_deg.clear();
   s->pair_set.clear();
   s->act_pairs.clear();
   s->active_src.clear();
@@ -1637,7 +1730,7 @@ int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
       info->stream_block_size = part.v->block;
       info->n_stream_robots = part.n_robots;
       for (const mh::PackDesc& d : part.packs) info->n_stream_springs += static_cast<int>(d.n_spring);
-      info->stream_chunk = part.v->chunked ? part.spring_cap : 0;
+      info->stream_chunk = part.v->chunked == 1 ? part.spring_cap : 0;
     }
   }
   info->robots_per_block = s->plan.rpb;
@@ -1711,7 +1804,7 @@ bool apply_choice(mh_sim* s, const TuneChoice& c) {
     auto cs = candidates(s, kinds);
     return std::find(cs.begin(), cs.end(), v) != cs.end();
   };
-  if (!allowed(c.lds, kResident) || !allowed(c.stream, MH_KERNEL_STREAM)) return false;
+  if (!allowed(c.lds, kResident) || !allowed(c.stream, kStreaming)) return false;
   Plan p;
   if (!make_plan(s, c.lds, c.stream, c.rpb, p)) return false;
   upload_plan(s, p);
@@ -1836,7 +1929,7 @@ int mh_sim_autotune(mh_sim* s, int n_steps) {
   s->cfg.use_graph = 0;
   s->substeps = 0;
   auto lc = mh::candidates(s, mh::kResident);
-  auto sc = mh::candidates(s, MH_KERNEL_STREAM);
+  auto sc = mh::candidates(s, mh::kStreaming);
   lc.push_back(nullptr);  // everything streamed
   // streaming variant for the LDS search: the first that can take every robot
   const mh::KernelVariant* sv0 = sc.empty() ? nullptr : sc.front();

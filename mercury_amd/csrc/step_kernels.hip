@@ -567,6 +567,135 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
   }
 }
 
+// ------------------------------------------- mass-parallel streaming kernel
+// For robots too large for the register-resident mass kernel (NN robots:
+// 25 springs per mass, up to 2000 masses).  Like step_mass, every spring is
+// evaluated by both of its endpoints from their own sides and each mass adds
+// its list in ascending spring index (the canonical sum, one accumulator),
+// so there is no term buffer, no gather list and one barrier per substep;
+// unlike it, the records stream from memory every substep.  Each thread
+// serves MPT virtual lanes v = tid + j BLOCK; virtual lanes are the robot's
+// masses by degree, most springs first (p.mperm).  A record is 8 bytes:
+// {16 * slot of the other endpoint | 8 * material << 16, rest length bits},
+// both LDS byte offsets as in step_mass; the actuated length is
+// fma(rest, R[material], rest).  Records of the 64 virtual lanes of group g
+// lie at p.row[d.row_begin + g] (in records, from d.csr_begin) as [k][64],
+// k below the group's longest list; shorter lists are padded with records
+// of the null material (stiffness 0, a ghost far from every mass), whose
+// terms are signed zeros.  A lane loads its records four at a time, the
+// next four in flight while the current ones are evaluated.  LDS: position
+// buffers float4[2][mass_slots(BLOCK MPT)] (read one, write the other) |
+// per material {R(t), k} float2[2][kMStreamMats] (the null material last).
+constexpr int kMStreamBatch = 4;
+template <int BLOCK, int MPT>
+constexpr int mstream_pos_bytes() { return 16 * mass_slots(BLOCK * MPT); }
+
+// WPE: waves per SIMD the register allocation must allow (8 at BLOCK 1024:
+// at most 64 VGPRs, so two workgroups share a CU).
+template <int SOLVER, int BLOCK, int MPT, int WPE>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void step_mstream(StepParams p) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const PackDesc d = p.packs[blockIdx.x];
+  const int nm = static_cast<int>(d.n_mass);
+  constexpr int kSlots = mass_slots(BLOCK * MPT);
+  constexpr int kGhost = kSlots - 1;
+  constexpr int kPos = mstream_pos_bytes<BLOCK, MPT>();
+  constexpr int kMat = 2 * kPos;
+  const int tid = threadIdx.x;
+  const int n_mat = p.n_mat;
+  const uint2* recs = reinterpret_cast<const uint2*>(p.mrec) + d.csr_begin;
+
+  MassState ms[MPT];
+  int slot[MPT], mass[MPT], nr_w[MPT];
+  uint32_t gbase[MPT];
+#pragma unroll
+  for (int j = 0; j < MPT; ++j) {
+    const int v = tid + j * BLOCK;
+    const bool live = v < nm;
+    mass[j] = live ? p.mperm[d.mass_begin + v] : 0;
+    slot[j] = live ? mass_slot(mass[j]) : kGhost;
+    if (live) load_mass(ms[j], p.pos, p.vel, d.mass_begin + mass[j], p);
+    int n = live ? p.mcnt[d.row_begin + v] : 0;
+    for (int off = 32; off > 0; off >>= 1) n = max(n, __shfl_xor(n, off));
+    nr_w[j] = n;
+    gbase[j] = (v < ((nm + 63) & ~63)) ? p.row[d.pad0 + (v >> 6)] + (v & 63) : 0u;
+    if (live) reinterpret_cast<float4*>(smem)[slot[j]] = lds_position<SOLVER>(ms[j]);
+  }
+  if (tid == 0) {
+    reinterpret_cast<float4*>(smem)[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
+    reinterpret_cast<float4*>(smem + kPos)[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
+  }
+  float t = p.clock[p.launch];
+  auto mat_table = [&](int buf, float tm) {
+    for (int i = tid; i <= n_mat; i += BLOCK) {
+      float2 e = make_float2(0.f, 0.f);  // the null material: R 0, stiffness 0
+      if (i < n_mat)
+        e = make_float2(actuation(p.mat_dL0[i], p.mat_omega[i], p.mat_phi[i], tm),
+                        (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[i] : p.mat_k[i]);
+      reinterpret_cast<float2*>(smem + kMat + buf * 8 * kMStreamMats)[i] = e;
+    }
+  };
+  mat_table(0, t);
+  __syncthreads();
+
+  for (int step = 0; step < p.n_steps; ++step) {
+    const int cur = step & 1;
+    const unsigned char* pos_r = smem + cur * kPos;
+    unsigned char* pos_w = smem + (1 - cur) * kPos;
+    const unsigned char* mat_r = smem + kMat + cur * 8 * kMStreamMats;
+#pragma unroll
+    for (int j = 0; j < MPT; ++j) {
+      const int v = tid + j * BLOCK;
+      const bool live = v < nm;
+      const float4 own = live ? *reinterpret_cast<const float4*>(pos_r + 16 * slot[j])
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+      const uint2* rv = recs + gbase[j];
+      const int n = nr_w[j];
+      float ax = 0.f, ay = 0.f, az = 0.f;
+      uint2 q[kMStreamBatch];
+#pragma unroll
+      for (int i = 0; i < kMStreamBatch; ++i) q[i] = i < n ? rv[64 * i] : make_uint2(0u, 0u);
+      for (int k0 = 0; k0 < n; k0 += kMStreamBatch) {
+        uint2 nq[kMStreamBatch];
+#pragma unroll
+        for (int i = 0; i < kMStreamBatch; ++i) {
+          const int k = k0 + kMStreamBatch + i;
+          nq[i] = k < n ? rv[64 * k] : make_uint2(0u, 0u);  // past the list: never evaluated
+        }
+#pragma unroll
+        for (int i = 0; i < kMStreamBatch; ++i) {
+          if (k0 + i < n) {
+            const uint32_t r = q[i].x;
+            const float lbar = __uint_as_float(q[i].y);
+            const float2 Rk = *reinterpret_cast<const float2*>(mat_r + (r >> 16));
+            const float4 other = *reinterpret_cast<const float4*>(pos_r + (r & 0xffffu));
+            float cx, cy, cz, lambda;
+            spring_term<SOLVER>(own, other, lbar, Rk.x, Rk.y, cx, cy, cz, lambda);
+            ax = ax + cx; ay = ay + cy; az = az + cz;
+          }
+          // one record at a time: keeps the registers of the unrolled batch bounded
+          __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int i = 0; i < kMStreamBatch; ++i) q[i] = nq[i];
+      }
+      if (live) *reinterpret_cast<float4*>(pos_w + 16 * slot[j]) = integrate<SOLVER>(ms[j], ax, ay, az, p);
+    }
+    t = t + p.dt;
+    mat_table(1 - cur, t);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int j = 0; j < MPT; ++j) {
+    if (tid + j * BLOCK < nm) {
+      const uint32_t gi = d.mass_begin + mass[j];
+      p.pos[gi] = make_float4(ms[j].x, ms[j].y, ms[j].z, ms[j].m);
+      p.vel[gi] = make_float4(ms[j].vx, ms[j].vy, ms[j].vz, 0.f);
+    }
+  }
+}
+
 // ------------------------------------------------------- mass-parallel kernel
 // One lane per mass; every spring is evaluated by both of its endpoints
 // (the same operands in the same order, so both get the same bits), which
@@ -1014,6 +1143,9 @@ __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint3
   {SOLVER, MH_KERNEL_STREAM, 1, B, 0, M, reinterpret_cast<const void*>(&step_chunk<SOLVER, B, M>)}
 #define MH_STREAM(SOLVER, B) \
   {SOLVER, MH_KERNEL_STREAM, 0, B, 0, 0, reinterpret_cast<const void*>(&step_stream<SOLVER, B>)}
+#define MH_MSTREAM(B, M, W)                                                                    \
+  {MH_SOLVER_FORCE, MH_KERNEL_STREAM, 2, B, 0, M,                                           \
+   reinterpret_cast<const void*>(&step_mstream<MH_SOLVER_FORCE, B, M, W>)}
 #define MH_LDS_SET(SOLVER)                                                         \
   MH_MASSG(SOLVER, 768, 13), MH_MASSG(SOLVER, 512, 16), MH_MASSG(SOLVER, 256, 16),     \
       MH_MASS(SOLVER, 768, 13), MH_MASS(SOLVER, 1024, 13), MH_MASS(SOLVER, 512, 16),   \
@@ -1023,7 +1155,10 @@ __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint3
       MH_CHUNK(SOLVER, 1024, 2), MH_CHUNK(SOLVER, 512, 4),                            \
       MH_STREAM(SOLVER, 1024), MH_STREAM(SOLVER, 512), MH_STREAM(SOLVER, 256)
 
-const KernelVariant kVariants[] = {MH_LDS_SET(MH_SOLVER_XPBD_REF), MH_LDS_SET(MH_SOLVER_FORCE)};
+// the mass-parallel streaming kernel serves the force solver (the XPBD
+// step's stress output needs each spring's index)
+const KernelVariant kVariants[] = {MH_LDS_SET(MH_SOLVER_XPBD_REF), MH_LDS_SET(MH_SOLVER_FORCE),
+                                   MH_MSTREAM(1024, 2, 4), MH_MSTREAM(1024, 2, 8), MH_MSTREAM(512, 4, 4)};
 
 }  // namespace
 
@@ -1061,6 +1196,10 @@ size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat) {
   b += 8u * static_cast<size_t>(n_mat);                           // actuation, stiffness
   b += 4u * static_cast<size_t>(kChunkMax + 1);                   // chunk entry offsets
   return b;
+}
+
+size_t lds_bytes_mstream(int block, int mpt) {
+  return 2u * 16u * static_cast<size_t>(mass_slots(block * mpt)) + 2u * 8u * kMStreamMats;
 }
 
 size_t lds_bytes_stream(int mass_cap, int n_mat) {

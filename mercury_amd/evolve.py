@@ -451,6 +451,8 @@ def main(argv=None):
                    help="retrain every generation while the surrogate ranks below the fallback rho")
     c.add_argument("--mirror", action="store_true",
                    help="train the surrogate on every genome and its z-mirror, predict from both")
+    c.add_argument("--max-train", type=int, default=200000,
+                   help="each retrain uses the most recent this many simulated genomes")
     rp = sub.add_parser("report", help="evaluations-to-threshold table from compare logs")
     rp.add_argument("logs", nargs="+", help="[LABEL=]path of a compare log (JSON lines); rows are "
                     "labelled LABEL:plain / LABEL:surrogate")
@@ -493,7 +495,7 @@ def main(argv=None):
                        voxels=a.voxels, duration=a.duration, screen_factor=a.screen_factor,
                        train_epochs=a.epochs, seed_start=a.seed_start, surrogate_kind=a.surrogate,
                        audit=a.audit, warm_start=a.warm_start, fallback_retrain=a.fallback_retrain,
-                       mirror=a.mirror,
+                       mirror=a.mirror, max_train=a.max_train,
                        modes={"both": (False, True), "plain": (False,), "surrogate": (True,)}[a.only])
         for use in (False, True):
             e = [x["evals_to_threshold"] for x in rows if x["surrogate"] == use]

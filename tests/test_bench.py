@@ -45,9 +45,11 @@ def test_kernel_labels():
     assert label(mh.MH_KERNEL_MASS_GATHER | mh.MH_KERNEL_STREAM) == "mass-gather+stream"
     assert label(mh.MH_KERNEL_STREAM, chunk=5120) == "stream (chunked)"
     assert label(mh.MH_KERNEL_LDS | mh.MH_KERNEL_STREAM, chunk=512) == "lds+stream (chunked)"
-    # every --kernel choice names a family the library knows, plus streaming
+    assert label(mh.MH_KERNEL_MASS_STREAM) == "mass-stream"
+    # every --kernel choice names a family the library knows, plus a
+    # streaming one
     for name, bits in bench.KERNEL_FAMILIES.items():
-        assert bits == 0 or bits & mh.MH_KERNEL_STREAM
+        assert bits == 0 or bits & (mh.MH_KERNEL_STREAM | mh.MH_KERNEL_MASS_STREAM)
 
 
 def test_cpu_baseline_leg_runs_the_mirror():

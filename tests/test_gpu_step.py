@@ -97,6 +97,23 @@ def test_launch_plan_does_not_change_results(cfg):
     assert np.array_equal(bits(sa), bits(sb))
 
 
+@pytest.mark.parametrize("cfg", [dict(kernel=mh.MH_KERNEL_MASS_STREAM),
+                                 dict(kernel=mh.MH_KERNEL_MASS_STREAM, block_size=512, substeps_per_launch=7),
+                                 dict(kernel=mh.MH_KERNEL_MASS | mh.MH_KERNEL_MASS_STREAM, use_graph=0)],
+                         ids=lambda c: "-".join(f"{k}{v}" for k, v in c.items()))
+def test_mass_stream_kernel_on_land_bitexact(cfg):
+    """The mass-parallel streaming kernel serves the force solver only: on
+    an uneven voxel batch on land (ground contact, friction) it gives the
+    default plan's trajectory bit for bit."""
+    print(f"mass-parallel streaming kernel, land, {cfg}", flush=True)
+    case = uneven_case(seed=9)
+    _, pos_a, vel_a, _ = run_gpu(case, FORCE, 120, 1e-4, LAND)
+    sim, pos_b, vel_b, _ = run_gpu(case, FORCE, 120, 1e-4, LAND, **cfg)
+    assert sim.info().kernel & mh.MH_KERNEL_MASS_STREAM or cfg["kernel"] != mh.MH_KERNEL_MASS_STREAM
+    assert np.array_equal(bits(pos_a), bits(pos_b))
+    assert np.array_equal(bits(vel_a), bits(vel_b))
+
+
 def test_graph_replays_match_eager():
     print("graph replay", flush=True)
     case = workload_case(n_robots=5, voxels=4, seed=4)

@@ -105,3 +105,32 @@ def test_large_nn_robots_many_chunks_bitexact():
     p, v, _ = mirror(case, mh.MH_SOLVER_FORCE, 100, 1e-4, WATER)
     assert np.array_equal(bits(pos), bits(p))
     assert np.array_equal(bits(vel), bits(v))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block", [1024, 512])
+def test_nn_robots_mass_stream_kernel_bitexact(block):
+    """step_mstream (one lane per mass, each evaluating all of its springs,
+    records streamed every substep): robots of 60-2000 masses, so threads
+    serve one to four masses, give the mirror's trajectory bit for bit, as
+    does the chunked kernel on the same robots."""
+    print(f"nn robots on the mass-parallel streaming kernel, block {block}", flush=True)
+    ms = mh.workload_materials()
+    els = robots.nn_population(ms, 4, mass_range=(60, 2000), K=25, seed=5)
+    assert max(e.n_masses for e in els) > 1024
+    case = Case(els, ms, mh.materials.WORKLOAD_ROWS)
+    sim = gpu_sim(case, mh.MH_SOLVER_FORCE, 1e-4, WATER, kernel=mh.MH_KERNEL_MASS_STREAM, block_size=block)
+    info = sim.info()
+    assert info.kernel == mh.MH_KERNEL_MASS_STREAM and info.n_stream_robots == 4
+    assert info.stream_block_size == block
+    pos0 = sim.collect_arrays()[0]
+    sim.Step(150)
+    pos, vel, _ = sim.collect_arrays()
+    p, v, _ = mirror(case, mh.MH_SOLVER_FORCE, 150, 1e-4, WATER)
+    assert np.array_equal(bits(pos), bits(p))
+    assert np.array_equal(bits(vel), bits(v))
+    assert np.abs(pos - pos0).max() > 1e-6
+    chunk = gpu_sim(case, mh.MH_SOLVER_FORCE, 1e-4, WATER, kernel=mh.MH_KERNEL_STREAM)
+    assert chunk.info().stream_chunk > 0
+    chunk.Step(150)
+    assert np.array_equal(bits(chunk.collect_arrays()[0]), bits(pos))
