@@ -1158,7 +1158,7 @@ __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint3
 // the mass-parallel streaming kernel serves the force solver (the XPBD
 // step's stress output needs each spring's index)
 const KernelVariant kVariants[] = {MH_LDS_SET(MH_SOLVER_XPBD_REF), MH_LDS_SET(MH_SOLVER_FORCE),
-                                   MH_MSTREAM(1024, 2, 4), MH_MSTREAM(1024, 2, 8), MH_MSTREAM(512, 4, 4)};
+                                   MH_MSTREAM(1024, 2, 8), MH_MSTREAM(1024, 2, 4), MH_MSTREAM(512, 4, 4)};
 
 }  // namespace
 
