@@ -34,9 +34,10 @@ STREAM_BYTES_PER_SPRING = 65.0
 # chunked streaming kernel (terms in LDS chunks), HBM bytes per
 # spring-update: record 9 B + two u16 gather entries 4 B
 CHUNK_HBM_BYTES_PER_SPRING = 13.0
-# mass-parallel streaming kernel: each endpoint reads its 8 B record of the
-# spring (other endpoint and material, rest length) every substep
-MSTREAM_HBM_BYTES_PER_SPRING = 16.0
+# mass-parallel streaming kernel: each endpoint reads its record of the
+# spring (other endpoint and material, rest length) every substep, 8 B, or
+# 6 B in the compact format (mh_sim_info.stream_record_bytes)
+MSTREAM_RECORD_ENDPOINTS = 2
 # the spring record alone (two u16 endpoints, f32 rest length, u8 material):
 # the least any kernel that streams the springs reads per spring-update
 SPRING_RECORD_BYTES = 9.0
@@ -46,6 +47,9 @@ HBM_PEAK_GBS = 8000.0
 # write 12 B and gather read 12 B) + 48 B per mass (pos/vel read + write).
 LDS_BYTES_PER_SPRING = 57.0
 LDS_BYTES_PER_MASS = 48.0
+# the force solver's stored term is {D, f}, 16 B instead of 12: 4 B more on
+# the write and on the gather read
+FORCE_TERM_EXTRA_BYTES = 8.0
 # mass-parallel kernel: both endpoints evaluate a spring, each reading the
 # other endpoint's position (12 B) and its (rest length, material) pair's
 # actuated length and stiffness (8 B); per mass its own position read and
@@ -174,13 +178,14 @@ def measured_traffic(workload, kernel):
 def roofline_head(bound, achieved, peak, unit, extra):
     """bound / achieved / peak / unit / frac of the dominant kernel.  With
     hardware counters for it, the bound is VALU issue when the kernel
-    issues at least 0.9 wave64 VALU instructions per CU-cycle (independent
-    FMA chains sustain 0.92 on MI355X): then the kernel is at the issue
-    limit, and the byte model's fraction is reported beside it; frac_survey
+    issues at least 0.9 wave64 VALU instructions, or keeps the VALU busy at
+    least 0.9 of the cycles, per CU-cycle (independent FMA chains sustain
+    0.92 instructions on MI355X): then the kernel is at the issue limit,
+    and the byte model's fraction is reported beside it; frac_survey
     is the fraction on the algorithmic LDS model of 57 B per spring plus
     48 B per mass per substep."""
     v = extra.get("valu_issue")
-    if v and v["per_cu_cycle"] >= 0.9:
+    if v and max(v["per_cu_cycle"], v.get("busy_per_cu_cycle") or 0.0) >= 0.9:
         out = {"bound": "valu", "achieved": v["achieved"], "peak": v["peak"], "unit": v["unit"],
                "frac": v["frac"], "byte_model_bound": bound, "byte_model_achieved": achieved,
                "byte_model_peak": peak, "byte_model_unit": unit,
@@ -363,8 +368,8 @@ def main():
     if info.kernel & mh.MH_KERNEL_MASS_STREAM and info.n_stream_springs > 0:
         # the streamed robots carry the work: HBM bytes of their records
         bound, peak, unit = "hbm", HBM_PEAK_GBS, "GB/s"
-        per_launch = MSTREAM_HBM_BYTES_PER_SPRING * info.n_stream_springs * steps_per_launch
-        per_unit = MSTREAM_HBM_BYTES_PER_SPRING
+        per_unit = MSTREAM_RECORD_ENDPOINTS * float(info.stream_record_bytes)
+        per_launch = per_unit * info.n_stream_springs * steps_per_launch
         if kernel_ms > 0:
             rec = SPRING_RECORD_BYTES * info.n_stream_springs * steps_per_launch / (kernel_ms * 1e-3) / 1e9
             hbm_extra = {"record_model": {"bytes_per_spring_update": SPRING_RECORD_BYTES,
@@ -378,8 +383,9 @@ def main():
         # LDS-resident kernel, or chunked streaming kernel whose terms also
         # live in LDS: LDS bytes over all springs
         bound, peak, unit = "lds", lds_peak, "GB/s"
-        per_launch = (LDS_BYTES_PER_SPRING * n_springs + LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
-        per_unit = LDS_BYTES_PER_SPRING + LDS_BYTES_PER_MASS * n_masses / n_springs
+        bps = LDS_BYTES_PER_SPRING + (FORCE_TERM_EXTRA_BYTES if solver == mh.MH_SOLVER_FORCE else 0.0)
+        per_launch = (bps * n_springs + LDS_BYTES_PER_MASS * n_masses) * steps_per_launch
+        per_unit = bps + LDS_BYTES_PER_MASS * n_masses / n_springs
         if info.stream_chunk > 0 and kernel_ms > 0:
             hb = CHUNK_HBM_BYTES_PER_SPRING * info.n_stream_springs * steps_per_launch
             hbm_extra = {"hbm_bytes_per_spring_update": CHUNK_HBM_BYTES_PER_SPRING,
@@ -418,6 +424,11 @@ def main():
             "achieved": valu, "peak": VALU_PEAK_WAVE_INSTR, "unit": "wave64 instr/s",
             "frac": valu / VALU_PEAK_WAVE_INSTR,
             "per_cu_cycle": per_cu_cycle,
+            # issue cycles: a transcendental holds the VALU two cycles
+            # (SQ_ACTIVE_INST_VALU = SQ_INSTS_VALU + SQ_INSTS_VALU_TRANS_F32)
+            "busy_per_cu_cycle": (counters["active_valu_per_launch"] /
+                                  (MI355X_CUS * counters["gpu_cycles_per_launch"])
+                                  if counters.get("active_valu_per_launch") else None),
             "fma_chains_per_cu_cycle": counters.get("fma_chains_per_cu_cycle"),
             "transcendental_share": counters.get("trans_per_launch", 0) / counters["valu_per_launch"],
             "lane_instr_per_active_spring_update": valu_launch * 64 / (n_active * steps_per_launch)}

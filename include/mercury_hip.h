@@ -252,6 +252,10 @@ typedef struct {
                             simulate call replays the last one when its
                             step count, stress flag, substeps per launch and
                             plan are unchanged, whatever the clock) */
+  int stream_record_bytes; /* bytes per spring record of the mass-parallel
+                              streaming kernel (8, or 6 for populations of
+                              at most 15 compact materials); 0 when that
+                              kernel is not in the plan */
 } mh_sim_info;
 int mh_sim_get_info(const mh_sim* sim, mh_sim_info* info);
 /* Times the candidate launch plans (LDS variant x robots per block, then

@@ -53,7 +53,8 @@ class SimInfo(ctypes.Structure):
         "n_robots", "n_masses", "n_springs", "n_active_springs", "kernel", "block_size",
         "n_blocks", "substeps_per_launch", "lds_bytes", "max_masses_per_block",
         "max_springs_per_block", "n_lds_robots", "n_stream_robots", "stream_block_size",
-        "robots_per_block", "n_stream_springs", "stream_chunk", "graph_builds")]
+        "robots_per_block", "n_stream_springs", "stream_chunk", "graph_builds",
+        "stream_record_bytes")]
 
 
 # name -> (restype, argtypes)

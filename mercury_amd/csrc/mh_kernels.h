@@ -70,7 +70,8 @@ struct KernelVariant {
                   // MASS kind: 1 = terms computed once and gathered (step_massg)
   int block;      // threads per workgroup
   int nspt;       // springs per thread held in registers (LDS kind); records
-                  // per endpoint side (MASS kind)
+                  // per endpoint side (MASS kind); record bytes, 8 or 6
+                  // (mass-parallel streaming kind)
   int mpt;        // masses per thread
   const void* fn; // kernel symbol
   const void* fn_stress = nullptr;  // variant used when stress is tracked (null: fn)
@@ -118,12 +119,15 @@ __host__ __device__ constexpr int massg_term_base(int block) {
 __host__ __device__ constexpr uint32_t massg_term_offset(int block, int maxp, uint32_t lane, uint32_t k) {
   return static_cast<uint32_t>(massg_term_base(block)) + 12u * (static_cast<uint32_t>(maxp) * lane + k);
 }
-// Bytes of the LDS spring-term array (3 floats per slot, spring_cap + 2
+// Words of a stored spring term by solver: xpbd (cx, cy, cz), force (Dx,
+// Dy, Dz, f).
+__host__ __device__ constexpr int solver_term_words(int solver) { return solver == MH_SOLVER_FORCE ? 4 : 3; }
+// Bytes of the LDS spring-term array (tw floats per slot, spring_cap + 2
 // slots), rounded up to 16 B so the float4 positions after it stay aligned.
-__host__ __device__ inline size_t lds_term_bytes(int spring_cap) {
-  return (12u * static_cast<size_t>(spring_cap + 2) + 15u) & ~static_cast<size_t>(15);
+__host__ __device__ inline size_t lds_term_bytes(int spring_cap, int tw) {
+  return (4u * static_cast<size_t>(tw) * static_cast<size_t>(spring_cap + 2) + 15u) & ~static_cast<size_t>(15);
 }
-size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat);
+size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat, int tw);
 size_t lds_bytes_stream(int mass_cap, int n_mat);
 size_t lds_bytes_mass(int block, int n_mat);
 // Chunked streaming kernel: per lane and chunk at most kChunkSpl springs,
@@ -132,11 +136,14 @@ size_t lds_bytes_mass(int block, int n_mat);
 // n_mass + 1 entries padded to whole quads.
 constexpr int kChunkSpl = 8, kChunkEpl = 8, kChunkRpl = 2, kChunkMax = 256;
 __host__ __device__ constexpr int chunk_row_quads(int n_mass) { return (n_mass + 1 + 3) / 4; }
-size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat);
+size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat, int tw);
 size_t lds_bytes_massg(int block, int maxp);
 // Mass-parallel streaming kernel (step_mstream): its per-material table
-// holds kMStreamMats entries, the compact materials and a null one.
+// holds kMStreamMats entries, the compact materials and a null one.  The
+// 6-byte record format holds the material in 4 bits (kMStream6Mats
+// entries) and 16 * slot in the other 12 of a u16.
 constexpr int kMStreamMats = 256;
+constexpr int kMStream6Mats = 16;
 size_t lds_bytes_mstream(int block, int mpt);
 
 hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes,

@@ -249,8 +249,9 @@ int robot_active(const mh_sim* s, int r) { return static_cast<int>(s->active_off
 
 bool fits_lds_alone(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
   const int m = robot_masses(s, r), sp = robot_active(s, r), g = static_cast<int>(s->gather_pad[r]);
-  return m <= v.block * v.mpt && sp <= v.block * v.nspt && 6 * (sp + 1) + 1 <= 65535 && g <= 65535 &&
-         lds_bytes_lds(std::max(m, 1), std::max(sp, 1), g, n_mat) <= static_cast<size_t>(s->lds_limit);
+  const int tw = solver_term_words(s->cfg.solver);
+  return m <= v.block * v.mpt && sp <= v.block * v.nspt && 2 * tw * (sp + 1) + 1 <= 65535 && g <= 65535 &&
+         lds_bytes_lds(std::max(m, 1), std::max(sp, 1), g, n_mat, tw) <= static_cast<size_t>(s->lds_limit);
 }
 
 bool fits_mass(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
@@ -281,6 +282,8 @@ bool fits_resident(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
 // block * mpt virtual lanes, no mass has more than 255 springs (u8
 // counts), the materials and the null one fit its table.
 bool fits_mstream(const mh_sim* s, const KernelVariant& v, int r, int n_mat) {
+  const bool compact = v.nspt == 6;
+  if (compact && (n_mat + 1 > kMStream6Mats || 16 * mass_slots(v.block * v.mpt) > 65536)) return false;
   return robot_masses(s, r) <= v.block * v.mpt && s->max_deg[r] <= 255 && n_mat + 1 <= kMStreamMats &&
          lds_bytes_mstream(v.block, v.mpt) <= static_cast<size_t>(s->lds_limit);
 }
@@ -297,10 +300,12 @@ int chunk_springs(const mh_sim* s, const KernelVariant& v, int m, int n_mat) {
   if (m > v.block * v.mpt || chunk_row_quads(m) > kChunkRpl * v.block) return 0;
   for (int C : {8192, 6144, 5120, 4096, 3072, 2048, 1536, 1024, 512}) {
     const int entries = chunk_csr_cap(C, m);
-    if (entries > 65535 || 3 * (C + 1) > 65535) continue;
+    if (entries > 65535 || 2 * solver_term_words(s->cfg.solver) * (C + 1) + 1 > 65535) continue;
     // what one chunk puts in flight per lane
     if (C > kChunkSpl * v.block || entries / 4 > kChunkEpl * v.block) continue;
-    if (lds_bytes_chunk(std::max(m, 1), C, entries, n_mat) <= static_cast<size_t>(s->lds_limit)) return C;
+    if (lds_bytes_chunk(std::max(m, 1), C, entries, n_mat, solver_term_words(s->cfg.solver)) <=
+        static_cast<size_t>(s->lds_limit))
+      return C;
   }
   return 0;
 }
@@ -313,6 +318,7 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
   plan = Plan{};
   plan.rpb = rpb;
   const int n_mat = static_cast<int>(s->compact_to_id.size());
+  const int tw = solver_term_words(s->cfg.solver);
   const int cap = rpb > 0 ? rpb : std::numeric_limits<int>::max();
   Part lp, stp;
   lp.v = lv;
@@ -374,11 +380,11 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
       const int m = static_cast<int>(d.n_mass) + robot_masses(s, r);
       const int sp = static_cast<int>(d.n_spring) + robot_active(s, r);
       const int g = gathers + static_cast<int>(s->gather_pad[r]);
-      // gather entries hold 2 * (3 * slot) + 1 in 16 bits: slots up to
+      // gather entries hold 2 * (tw * slot) + 1 in 16 bits: slots up to
       // spring_cap + 1
-      const bool fits = m <= mcap && sp <= scap && 6 * (sp + 1) + 1 <= 65535 && g <= 65535 &&
+      const bool fits = m <= mcap && sp <= scap && 2 * tw * (sp + 1) + 1 <= 65535 && g <= 65535 &&
                         lds_bytes_lds(std::max(lp.mass_cap, m), std::max(lp.spring_cap, sp),
-                                      std::max(lp.csr_cap, g), n_mat) <= static_cast<size_t>(s->lds_limit);
+                                      std::max(lp.csr_cap, g), n_mat, tw) <= static_cast<size_t>(s->lds_limit);
       if (!fits) break;
       d.n_mass = static_cast<uint32_t>(m);
       d.n_spring = static_cast<uint32_t>(sp);
@@ -395,7 +401,7 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
   if (!lp.packs.empty()) {
     lp.lds = lv->kind == MH_KERNEL_MASS
                  ? (lv->chunked ? lds_bytes_massg(lv->block, lv->nspt) : lds_bytes_mass(lv->block, n_mat))
-                 : lds_bytes_lds(std::max(lp.mass_cap, 1), std::max(lp.spring_cap, 1), lp.csr_cap, n_mat);
+                 : lds_bytes_lds(std::max(lp.mass_cap, 1), std::max(lp.spring_cap, 1), lp.csr_cap, n_mat, tw);
     plan.parts.push_back(std::move(lp));
   }
   if (!stp.packs.empty()) {
@@ -405,7 +411,7 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
       // one chunk size for the part: the one that fits its largest robot
       stp.spring_cap = chunk_springs(s, *sv, std::max(stp.mass_cap, 1), n_mat);
       stp.csr_cap = chunk_csr_cap(stp.spring_cap, std::max(stp.mass_cap, 1));
-      stp.lds = lds_bytes_chunk(std::max(stp.mass_cap, 1), stp.spring_cap, stp.csr_cap, n_mat);
+      stp.lds = lds_bytes_chunk(std::max(stp.mass_cap, 1), stp.spring_cap, stp.csr_cap, n_mat, tw);
       for (const PackDesc& d : stp.packs)
         if ((d.n_spring + stp.spring_cap - 1) / stp.spring_cap > static_cast<uint32_t>(kChunkMax)) return false;
     } else {
@@ -424,7 +430,8 @@ bool make_plan(const mh_sim* s, const KernelVariant* lv, const KernelVariant* sv
 void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::vector<uint16_t>& ccsr,
                     std::vector<uint16_t>& crow, std::vector<uint32_t>& ctab) {
   const uint32_t C = static_cast<uint32_t>(part.spring_cap);
-  const uint16_t zero_entry = static_cast<uint16_t>(6u * (C + 1));
+  const uint32_t tw2 = 2u * static_cast<uint32_t>(solver_term_words(s->cfg.solver));
+  const uint16_t zero_entry = static_cast<uint16_t>(tw2 * (C + 1));
   auto pad = [](uint32_t n) { return (n + 3u) & ~3u; };
   std::vector<uint32_t> lpair, count, fill_plus;
   for (PackDesc& d : part.packs) {
@@ -461,7 +468,7 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
       ccsr.resize(base + count[nm], zero_entry);
       fill_plus.assign(count.begin(), count.end() - 1);
       for (uint32_t k = s0; k < s1; k++) {
-        const uint16_t e = static_cast<uint16_t>(6u * (k - s0));
+        const uint16_t e = static_cast<uint16_t>(tw2 * (k - s0));
         ccsr[base + fill_plus[lpair[2 * k]]++] = e;
         ccsr[base + fill_plus[lpair[2 * k + 1]]++] = static_cast<uint16_t>(e + 1u);
       }
@@ -488,10 +495,13 @@ void upload_chunked(mh_sim* s, Part& part, std::vector<uint32_t>& spair, std::ve
 void upload_mstream(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector<uint32_t>& mrec,
                     std::vector<uint8_t>& mcnt, std::vector<uint32_t>& row) {
   const int n_mat = static_cast<int>(s->compact_to_id.size());
+  const bool compact = part.v->nspt == 6;
   const uint32_t ghost = static_cast<uint32_t>(mass_slots(part.v->block * part.v->mpt) - 1);
   const uint32_t pad_w0 = 16u * ghost | (8u * static_cast<uint32_t>(n_mat)) << 16;
+  const uint16_t pad_code = static_cast<uint16_t>(16u * ghost | static_cast<uint32_t>(n_mat));
   std::vector<uint32_t> deg, order, lane, fill, start;
-  if (mrec.size() & 1u) mrec.push_back(0u);  // records are 8-byte aligned
+  // records are 8-byte aligned, the compact format's rest-length quads 16-byte
+  while (mrec.size() % (compact ? 4u : 2u)) mrec.push_back(0u);
   for (PackDesc& d : part.packs) {
     const uint32_t nm = d.n_mass, ns = d.n_spring;  // one robot: pack-local = robot-local
     deg.assign(nm, 0);
@@ -513,18 +523,32 @@ void upload_mstream(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vec
     d.csr_begin = static_cast<uint32_t>(mrec.size() / 2);
     d.pad0 = static_cast<uint32_t>(row.size());
     const uint32_t groups = (nm + 63) / 64;
+    // per group, in 8-byte units: 64 records per k ([k][64]), or in the
+    // compact format 64 code quads then 64 rest-length quads (2 units
+    // each) per quad of k
     start.assign(groups + 1, 0);
+    std::vector<uint32_t> quads(groups);
     for (uint32_t gi = 0; gi < groups; gi++) {
       uint32_t longest = 0;
       for (uint32_t v = 64 * gi; v < std::min(nm, 64 * gi + 64); v++) longest = std::max(longest, deg[order[v]]);
       row.push_back(start[gi]);
-      start[gi + 1] = start[gi] + 64 * longest;
+      quads[gi] = (longest + 3) / 4;
+      start[gi + 1] = start[gi] + (compact ? 192 * quads[gi] : 64 * longest);
     }
     const size_t base = mrec.size();
     mrec.resize(base + 2 * static_cast<size_t>(start[groups]));
-    for (size_t i = base; i < mrec.size(); i += 2) {
-      mrec[i] = pad_w0;
-      mrec[i + 1] = 0x3f800000u;
+    if (compact) {
+      for (uint32_t gi = 0; gi < groups; gi++) {
+        uint32_t* w = mrec.data() + base + 2 * static_cast<size_t>(start[gi]);
+        const size_t nq = 64u * quads[gi];
+        std::fill(w, w + 2 * nq, static_cast<uint32_t>(pad_code) * 0x10001u);
+        std::fill(w + 2 * nq, w + 6 * nq, 0x3f800000u);
+      }
+    } else {
+      for (size_t i = base; i < mrec.size(); i += 2) {
+        mrec[i] = pad_w0;
+        mrec[i + 1] = 0x3f800000u;
+      }
     }
     fill.assign(nm, 0);
     for (uint32_t k = 0; k < ns; k++) {  // ascending spring index
@@ -536,9 +560,21 @@ void upload_mstream(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vec
       for (int e = 0; e < 2; e++) {
         const uint32_t own = e ? b : a, other = e ? a : b;
         const uint32_t v = lane[own];
-        const size_t at = base + 2 * (static_cast<size_t>(start[v / 64]) + 64u * fill[own]++ + (v & 63u));
-        mrec[at] = 16u * static_cast<uint32_t>(mass_slot(static_cast<int>(other))) | m8;
-        mrec[at + 1] = lb;
+        const uint32_t kk = fill[own]++;
+        const uint32_t oslot = 16u * static_cast<uint32_t>(mass_slot(static_cast<int>(other)));
+        if (compact) {
+          // code quad (kk / 4, lane) holds 4 u16 codes, rest-length quad 4 floats
+          uint32_t* w = mrec.data() + base + 2 * static_cast<size_t>(start[v / 64]);
+          const size_t q = 64u * (kk / 4) + (v & 63u);
+          uint32_t& cw = w[2 * q + ((kk & 3u) >> 1)];
+          const uint32_t sh = 16u * (kk & 1u);
+          cw = (cw & ~(0xffffu << sh)) | (oslot | s->act_mat[g]) << sh;
+          w[2 * 64u * quads[v / 64] + 4 * q + (kk & 3u)] = lb;
+        } else {
+          const size_t at = base + 2 * (static_cast<size_t>(start[v / 64]) + 64u * kk + (v & 63u));
+          mrec[at] = oslot | m8;
+          mrec[at + 1] = lb;
+        }
       }
     }
   }
@@ -699,11 +735,14 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
 }
 
 // Device arrays that depend on the packing: pack-local spring pairs, the
-// gather lists, row bounds and phase-2 order.  Mass i's row is two parts:
-// the springs whose first endpoint it is, then those whose second endpoint
-// it is, each in ascending spring index; row[2i], row[2i+1], row[2i+2]
-// bound them.  LDS parts store entries as 3 * slot and pad each part to a
-// multiple of 4 with the zero slot; streaming parts store the slot.
+// gather lists, row bounds and phase-2 order.  Mass i's row holds its
+// springs in ascending spring index, row[i] and row[i+1] bound it; an entry
+// is twice the term's offset, + 1 where the mass is the spring's second
+// endpoint.  LDS parts store the offset as tw * slot (tw = 3 words per xpbd
+// term, 4 per force term) and pad each row to a multiple of 4 with the zero
+// slot; streaming parts store the slot.  The mass-parallel and chunked
+// parts have layouts of their own (upload_mstream, upload_mass,
+// upload_chunked).
 void upload_plan(mh_sim* s, Plan& plan) {
   const size_t na = s->n_active;
   std::vector<uint32_t> spair(na), csr, row;
@@ -728,10 +767,11 @@ void upload_plan(mh_sim* s, Plan& plan) {
     } else if (part.v->chunked == 1) {
       upload_chunked(s, part, spair, ccsr, crow, ctab);
     } else {
-      // entries: 2 * (3 * slot) (LDS parts) or 2 * slot (streaming parts),
+      // entries: 2 * (tw * slot) (LDS parts) or 2 * slot (streaming parts),
       // + 1 where the mass is the spring's second endpoint
-      const uint32_t zero_entry = 6u * static_cast<uint32_t>(part.spring_cap + 1);
-      const uint32_t scale = padded ? 6u : 2u;
+      const uint32_t tw2 = 2u * static_cast<uint32_t>(solver_term_words(s->cfg.solver));
+      const uint32_t zero_entry = tw2 * static_cast<uint32_t>(part.spring_cap + 1);
+      const uint32_t scale = padded ? tw2 : 2u;
       auto pad = [padded](uint32_t n) { return padded ? (n + 3u) & ~3u : n; };
       for (PackDesc& d : part.packs) {
         d.csr_begin = static_cast<uint32_t>(csr.size());
@@ -822,8 +862,9 @@ std::vector<const KernelVariant*> candidates(const mh_sim* s, int kinds) {
 }
 
 // Default plan: the LDS variant (table order, largest blocks first) that
-// takes the most robots, and for the rest the first streaming variant
-// (chunked ones first) that fits them.
+// takes the most robots, and for the rest the first streaming variant that
+// fits them (table order: for the force solver the mass-parallel streaming
+// kernel, then the chunked and global-memory ones).
 void choose_default_plan(mh_sim* s) {
   auto lc = candidates(s, kResident);
   auto sc = candidates(s, kStreaming);
@@ -1731,6 +1772,7 @@ int mh_sim_get_info(const mh_sim* s, mh_sim_info* info) {
       info->n_stream_robots = part.n_robots;
       for (const mh::PackDesc& d : part.packs) info->n_stream_springs += static_cast<int>(d.n_spring);
       info->stream_chunk = part.v->chunked == 1 ? part.spring_cap : 0;
+      info->stream_record_bytes = part.v->chunked == 2 ? part.v->nspt : 0;
     }
   }
   info->robots_per_block = s->plan.rpb;

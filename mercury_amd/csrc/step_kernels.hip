@@ -3,35 +3,40 @@
 // One workgroup owns one pack (a contiguous range of robots) for n_steps
 // substeps; the state is read from HBM once per launch and written back
 // once.  No atomics: every mass receives one sum from +0 over its springs
-// in ascending spring index, each spring's term added where the mass is
-// the first endpoint and subtracted where it is the second, so results are
-// bit-reproducible, independent of the launch plan and equal to the CPU
-// mirror of the same step.
+// in ascending spring index, each spring's contribution added where the
+// mass is the first endpoint and subtracted where it is the second, so
+// results are bit-reproducible, independent of the launch plan and equal
+// to the CPU mirror of the same step.
 //
 // Step semantics (reference sim_kernel.cu:141-269 for the XPBD solver;
 // force solver per the project spec written in the header of oracle/step.c):
 //   XPBD  x~ = x + v dt;  lambda = -(d - L)/(2 + ((1/k)/dt)/dt);
 //         dp = lambda * (x~a - x~b)/(d + 1e-12);  x' = x~ + sum(+-dp);
 //         v' = (float)(0.99 * (double)(x' - x) / (double)dt)
-//   force c = ((k (L - d)) (1 / (d + 1e-12))) (xa - xb);  F = sum(+-c) - m g y^ - c_d v
-//         + ground normal/friction;  u = v + (F (1/m)) dt;  x' = x + u dt;
+//   force D = xa - xb;  f = (k (L - d)) (1 / (d + 1e-12));
+//         F = sum fma(+-f, D, .) - m g y^ - c_d v + ground normal/friction
+//         (each spring's f and D folded into the running sum with one
+//         rounding per component);  u = v + (F (1/m)) dt;  x' = x + u dt;
 //         v' = damping * u
 //   L = fma(Lbar, A sin(fma(omega, t, phi)), Lbar)
 //
 // Kernels:
-//   step_mass   one lane per mass, robot resident in LDS; each spring is
-//               evaluated by both endpoints from records held in VGPRs;
-//               one barrier per substep (voxel robots);
-//   step_massg  the same lanes; each spring evaluated once by its first
-//               endpoint, the term passed to the second through LDS; two
-//               barriers per substep;
-//   step_lds    one lane per spring (records in VGPRs), terms in LDS, then
-//               one lane per mass gathering them through CSR lists;
-//   step_chunk  robots too large for LDS: positions in LDS, springs
-//               streamed from HBM in LDS-sized chunks, the next chunk's
-//               inputs loaded while the current one is worked on;
-//   step_stream the fallback for robots too large for step_chunk: spring
-//               terms in HBM.
+//   step_mass    one lane per mass, robot resident in LDS; each spring is
+//                evaluated by both endpoints from records held in VGPRs;
+//                one barrier per substep (voxel robots);
+//   step_massg   the same lanes, XPBD only; each spring evaluated once by
+//                its first endpoint, the term passed to the second through
+//                LDS; two barriers per substep;
+//   step_lds     one lane per spring (records in VGPRs), terms in LDS, then
+//                one lane per mass gathering them through CSR lists;
+//   step_mstream robots over the register budget, force solver: one lane
+//                per mass walks its springs from its own side, records
+//                streamed from HBM every substep; one barrier per substep;
+//   step_chunk   robots too large for LDS: positions in LDS, springs
+//                streamed from HBM in LDS-sized chunks, the next chunk's
+//                inputs loaded while the current one is worked on;
+//   step_stream  the fallback for robots too large for step_chunk: spring
+//                terms in HBM.
 #include <hip/hip_runtime.h>
 
 #include "mercury_hip.h"
@@ -124,34 +129,23 @@ __device__ __forceinline__ float4 lds_position(const MassState& s) {
   return make_float4(s.x, s.y, s.z, 0.f);
 }
 
-// Spring term of one spring with actuated rest length L: returns
-// (cx, cy, cz); xpbd also yields lambda.
+// XPBD term of one spring with actuated rest length L: the correction
+// (cx, cy, cz) = lambda n and lambda.
 template <int SOLVER>
 __device__ __forceinline__ void spring_term_L(float4 pa, float4 pb, float L, float kk, float& cx, float& cy,
                                               float& cz, float& lambda) {
+  static_assert(SOLVER == MH_SOLVER_XPBD_REF, "the force solver's term is force_factor");
   const float dx = pa.x - pb.x;
   const float dy = pa.y - pb.y;
   const float dz = pa.z - pb.z;
-  if constexpr (SOLVER == MH_SOLVER_XPBD_REF) {
-    const float d = norm3(dx, dy, dz);
-    const float den = d + kEps;
-    const float nx = dx / den, ny = dy / den, nz = dz / den;
-    const float C = d - L;
-    lambda = -C / kk;
-    cx = lambda * nx;
-    cy = lambda * ny;
-    cz = lambda * nz;
-  } else {
-    // Hooke's law with one reciprocal: f = k (L - d) * (1 / (d + eps)),
-    // c = f * delta
-    float d, inv;
-    length_and_inverse(dx, dy, dz, kEps, d, inv);
-    const float f = (kk * (L - d)) * inv;
-    lambda = 0.f;
-    cx = f * dx;
-    cy = f * dy;
-    cz = f * dz;
-  }
+  const float d = norm3(dx, dy, dz);
+  const float den = d + kEps;
+  const float nx = dx / den, ny = dy / den, nz = dz / den;
+  const float C = d - L;
+  lambda = -C / kk;
+  cx = lambda * nx;
+  cy = lambda * ny;
+  cz = lambda * nz;
 }
 
 // The same from the rest length lbar and actuation factor R:
@@ -163,14 +157,69 @@ __device__ __forceinline__ void spring_term(float4 pa, float4 pb, float lbar, fl
   spring_term_L<SOLVER>(pa, pb, fmaf(lbar, R, lbar), kk, cx, cy, cz, lambda);
 }
 
+// Force solver, one spring with actuated rest length L and stiffness kk:
+// Hooke's law with one correctly rounded reciprocal, f = (k (L - d)) *
+// (1 / (d + eps)), and the difference D = pa - pb.  Its term f D enters a
+// mass's sum as fmaf(f, D, sum) per component (one rounding), or
+// fmaf(-f, D, sum) where the mass is the second endpoint; from the second
+// endpoint's own side D comes out as -D exactly, so both are fmaf(f, D,
+// sum) with that side's D.
+__device__ __forceinline__ void force_factor(float4 pa, float4 pb, float L, float kk, float& f, float& dx,
+                                             float& dy, float& dz) {
+  dx = pa.x - pb.x;
+  dy = pa.y - pb.y;
+  dz = pa.z - pb.z;
+  float d, inv;
+  length_and_inverse(dx, dy, dz, kEps, d, inv);
+  f = (kk * (L - d)) * inv;
+}
+
+// Words of a stored spring term: xpbd (cx, cy, cz); force (Dx, Dy, Dz, f).
+template <int SOLVER>
+constexpr int term_words() { return solver_term_words(SOLVER); }
+
+// Adds four gathered terms to a sum, in entry order.  An entry (16 bits) is
+// twice the word offset of a stored term, + 1 where the mass is the
+// spring's second endpoint; g = +-1 from that bit.  xpbd terms (cx, cy, cz):
+// fma(term, g, sum) is sum +- term rounded once.  force terms (Dx, Dy, Dz,
+// f): fma(g f, D, sum), one rounding per component.
+template <int SOLVER>
+__device__ __forceinline__ void gather4(const float* s_term, uint2 w, float& ax, float& ay, float& az) {
+  const float* q[4] = {s_term + ((w.x & 0xffffu) >> 1), s_term + (w.x >> 17), s_term + ((w.y & 0xffffu) >> 1),
+                       s_term + (w.y >> 17)};
+  const float g[4] = {__uint_as_float(0x3f800000u | (w.x << 31)),
+                      __uint_as_float(0x3f800000u | ((w.x & 0x10000u) << 15)),
+                      __uint_as_float(0x3f800000u | (w.y << 31)),
+                      __uint_as_float(0x3f800000u | ((w.y & 0x10000u) << 15))};
+  if constexpr (SOLVER == MH_SOLVER_FORCE) {
+    float4 t[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t[i] = *reinterpret_cast<const float4*>(q[i]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float sf = t[i].w * g[i];
+      ax = fmaf(sf, t[i].x, ax); ay = fmaf(sf, t[i].y, ay); az = fmaf(sf, t[i].z, az);
+    }
+  } else {
+    float x[4], y[4], z[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { x[i] = q[i][0]; y[i] = q[i][1]; z[i] = q[i][2]; }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      ax = fmaf(x[i], g[i], ax); ay = fmaf(y[i], g[i], ay); az = fmaf(z[i], g[i], az);
+    }
+  }
+}
+
 // ----------------------------------------------------------------- LDS kernel
-// LDS layout (from LDS address 0): spring terms float[3][spring_cap + 2]
-// (12 B per slot, rounded up to 16 B) | positions float4[pos_slots(mass_cap)] |
-// gather entries u16[csr_cap] | row bounds u16[mass_cap + 1] (the layout
-// reserves 2 mass_cap + 2) | actuation float[n_mat].  Slot spring_cap is
-// the sink of lanes without a spring; slot spring_cap + 1 holds zeros.  A
-// gather entry is 2 * (3 * slot) of a term, + 1 where the mass is the
-// spring's second endpoint.  Mass i's row [row[i], row[i+1]) lists its
+// LDS layout (from LDS address 0): spring terms, TW floats per slot
+// (xpbd (cx, cy, cz), force (Dx, Dy, Dz, f)), spring_cap + 2 slots rounded
+// up to 16 B | positions float4[pos_slots(mass_cap)] | gather entries
+// u16[csr_cap] | row bounds u16[mass_cap + 1] (the layout reserves
+// 2 mass_cap + 2) | actuation float[n_mat].  Slot spring_cap is the sink of
+// lanes without a spring; slot spring_cap + 1 holds zeros.  A gather entry
+// is 2 * (TW * slot) of a term, + 1 where the mass is the spring's second
+// endpoint.  Mass i's row [row[i], row[i+1]) lists its
 // springs in ascending spring index, padded to a multiple of 4 with the
 // zero slot (adding +0 leaves a sum that started at +0 unchanged); the
 // mass receives the canonical sum.
@@ -187,9 +236,10 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
   const PackDesc d = p.packs[blockIdx.x];
   const int nm = static_cast<int>(d.n_mass);
   const int ns = static_cast<int>(d.n_spring);
+  constexpr int TW = term_words<SOLVER>();
   const int sink = p.spring_cap, zero = p.spring_cap + 1;
   float* s_term = reinterpret_cast<float*>(smem);
-  float4* s_pos = reinterpret_cast<float4*>(smem + lds_term_bytes(p.spring_cap));
+  float4* s_pos = reinterpret_cast<float4*>(smem + lds_term_bytes(p.spring_cap, TW));
   uint16_t* s_csr = reinterpret_cast<uint16_t*>(s_pos + pos_slots(p.mass_cap));
   uint16_t* s_row = s_csr + p.csr_cap;
   float* s_R = reinterpret_cast<float*>(s_row + 2 * p.mass_cap + 2);
@@ -201,7 +251,7 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
   const int n_csr = static_cast<int>(p.row[d.row_begin + nm]);
   for (int i = tid; i < n_csr; i += BLOCK) s_csr[i] = static_cast<uint16_t>(p.csr[d.csr_begin + i]);
   for (int i = tid; i <= nm; i += BLOCK) s_row[i] = static_cast<uint16_t>(p.row[d.row_begin + i]);
-  if (tid < 3) s_term[3 * zero + tid] = 0.f;
+  if (tid < TW) s_term[TW * zero + tid] = 0.f;
 
   // spring records -> registers; lanes past the pack's springs compute a
   // dummy spring (0,0) whose term goes to the sink slot
@@ -211,7 +261,7 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
 #pragma unroll
   for (int k = 0; k < NSPT; ++k) {
     const int s = wave_first + k * 64 + lane;
-    sp[k] = 0; lb[k] = 1.f; kk[k] = 1.f; st[k] = 0.f; mi[k] = 0; slot[k] = 3 * sink;
+    sp[k] = 0; lb[k] = 1.f; kk[k] = 1.f; st[k] = 0.f; mi[k] = 0; slot[k] = TW * sink;
     if (k < iters && s < ns) {
       const uint32_t g = d.spring_begin + s;
       sp[k] = p.spair[g];
@@ -219,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
       mi[k] = p.smat[g];
       kk[k] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[mi[k]] : p.mat_k[mi[k]];
       if (p.track_stress) st[k] = p.sstress[g];
-      slot[k] = 3 * s;
+      slot[k] = TW * s;
     }
   }
   // masses -> registers, first positions -> LDS
@@ -247,12 +297,18 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
       if (k < iters) {
         const float4 pa = s_pos[sp[k] & 0xffffu];
         const float4 pb = s_pos[sp[k] >> 16];
-        float cx, cy, cz, lambda;
-        spring_term<SOLVER>(pa, pb, lb[k], s_R[mi[k]], kk[k], cx, cy, cz, lambda);
-        if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) st[k] = st[k] + lambda / lb[k];
-        s_term[slot[k]] = cx;
-        s_term[slot[k] + 1] = cy;
-        s_term[slot[k] + 2] = cz;
+        if constexpr (SOLVER == MH_SOLVER_FORCE) {
+          float f, dx, dy, dz;
+          force_factor(pa, pb, fmaf(lb[k], s_R[mi[k]], lb[k]), kk[k], f, dx, dy, dz);
+          *reinterpret_cast<float4*>(s_term + slot[k]) = make_float4(dx, dy, dz, f);
+        } else {
+          float cx, cy, cz, lambda;
+          spring_term<SOLVER>(pa, pb, lb[k], s_R[mi[k]], kk[k], cx, cy, cz, lambda);
+          if (p.track_stress) st[k] = st[k] + lambda / lb[k];
+          s_term[slot[k]] = cx;
+          s_term[slot[k] + 1] = cy;
+          s_term[slot[k] + 2] = cz;
+        }
       }
     }
     __syncthreads();
@@ -264,25 +320,11 @@ __global__ __launch_bounds__(BLOCK) void step_lds(StepParams p) {
         const int e2 = s_row[i + 1];
         float ax = 0.0f, ay = 0.0f, az = 0.0f;
         // the mass's springs in ascending spring order, four gathers in
-        // flight; an entry is 2 * (3 * slot) + 1 where the mass is the
-        // spring's second endpoint: fma(term, -1, sum) is sum - term
-        // rounded once
+        // flight; an entry is 2 * (TW * slot) + 1 where the mass is the
+        // spring's second endpoint
         for (int e = s_row[i]; e < e2; e += 4) {
           const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
-          const float* q0 = s_term + ((w.x & 0xffffu) >> 1);
-          const float* q1 = s_term + (w.x >> 17);
-          const float* q2 = s_term + ((w.y & 0xffffu) >> 1);
-          const float* q3 = s_term + (w.y >> 17);
-          const float g0 = __uint_as_float(0x3f800000u | (w.x << 31));
-          const float g1 = __uint_as_float(0x3f800000u | ((w.x & 0x10000u) << 15));
-          const float g2 = __uint_as_float(0x3f800000u | (w.y << 31));
-          const float g3 = __uint_as_float(0x3f800000u | ((w.y & 0x10000u) << 15));
-          const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
-          const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
-          ax = fmaf(x0, g0, ax); ay = fmaf(y0, g0, ay); az = fmaf(z0, g0, az);
-          ax = fmaf(x1, g1, ax); ay = fmaf(y1, g1, ay); az = fmaf(z1, g1, az);
-          ax = fmaf(x2, g2, ax); ay = fmaf(y2, g2, ay); az = fmaf(z2, g2, az);
-          ax = fmaf(x3, g3, ax); ay = fmaf(y3, g3, ay); az = fmaf(z3, g3, az);
+          gather4<SOLVER>(s_term, w, ax, ay, az);
         }
         s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax, ay, az, p);
       }
@@ -342,14 +384,20 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
       const uint32_t pr = spair[s];
       const int m = smat[s];
       const float kk = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[m] : p.mat_k[m];
-      float cx, cy, cz, lambda;
-      spring_term<SOLVER>(s_pos[pr & 0xffffu], s_pos[pr >> 16], slbar[s], s_R[m], kk, cx, cy,
-                          cz, lambda);
-      if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
-        float* stp = p.sstress + d.spring_begin + s;
-        *stp = *stp + lambda / slbar[s];
+      if constexpr (SOLVER == MH_SOLVER_FORCE) {
+        float f, dx, dy, dz;
+        force_factor(s_pos[pr & 0xffffu], s_pos[pr >> 16], fmaf(slbar[s], s_R[m], slbar[s]), kk, f, dx, dy, dz);
+        sc[s] = make_float4(dx, dy, dz, f);
+      } else {
+        float cx, cy, cz, lambda;
+        spring_term<SOLVER>(s_pos[pr & 0xffffu], s_pos[pr >> 16], slbar[s], s_R[m], kk, cx, cy,
+                            cz, lambda);
+        if (p.track_stress) {
+          float* stp = p.sstress + d.spring_begin + s;
+          *stp = *stp + lambda / slbar[s];
+        }
+        sc[s] = make_float4(cx, cy, cz, 0.f);
       }
-      sc[s] = make_float4(cx, cy, cz, 0.f);
     }
     __syncthreads();
     const float tn = t + p.dt;
@@ -372,9 +420,16 @@ __global__ __launch_bounds__(BLOCK) void step_stream(StepParams p) {
         const uint32_t c = csr[e];
         const float4 q = sc[c >> 1];
         const float g = __uint_as_float(0x3f800000u | (c << 31));
-        ax = fmaf(q.x, g, ax);
-        ay = fmaf(q.y, g, ay);
-        az = fmaf(q.z, g, az);
+        if constexpr (SOLVER == MH_SOLVER_FORCE) {
+          const float sf = q.w * g;  // (Dx, Dy, Dz, f): fma(+-f, D, sum)
+          ax = fmaf(sf, q.x, ax);
+          ay = fmaf(sf, q.y, ay);
+          az = fmaf(sf, q.z, az);
+        } else {
+          ax = fmaf(q.x, g, ax);
+          ay = fmaf(q.y, g, ay);
+          az = fmaf(q.z, g, az);
+        }
       }
       s_pos[i] = integrate<SOLVER>(s, ax, ay, az, p);
       p.pos[d.mass_begin + i] = make_float4(s.x, s.y, s.z, s.m);
@@ -411,10 +466,11 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
   const int ns = static_cast<int>(d.n_spring);
   const int C = p.spring_cap;
   const int n_chunks = ns > 0 ? (ns + C - 1) / C : 1;
+  constexpr int TW = term_words<SOLVER>();
   const int zero = C + 1;
   const int rq = chunk_row_quads(nm);
   float* s_term = reinterpret_cast<float*>(smem);
-  float4* s_pos = reinterpret_cast<float4*>(smem + lds_term_bytes(C));
+  float4* s_pos = reinterpret_cast<float4*>(smem + lds_term_bytes(C, TW));
   uint16_t* s_csr = reinterpret_cast<uint16_t*>(s_pos + pos_slots(p.mass_cap));
   uint16_t* s_row = s_csr + p.csr_cap;
   float* s_R = reinterpret_cast<float*>(s_row + 4 * chunk_row_quads(p.mass_cap));
@@ -426,7 +482,7 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
   const uint8_t* smat = p.smat + d.spring_begin;
   const uint16_t* ccsr = p.ccsr + d.csr_begin;
   const uint16_t* crow = p.crow + d.row_begin;
-  if (tid < 3) s_term[3 * zero + tid] = 0.f;
+  if (tid < TW) s_term[TW * zero + tid] = 0.f;
   for (int i = tid; i <= n_chunks; i += BLOCK) s_eoff[i] = p.ctab[d.pad0 + i];
   for (int i = tid; i < p.n_mat; i += BLOCK) s_K[i] = (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[i] : p.mat_k[i];
 
@@ -501,17 +557,24 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
         const int s = s0 + tid + j * BLOCK;
         if (s < s1) {
           const int m = static_cast<int>(mt[j]);
-          float cx, cy, cz, lambda;
-          spring_term<SOLVER>(s_pos[pr[j] & 0xffffu], s_pos[pr[j] >> 16], lbv[j], s_R[m], s_K[m], cx, cy,
-                              cz, lambda);
-          if (SOLVER == MH_SOLVER_XPBD_REF && p.track_stress) {
-            float* stp = p.sstress + d.spring_begin + s;
-            *stp = *stp + lambda / lbv[j];
+          const int o = TW * (s - s0);
+          if constexpr (SOLVER == MH_SOLVER_FORCE) {
+            float f, dx, dy, dz;
+            force_factor(s_pos[pr[j] & 0xffffu], s_pos[pr[j] >> 16], fmaf(lbv[j], s_R[m], lbv[j]), s_K[m], f,
+                         dx, dy, dz);
+            *reinterpret_cast<float4*>(s_term + o) = make_float4(dx, dy, dz, f);
+          } else {
+            float cx, cy, cz, lambda;
+            spring_term<SOLVER>(s_pos[pr[j] & 0xffffu], s_pos[pr[j] >> 16], lbv[j], s_R[m], s_K[m], cx, cy,
+                                cz, lambda);
+            if (p.track_stress) {
+              float* stp = p.sstress + d.spring_begin + s;
+              *stp = *stp + lambda / lbv[j];
+            }
+            s_term[o] = cx;
+            s_term[o + 1] = cy;
+            s_term[o + 2] = cz;
           }
-          const int o = 3 * (s - s0);
-          s_term[o] = cx;
-          s_term[o + 1] = cy;
-          s_term[o + 2] = cz;
         }
       }
       // the following chunk (the first one again after the last: the next
@@ -524,26 +587,13 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
         const int i = tid + j * BLOCK;
         if (i < nm) {
           // the chunk's springs of the mass in ascending spring order; an
-          // entry is 2 * (3 * (spring - chunk start)), + 1 where the mass
-          // is the second endpoint: fma(term, -1, sum) is sum - term
+          // entry is 2 * (TW * (spring - chunk start)), + 1 where the mass
+          // is the second endpoint
           const int e2 = s_row[i + 1];
           float ax = A[j][0], ay = A[j][1], az = A[j][2];
           for (int e = s_row[i]; e < e2; e += 4) {
             const uint2 w = *reinterpret_cast<const uint2*>(s_csr + e);
-            const float* q0 = s_term + ((w.x & 0xffffu) >> 1);
-            const float* q1 = s_term + (w.x >> 17);
-            const float* q2 = s_term + ((w.y & 0xffffu) >> 1);
-            const float* q3 = s_term + (w.y >> 17);
-            const float g0 = __uint_as_float(0x3f800000u | (w.x << 31));
-            const float g1 = __uint_as_float(0x3f800000u | ((w.x & 0x10000u) << 15));
-            const float g2 = __uint_as_float(0x3f800000u | (w.y << 31));
-            const float g3 = __uint_as_float(0x3f800000u | ((w.y & 0x10000u) << 15));
-            const float x0 = q0[0], y0 = q0[1], z0 = q0[2], x1 = q1[0], y1 = q1[1], z1 = q1[2];
-            const float x2 = q2[0], y2 = q2[1], z2 = q2[2], x3 = q3[0], y3 = q3[1], z3 = q3[2];
-            ax = fmaf(x0, g0, ax); ay = fmaf(y0, g0, ay); az = fmaf(z0, g0, az);
-            ax = fmaf(x1, g1, ax); ay = fmaf(y1, g1, ay); az = fmaf(z1, g1, az);
-            ax = fmaf(x2, g2, ax); ay = fmaf(y2, g2, ay); az = fmaf(z2, g2, az);
-            ax = fmaf(x3, g3, ax); ay = fmaf(y3, g3, ay); az = fmaf(z3, g3, az);
+            gather4<SOLVER>(s_term, w, ax, ay, az);
           }
           A[j][0] = ax; A[j][1] = ay; A[j][2] = az;
           if (last) s_pos[pslot[j]] = integrate<SOLVER>(ms[j], ax, ay, az, p);
@@ -575,25 +625,50 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
 // so there is no term buffer, no gather list and one barrier per substep;
 // unlike it, the records stream from memory every substep.  Each thread
 // serves MPT virtual lanes v = tid + j BLOCK; virtual lanes are the robot's
-// masses by degree, most springs first (p.mperm).  A record is 8 bytes:
-// {16 * slot of the other endpoint | 8 * material << 16, rest length bits},
-// both LDS byte offsets as in step_mass; the actuated length is
-// fma(rest, R[material], rest).  Records of the 64 virtual lanes of group g
-// lie at p.row[d.row_begin + g] (in records, from d.csr_begin) as [k][64],
-// k below the group's longest list; shorter lists are padded with records
+// masses by degree, most springs first (p.mperm).  A record holds the
+// other endpoint's slot, the spring's material and its rest length; the
+// actuated length is fma(rest, R[material], rest).  The records of the 64
+// virtual lanes of group g start at p.row[d.pad0 + g] (in 8-byte units,
+// from d.csr_begin).  Lists are padded to the group's longest with records
 // of the null material (stiffness 0, a ghost far from every mass), whose
-// terms are signed zeros.  A lane loads its records four at a time, the
-// next four in flight while the current ones are evaluated.  LDS: position
-// buffers float4[2][mass_slots(BLOCK MPT)] (read one, write the other) |
-// per material {R(t), k} float2[2][kMStreamMats] (the null material last).
-constexpr int kMStreamBatch = 4;
+// terms are signed zeros.  Two record formats (REC):
+//   8  one uint2 per record {16 * slot | 8 * material << 16, rest length
+//      bits}, both LDS byte offsets, laid out [k][64];
+//   6  a u16 code 16 * slot | material (at most 15 materials and the null
+//      one) and the f32 rest length, in two arrays of quads: the codes
+//      uint2[n4][64] (4 codes per lane), then the rest lengths
+//      float4[n4][64], n4 = ceil(longest / 4).  12 bytes per spring-update
+//      instead of 16, in two loads per 4 records.
+// A lane loads its records BATCH at a time, the next BATCH in flight while
+// the current ones are evaluated.  LDS: position buffers
+// float4[2][mass_slots(BLOCK MPT)] (read one, write the other) | per
+// material {R(t), k} float2[2][kMStreamMats] (the null material last).
 template <int BLOCK, int MPT>
 constexpr int mstream_pos_bytes() { return 16 * mass_slots(BLOCK * MPT); }
 
+// Adds one record's spring to (ax, ay, az): the other endpoint's position
+// at LDS byte offset po, its material's {R(t), k} at mo.
+__device__ __forceinline__ void mstream_record(const unsigned char* pos_r, const unsigned char* mat_r, float4 own,
+                                               uint32_t po, uint32_t mo, float lbar, float& ax, float& ay,
+                                               float& az) {
+  const float2 Rk = *reinterpret_cast<const float2*>(mat_r + mo);
+  const float4 other = *reinterpret_cast<const float4*>(pos_r + po);
+  float f, dx, dy, dz;
+  force_factor(own, other, fmaf(lbar, Rk.x, lbar), Rk.y, f, dx, dy, dz);
+  ax = fmaf(f, dx, ax);
+  ay = fmaf(f, dy, ay);
+  az = fmaf(f, dz, az);
+}
+
 // WPE: waves per SIMD the register allocation must allow (8 at BLOCK 1024:
-// at most 64 VGPRs, so two workgroups share a CU).
-template <int SOLVER, int BLOCK, int MPT, int WPE>
+// at most 64 VGPRs, so two workgroups share a CU); BATCH: records per load
+// batch (a multiple of 4 for REC 6); REC: record format, 8 or 6 bytes.
+template <int SOLVER, int BLOCK, int MPT, int WPE, int BATCH, int REC>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) void step_mstream(StepParams p) {
+  static_assert(SOLVER == MH_SOLVER_FORCE, "the mass-parallel streaming kernel serves the force solver");
+  static_assert(REC == 8 || (REC == 6 && BATCH % 4 == 0), "record format");
+  static_assert(BLOCK % 64 == 0, "a wave serves one group of 64 virtual lanes");
+  constexpr int kMStreamBatch = BATCH;
   extern __shared__ __align__(16) unsigned char smem[];
   const PackDesc d = p.packs[blockIdx.x];
   const int nm = static_cast<int>(d.n_mass);
@@ -617,8 +692,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
     if (live) load_mass(ms[j], p.pos, p.vel, d.mass_begin + mass[j], p);
     int n = live ? p.mcnt[d.row_begin + v] : 0;
     for (int off = 32; off > 0; off >>= 1) n = max(n, __shfl_xor(n, off));
-    nr_w[j] = n;
-    gbase[j] = (v < ((nm + 63) & ~63)) ? p.row[d.pad0 + (v >> 6)] + (v & 63) : 0u;
+    // wave-uniform (a wave serves one group): scalar registers and branches
+    nr_w[j] = __builtin_amdgcn_readfirstlane(n);
+    gbase[j] = __builtin_amdgcn_readfirstlane((v < ((nm + 63) & ~63)) ? p.row[d.pad0 + (v >> 6)] : 0u);
     if (live) reinterpret_cast<float4*>(smem)[slot[j]] = lds_position<SOLVER>(ms[j]);
   }
   if (tid == 0) {
@@ -649,35 +725,87 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
       const bool live = v < nm;
       const float4 own = live ? *reinterpret_cast<const float4*>(pos_r + 16 * slot[j])
                               : make_float4(0.f, 0.f, 0.f, 0.f);
-      const uint2* rv = recs + gbase[j];
       const int n = nr_w[j];
+      const int lane = tid & 63;  // v & 63: BLOCK is a multiple of 64
       float ax = 0.f, ay = 0.f, az = 0.f;
-      uint2 q[kMStreamBatch];
+      // The loads inside the loops are unconditional (an index past the
+      // list reads the list's last record or quad again, never evaluated),
+      // so the compiler waits only for the batch in use, not for the one
+      // in flight.  n is wave-uniform.
+      if constexpr (REC == 8) {
+        const uint2* rv = recs + gbase[j] + lane;
+        uint2 q[kMStreamBatch];
 #pragma unroll
-      for (int i = 0; i < kMStreamBatch; ++i) q[i] = i < n ? rv[64 * i] : make_uint2(0u, 0u);
-      for (int k0 = 0; k0 < n; k0 += kMStreamBatch) {
-        uint2 nq[kMStreamBatch];
+        for (int i = 0; i < kMStreamBatch; ++i) q[i] = make_uint2(0u, 0u);
+        if (n > 0) {
 #pragma unroll
-        for (int i = 0; i < kMStreamBatch; ++i) {
-          const int k = k0 + kMStreamBatch + i;
-          nq[i] = k < n ? rv[64 * k] : make_uint2(0u, 0u);  // past the list: never evaluated
+          for (int i = 0; i < kMStreamBatch; ++i) q[i] = rv[64 * min(i, n - 1)];
         }
+        for (int k0 = 0; k0 < n; k0 += kMStreamBatch) {
+          uint2 nq[kMStreamBatch];
 #pragma unroll
-        for (int i = 0; i < kMStreamBatch; ++i) {
-          if (k0 + i < n) {
-            const uint32_t r = q[i].x;
-            const float lbar = __uint_as_float(q[i].y);
-            const float2 Rk = *reinterpret_cast<const float2*>(mat_r + (r >> 16));
-            const float4 other = *reinterpret_cast<const float4*>(pos_r + (r & 0xffffu));
-            float cx, cy, cz, lambda;
-            spring_term<SOLVER>(own, other, lbar, Rk.x, Rk.y, cx, cy, cz, lambda);
-            ax = ax + cx; ay = ay + cy; az = az + cz;
+          for (int i = 0; i < kMStreamBatch; ++i) nq[i] = rv[64 * min(k0 + kMStreamBatch + i, n - 1)];
+#pragma unroll
+          for (int i = 0; i < kMStreamBatch; ++i) {
+            if (k0 + i < n)
+              mstream_record(pos_r, mat_r, own, q[i].x & 0xffffu, q[i].x >> 16, __uint_as_float(q[i].y), ax, ay,
+                             az);
+            // one record at a time: keeps the registers of the unrolled batch bounded
+            __builtin_amdgcn_sched_barrier(0);
           }
-          // one record at a time: keeps the registers of the unrolled batch bounded
-          __builtin_amdgcn_sched_barrier(0);
-        }
 #pragma unroll
-        for (int i = 0; i < kMStreamBatch; ++i) q[i] = nq[i];
+          for (int i = 0; i < kMStreamBatch; ++i) q[i] = nq[i];
+        }
+      } else {
+        // quads: codes cv[64 b], rest lengths lv[64 b]
+        constexpr int kQ = kMStreamBatch / 4;
+        const int n4 = (n + 3) >> 2;
+        const uint2* cv = recs + gbase[j] + lane;
+        const float4* lv = reinterpret_cast<const float4*>(recs + gbase[j] + 64 * n4) + lane;
+        uint2 qc[kQ];
+        float4 ql[kQ];
+#pragma unroll
+        for (int i = 0; i < kQ; ++i) {
+          qc[i] = make_uint2(0u, 0u);
+          ql[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        if (n4 > 0) {
+#pragma unroll
+          for (int i = 0; i < kQ; ++i) {
+            qc[i] = cv[64 * min(i, n4 - 1)];
+            ql[i] = lv[64 * min(i, n4 - 1)];
+          }
+        }
+        for (int b0 = 0; b0 < n4; b0 += kQ) {
+          uint2 nc[kQ];
+          float4 nl[kQ];
+#pragma unroll
+          for (int i = 0; i < kQ; ++i) {
+            const int b = min(b0 + kQ + i, n4 - 1);
+            nc[i] = cv[64 * b];
+            nl[i] = lv[64 * b];
+          }
+#pragma unroll
+          for (int i = 0; i < kQ; ++i) {
+            const int k = 4 * (b0 + i);
+            const uint32_t w0 = qc[i].x, w1 = qc[i].y;
+            if (k < n) mstream_record(pos_r, mat_r, own, w0 & 0xfff0u, (w0 << 3) & 0x78u, ql[i].x, ax, ay, az);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 1 < n)
+              mstream_record(pos_r, mat_r, own, (w0 >> 16) & 0xfff0u, (w0 >> 13) & 0x78u, ql[i].y, ax, ay, az);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 2 < n) mstream_record(pos_r, mat_r, own, w1 & 0xfff0u, (w1 << 3) & 0x78u, ql[i].z, ax, ay, az);
+            __builtin_amdgcn_sched_barrier(0);
+            if (k + 3 < n)
+              mstream_record(pos_r, mat_r, own, (w1 >> 16) & 0xfff0u, (w1 >> 13) & 0x78u, ql[i].w, ax, ay, az);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+#pragma unroll
+          for (int i = 0; i < kQ; ++i) {
+            qc[i] = nc[i];
+            ql[i] = nl[i];
+          }
+        }
       }
       if (live) *reinterpret_cast<float4*>(pos_w + 16 * slot[j]) = integrate<SOLVER>(ms[j], ax, ay, az, p);
     }
@@ -795,14 +923,20 @@ __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepPara
       const uint32_t r = rr[k];
       const float2 Lk = *reinterpret_cast<const float2*>(smem + kD0 + (r >> 16));
       const float4 other = *reinterpret_cast<const float4*>(smem + kP0 + (r & 0xffffu));
-      float cx, cy, cz, lambda;
-      spring_term_L<SOLVER>(own, other, Lk.x, Lk.y, cx, cy, cz, lambda);
       // padding records add a signed zero, which leaves a sum that
       // started at +0 unchanged
-      ax = ax + cx; ay = ay + cy; az = az + cz;
-      if constexpr (STRESS)
-        if (k < nr && ((first >> k) & 1u))
-          st[k] = st[k] + lambda / *reinterpret_cast<const float*>(smem + kAux + (r >> 16) + 4);
+      if constexpr (SOLVER == MH_SOLVER_FORCE) {
+        float f, dx, dy, dz;
+        force_factor(own, other, Lk.x, Lk.y, f, dx, dy, dz);
+        ax = fmaf(f, dx, ax); ay = fmaf(f, dy, ay); az = fmaf(f, dz, az);
+      } else {
+        float cx, cy, cz, lambda;
+        spring_term_L<SOLVER>(own, other, Lk.x, Lk.y, cx, cy, cz, lambda);
+        ax = ax + cx; ay = ay + cy; az = az + cz;
+        if constexpr (STRESS)
+          if (k < nr && ((first >> k) & 1u))
+            st[k] = st[k] + lambda / *reinterpret_cast<const float*>(smem + kAux + (r >> 16) + 4);
+      }
     }
     // one record at a time: keeps the unrolled loop's registers bounded
     __builtin_amdgcn_sched_barrier(0);
@@ -966,6 +1100,8 @@ __device__ __forceinline__ bool position_in_div_range(float4 q) {
 // that substep left the fast division's range.
 template <int SOLVER, int BLOCK, int MAXP, bool STRESS>
 __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
+  static_assert(SOLVER == MH_SOLVER_XPBD_REF,
+                "step_massg stores 12-byte terms: the xpbd step only (the force step_mass is faster)");
   extern __shared__ __align__(16) unsigned char smem[];
   const PackDesc d = p.packs[blockIdx.x];
   const int nm = static_cast<int>(d.n_mass);
@@ -1047,15 +1183,10 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
         const uint32_t r = rp[k];
         const float4 other = *reinterpret_cast<const float4*>(smem + kMassPos + (r & 0xffffu));
         float cx, cy, cz, lambda;
-        if constexpr (SOLVER == MH_SOLVER_XPBD_REF) {
-          const float2 e = *reinterpret_cast<const float2*>(smem + kDyn + (r >> 16));
-          const float2 a = *reinterpret_cast<const float2*>(smem + kAux + (r >> 16));
-          xpbd_term_L(own, other, e.x, e.y, a.x, fast_ok, cx, cy, cz, lambda);
-          if (STRESS && k < np) st[k] = st[k] + lambda / a.y;
-        } else {
-          const float2 Lk = *reinterpret_cast<const float2*>(smem + kDyn + (r >> 16));
-          spring_term_L<SOLVER>(own, other, Lk.x, Lk.y, cx, cy, cz, lambda);
-        }
+        const float2 e = *reinterpret_cast<const float2*>(smem + kDyn + (r >> 16));
+        const float2 a = *reinterpret_cast<const float2*>(smem + kAux + (r >> 16));
+        xpbd_term_L(own, other, e.x, e.y, a.x, fast_ok, cx, cy, cz, lambda);
+        if (STRESS && k < np) st[k] = st[k] + lambda / a.y;
         float* slot = reinterpret_cast<float*>(mine + 12 * k);
         slot[0] = cx;
         slot[1] = cy;
@@ -1143,11 +1274,10 @@ __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint3
   {SOLVER, MH_KERNEL_STREAM, 1, B, 0, M, reinterpret_cast<const void*>(&step_chunk<SOLVER, B, M>)}
 #define MH_STREAM(SOLVER, B) \
   {SOLVER, MH_KERNEL_STREAM, 0, B, 0, 0, reinterpret_cast<const void*>(&step_stream<SOLVER, B>)}
-#define MH_MSTREAM(B, M, W)                                                                    \
-  {MH_SOLVER_FORCE, MH_KERNEL_STREAM, 2, B, 0, M,                                           \
-   reinterpret_cast<const void*>(&step_mstream<MH_SOLVER_FORCE, B, M, W>)}
+#define MH_MSTREAM(B, M, W, BT, R)                                                             \
+  {MH_SOLVER_FORCE, MH_KERNEL_STREAM, 2, B, R, M,                                           \
+   reinterpret_cast<const void*>(&step_mstream<MH_SOLVER_FORCE, B, M, W, BT, R>)}
 #define MH_LDS_SET(SOLVER)                                                         \
-  MH_MASSG(SOLVER, 768, 13), MH_MASSG(SOLVER, 512, 16), MH_MASSG(SOLVER, 256, 16),     \
       MH_MASS(SOLVER, 768, 13), MH_MASS(SOLVER, 1024, 13), MH_MASS(SOLVER, 512, 16),   \
       MH_MASS(SOLVER, 256, 16), MH_LDS(SOLVER, 1024, 8, 1), MH_LDS(SOLVER, 1024, 8, 2), MH_LDS(SOLVER, 1024, 4, 1), \
       MH_LDS(SOLVER, 512, 8, 2), MH_LDS(SOLVER, 512, 16, 2), \
@@ -1155,10 +1285,18 @@ __global__ void robot_com(const float4* pos, const uint8_t* mass_id, const uint3
       MH_CHUNK(SOLVER, 1024, 2), MH_CHUNK(SOLVER, 512, 4),                            \
       MH_STREAM(SOLVER, 1024), MH_STREAM(SOLVER, 512), MH_STREAM(SOLVER, 256)
 
-// the mass-parallel streaming kernel serves the force solver (the XPBD
-// step's stress output needs each spring's index)
-const KernelVariant kVariants[] = {MH_LDS_SET(MH_SOLVER_XPBD_REF), MH_LDS_SET(MH_SOLVER_FORCE),
-                                   MH_MSTREAM(1024, 2, 8), MH_MSTREAM(1024, 2, 4), MH_MSTREAM(512, 4, 4)};
+// The mass-parallel streaming kernel serves the force solver (the XPBD
+// step's stress output needs each spring's index); it precedes the other
+// streaming variants, so plans that are not autotuned stream with it.
+// The gather flavour of the mass-parallel kernel serves the xpbd step (its
+// division-heavy term is computed once); the force solver's step_mass is
+// faster than it.
+const KernelVariant kVariants[] = {MH_MASSG(MH_SOLVER_XPBD_REF, 768, 13), MH_MASSG(MH_SOLVER_XPBD_REF, 512, 16),
+                                   MH_MASSG(MH_SOLVER_XPBD_REF, 256, 16), MH_LDS_SET(MH_SOLVER_XPBD_REF),
+                                   MH_MSTREAM(1024, 2, 8, 4, 6), MH_MSTREAM(512, 4, 4, 4, 6),
+                                   MH_MSTREAM(512, 4, 4, 8, 6), MH_MSTREAM(1024, 2, 8, 4, 8),
+                                   MH_MSTREAM(1024, 2, 8, 2, 8), MH_MSTREAM(512, 4, 4, 4, 8),
+                                   MH_LDS_SET(MH_SOLVER_FORCE)};
 
 }  // namespace
 
@@ -1167,8 +1305,8 @@ const KernelVariant* kernel_variants(int* count) {
   return kVariants;
 }
 
-size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat) {
-  size_t b = lds_term_bytes(spring_cap);                // spring terms, sink, zero
+size_t lds_bytes_lds(int mass_cap, int spring_cap, int csr_cap, int n_mat, int tw) {
+  size_t b = lds_term_bytes(spring_cap, tw);            // spring terms, sink, zero
   b += 16u * static_cast<size_t>(pos_slots(mass_cap));  // float4 positions (skewed)
   b += 2u * static_cast<size_t>(csr_cap);               // gather entries (u16)
   b += 2u * static_cast<size_t>(2 * mass_cap + 2);      // row bounds
@@ -1188,8 +1326,8 @@ size_t lds_bytes_massg(int block, int maxp) {
   return static_cast<size_t>(massg_term_base(block)) + 12u * static_cast<size_t>(maxp) * block;
 }
 
-size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat) {
-  size_t b = lds_term_bytes(spring_cap);                          // spring terms, sink, zero
+size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat, int tw) {
+  size_t b = lds_term_bytes(spring_cap, tw);                      // spring terms, sink, zero
   b += 16u * static_cast<size_t>(pos_slots(mass_cap));            // float4 positions (skewed)
   b += 2u * static_cast<size_t>(csr_cap);                         // gather entries (u16)
   b += 8u * static_cast<size_t>(chunk_row_quads(mass_cap));       // row bounds

@@ -30,11 +30,13 @@
  *
  * force (project spec; the reference snapshot has no force solver -- paper
  * p.2 Alg. 1 and the land/water environment of paper p.2 Sec. II.E):
- *   per non-air spring: L and d as above;
- *        f = (k * (L - d)) * (1 / (d + 1e-12f));  c = f * (x~a - x~b)
+ *   per non-air spring: L and d as above, D = x~a - x~b;
+ *        f = (k * (L - d)) * (1 / (d + 1e-12f))
  *        (Hooke's law with one correctly rounded reciprocal per spring);
- *        +c on a, -c on b  (canonical order above)
- *   per mass: F = sum(c)
+ *        the term f D enters a's sum as sum = fmaf(f, D, sum) and b's as
+ *        sum = fmaf(-f, D, sum), per component, one rounding each
+ *        (canonical order above; from b's own side, D is -D exactly)
+ *   per mass: F = that sum
  *        F.y = F.y - m*g
  *        F   = F - drag*v                       (if drag != 0)
  *        if ground_k != 0 and y < 0:
@@ -143,6 +145,18 @@ static int run_jobs(job_t proto, int nthreads) {
  * the first endpoint, - the term where it is the second (first, then
  * second, for a spring from a mass to itself).  Air springs contribute
  * nothing.  acc must hold zeros on entry. */
+/* force terms {f, Dx, Dy, Dz} per spring: sum = fmaf(+-f, D, sum) */
+static void gather_force_terms(const orc_topology* tp, uint32_t sb, uint32_t ns, const float* term,
+                               float* acc) {
+  for (uint32_t s = 0; s < ns; s++) {
+    if (tp->mat[sb + s] == 0) continue;
+    uint32_t a = tp->pairs[2 * (sb + s)], b = tp->pairs[2 * (sb + s) + 1];
+    const float* t = term + 4 * (size_t)s;
+    for (int c = 0; c < 3; c++) acc[3 * a + c] = fmaf(t[0], t[1 + c], acc[3 * a + c]);
+    for (int c = 0; c < 3; c++) acc[3 * b + c] = fmaf(-t[0], t[1 + c], acc[3 * b + c]);
+  }
+}
+
 static void gather_terms(const orc_topology* tp, uint32_t sb, uint32_t ns, uint32_t nm,
                          const float* term, float* acc, float* accm) {
   (void)nm;
@@ -286,8 +300,7 @@ static void force32_robot(job_t* j, int r) {
   float* v = (float*)j->vel + 3 * (size_t)mb;
   const float* ms = tp->mass + mb;
   float* acc = (float*)malloc(sizeof(float) * 3 * nm);
-  float* term = (float*)malloc(sizeof(float) * 3 * (ns + 1));
-  float* accm = (float*)malloc(sizeof(float) * 3 * nm);
+  float* term = (float*)malloc(sizeof(float) * 4 * (ns + 1));
   float* R = (float*)malloc(sizeof(float) * (size_t)mt->n);
   float t = j->t0;
   for (int step = 0; step < j->nsteps; step++) {
@@ -308,9 +321,9 @@ static void force32_robot(job_t* j, int r) {
       float d = nrm3f(dx, dy, dz);
       float inv = 1.0f / (d + ORC_EPS);
       float f = (mt->k[m] * (L - d)) * inv;
-      term[3 * s] = f * dx; term[3 * s + 1] = f * dy; term[3 * s + 2] = f * dz;
+      term[4 * s] = f; term[4 * s + 1] = dx; term[4 * s + 2] = dy; term[4 * s + 3] = dz;
     }
-    gather_terms(tp, sb, ns, nm, term, acc, accm);
+    gather_force_terms(tp, sb, ns, term, acc);
     for (uint32_t i = 0; i < nm; i++) {
       float m = ms[i];
       float fx = acc[3 * i], fy = acc[3 * i + 1], fz = acc[3 * i + 2];
@@ -348,7 +361,7 @@ static void force32_robot(job_t* j, int r) {
     }
     t = t + dt;
   }
-  free(acc); free(term); free(accm); free(R);
+  free(acc); free(term); free(R);
 }
 
 /* ----------------------------------------------------------- force fp64 */

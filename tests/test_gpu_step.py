@@ -261,8 +261,8 @@ def ring_robot(ms, rng, n, k, spacing=0.01):
                    rng.choice(mats, len(pairs)).astype(np.uint8), mass=np.full(n, 0.001, np.float32))
 
 
-@pytest.mark.parametrize("family", [mh.MH_KERNEL_MASS, mh.MH_KERNEL_MASS_GATHER])
-@pytest.mark.parametrize("solver", [FORCE, XPBD])
+@pytest.mark.parametrize("family,solver", [(mh.MH_KERNEL_MASS, FORCE), (mh.MH_KERNEL_MASS, XPBD),
+                                           (mh.MH_KERNEL_MASS_GATHER, XPBD)])
 def test_mass_pair_table_limits(family, solver):
     """Robots of 160 distinct (rest length, material) pairs: packs of the
     mass-parallel kernels close when the union of their pairs would pass
@@ -314,27 +314,28 @@ def grid_robot(ms, dims, spacing=0.01):
                    smat, mass=np.full(len(pos), 0.001, np.float32))
 
 
-@pytest.mark.parametrize("dims,block,family,fits", [
-    ((8, 8, 12), 768, mh.MH_KERNEL_MASS, True),          # 768 masses: a full 768-lane workgroup
-    ((8, 8, 16), 1024, mh.MH_KERNEL_MASS, True),         # 1024 masses: a full 1024-lane workgroup
-    ((8, 8, 12), 768, mh.MH_KERNEL_MASS_GATHER, True),
-    ((7, 10, 11), 768, mh.MH_KERNEL_MASS, False),        # 770 masses: over the 768-lane kernel
-    ((7, 10, 11), 0, mh.MH_KERNEL_MASS, True),           # ... which the 1024-lane one takes
+@pytest.mark.parametrize("dims,block,family,fits,solver", [
+    ((8, 8, 12), 768, mh.MH_KERNEL_MASS, True, FORCE),          # 768 masses: a full 768-lane workgroup
+    ((8, 8, 16), 1024, mh.MH_KERNEL_MASS, True, FORCE),         # 1024 masses: a full 1024-lane workgroup
+    ((8, 8, 12), 768, mh.MH_KERNEL_MASS_GATHER, True, XPBD),
+    ((7, 10, 11), 768, mh.MH_KERNEL_MASS, False, FORCE),        # 770 masses: over the 768-lane kernel
+    ((7, 10, 11), 0, mh.MH_KERNEL_MASS, True, FORCE),           # ... which the 1024-lane one takes
 ])
-def test_mass_kernel_capacity_edges(dims, block, family, fits):
-    print(f"mass kernel capacity dims={dims} block={block} family={family}", flush=True)
+def test_mass_kernel_capacity_edges(dims, block, family, fits, solver):
+    print(f"mass kernel capacity dims={dims} block={block} family={family} solver={solver}", flush=True)
     from helpers import Case
     ms = mh.workload_materials()
     case = Case([grid_robot(ms, dims)], ms, mh.materials.WORKLOAD_ROWS)
+    n, dt, env = (300, 1e-4, LAND) if solver == FORCE else (60, 1e-3, None)
     if not fits:
         with pytest.raises(mh.MhError):
-            run_gpu(case, FORCE, 10, 1e-4, LAND, kernel=family, block_size=block)
+            run_gpu(case, solver, 10, dt, env, kernel=family, block_size=block)
         return
-    sim, pos, vel, _ = run_gpu(case, FORCE, 300, 1e-4, LAND, kernel=family, block_size=block)
+    sim, pos, vel, _ = run_gpu(case, solver, n, dt, env, kernel=family, block_size=block)
     info = sim.info()
     assert info.n_lds_robots == 1 and info.n_stream_robots == 0
     assert info.block_size == (block or 1024)
-    p, v, _ = mirror(case, FORCE, 300, 1e-4, LAND)
+    p, v, _ = mirror(case, solver, n, dt, env)
     assert np.array_equal(bits(pos), bits(p))
     assert np.array_equal(bits(vel), bits(v))
     assert np.abs(pos - case.pos0).max() > 1e-6

@@ -107,22 +107,38 @@ def test_large_nn_robots_many_chunks_bitexact():
     assert np.array_equal(bits(vel), bits(v))
 
 
+# six base materials: 6 single and 15 two-material composites, over the 15
+# that the 6-byte record format holds
+SIX_ROWS = mh.materials.WORKLOAD_ROWS + [(3.0e3, 0.1, mh.materials.ACT_OMEGA, 1.5),
+                                         (7.0e3, 0.0, mh.materials.ACT_OMEGA, 0.0)]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("block", [1024, 512])
-def test_nn_robots_mass_stream_kernel_bitexact(block):
+@pytest.mark.parametrize("block,n_base", [(1024, 4), (512, 4), (1024, 6)])
+def test_nn_robots_mass_stream_kernel_bitexact(block, n_base):
     """step_mstream (one lane per mass, each evaluating all of its springs,
     records streamed every substep): robots of 60-2000 masses, so threads
     serve one to four masses, give the mirror's trajectory bit for bit, as
-    does the chunked kernel on the same robots."""
-    print(f"nn robots on the mass-parallel streaming kernel, block {block}", flush=True)
-    ms = mh.workload_materials()
-    els = robots.nn_population(ms, 4, mass_range=(60, 2000), K=25, seed=5)
+    does the chunked kernel on the same robots.  Four base materials give
+    at most 10 compact materials and the 6-byte records; six give more than
+    15 and the 8-byte records."""
+    print(f"nn robots on the mass-parallel streaming kernel, block {block}, {n_base} base materials", flush=True)
+    rows = mh.materials.WORKLOAD_ROWS if n_base == 4 else SIX_ROWS
+    ms = mh.MaterialSet.custom(rows)
+    els = robots.nn_population(ms, 4, mass_range=(60, 2000), K=25, n_materials=n_base, seed=5)
     assert max(e.n_masses for e in els) > 1024
-    case = Case(els, ms, mh.materials.WORKLOAD_ROWS)
+    if n_base == 6:  # every composite of the six (ids 1-21) on some springs
+        rng = np.random.default_rng(6)
+        for e in els:
+            e.spring_material = rng.integers(1, ms.count(), e.n_springs).astype(np.uint8)
+    n_comp = len(np.unique(np.concatenate([e.spring_material for e in els])))
+    assert (n_comp <= 15) == (n_base == 4)
+    case = Case(els, ms, rows)
     sim = gpu_sim(case, mh.MH_SOLVER_FORCE, 1e-4, WATER, kernel=mh.MH_KERNEL_MASS_STREAM, block_size=block)
     info = sim.info()
     assert info.kernel == mh.MH_KERNEL_MASS_STREAM and info.n_stream_robots == 4
     assert info.stream_block_size == block
+    assert info.stream_record_bytes == (6 if n_base == 4 else 8)
     pos0 = sim.collect_arrays()[0]
     sim.Step(150)
     pos, vel, _ = sim.collect_arrays()
@@ -134,3 +150,8 @@ def test_nn_robots_mass_stream_kernel_bitexact(block):
     assert chunk.info().stream_chunk > 0
     chunk.Step(150)
     assert np.array_equal(bits(chunk.collect_arrays()[0]), bits(pos))
+    # the variant the autotuner picks among the mass-parallel streaming ones
+    tuned = gpu_sim(case, mh.MH_SOLVER_FORCE, 1e-4, WATER, kernel=mh.MH_KERNEL_MASS_STREAM)
+    tuned.autotune(40)
+    tuned.Step(150)
+    assert np.array_equal(bits(tuned.collect_arrays()[0]), bits(pos))

@@ -251,7 +251,8 @@ def test_step_times_match_float_loop():
 
 def numpy_force_step_f32(pos, vel, mass, pairs, lbar, mat, k, R, env):
     """Independent float32 restatement of one force-solver step: per-mass
-    gather of f * delta terms, gravity, drag, ground contact and friction."""
+    gather of the f * delta terms as fused multiply-adds, gravity, drag,
+    ground contact and friction."""
     f = np.float32
     dt, g, kg, mus, muk, c, damp = (f(env[key]) for key in
                                     ("dt", "gravity", "ground_k", "mu_s", "mu_k", "drag", "damping"))
@@ -264,15 +265,15 @@ def numpy_force_step_f32(pos, vel, mass, pairs, lbar, mat, k, R, env):
         ss = fma32(d3[1], d3[1], f(d3[0] * d3[0]))
         d = f(np.sqrt(fma32(d3[2], d3[2], ss)))
         fs = f(f(k[mat[s]] * f(L - d)) * f(f(1) / f(d + f(1e-12))))
-        terms[s] = (fs * d3).astype(np.float32)
+        terms[s] = (fs, d3)
     out_p, out_v = pos.copy(), vel.copy()
     for i in range(len(pos)):
         F = np.zeros(3, np.float32)
         for s, (a, b) in enumerate(pairs):      # ascending spring index
-            if s in terms and a == i:           # first endpoint: + term
-                F = (F + terms[s]).astype(np.float32)
-            if s in terms and b == i:           # second endpoint: - term
-                F = (F - terms[s]).astype(np.float32)
+            if s in terms and a == i:           # first endpoint: fma(+f, delta, sum)
+                F = np.array([fma32(terms[s][0], terms[s][1][c], F[c]) for c in range(3)], np.float32)
+            if s in terms and b == i:           # second endpoint: fma(-f, delta, sum)
+                F = np.array([fma32(-terms[s][0], terms[s][1][c], F[c]) for c in range(3)], np.float32)
         v = vel[i].copy()
         F[1] = f(F[1] - f(mass[i] * g))
         if c != 0:
