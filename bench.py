@@ -159,11 +159,13 @@ def measured_counters(workload, kernel):
         return json.load(fh).get(f"{workload}/{kernel}")
 
 
-def measured_traffic(workload, kernel):
+def measured_traffic(workload, kernel, substeps_per_launch):
     """HBM bytes per step-kernel launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by what the
     counter reads for a copy kernel of known size) for this workload and
-    kernel, if one exists."""
+    kernel, if one exists.  Kernels that stream their records every substep
+    have an entry per substep, scaled to this launch's substeps; the others
+    read and write their state once per launch."""
     path = os.path.join(REPO, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -171,28 +173,30 @@ def measured_traffic(workload, kernel):
         entry = json.load(fh).get(f"{workload}/{kernel}")
     if not entry:
         return None
-    return {"bytes_per_launch": entry["bytes_per_launch"],
-            "source": "profiles/traffic.json: " + entry["summary"]}
+    per_launch = (entry["bytes_per_substep"] * substeps_per_launch if "bytes_per_substep" in entry
+                  else entry["bytes_per_launch"])
+    return {"bytes_per_launch": per_launch, "source": "profiles/traffic.json: " + entry["summary"]}
 
 
 def roofline_head(bound, achieved, peak, unit, extra):
     """bound / achieved / peak / unit / frac of the dominant kernel.  With
-    hardware counters for it, the bound is VALU issue when the kernel
-    issues at least 0.9 wave64 VALU instructions, or keeps the VALU busy at
-    least 0.9 of the cycles, per CU-cycle (independent FMA chains sustain
-    0.92 instructions on MI355X): then the kernel is at the issue limit,
-    and the byte model's fraction is reported beside it; frac_survey
-    is the fraction on the algorithmic LDS model of 57 B per spring plus
-    48 B per mass per substep."""
+    hardware counters for it, two rooflines are compared: VALU issue (wave64
+    VALU instructions per second against one per CU-cycle at 2.4 GHz;
+    independent FMA chains sustain 0.92 of that on MI355X) and the kernel's
+    byte model (LDS or HBM).  The bound is the one the kernel reaches the
+    larger fraction of; the other is reported beside it.  frac_survey is
+    the fraction on the algorithmic LDS model of 57 B per spring plus 48 B
+    per mass per substep."""
     v = extra.get("valu_issue")
-    if v and max(v["per_cu_cycle"], v.get("busy_per_cu_cycle") or 0.0) >= 0.9:
+    byte_frac = (achieved / peak) if achieved else None
+    if v and (byte_frac is None or v["frac"] > byte_frac):
         out = {"bound": "valu", "achieved": v["achieved"], "peak": v["peak"], "unit": v["unit"],
                "frac": v["frac"], "byte_model_bound": bound, "byte_model_achieved": achieved,
-               "byte_model_peak": peak, "byte_model_unit": unit,
-               "byte_model_frac": (achieved / peak) if achieved else None}
+               "byte_model_peak": peak, "byte_model_unit": unit, "byte_model_frac": byte_frac}
     else:
-        out = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-               "frac": (achieved / peak) if achieved else None}
+        out = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": byte_frac}
+        if v:
+            out.update({"valu_achieved": v["achieved"], "valu_peak": v["peak"], "valu_frac": v["frac"]})
     if "survey_model" in extra:
         out["frac_survey"] = extra["survey_model"]["frac"]
     return out
@@ -398,8 +402,10 @@ def main():
         per_unit = STREAM_BYTES_PER_SPRING
     achieved = per_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else None
     kernel_name = kernel_label(info)
-    traffic = measured_traffic(args.workload, kernel_name)
-    counters = measured_counters(args.workload, kernel_name)
+    # the dominant kernel's family names its committed profiles
+    dominant = "mass-stream" if info.kernel & mh.MH_KERNEL_MASS_STREAM and info.n_stream_springs > 0 else kernel_name
+    traffic = measured_traffic(args.workload, dominant, steps_per_launch)
+    counters = measured_counters(args.workload, dominant)
     extra = {}
     if bound == "lds" and kernel_ms > 0:
         # the algorithmic LDS model: 57 B per spring (two endpoint
@@ -424,6 +430,8 @@ def main():
             "achieved": valu, "peak": VALU_PEAK_WAVE_INSTR, "unit": "wave64 instr/s",
             "frac": valu / VALU_PEAK_WAVE_INSTR,
             "per_cu_cycle": per_cu_cycle,
+            # the clock the profiled cycles imply for this launch's time
+            "clock_ghz": counters["gpu_cycles_per_launch"] * scale / (kernel_ms * 1e-3) / 1e9,
             # issue cycles: a transcendental holds the VALU two cycles
             # (SQ_ACTIVE_INST_VALU = SQ_INSTS_VALU + SQ_INSTS_VALU_TRANS_F32)
             "busy_per_cu_cycle": (counters["active_valu_per_launch"] /

@@ -822,7 +822,9 @@ void upload_plan(mh_sim* s, Plan& plan) {
     }
     part.pack_offset = static_cast<uint32_t>(all.size());
     all.insert(all.end(), part.packs.begin(), part.packs.end());
-    if (part.lds > 64 * 1024) {
+    // dynamic LDS above 64 KB needs the attribute (the mass-parallel
+    // streaming kernel's LDS is static)
+    if (part.lds > 64 * 1024 && !(part.v->kind == MH_KERNEL_STREAM && part.v->chunked == 2)) {
       for (const void* fn : {part.v->fn, part.v->fn_stress})
         if (fn)
           (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(part.lds));

@@ -39,6 +39,8 @@
 //                terms in HBM.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "mercury_hip.h"
 #include "mh_device.h"
 #include "mh_kernels.h"
@@ -177,6 +179,14 @@ __device__ __forceinline__ void force_factor(float4 pa, float4 pb, float L, floa
 // Words of a stored spring term: xpbd (cx, cy, cz); force (Dx, Dy, Dz, f).
 template <int SOLVER>
 constexpr int term_words() { return solver_term_words(SOLVER); }
+
+// Marks the unused w of a position read from LDS as used, with no
+// instruction.  Left alone, the compiler narrows such float4 reads to
+// ds_read_b96, which on gfx950 costs more LDS cycles than ds_read_b128
+// when a wave's addresses are scattered (a wave reading random 16-byte
+// slots: 17 cycles against 12).  Placed after x, y and z have been used,
+// so it adds no wait.
+__device__ __forceinline__ void keep_b128(float w) { asm volatile("" ::"v"(w)); }
 
 // Adds four gathered terms to a sum, in entry order.  An entry (16 bits) is
 // twice the word offset of a stored term, + 1 where the mass is the
@@ -640,11 +650,16 @@ __global__ __launch_bounds__(BLOCK) void step_chunk(StepParams p) {
 //      float4[n4][64], n4 = ceil(longest / 4).  12 bytes per spring-update
 //      instead of 16, in two loads per 4 records.
 // A lane loads its records BATCH at a time, the next BATCH in flight while
-// the current ones are evaluated.  LDS: position buffers
-// float4[2][mass_slots(BLOCK MPT)] (read one, write the other) | per
-// material {R(t), k} float2[2][kMStreamMats] (the null material last).
+// the current ones are evaluated.  LDS, a static array so that every base
+// is an instruction offset: per material {R(t), k} float2[2][kMStreamMats]
+// (the null material last) | position buffers
+// float4[2][mass_slots(BLOCK MPT)] (read one, write the other).  Substeps
+// run in pairs, so the buffer a substep reads is a compile-time choice.
 template <int BLOCK, int MPT>
 constexpr int mstream_pos_bytes() { return 16 * mass_slots(BLOCK * MPT); }
+constexpr int kMStreamMatBytes = 2 * 8 * kMStreamMats;
+template <int BLOCK, int MPT>
+constexpr int mstream_lds_bytes() { return kMStreamMatBytes + 2 * mstream_pos_bytes<BLOCK, MPT>(); }
 
 // Adds one record's spring to (ax, ay, az): the other endpoint's position
 // at LDS byte offset po, its material's {R(t), k} at mo.
@@ -658,6 +673,7 @@ __device__ __forceinline__ void mstream_record(const unsigned char* pos_r, const
   ax = fmaf(f, dx, ax);
   ay = fmaf(f, dy, ay);
   az = fmaf(f, dz, az);
+  keep_b128(other.w);
 }
 
 // WPE: waves per SIMD the register allocation must allow (8 at BLOCK 1024:
@@ -669,13 +685,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
   static_assert(REC == 8 || (REC == 6 && BATCH % 4 == 0), "record format");
   static_assert(BLOCK % 64 == 0, "a wave serves one group of 64 virtual lanes");
   constexpr int kMStreamBatch = BATCH;
-  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ __align__(16) unsigned char lds[mstream_lds_bytes<BLOCK, MPT>()];
+  unsigned char* const smem = lds + kMStreamMatBytes;  // the position buffers
   const PackDesc d = p.packs[blockIdx.x];
   const int nm = static_cast<int>(d.n_mass);
   constexpr int kSlots = mass_slots(BLOCK * MPT);
   constexpr int kGhost = kSlots - 1;
   constexpr int kPos = mstream_pos_bytes<BLOCK, MPT>();
-  constexpr int kMat = 2 * kPos;
   const int tid = threadIdx.x;
   const int n_mat = p.n_mat;
   const uint2* recs = reinterpret_cast<const uint2*>(p.mrec) + d.csr_begin;
@@ -708,17 +724,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
       if (i < n_mat)
         e = make_float2(actuation(p.mat_dL0[i], p.mat_omega[i], p.mat_phi[i], tm),
                         (SOLVER == MH_SOLVER_XPBD_REF) ? p.mat_kx[i] : p.mat_k[i]);
-      reinterpret_cast<float2*>(smem + kMat + buf * 8 * kMStreamMats)[i] = e;
+      reinterpret_cast<float2*>(lds + buf * 8 * kMStreamMats)[i] = e;
     }
   };
   mat_table(0, t);
   __syncthreads();
 
-  for (int step = 0; step < p.n_steps; ++step) {
-    const int cur = step & 1;
+  auto substep = [&](auto cur_c) {
+    constexpr int cur = decltype(cur_c)::value;
     const unsigned char* pos_r = smem + cur * kPos;
     unsigned char* pos_w = smem + (1 - cur) * kPos;
-    const unsigned char* mat_r = smem + kMat + cur * 8 * kMStreamMats;
+    const unsigned char* mat_r = lds + cur * 8 * kMStreamMats;
 #pragma unroll
     for (int j = 0; j < MPT; ++j) {
       const int v = tid + j * BLOCK;
@@ -812,7 +828,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
     t = t + p.dt;
     mat_table(1 - cur, t);
     __syncthreads();
+  };
+  int step = 0;
+  for (; step + 1 < p.n_steps; step += 2) {
+    substep(std::integral_constant<int, 0>{});
+    substep(std::integral_constant<int, 1>{});
   }
+  if (step < p.n_steps) substep(std::integral_constant<int, 0>{});
 
 #pragma unroll
   for (int j = 0; j < MPT; ++j) {
@@ -937,6 +959,7 @@ __device__ __forceinline__ void mass_substep(unsigned char* smem, const StepPara
           if (k < nr && ((first >> k) & 1u))
             st[k] = st[k] + lambda / *reinterpret_cast<const float*>(smem + kAux + (r >> 16) + 4);
       }
+      keep_b128(other.w);
     }
     // one record at a time: keeps the unrolled loop's registers bounded
     __builtin_amdgcn_sched_barrier(0);
@@ -1186,6 +1209,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
         const float2 e = *reinterpret_cast<const float2*>(smem + kDyn + (r >> 16));
         const float2 a = *reinterpret_cast<const float2*>(smem + kAux + (r >> 16));
         xpbd_term_L(own, other, e.x, e.y, a.x, fast_ok, cx, cy, cz, lambda);
+        keep_b128(other.w);
         if (STRESS && k < np) st[k] = st[k] + lambda / a.y;
         float* slot = reinterpret_cast<float*>(mine + 12 * k);
         slot[0] = cx;
@@ -1337,7 +1361,7 @@ size_t lds_bytes_chunk(int mass_cap, int spring_cap, int csr_cap, int n_mat, int
 }
 
 size_t lds_bytes_mstream(int block, int mpt) {
-  return 2u * 16u * static_cast<size_t>(mass_slots(block * mpt)) + 2u * 8u * kMStreamMats;
+  return kMStreamMatBytes + 2u * 16u * static_cast<size_t>(mass_slots(block * mpt));
 }
 
 size_t lds_bytes_stream(int mass_cap, int n_mat) {
@@ -1349,7 +1373,9 @@ hipError_t launch_step(const KernelVariant& v, int n_packs, size_t lds_bytes, hi
   StepParams args = p;
   void* kargs[] = {&args};
   const void* fn = (p.track_stress && v.fn_stress) ? v.fn_stress : v.fn;
-  return hipLaunchKernel(fn, dim3(n_packs), dim3(v.block), kargs, lds_bytes, stream);
+  // the mass-parallel streaming kernel's LDS is a static array
+  const size_t dyn = (v.kind == MH_KERNEL_STREAM && v.chunked == 2) ? 0 : lds_bytes;
+  return hipLaunchKernel(fn, dim3(n_packs), dim3(v.block), kargs, dyn, stream);
 }
 
 hipError_t launch_clock(float* times, float t0, float dt, int n_steps, int K, hipStream_t stream) {

@@ -31,11 +31,18 @@ def test_substep_count_follows_the_float_time_loop():
 
 
 def test_traffic_lookup_reads_the_committed_counter_summary():
-    for workload, kernel in (("voxel", "mass"), ("voxel", "mass-gather"), ("nn", "stream (chunked)")):
-        t = bench.measured_traffic(workload, kernel)
+    for workload, kernel in (("voxel", "mass"), ("voxel", "mass-gather"), ("nn", "stream (chunked)"),
+                             ("nn", "mass-stream")):
+        t = bench.measured_traffic(workload, kernel, 1000)
         assert t is not None and t["bytes_per_launch"] > 0
         assert t["source"].startswith("profiles/traffic.json: ")
-    assert bench.measured_traffic("voxel", "no-such-kernel") is None
+    # streaming kernels read their records every substep; the mass-parallel
+    # kernel reads its state once per launch
+    assert bench.measured_traffic("nn", "mass-stream", 2000)["bytes_per_launch"] == \
+        2 * bench.measured_traffic("nn", "mass-stream", 1000)["bytes_per_launch"]
+    assert bench.measured_traffic("voxel", "mass", 2000)["bytes_per_launch"] == \
+        bench.measured_traffic("voxel", "mass", 1000)["bytes_per_launch"]
+    assert bench.measured_traffic("voxel", "no-such-kernel", 1000) is None
 
 
 def test_kernel_labels():
