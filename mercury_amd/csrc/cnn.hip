@@ -68,29 +68,74 @@ __device__ __forceinline__ uint16_t onehot_col(const uint8_t* g, int r, int kk) 
   if (!inside(z, y, x, kG)) return 0;
   return g[static_cast<size_t>(n) * kP1 + (z * kG + y) * kG + x] == c ? uint16_t(0x3f80) : uint16_t(0);
 }
+// Columns kk .. kk + 7 of row r (kk a multiple of 8, C a multiple of 8: one
+// kernel offset, 8 consecutive channels, 32-byte aligned in the activation
+// buffers).
+template <int D, int C>
+__device__ __forceinline__ u16x8_t conv_col8(const float* act, int r, int kk) {
+  static_assert(C % 8 == 0, "8 channels of one kernel offset");
+  constexpr int P = D * D * D;
+  const int k = kk / C, c = kk - k * C;
+  const int n = r / P, pos = r - n * P;
+  const int x = pos % D + k % 3 - 1, y = (pos / D) % D + (k / 3) % 3 - 1, z = pos / (D * D) + k / 9 - 1;
+  if (!inside(z, y, x, D)) return u16x8_t{0, 0, 0, 0, 0, 0, 0, 0};
+  return ld8_bf16_a16(act + (static_cast<size_t>(n) * P + (z * D + y) * D + x) * C + c);
+}
 // im2col matrices as the forward GEMM's A operand (rows r, columns k*C + c)
 // and as the weight gradient's X operand (the same matrix)
 template <int D, int C>
 struct ConvColsA {
   const float* act;
   static constexpr bool kContigK = true;
+  static constexpr bool kVec8 = true;
   __device__ uint16_t operator()(int m, int k) const { return conv_col<D, C>(act, m, k); }
+  __device__ u16x8_t vec8(int m, int k) const { return conv_col8<D, C>(act, m, k); }
 };
 template <int D, int C>
 struct ConvColsB {
   const float* act;
   static constexpr bool kContigK = false;
+  static constexpr bool kVec8 = true;
   __device__ uint16_t operator()(int k, int n) const { return conv_col<D, C>(act, k, n); }
+  __device__ u16x8_t vec8(int k, int n) const { return conv_col8<D, C>(act, k, n); }
 };
+// Columns kk .. kk + 7 of row r of the one-hot im2col matrix: 8 columns
+// cross kernel offsets (5 channels each), so the row is decoded once and
+// the offset and channel advance column by column.
+__device__ __forceinline__ u16x8_t onehot_col8(const uint8_t* g, int r, int kk) {
+  const int n = r / kP1, pos = r - n * kP1;
+  const int px = pos % kG, py = (pos / kG) % kG, pz = pos / (kG * kG);
+  const uint8_t* gn = g + static_cast<size_t>(n) * kP1;
+  int k = kk / kC0, c = kk - k * kC0;
+  u16x8_t v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    uint16_t e = 0;
+    if (k < 27) {
+      const int x = px + k % 3 - 1, y = py + (k / 3) % 3 - 1, z = pz + k / 9 - 1;
+      if (inside(z, y, x, kG) && gn[(z * kG + y) * kG + x] == c) e = 0x3f80;
+    }
+    v[j] = e;
+    if (++c == kC0) {
+      c = 0;
+      ++k;
+    }
+  }
+  return v;
+}
 struct OneHotA {
   const uint8_t* g;
   static constexpr bool kContigK = true;
+  static constexpr bool kVec8 = true;
   __device__ uint16_t operator()(int m, int k) const { return onehot_col(g, m, k); }
+  __device__ u16x8_t vec8(int m, int k) const { return onehot_col8(g, m, k); }
 };
 struct OneHotB {
   const uint8_t* g;
   static constexpr bool kContigK = false;
+  static constexpr bool kVec8 = true;
   __device__ uint16_t operator()(int k, int n) const { return onehot_col(g, k, n); }
+  __device__ u16x8_t vec8(int k, int n) const { return onehot_col8(g, k, n); }
 };
 // Input gradient of a convolution, d[n, pos, c] = sum over k, o of
 // dZ[n, pos - offset(k), o] W[o][k*C + c] (positions inside the grid), as a
@@ -100,22 +145,38 @@ template <int D, int O>
 struct ShiftedGrad {
   const float* dz;
   static constexpr bool kContigK = true;
-  __device__ uint16_t operator()(int m, int kk) const {
+  static constexpr bool kVec8 = O % 8 == 0;
+  __device__ int row(int m, int k) const {  // -1 outside the grid
     constexpr int P = D * D * D;
-    const int k = kk / O, o = kk - k * O;
     const int n = m / P, pos = m - n * P;
     const int x = pos % D - (k % 3 - 1), y = (pos / D) % D - ((k / 3) % 3 - 1), z = pos / (D * D) - (k / 9 - 1);
-    if (!inside(z, y, x, D)) return 0;
-    return bf16_bits(dz[(static_cast<size_t>(n) * P + (z * D + y) * D + x) * O + o]);
+    return inside(z, y, x, D) ? n * P + (z * D + y) * D + x : -1;
+  }
+  __device__ uint16_t operator()(int m, int kk) const {
+    const int k = kk / O, o = kk - k * O;
+    const int r = row(m, k);
+    return r < 0 ? uint16_t(0) : bf16_bits(dz[static_cast<size_t>(r) * O + o]);
+  }
+  // kk a multiple of 8: 8 consecutive output channels of one offset
+  __device__ u16x8_t vec8(int m, int kk) const {
+    const int k = kk / O, o = kk - k * O;
+    const int r = row(m, k);
+    return r < 0 ? u16x8_t{0, 0, 0, 0, 0, 0, 0, 0} : ld8_bf16_a16(dz + static_cast<size_t>(r) * O + o);
   }
 };
 template <int C, int O>
 struct ConvWeightT {
   const __hip_bfloat16* w;
   static constexpr bool kContigK = false;
+  static constexpr bool kVec8 = C % 8 == 0;
   __device__ uint16_t operator()(int kk, int c) const {
     const int k = kk / O, o = kk - k * O;
     return ld_bf16(w, static_cast<size_t>(o) * (27 * C) + k * C + c);
+  }
+  // c a multiple of 8: 8 consecutive input channels
+  __device__ u16x8_t vec8(int kk, int c) const {
+    const int k = kk / O, o = kk - k * O;
+    return ld8_bf16(w + static_cast<size_t>(o) * (27 * C) + k * C + c);
   }
 };
 

@@ -61,10 +61,35 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef uint16_t u16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
+// bf16 bits of 8 consecutive elements at p: 16-byte loads where p is
+// 16-byte aligned, element loads otherwise.
+__device__ __forceinline__ u16x8_t ld8_bf16_a16(const float* p) {
+  const float4 a = reinterpret_cast<const float4*>(p)[0], b = reinterpret_cast<const float4*>(p)[1];
+  return u16x8_t{bf16_bits(a.x), bf16_bits(a.y), bf16_bits(a.z), bf16_bits(a.w),
+                 bf16_bits(b.x), bf16_bits(b.y), bf16_bits(b.z), bf16_bits(b.w)};
+}
+__device__ __forceinline__ u16x8_t ld8_bf16(const float* p) {
+  if ((reinterpret_cast<uintptr_t>(p) & 15u) == 0) return ld8_bf16_a16(p);
+  u16x8_t r;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = bf16_bits(p[j]);
+  return r;
+}
+__device__ __forceinline__ u16x8_t ld8_bf16(const __hip_bfloat16* p) {
+  if ((reinterpret_cast<uintptr_t>(p) & 15u) == 0) return *reinterpret_cast<const u16x8_t*>(p);
+  u16x8_t r;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = __builtin_bit_cast(uint16_t, p[j]);
+  return r;
+}
+
 // Operand loaders of the matrix-core tile: A(m, k) and B(k, n) return the
 // bf16 bits of op(A)[m][k] and op(B)[k][n] (callers stay within M, N and
 // the k range).  kContigK says which index is contiguous in memory (k, or
-// m / n), so a tile's elements are read along it.
+// m / n), so a tile's elements are read along it.  A loader with kVec8
+// also reads 8 consecutive elements along that index in one call,
+// vec8(a, b), with (a, b) the first element's indices in the order of
+// operator(); callers use it only when all 8 lie inside the operand.
 //   StridedA: op(A)[m][k] = TR ? p[k*ld + m] : p[m*ld + k]
 //   StridedB: op(B)[k][n] = TR ? p[n*ld + k] : p[k*ld + n]
 template <typename T, bool TR>
@@ -72,8 +97,12 @@ struct StridedA {
   const T* p;
   int ld;
   static constexpr bool kContigK = !TR;
+  static constexpr bool kVec8 = true;
   __device__ uint16_t operator()(int m, int k) const {
     return TR ? ld_bf16(p, static_cast<size_t>(k) * ld + m) : ld_bf16(p, static_cast<size_t>(m) * ld + k);
+  }
+  __device__ u16x8_t vec8(int m, int k) const {
+    return ld8_bf16(p + (TR ? static_cast<size_t>(k) * ld + m : static_cast<size_t>(m) * ld + k));
   }
 };
 template <typename T, bool TR>
@@ -81,10 +110,67 @@ struct StridedB {
   const T* p;
   int ld;
   static constexpr bool kContigK = TR;
+  static constexpr bool kVec8 = true;
   __device__ uint16_t operator()(int k, int n) const {
     return TR ? ld_bf16(p, static_cast<size_t>(n) * ld + k) : ld_bf16(p, static_cast<size_t>(k) * ld + n);
   }
+  __device__ u16x8_t vec8(int k, int n) const {
+    return ld8_bf16(p + (TR ? static_cast<size_t>(n) * ld + k : static_cast<size_t>(k) * ld + n));
+  }
 };
+
+// Stages one 64 x 32 slice of an operand in LDS (rows along m or n, k
+// contiguous), 8 elements per thread of 256.  ROWS = M (A) or N (B); A
+// loaders take (row, k), B loaders (k, row).
+template <bool IS_A, typename L>
+__device__ __forceinline__ uint16_t load_at(const L& l, int row, int k) {
+  if constexpr (IS_A) return l(row, k);
+  else return l(k, row);
+}
+template <bool IS_A, typename L>
+__device__ __forceinline__ u16x8_t vec8_at(const L& l, int row, int k) {
+  if constexpr (IS_A) return l.vec8(row, k);
+  else return l.vec8(k, row);
+}
+template <bool IS_A, typename L>
+__device__ __forceinline__ void stage_slice(const L& l, int ROWS, int r0, int k0, int k_end, uint16_t (*s)[40]) {
+  if constexpr (L::kVec8) {
+    if constexpr (L::kContigK) {
+      // thread -> one row, 8 consecutive k
+      const int rr = threadIdx.x >> 2, kk = (threadIdx.x & 3) << 3;
+      const int r = r0 + rr, k = k0 + kk;
+      u16x8_t v;
+      if (r < ROWS && k + 8 <= k_end) {
+        v = vec8_at<IS_A>(l, r, k);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (r < ROWS && k + j < k_end) ? load_at<IS_A>(l, r, k + j) : uint16_t(0);
+      }
+      *reinterpret_cast<u16x8_t*>(&s[rr][kk]) = v;
+    } else {
+      // thread -> one k, 8 consecutive rows
+      const int kk = threadIdx.x >> 3, rr = (threadIdx.x & 7) << 3;
+      const int r = r0 + rr, k = k0 + kk;
+      u16x8_t v;
+      if (r + 8 <= ROWS && k < k_end) {
+        v = vec8_at<IS_A>(l, r, k);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (r + j < ROWS && k < k_end) ? load_at<IS_A>(l, r + j, k) : uint16_t(0);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s[rr + j][kk] = v[j];
+    }
+  } else {
+    // element by element, read along the operand's contiguous index
+    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+      int rr, kk;
+      if (L::kContigK) { kk = e % 32; rr = e / 32; } else { rr = e % 64; kk = e / 64; }
+      const int r = r0 + rr, k = k0 + kk;
+      s[rr][kk] = (r < ROWS && k < k_end) ? load_at<IS_A>(l, r, k) : uint16_t(0);
+    }
+  }
+}
 
 // One 64 x 64 tile of op(A) op(B) over k in [k_begin, k_end) on the matrix
 // cores (v_mfma_f32_16x16x32_bf16: bf16 operands, fp32 accumulate), 256
@@ -107,19 +193,12 @@ __device__ __forceinline__ void mfma_tile(const LA& la, const LB& lb, int M, int
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  static_assert(kTile == 64 && kMk == 32 && kMRow == 40, "stage_slice's shape");
   for (int k0 = k_begin; k0 < k_end; k0 += kMk) {
     // 64 x 32 elements of each operand, 8 per thread, read along the
     // operand's contiguous dimension
-    for (int e = threadIdx.x; e < kTile * kMk; e += 256) {
-      int mm, kk;
-      if (LA::kContigK) { kk = e % kMk; mm = e / kMk; } else { mm = e % kTile; kk = e / kTile; }
-      const int m = m0 + mm, k = k0 + kk;
-      sa[mm][kk] = (m < M && k < k_end) ? la(m, k) : uint16_t(0);
-      int nn, kb;
-      if (LB::kContigK) { kb = e % kMk; nn = e / kMk; } else { nn = e % kTile; kb = e / kTile; }
-      const int n = n0 + nn, kq = k0 + kb;
-      sb[nn][kb] = (n < N && kq < k_end) ? lb(kq, n) : uint16_t(0);
-    }
+    stage_slice<true>(la, M, m0, k0, k_end, sa);
+    stage_slice<false>(lb, N, n0, k0, k_end, sb);
     __syncthreads();
     bf16x8_t fa[2], fb[2];
 #pragma unroll

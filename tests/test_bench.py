@@ -31,8 +31,7 @@ def test_substep_count_follows_the_float_time_loop():
 
 
 def test_traffic_lookup_reads_the_committed_counter_summary():
-    for workload, kernel in (("voxel", "mass"), ("voxel", "mass-gather"), ("nn", "stream (chunked)"),
-                             ("nn", "mass-stream")):
+    for workload, kernel in (("voxel", "mass"), ("nn", "stream (chunked)"), ("nn", "mass-stream")):
         t = bench.measured_traffic(workload, kernel, 1000)
         assert t is not None and t["bytes_per_launch"] > 0
         assert t["source"].startswith("profiles/traffic.json: ")
