@@ -698,7 +698,9 @@ void upload_mass(mh_sim* s, Part& part, std::vector<uint16_t>& perm, std::vector
         last_key = key;
         last_pi = index_of(key);
       }
-      const uint32_t tail = (8u * last_pi) << 16;
+      // the pair's byte offset in the kernel's table: float2 entries
+      // (step_mass), float4 (step_massg)
+      const uint32_t tail = ((gather ? 16u : 8u) * last_pi) << 16;
       const uint32_t rec_a = 16u * static_cast<uint32_t>(mass_slot(static_cast<int>(b))) | tail;
       if (!gather) {
         // both endpoints hold a record, in ascending spring index; the

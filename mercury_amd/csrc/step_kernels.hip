@@ -908,12 +908,35 @@ __device__ __forceinline__ void mass_pairs_init(unsigned char* smem, const StepP
   }
 }
 
-// The next substep's actuated rest lengths: L(t) of every pair into dyn.
-template <int BLOCK>
+// The next substep's actuated rest lengths: L(t) of every pair into the
+// first word of its dyn entry (STRIDE words per entry).
+template <int BLOCK, int STRIDE = 2>
 __device__ __forceinline__ void mass_pairs_advance(unsigned char* smem, int n_pairs, int dyn, int cst, float t) {
   for (int i = static_cast<int>(threadIdx.x); i < n_pairs; i += BLOCK) {
     const float4 cs = reinterpret_cast<const float4*>(smem + cst)[i];
-    reinterpret_cast<float*>(smem + dyn)[2 * i] = fmaf(cs.w, actuation(cs.x, cs.y, cs.z, t), cs.w);
+    reinterpret_cast<float*>(smem + dyn)[STRIDE * i] = fmaf(cs.w, actuation(cs.x, cs.y, cs.z, t), cs.w);
+  }
+}
+
+// step_massg's pair table: one float4 {L(t), k, 1/k, rest length} per pair
+// at dyn (the null pair's stiffness 1 with reciprocal 0: a signed-zero
+// term), so a record reads its pair with one ds_read_b128; the constants
+// {dL0, omega, phi, rest length} at cst.
+template <int BLOCK>
+__device__ __forceinline__ void massg_pairs_init(unsigned char* smem, const StepParams& p, const PackDesc& d,
+                                                 int dyn, int cst, float t) {
+  const int n = static_cast<int>(d.pad1);
+  for (int i = static_cast<int>(threadIdx.x); i < n; i += BLOCK) {
+    const float lb = p.lbtab[d.pad0 + 2 * i];
+    const uint32_t c = __float_as_uint(p.lbtab[d.pad0 + 2 * i + 1]);
+    const float k = p.mat_kx[c];
+    const float4 cs = make_float4(p.mat_dL0[c], p.mat_omega[c], p.mat_phi[c], lb);
+    reinterpret_cast<float4*>(smem + cst)[i] = cs;
+    reinterpret_cast<float4*>(smem + dyn)[i] = make_float4(fmaf(lb, actuation(cs.x, cs.y, cs.z, t), lb), k, 1.0f / k, lb);
+  }
+  if (threadIdx.x == 0) {
+    reinterpret_cast<float4*>(smem + cst)[kMassPairNull] = make_float4(0.f, 0.f, 0.f, 1.f);
+    reinterpret_cast<float4*>(smem + dyn)[kMassPairNull] = make_float4(1.f, 1.f, 0.f, 1.f);
   }
 }
 
@@ -1113,12 +1136,13 @@ __device__ __forceinline__ bool position_in_div_range(float4 q) {
 // writes its position in place; the next substep's actuated rest lengths
 // are written in place too.  Barrier.  Two barriers per substep, half the
 // spring arithmetic of step_mass.  LDS: a zero term at kMassZero and two
-// flags at kMassZero + 16 | positions float4[mass_slots(BLOCK)] | the pair
-// tables {L, k}, {1/k, rest length} and the constants, as in step_mass but
-// one {L, k} buffer | terms.  p.mrec parts: 0..MAXP - 1 the first-endpoint
-// records (as step_mass's), MAXP..3 MAXP - 1 the entries (padding: the
-// zero term), 3 MAXP + k the pack-local spring index of first-endpoint
-// record k; p.mcnt: two counts per lane (records, entries).  XPBD: the
+// flags at kMassZero + 16 | positions float4[mass_slots(BLOCK)] | one pair
+// table float4 {L, k, 1/k, rest length} and the constants | terms.  p.mrec
+// parts: 0..MAXP - 1 the first-endpoint records (as step_mass's, with the
+// pair's offset 16 * pair for the float4 table), MAXP..3 MAXP - 1 the
+// entries (padding: the zero term), 3 MAXP + k the pack-local spring index
+// of first-endpoint record k; p.mcnt: two counts per lane (records,
+// entries).  XPBD: the
 // flags (one per substep parity) record whether any position written for
 // that substep left the fast division's range.
 template <int SOLVER, int BLOCK, int MAXP, bool STRESS>
@@ -1130,8 +1154,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   const int nm = static_cast<int>(d.n_mass);
   constexpr int kSlots = mass_slots(BLOCK);
   constexpr int kGhost = kSlots - 1;
-  constexpr int kDyn = mass_pair_base(BLOCK, 1);
-  constexpr int kAux = kDyn + kMassDynBytes;
+  constexpr int kDyn = mass_pair_base(BLOCK, 1);  // float4 per pair, over two float2 tables' room
   constexpr int kCst = kDyn + 2 * kMassDynBytes;
   constexpr int kTerm = massg_term_base(BLOCK);
   const int tid = threadIdx.x;
@@ -1142,7 +1165,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   uint32_t rp[MAXP], re[2 * MAXP];
   float st[MAXP];
   int np = 0, ne = 0, mass = 0;
-  const uint32_t pad = 16u * kGhost | (8u * kMassPairNull) << 16;
+  const uint32_t pad = 16u * kGhost | (16u * kMassPairNull) << 16;
 #pragma unroll
   for (int k = 0; k < MAXP; ++k) {
     rp[k] = pad;
@@ -1169,10 +1192,8 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
   if (live) pos[pslot] = lds_position<SOLVER>(ms);
   uint32_t* s_flag = reinterpret_cast<uint32_t*>(smem + kMassZero + 16);
   float t = p.clock[p.launch];
-  // one dynamic buffer, updated in place in phase 2; null pair: stiffness 0
-  // (force) or 1 with reciprocal 0 (xpbd), a signed zero either way
-  mass_pairs_init<SOLVER, BLOCK>(smem, p, d, kDyn, kDyn, kAux, kCst, t,
-                                 (SOLVER == MH_SOLVER_XPBD_REF) ? 1.0f : 0.0f, 0.0f);
+  // one pair table, its rest lengths updated in place in phase 2
+  massg_pairs_init<BLOCK>(smem, p, d, kDyn, kCst, t);
   const int n_pairs = static_cast<int>(d.pad1);
   if (tid == 0) {
     pos[kGhost] = make_float4(1.0e3f, 1.0e3f, 1.0e3f, 0.f);
@@ -1206,11 +1227,11 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
         const uint32_t r = rp[k];
         const float4 other = *reinterpret_cast<const float4*>(smem + kMassPos + (r & 0xffffu));
         float cx, cy, cz, lambda;
-        const float2 e = *reinterpret_cast<const float2*>(smem + kDyn + (r >> 16));
-        const float2 a = *reinterpret_cast<const float2*>(smem + kAux + (r >> 16));
-        xpbd_term_L(own, other, e.x, e.y, a.x, fast_ok, cx, cy, cz, lambda);
+        const float4 e = *reinterpret_cast<const float4*>(smem + kDyn + (r >> 16));  // {L, k, 1/k, rest}
+        xpbd_term_L(own, other, e.x, e.y, e.z, fast_ok, cx, cy, cz, lambda);
         keep_b128(other.w);
-        if (STRESS && k < np) st[k] = st[k] + lambda / a.y;
+        keep_b128(e.w);
+        if (STRESS && k < np) st[k] = st[k] + lambda / e.w;
         float* slot = reinterpret_cast<float*>(mine + 12 * k);
         slot[0] = cx;
         slot[1] = cy;
@@ -1235,7 +1256,7 @@ __global__ __launch_bounds__(BLOCK) void step_massg(StepParams p) {
       pos[pslot] = np4;
       if (SOLVER == MH_SOLVER_XPBD_REF && !position_in_div_range(np4)) atomicOr(&s_flag[(step + 1) & 1], 1u);
     }
-    mass_pairs_advance<BLOCK>(smem, n_pairs, kDyn, kCst, t + p.dt);
+    mass_pairs_advance<BLOCK, 4>(smem, n_pairs, kDyn, kCst, t + p.dt);
     t = t + p.dt;
     __syncthreads();
   }
