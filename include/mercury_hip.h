@@ -125,7 +125,7 @@ enum { MH_SOLVER_XPBD_REF = 0, MH_SOLVER_FORCE = 1 };
  *           its springs (both endpoints evaluate a spring);
  *   MASS_GATHER  robots resident in LDS, one lane per mass; a spring is
  *           evaluated by its first endpoint and its second endpoint reads
- *           the stored term;
+ *           the stored term (XPBD solver);
  *   MASS_STREAM  robots over the LDS budget (force solver, up to 2048
  *           masses of at most 255 springs each), one lane per mass
  *           evaluating each of its springs, spring records streamed per

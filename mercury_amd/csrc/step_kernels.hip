@@ -836,11 +836,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE))) vo
   }
   if (step < p.n_steps) substep(std::integral_constant<int, 0>{});
 
+  // the mass index and the mass itself are read again here rather than
+  // kept in registers through the substep loop
 #pragma unroll
   for (int j = 0; j < MPT; ++j) {
     if (tid + j * BLOCK < nm) {
-      const uint32_t gi = d.mass_begin + mass[j];
-      p.pos[gi] = make_float4(ms[j].x, ms[j].y, ms[j].z, ms[j].m);
+      const uint32_t gi = d.mass_begin + p.mperm[d.mass_begin + tid + j * BLOCK];
+      p.pos[gi] = make_float4(ms[j].x, ms[j].y, ms[j].z, p.pos[gi].w);
       p.vel[gi] = make_float4(ms[j].vx, ms[j].vy, ms[j].vz, 0.f);
     }
   }
