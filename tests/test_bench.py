@@ -66,3 +66,16 @@ def test_cpu_baseline_leg_runs_the_mirror():
     rate64, _ = bench.cpu_baseline(els, mh.MH_SOLVER_XPBD_REF, 1e-3, 20, 1, {}, f64=True)
     assert rate64 > 0
     assert np.isfinite(rate) and np.isfinite(rate64)
+
+
+def test_roofline_head_takes_the_larger_fraction():
+    valu = {"achieved": 5.9e11, "peak": 6.144e11, "unit": "wave64 instr/s", "frac": 5.9e11 / 6.144e11}
+    head = bench.roofline_head("lds", 3.3e4, 1.4e5, "GB/s", {"valu_issue": valu})
+    assert head["bound"] == "valu" and head["frac"] == valu["frac"]
+    assert head["byte_model_bound"] == "lds" and abs(head["byte_model_frac"] - 3.3e4 / 1.4e5) < 1e-12
+    low = dict(valu, achieved=4.0e11, frac=4.0e11 / 6.144e11)
+    head = bench.roofline_head("hbm", 6.5e3, 8.0e3, "GB/s", {"valu_issue": low})
+    assert head["bound"] == "hbm" and abs(head["frac"] - 6.5e3 / 8.0e3) < 1e-12
+    assert head["valu_frac"] == low["frac"]
+    head = bench.roofline_head("hbm", 6.5e3, 8.0e3, "GB/s", {})
+    assert head["bound"] == "hbm" and "valu_frac" not in head
