@@ -261,6 +261,7 @@ struct mh_cnn {
   Dev<uint8_t> g;
   Dev<float> z1, a1, z2, a2, p2, z3, a3, feat, zh1, ah1, zh2, ah2, y, t;
   // gradients
+  Dev<float> epoch_terms;  // loss terms of every batch of a training run's last epoch
   Dev<float> dy, terms, dzh2, dzh1, dfeat, dz3, dp2, dz2, dz1;
   Dev<uint8_t> data_g;
   Dev<float> data_t;
@@ -339,20 +340,28 @@ void forward(mh_cnn* net, int B) {
 
 // gradient of layer l's weights and bias from dZ [M x out] and its input
 // (loader lx of the [M x in] matrix), reduced over rows in kSplit
-// fixed-order partials
+// fixed-order partials: summed into net->grads, or with to_adam left in
+// net->part at the parameters' offsets, n_params apart, for
+// adam_step_splits to sum
 template <typename LX>
-void layer_grads(mh_cnn* net, int l, const float* dZ, LX lx, int M) {
+void layer_grads(mh_cnn* net, int l, const float* dZ, LX lx, int M, bool to_adam) {
   const int No = kLayers[l].out, Ni = kLayers[l].in;
   const int rows = (M + kSplit - 1) / kSplit;
   hipStream_t s = net->stream;
+  float* pW = to_adam ? net->part.p + net->w_off[l] : net->part.p;
+  float* pb = to_adam ? net->part.p + net->b_off[l] : net->part.p;
+  const size_t sW = to_adam ? net->n_params : static_cast<size_t>(No) * Ni;
+  const size_t sb = to_adam ? net->n_params : static_cast<size_t>(No);
   hipLaunchKernelGGL(wgrad_l<LX>, dim3((Ni + kTile - 1) / kTile, (No + kTile - 1) / kTile, kSplit), dim3(256), 0, s,
-                     dZ, No, lx, Ni, M, rows, net->part.p);
-  hipLaunchKernelGGL(sum_splits, dim3(blocks_for(static_cast<size_t>(No) * Ni)), dim3(256), 0, s, net->part.p,
-                     No * Ni, kSplit, net->grads.p + net->w_off[l]);
+                     dZ, No, lx, Ni, M, rows, pW, sW);
+  if (!to_adam)
+    hipLaunchKernelGGL(sum_splits, dim3(blocks_for(static_cast<size_t>(No) * Ni)), dim3(256), 0, s, net->part.p,
+                       No * Ni, kSplit, net->grads.p + net->w_off[l]);
   hipLaunchKernelGGL(col_partial, dim3((No + kColTile - 1) / kColTile, kSplit), dim3(kColThreads), 0, s, dZ, M, No,
-                     rows, net->part.p);
-  hipLaunchKernelGGL(sum_splits, dim3(blocks_for(No)), dim3(256), 0, s, net->part.p, No, kSplit,
-                     net->grads.p + net->b_off[l]);
+                     rows, pb, sb);
+  if (!to_adam)
+    hipLaunchKernelGGL(sum_splits, dim3(blocks_for(No)), dim3(256), 0, s, net->part.p, No, kSplit,
+                       net->grads.p + net->b_off[l]);
 }
 
 // input gradient of convolution l (input grid D, C channels, O outputs)
@@ -373,41 +382,52 @@ void dense_back(mh_cnn* net, int l, const float* dZ, int M, float* dIn, const fl
                      static_cast<float*>(nullptr), Zprev);
 }
 
-void backward(mh_cnn* net, int B) {
+void backward(mh_cnn* net, int B, bool to_adam = false) {
   const size_t R1 = static_cast<size_t>(B) * kP1, R2 = static_cast<size_t>(B) * kP2;
   hipStream_t s = net->stream;
   using Rows = StridedB<float, false>;
-  layer_grads(net, 5, net->dy.p, Rows{net->ah2.p, kH}, B);
+  layer_grads(net, 5, net->dy.p, Rows{net->ah2.p, kH}, B, to_adam);
   dense_back<3>(net, 5, net->dy.p, B, net->dzh2.p, net->zh2.p);
-  layer_grads(net, 4, net->dzh2.p, Rows{net->ah1.p, kH}, B);
+  layer_grads(net, 4, net->dzh2.p, Rows{net->ah1.p, kH}, B, to_adam);
   dense_back<3>(net, 4, net->dzh2.p, B, net->dzh1.p, net->zh1.p);
-  layer_grads(net, 3, net->dzh1.p, Rows{net->feat.p, kC3}, B);
+  layer_grads(net, 3, net->dzh1.p, Rows{net->feat.p, kC3}, B, to_adam);
   dense_back<0>(net, 3, net->dzh1.p, B, net->dfeat.p, nullptr);
   hipLaunchKernelGGL(gap_bwd_gelu, dim3(blocks_for(R2 * kC3)), dim3(256), 0, s, net->dfeat.p, net->z3.p, B,
                      net->dz3.p);
-  layer_grads(net, 2, net->dz3.p, ConvColsB<kG2, kC2>{net->p2.p}, static_cast<int>(R2));
+  layer_grads(net, 2, net->dz3.p, ConvColsB<kG2, kC2>{net->p2.p}, static_cast<int>(R2), to_adam);
   conv_back<kG2, kC2, kC3, 0>(net, 2, net->dz3.p, static_cast<int>(R2), net->dp2.p, nullptr);
   hipLaunchKernelGGL(pool2_bwd_gelu, dim3(blocks_for(R1 * kC2)), dim3(256), 0, s, net->dp2.p, net->z2.p, B,
                      net->dz2.p);
-  layer_grads(net, 1, net->dz2.p, ConvColsB<kG, kC1>{net->a1.p}, static_cast<int>(R1));
+  layer_grads(net, 1, net->dz2.p, ConvColsB<kG, kC1>{net->a1.p}, static_cast<int>(R1), to_adam);
   conv_back<kG, kC1, kC2, 3>(net, 1, net->dz2.p, static_cast<int>(R1), net->dz1.p, net->z1.p);
-  layer_grads(net, 0, net->dz1.p, OneHotB{net->g.p}, static_cast<int>(R1));
+  layer_grads(net, 0, net->dz1.p, OneHotB{net->g.p}, static_cast<int>(R1), to_adam);
   hipcheck(hipGetLastError(), "cnn backward");
 }
 
-// loss terms and dy = (y - t) / B for the batch in net->g / net->t
-float loss_backward(mh_cnn* net, int B, bool want_loss) {
+// Loss terms (into `terms`, B floats on the device) and dy = (y - t) / B for
+// the batch in net->g / net->t, then the backward pass (to_adam as in
+// layer_grads).
+void loss_backward_to(mh_cnn* net, int B, float* terms, bool to_adam = false) {
   forward(net, B);
   hipLaunchKernelGGL(loss_grad, dim3(blocks_for(B)), dim3(256), 0, net->stream, net->y.p, net->t.p, B, 1.0f / B,
-                     net->dy.p, net->terms.p);
-  backward(net, B);
+                     net->dy.p, terms);
+  backward(net, B, to_adam);
+}
+
+// A batch's loss from its B terms on the host: their sum in order / B.
+float batch_loss(const float* terms, int B) {
+  double sum = 0.0;
+  for (int i = 0; i < B; i++) sum += terms[i];
+  return static_cast<float>(sum / B);
+}
+
+float loss_backward(mh_cnn* net, int B, bool want_loss) {
+  loss_backward_to(net, B, net->terms.p);
   if (!want_loss) return 0.f;
   std::vector<float> terms(B);
   hipcheck(hipMemcpyAsync(terms.data(), net->terms.p, B * sizeof(float), hipMemcpyDeviceToHost, net->stream), "D2H");
   hipcheck(hipStreamSynchronize(net->stream), "sync");
-  double sum = 0.0;
-  for (float v : terms) sum += v;
-  return static_cast<float>(sum / B);
+  return batch_loss(terms.data(), B);
 }
 
 int fail(const mh::Error& e) {
@@ -447,7 +467,8 @@ int mh_cnn_create(int device, uint64_t seed, mh_cnn** out) {
     net->m.ensure(off);
     net->v.ensure(off);
     net->w16.ensure(off);
-    net->part.ensure(static_cast<size_t>(kSplit) * max_w);
+    // split partials of one tensor, or of every parameter for adam_step_splits
+    net->part.ensure(static_cast<size_t>(kSplit) * std::max(max_w, off));
     std::vector<float> zeros(off, 0.f);
     hipcheck(hipMemcpy(net->params.p, p.data(), off * sizeof(float), hipMemcpyHostToDevice), "H2D");
     hipcheck(hipMemcpy(net->m.p, zeros.data(), off * sizeof(float), hipMemcpyHostToDevice), "H2D");
@@ -551,36 +572,43 @@ int mh_cnn_train(mh_cnn* net, const uint8_t* genomes, const float* target, int n
     hipcheck(hipMemcpyAsync(net->data_t.p, target, n * sizeof(float), hipMemcpyHostToDevice, net->stream), "H2D");
     std::mt19937_64 rng(seed);
     std::vector<int> perm(n);
-    double epoch_loss = 0.0;
+    // the last epoch keeps every batch's loss terms on the device, so no
+    // batch waits for a copy to the host; they are summed after the loop
+    const int n_batches = n / batch;
+    net->epoch_terms.ensure(static_cast<size_t>(n_batches) * batch);
+    int batches = 0;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int ep = 0; ep < n_epochs; ep++) {
       for (int i = 0; i < n; i++) perm[i] = i;
       std::shuffle(perm.begin(), perm.end(), rng);
       hipcheck(hipMemcpyAsync(net->idx.p, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, net->stream), "H2D");
       const bool last = ep == n_epochs - 1;
-      epoch_loss = 0.0;
-      int batches = 0;
       for (int b0 = 0; b0 + batch <= n; b0 += batch) {
         hipLaunchKernelGGL(gather_genomes, dim3(batch), dim3(128), 0, net->stream, net->data_g.p, net->idx.p + b0,
                            batch, net->g.p);
         hipLaunchKernelGGL(gather_scalar, dim3(blocks_for(batch)), dim3(256), 0, net->stream, net->data_t.p,
                            net->idx.p + b0, batch, net->t.p);
-        const float l = loss_backward(net, batch, last);
+        loss_backward_to(net, batch,
+                         last ? net->epoch_terms.p + static_cast<size_t>(batches) * batch : net->terms.p, true);
         net->adam_t++;
         const float bc1 = 1.f - std::pow(b1, static_cast<float>(net->adam_t));
         const float bc2 = 1.f - std::pow(b2, static_cast<float>(net->adam_t));
-        hipLaunchKernelGGL(adam_step, dim3(blocks_for(net->n_params)), dim3(256), 0, net->stream, net->params.p,
-                           net->grads.p, net->m.p, net->v.p, net->w16.p, static_cast<int>(net->n_params), lr, b1,
-                           b2, eps, bc1, bc2);
+        hipLaunchKernelGGL(adam_step_splits, dim3(blocks_for(net->n_params)), dim3(256), 0, net->stream,
+                           net->params.p, net->part.p, kSplit, net->n_params, net->m.p, net->v.p, net->w16.p,
+                           static_cast<int>(net->n_params), lr, b1, b2, eps, bc1, bc2);
         hipcheck(hipGetLastError(), "train step");
-        if (last) {
-          epoch_loss += l;
-          batches++;
-        }
+        if (last) batches++;
       }
-      if (batches) epoch_loss /= batches;
     }
+    // the last epoch's mean of the batch losses
+    std::vector<float> terms(static_cast<size_t>(batches) * batch);
+    if (batches)
+      hipcheck(hipMemcpyAsync(terms.data(), net->epoch_terms.p, terms.size() * sizeof(float),
+                              hipMemcpyDeviceToHost, net->stream), "D2H");
     hipcheck(hipStreamSynchronize(net->stream), "sync");
+    double epoch_loss = 0.0;
+    for (int bi = 0; bi < batches; bi++) epoch_loss += batch_loss(terms.data() + static_cast<size_t>(bi) * batch, batch);
+    if (batches) epoch_loss /= batches;
     if (final_loss) *final_loss = static_cast<float>(epoch_loss);
     return MH_OK;
   } catch (const mh::Error& e) {

@@ -292,14 +292,14 @@ __global__ void loss_grad(const float* Y, const float* T, int n, float inv_b, fl
   terms[i] = 0.5f * e * e;
 }
 
-// Column sums in a fixed order: part[s][n] = sum of X[r][n] over the rows r
-// of split s ([s rows, (s + 1) rows) within M).  A block of kColThreads
+// Column sums in a fixed order: part[s split_stride + n] = sum of X[r][n]
+// over the rows r of split s ([s rows, (s + 1) rows) within M).  A block of kColThreads
 // takes kColTile columns: thread (g, c) adds rows g, g + kColGroups, ... of
 // its split in ascending order, then one thread per column adds the
 // kColGroups partials in ascending g.
 constexpr int kColTile = 64, kColGroups = 16, kColThreads = kColTile * kColGroups;
 __global__ __launch_bounds__(kColThreads) void col_partial(const float* __restrict__ X, int M, int N, int rows,
-                                                           float* __restrict__ part) {
+                                                           float* __restrict__ part, size_t split_stride) {
   __shared__ float acc[kColGroups][kColTile];
   const int c = threadIdx.x % kColTile, g = threadIdx.x / kColTile;
   const int n = blockIdx.x * kColTile + c;
@@ -313,17 +313,17 @@ __global__ __launch_bounds__(kColThreads) void col_partial(const float* __restri
     float t = 0.f;
 #pragma unroll
     for (int k = 0; k < kColGroups; ++k) t += acc[k][c];
-    part[static_cast<size_t>(blockIdx.y) * N + n] = t;
+    part[blockIdx.y * split_stride + n] = t;
   }
 }
 
-// Weight gradient with split-K: part[s][i][j] = sum over rows r of split s
-// of D[r][i] * X(r, j) on the matrix cores (bf16 operands, fp32 sums in a
-// fixed order); D is [M x Ni] row-major, X a B-operand loader giving row r,
-// column j of an [M x Nj] matrix.
+// Weight gradient with split-K: part[s split_stride + i Nj + j] = sum over
+// rows r of split s of D[r][i] * X(r, j) on the matrix cores (bf16 operands,
+// fp32 sums in a fixed order); D is [M x Ni] row-major, X a B-operand loader
+// giving row r, column j of an [M x Nj] matrix.
 template <typename LX>
 __global__ __launch_bounds__(256) void wgrad_l(const float* __restrict__ Dm, int Ni, LX lx, int Nj, int M, int rows,
-                                               float* __restrict__ part) {
+                                               float* __restrict__ part, size_t split_stride) {
   __shared__ __align__(16) uint16_t sd[kTile][kMRow];
   __shared__ __align__(16) uint16_t sx[kTile][kMRow];
   const int i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
@@ -331,7 +331,7 @@ __global__ __launch_bounds__(256) void wgrad_l(const float* __restrict__ Dm, int
   f32x4_t acc[2][2];
   // op(A)[i][r] = D[r][i]
   mfma_tile(StridedA<float, true>{Dm, Ni}, lx, Ni, Nj, r_begin, r_end, i0, j0, sd, sx, acc);
-  float* out = part + static_cast<size_t>(blockIdx.z) * Ni * Nj;
+  float* out = part + blockIdx.z * split_stride;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -352,11 +352,8 @@ __global__ void sum_splits(const float* part, int n, int splits, float* out) {
   out[e] = s;
 }
 
-__global__ void adam_step(float* p, const float* g, float* m, float* v, __hip_bfloat16* w16, int n,
-                          float lr, float b1, float b2, float eps, float bc1, float bc2) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i];
+__device__ __forceinline__ void adam_update(int i, float gi, float* p, float* m, float* v, __hip_bfloat16* w16,
+                                            float lr, float b1, float b2, float eps, float bc1, float bc2) {
   const float mi = b1 * m[i] + (1.f - b1) * gi;
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi;
@@ -364,6 +361,26 @@ __global__ void adam_step(float* p, const float* g, float* m, float* v, __hip_bf
   const float pi = p[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
   p[i] = pi;
   if (w16) w16[i] = __float2bfloat16(pi);
+}
+
+__global__ void adam_step(float* p, const float* g, float* m, float* v, __hip_bfloat16* w16, int n,
+                          float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  adam_update(i, g[i], p, m, v, w16, lr, b1, b2, eps, bc1, bc2);
+}
+
+// Adam with each gradient summed from its split-K partials, part[k stride +
+// i] for k = 0 .. splits - 1, in sum_splits' order: the same gradient
+// without a sum_splits launch per weight and bias tensor.
+__global__ void adam_step_splits(float* p, const float* part, int splits, size_t stride, float* m, float* v,
+                                 __hip_bfloat16* w16, int n, float lr, float b1, float b2, float eps, float bc1,
+                                 float bc2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float g = 0.f;
+  for (int k = 0; k < splits; ++k) g += part[k * stride + i];
+  adam_update(i, g, p, m, v, w16, lr, b1, b2, eps, bc1, bc2);
 }
 
 __global__ void to_bf16(const float* src, __hip_bfloat16* dst, int n) {

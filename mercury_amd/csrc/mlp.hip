@@ -46,6 +46,7 @@ struct mh_mlp {
   Dev<__hip_bfloat16> w16;    // bf16 copy of the whole parameter vector (weights used)
   std::vector<Dev<float>> Z, A, dA;  // per layer activations / gradients
   Dev<float> x, t, dY, terms, data_x, data_t;
+  Dev<float> epoch_terms;     // loss terms of every batch of a training run's last epoch
   Dev<float> part;            // split-K partials of the weight and bias gradients
   Dev<int> idx;
   int batch_cap = 0;
@@ -103,8 +104,13 @@ void forward(mh_mlp* net, const float* in, int B) {
   }
 }
 
-// Backward from dY (already (Y - T)/B): gradients into net->grads.
-void backward(mh_mlp* net, const float* in, int B) {
+// Row splits of a batch of B for the weight and bias gradients.
+int mlp_splits(int B) { return std::max(1, std::min(kMlpSplit, B / 64)); }
+
+// Backward from dY (already (Y - T)/B): gradients into net->grads, or with
+// to_adam their split partials into net->part at the parameters' offsets,
+// n_params apart, for adam_step_splits to sum.
+void backward(mh_mlp* net, const float* in, int B, bool to_adam = false) {
   const int L = net->L();
   const float* delta = net->dY.p;  // dLoss/dZ_l for the current layer
   for (int l = L - 1; l >= 0; l--) {
@@ -114,16 +120,22 @@ void backward(mh_mlp* net, const float* in, int B) {
     float* gb = net->grads.p + net->b_off(l);
     // gW[no x ni] = delta^T[no x B] a_prev[B x ni] and gb = column sums of
     // delta, each as S row-split partials summed in order
-    const int S = std::max(1, std::min(kMlpSplit, B / 64));
+    const int S = mlp_splits(B);
     const int rows = (B + S - 1) / S;
+    float* pW = to_adam ? net->part.p + net->w_off[l] : net->part.p;
+    float* pb = to_adam ? net->part.p + net->b_off(l) : net->part.p;
+    const size_t sW = to_adam ? net->n_params : static_cast<size_t>(no) * ni;
+    const size_t sb = to_adam ? net->n_params : static_cast<size_t>(no);
     hipLaunchKernelGGL((wgrad_l<StridedB<float, false>>), dim3((ni + kTile - 1) / kTile, (no + kTile - 1) / kTile, S),
-                       dim3(256), 0, net->stream, delta, no, StridedB<float, false>{a_prev, ni}, ni, B, rows,
-                       net->part.p);
-    hipLaunchKernelGGL(sum_splits, dim3((no * ni + 255) / 256), dim3(256), 0, net->stream, net->part.p, no * ni, S,
-                       gW);
+                       dim3(256), 0, net->stream, delta, no, StridedB<float, false>{a_prev, ni}, ni, B, rows, pW,
+                       sW);
+    if (!to_adam)
+      hipLaunchKernelGGL(sum_splits, dim3((no * ni + 255) / 256), dim3(256), 0, net->stream, net->part.p, no * ni,
+                         S, gW);
     hipLaunchKernelGGL(col_partial, dim3((no + kColTile - 1) / kColTile, S), dim3(kColThreads), 0, net->stream,
-                       delta, B, no, rows, net->part.p);
-    hipLaunchKernelGGL(sum_splits, dim3((no + 255) / 256), dim3(256), 0, net->stream, net->part.p, no, S, gb);
+                       delta, B, no, rows, pb, sb);
+    if (!to_adam)
+      hipLaunchKernelGGL(sum_splits, dim3((no + 255) / 256), dim3(256), 0, net->stream, net->part.p, no, S, gb);
     if (l > 0) {
       // delta_prev[B x ni] = (delta[B x no] W[no x ni]) * gelu'(Z_{l-1})
       const __hip_bfloat16* W = net->w16.p + net->w_off[l];
@@ -197,7 +209,8 @@ int mh_mlp_create(int device, int n_layers, const int* sizes, uint64_t seed, mh_
     net->n_params = off;
     size_t widest = 0;
     for (int l = 0; l < n_layers; l++) widest = std::max<size_t>(widest, static_cast<size_t>(sizes[l]) * sizes[l + 1]);
-    net->part.ensure(static_cast<size_t>(kMlpSplit) * widest);
+    // split partials of one tensor, or of every parameter for adam_step_splits
+    net->part.ensure(static_cast<size_t>(kMlpSplit) * std::max(widest, net->n_params));
     // He-uniform weights (bound sqrt(6 / fan_in)), zero biases
     std::mt19937_64 rng(seed);
     std::vector<float> p(off, 0.f);
@@ -361,45 +374,51 @@ int mh_mlp_train(mh_mlp* net, const float* x, const float* target, int n, int ba
                             hipMemcpyHostToDevice, net->stream), "H2D");
     std::mt19937_64 rng(seed);
     std::vector<int> perm(n);
-    double epoch_loss = 0.0;
+    // the last epoch keeps every batch's loss terms on the device, so no
+    // batch waits for a copy to the host; they are summed after the loop
+    const int n_batches = n / batch;
+    const size_t nt = static_cast<size_t>(batch) * no;
+    net->epoch_terms.ensure(static_cast<size_t>(n_batches) * nt);
+    int batches = 0;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int ep = 0; ep < n_epochs; ep++) {
       for (int i = 0; i < n; i++) perm[i] = i;
       std::shuffle(perm.begin(), perm.end(), rng);
       hipcheck(hipMemcpyAsync(net->idx.p, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, net->stream), "H2D");
-      epoch_loss = 0.0;
-      int batches = 0;
+      const bool last = ep == n_epochs - 1;
       for (int b0 = 0; b0 + batch <= n; b0 += batch) {
         hipLaunchKernelGGL(gather_rows, dim3(batch), dim3(64), 0, net->stream, net->data_x.p, net->idx.p + b0,
                            batch, ni, net->x.p);
         hipLaunchKernelGGL(gather_rows, dim3(batch), dim3(64), 0, net->stream, net->data_t.p, net->idx.p + b0,
                            batch, no, net->t.p);
         forward(net, net->x.p, batch);
-        const int nt = batch * no;
-        hipLaunchKernelGGL(loss_grad, dim3((nt + 255) / 256), dim3(256), 0, net->stream, net->A[L - 1].p,
-                           net->t.p, nt, 1.0f / batch, net->dY.p, net->terms.p);
-        backward(net, net->x.p, batch);
+        float* terms = last ? net->epoch_terms.p + batches * nt : net->terms.p;
+        hipLaunchKernelGGL(loss_grad, dim3((static_cast<int>(nt) + 255) / 256), dim3(256), 0, net->stream,
+                           net->A[L - 1].p, net->t.p, static_cast<int>(nt), 1.0f / batch, net->dY.p, terms);
+        backward(net, net->x.p, batch, true);
         net->adam_t++;
         const float bc1 = 1.f - std::pow(b1, static_cast<float>(net->adam_t));
         const float bc2 = 1.f - std::pow(b2, static_cast<float>(net->adam_t));
-        hipLaunchKernelGGL(adam_step, dim3((net->n_params + 255) / 256), dim3(256), 0, net->stream, net->params.p,
-                           net->grads.p, net->m.p, net->v.p, net->w16.p, static_cast<int>(net->n_params), lr, b1,
-                           b2, eps, bc1, bc2);
+        hipLaunchKernelGGL(adam_step_splits, dim3((net->n_params + 255) / 256), dim3(256), 0, net->stream,
+                           net->params.p, net->part.p, mlp_splits(batch), net->n_params, net->m.p, net->v.p,
+                           net->w16.p, static_cast<int>(net->n_params), lr, b1, b2, eps, bc1, bc2);
         hipcheck(hipGetLastError(), "train step");
-        if (ep == n_epochs - 1) {
-          std::vector<float> terms(nt);
-          hipcheck(hipMemcpyAsync(terms.data(), net->terms.p, nt * sizeof(float), hipMemcpyDeviceToHost,
-                                  net->stream), "D2H");
-          hipcheck(hipStreamSynchronize(net->stream), "sync");
-          double s = 0.0;
-          for (float v : terms) s += v;
-          epoch_loss += s / batch;
-          batches++;
-        }
+        if (last) batches++;
       }
-      if (batches) epoch_loss /= batches;
     }
+    // the last epoch's mean over batches of each batch's summed terms / batch
+    std::vector<float> terms(static_cast<size_t>(batches) * nt);
+    if (batches)
+      hipcheck(hipMemcpyAsync(terms.data(), net->epoch_terms.p, terms.size() * sizeof(float),
+                              hipMemcpyDeviceToHost, net->stream), "D2H");
     hipcheck(hipStreamSynchronize(net->stream), "sync");
+    double epoch_loss = 0.0;
+    for (int bi = 0; bi < batches; bi++) {
+      double s = 0.0;
+      for (size_t j = 0; j < nt; j++) s += terms[bi * nt + j];
+      epoch_loss += s / batch;
+    }
+    if (batches) epoch_loss /= batches;
     if (final_loss) *final_loss = static_cast<float>(epoch_loss);
     return MH_OK;
   } catch (const mh::Error& e) {

@@ -97,3 +97,17 @@ def rel_err(x32, x64, mass_offset):
         if len(b):
             out.append(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
     return np.array(out)
+
+
+def adam_first_step_f32(p, g, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """The device's first Adam step (t = 1, zero moments) restated in float32,
+    one rounding per operation in the kernel's order: m = b1 0 + (1 - b1) g,
+    v = b2 0 + ((1 - b2) g) g, p - (lr (m / bc1)) / (sqrt(v / bc2) + eps)
+    with bc = 1 - b^1."""
+    f = np.float32
+    p, g = np.asarray(p, f), np.asarray(g, f)
+    b1, b2, eps, lr = f(b1), f(b2), f(eps), f(lr)
+    bc1, bc2 = f(1) - b1, f(1) - b2
+    m = b1 * f(0) + (f(1) - b1) * g
+    v = b2 * f(0) + ((f(1) - b2) * g) * g
+    return p - (lr * (m / bc1)) / (np.sqrt(v / bc2) + eps)

@@ -6,6 +6,7 @@ import pytest
 import oracle
 import mercury_amd as mh
 from mercury_amd import surrogate
+from helpers import adam_first_step_f32
 
 
 def test_spearman_matches_scipy_with_ties():
@@ -115,6 +116,26 @@ def test_mlp_training_learns_nonlinear_target():
     print("rho mlp", rho, "rho linear", surrogate.spearman(lin, y[7000:]), flush=True)
     assert rho > 0.75 and rho > surrogate.spearman(lin, y[7000:]) + 0.15
     assert surrogate.mae(pred, y[7000:]) < surrogate.mae(lin, y[7000:])
+
+
+@pytest.mark.gpu
+def test_mlp_train_step_is_adam_on_the_split_summed_gradient():
+    """mh_mlp_train leaves each gradient as 8 row-split partials and sums them
+    inside the Adam kernel; one step must equal, bit for bit, a float32 Adam
+    step on mh_mlp_loss_grad's gradient (summed by its own kernels), and the
+    reported loss must be that call's loss.  Every row is the same sample, so
+    the training batch's shuffle leaves the sums' terms unchanged."""
+    print("mlp train step", flush=True)
+    rng = np.random.default_rng(6)
+    net = surrogate.MLP([8, 16, 1], seed=7)
+    p0 = net.get_params()
+    x = np.repeat(rng.normal(size=(1, 8)).astype(np.float32), 512, axis=0)
+    t = np.repeat(rng.normal(size=(1, 1)).astype(np.float32), 512, axis=0)
+    loss, g = net.loss_grad(x, t)
+    assert np.count_nonzero(g) > len(g) // 2
+    final = net.train(x, t, epochs=1, batch=512, lr=1e-2, seed=0)
+    assert final == loss
+    np.testing.assert_array_equal(net.get_params(), adam_first_step_f32(p0, g, 1e-2))
 
 
 @pytest.mark.gpu

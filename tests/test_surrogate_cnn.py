@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from mercury_amd import evolve, surrogate
+from helpers import adam_first_step_f32
 
 
 def bf16_round(w):
@@ -132,3 +133,23 @@ def test_cnn_learns_spatial_target_that_summary_features_miss():
     print("rho cnn", rho, "rho summary-feature mlp", rho_mlp, flush=True)
     # measured on MI355X: 0.72 after 15 epochs against 0.08 for the MLP
     assert rho > 0.65 and rho > rho_mlp + 0.4
+
+
+@pytest.mark.gpu
+def test_cnn_train_step_is_adam_on_the_split_summed_gradient():
+    """mh_cnn_train leaves each gradient as 64 row-split partials and sums them
+    inside the Adam kernel; one step must equal, bit for bit, a float32 Adam
+    step on mh_cnn_loss_grad's gradient, and the reported loss must be that
+    call's loss.  Every genome of the batch is the same, so the shuffle leaves
+    the sums' terms unchanged."""
+    print("cnn train step", flush=True)
+    rng = np.random.default_rng(3)
+    net = surrogate.CNN(seed=4)
+    p0 = net.get_params()
+    g = np.repeat(evolve.random_genomes(rng, 1, 8), 256, axis=0)
+    t = np.full(256, 0.37, np.float32)
+    loss, grad = net.loss_grad(g, t)
+    assert np.count_nonzero(grad) > len(grad) // 4
+    final = net.train(g, t, epochs=1, batch=256, lr=1e-3, seed=0)
+    assert final == loss
+    np.testing.assert_array_equal(net.get_params(), adam_first_step_f32(p0, grad, 1e-3))
