@@ -83,6 +83,31 @@ __device__ __forceinline__ u16x8_t ld8_bf16(const __hip_bfloat16* p) {
   return r;
 }
 
+// 8 consecutive elements as loaded, converted to bf16 bits only by
+// bits8(): a tile step issues the loads of all its slices before the first
+// conversion waits for one.  p must be 16-byte aligned.
+template <typename T>
+struct Raw8;
+template <>
+struct Raw8<float> {
+  float4 a, b;
+};
+template <>
+struct Raw8<__hip_bfloat16> {
+  u16x8_t v;
+};
+__device__ __forceinline__ Raw8<float> ld_raw8(const float* p) {
+  return Raw8<float>{reinterpret_cast<const float4*>(p)[0], reinterpret_cast<const float4*>(p)[1]};
+}
+__device__ __forceinline__ Raw8<__hip_bfloat16> ld_raw8(const __hip_bfloat16* p) {
+  return Raw8<__hip_bfloat16>{*reinterpret_cast<const u16x8_t*>(p)};
+}
+__device__ __forceinline__ u16x8_t bits8(const Raw8<float>& r) {
+  return u16x8_t{bf16_bits(r.a.x), bf16_bits(r.a.y), bf16_bits(r.a.z), bf16_bits(r.a.w),
+                 bf16_bits(r.b.x), bf16_bits(r.b.y), bf16_bits(r.b.z), bf16_bits(r.b.w)};
+}
+__device__ __forceinline__ u16x8_t bits8(const Raw8<__hip_bfloat16>& r) { return r.v; }
+
 // Operand loaders of the matrix-core tile: A(m, k) and B(k, n) return the
 // bf16 bits of op(A)[m][k] and op(B)[k][n] (callers stay within M, N and
 // the k range).  kContigK says which index is contiguous in memory (k, or
@@ -104,6 +129,11 @@ struct StridedA {
   __device__ u16x8_t vec8(int m, int k) const {
     return ld8_bf16(p + (TR ? static_cast<size_t>(k) * ld + m : static_cast<size_t>(m) * ld + k));
   }
+  // every 8-element run a tile step reads starts 16-byte aligned
+  __device__ bool aligned16() const { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0 && ld % 8 == 0; }
+  __device__ Raw8<T> raw8(int m, int k) const {
+    return ld_raw8(p + (TR ? static_cast<size_t>(k) * ld + m : static_cast<size_t>(m) * ld + k));
+  }
 };
 template <typename T, bool TR>
 struct StridedB {
@@ -117,10 +147,16 @@ struct StridedB {
   __device__ u16x8_t vec8(int k, int n) const {
     return ld8_bf16(p + (TR ? static_cast<size_t>(n) * ld + k : static_cast<size_t>(k) * ld + n));
   }
+  __device__ bool aligned16() const { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0 && ld % 8 == 0; }
+  __device__ Raw8<T> raw8(int k, int n) const {
+    return ld_raw8(p + (TR ? static_cast<size_t>(n) * ld + k : static_cast<size_t>(k) * ld + n));
+  }
 };
 
-// Stages one 64 x 32 slice of an operand in LDS (rows along m or n, k
-// contiguous), 8 elements per thread of 256.  ROWS = M (A) or N (B); A
+// One 64 x 32 slice of an operand (rows along m or n, k contiguous), 8
+// elements per thread of 256: fetch_slice reads this thread's 8 as bf16
+// bits (zeros outside ROWS and the k range), raw_at reads them unconverted
+// with no checks, put_slice writes them to LDS.  ROWS = M (A) or N (B); A
 // loaders take (row, k), B loaders (k, row).
 template <bool IS_A, typename L>
 __device__ __forceinline__ uint16_t load_at(const L& l, int row, int k) {
@@ -133,41 +169,63 @@ __device__ __forceinline__ u16x8_t vec8_at(const L& l, int row, int k) {
   else return l.vec8(k, row);
 }
 template <bool IS_A, typename L>
-__device__ __forceinline__ void stage_slice(const L& l, int ROWS, int r0, int k0, int k_end, uint16_t (*s)[40]) {
+__device__ __forceinline__ auto raw_at(const L& l, int r0, int k0) {
+  if constexpr (L::kContigK) {
+    const int r = r0 + (threadIdx.x >> 2), k = k0 + ((threadIdx.x & 3) << 3);
+    if constexpr (IS_A) return l.raw8(r, k);
+    else return l.raw8(k, r);
+  } else {
+    const int r = r0 + ((threadIdx.x & 7) << 3), k = k0 + (threadIdx.x >> 3);
+    if constexpr (IS_A) return l.raw8(r, k);
+    else return l.raw8(k, r);
+  }
+}
+template <bool IS_A, typename L>
+__device__ __forceinline__ u16x8_t fetch_slice(const L& l, int ROWS, int r0, int k0, int k_end) {
+  u16x8_t v;
   if constexpr (L::kVec8) {
     if constexpr (L::kContigK) {
-      // thread -> one row, 8 consecutive k
-      const int rr = threadIdx.x >> 2, kk = (threadIdx.x & 3) << 3;
-      const int r = r0 + rr, k = k0 + kk;
-      u16x8_t v;
-      if (r < ROWS && k + 8 <= k_end) {
-        v = vec8_at<IS_A>(l, r, k);
-      } else {
+      const int r = r0 + (threadIdx.x >> 2), k = k0 + ((threadIdx.x & 3) << 3);
+      if (r < ROWS && k + 8 <= k_end) v = vec8_at<IS_A>(l, r, k);
+      else {
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = (r < ROWS && k + j < k_end) ? load_at<IS_A>(l, r, k + j) : uint16_t(0);
       }
-      *reinterpret_cast<u16x8_t*>(&s[rr][kk]) = v;
     } else {
-      // thread -> one k, 8 consecutive rows
-      const int kk = threadIdx.x >> 3, rr = (threadIdx.x & 7) << 3;
-      const int r = r0 + rr, k = k0 + kk;
-      u16x8_t v;
-      if (r + 8 <= ROWS && k < k_end) {
-        v = vec8_at<IS_A>(l, r, k);
-      } else {
+      const int r = r0 + ((threadIdx.x & 7) << 3), k = k0 + (threadIdx.x >> 3);
+      if (r + 8 <= ROWS && k < k_end) v = vec8_at<IS_A>(l, r, k);
+      else {
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = (r + j < ROWS && k < k_end) ? load_at<IS_A>(l, r + j, k) : uint16_t(0);
       }
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      const int rr = L::kContigK ? e / 32 : e % 64, kk = L::kContigK ? e % 32 : e / 64;
+      const int r = r0 + rr, k = k0 + kk;
+      v[j] = (r < ROWS && k < k_end) ? load_at<IS_A>(l, r, k) : uint16_t(0);
+    }
+  }
+  return v;
+}
+template <typename L>
+__device__ __forceinline__ void put_slice(u16x8_t v, uint16_t (*s)[40]) {
+  if constexpr (L::kVec8) {
+    if constexpr (L::kContigK) {
+      *reinterpret_cast<u16x8_t*>(&s[threadIdx.x >> 2][(threadIdx.x & 3) << 3]) = v;
+    } else {
+      const int kk = threadIdx.x >> 3, rr = (threadIdx.x & 7) << 3;
 #pragma unroll
       for (int j = 0; j < 8; ++j) s[rr + j][kk] = v[j];
     }
   } else {
-    // element by element, read along the operand's contiguous index
-    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-      int rr, kk;
-      if (L::kContigK) { kk = e % 32; rr = e / 32; } else { rr = e % 64; kk = e / 64; }
-      const int r = r0 + rr, k = k0 + kk;
-      s[rr][kk] = (r < ROWS && k < k_end) ? load_at<IS_A>(l, r, k) : uint16_t(0);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = threadIdx.x + 256 * j;
+      if constexpr (L::kContigK) s[e / 32][e % 32] = v[j];
+      else s[e % 64][e / 64] = v[j];
     }
   }
 }
@@ -175,17 +233,27 @@ __device__ __forceinline__ void stage_slice(const L& l, int ROWS, int r0, int k0
 // One 64 x 64 tile of op(A) op(B) over k in [k_begin, k_end) on the matrix
 // cores (v_mfma_f32_16x16x32_bf16: bf16 operands, fp32 accumulate), 256
 // threads.  Elements outside M, N or the k range are zeros.  Each step
-// stages a 64 x 32 slice of both operands in LDS as bf16, k-contiguous per
-// row of op(A) and per column of op(B); wave w owns the 32 x 32 quarter
+// stages S slices of 64 x 32 of both operands in LDS as bf16, k-contiguous
+// per row of op(A) and per column of op(B), between two barriers.  With
+// FAST (strided operands), a tile whose rows lie inside M and N, whose k
+// range is whole slices and whose operand runs are 16-byte aligned takes a
+// branch-free path that issues the loads of all S slices before the first
+// conversion to bf16 waits for one; otherwise each thread's 8 elements are
+// read with per-element bounds checks.  Wave w owns the 32 x 32 quarter
 // (32 (w / 2), 32 (w % 2)) as 2 x 2 blocks of 16 x 16.  Lane l feeds row
 // (or column) l % 16 of a block with the 8 k values 8 (l / 16) ..
 // 8 (l / 16) + 7 and receives rows 4 (l / 16) .. 4 (l / 16) + 3 of column
-// l % 16.  Every output sums its k in the same order on every run.
-constexpr int kMk = 32;                  // k per step
+// l % 16.  Every output sums its k in ascending slices of 32, the same
+// order on every path and every run.
+constexpr int kMk = 32;                  // k per slice
 constexpr int kMRow = kMk + 8;           // LDS row pitch in bf16 (80 B, 16 B aligned)
-template <typename LA, typename LB>
+// slices per step: 2 for the dense layers' GEMMs (k of 64 or 256); the
+// convolution kernels' gathered operands take one (a deeper stage lowers
+// their occupancy without shortening them)
+constexpr int kStageDense = 2, kStageConv = 1, kStageWgrad = 1;
+template <int S, bool FAST, typename LA, typename LB>
 __device__ __forceinline__ void mfma_tile(const LA& la, const LB& lb, int M, int N, int k_begin, int k_end, int m0,
-                                          int n0, uint16_t (*sa)[kMRow], uint16_t (*sb)[kMRow],
+                                          int n0, uint16_t (*sa)[kTile][kMRow], uint16_t (*sb)[kTile][kMRow],
                                           f32x4_t (&acc)[2][2]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -193,23 +261,61 @@ __device__ __forceinline__ void mfma_tile(const LA& la, const LB& lb, int M, int
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-  static_assert(kTile == 64 && kMk == 32 && kMRow == 40, "stage_slice's shape");
-  for (int k0 = k_begin; k0 < k_end; k0 += kMk) {
-    // 64 x 32 elements of each operand, 8 per thread, read along the
-    // operand's contiguous dimension
-    stage_slice<true>(la, M, m0, k0, k_end, sa);
-    stage_slice<false>(lb, N, n0, k0, k_end, sb);
-    __syncthreads();
-    bf16x8_t fa[2], fb[2];
+  bool fast = false;
+  if constexpr (FAST)
+    fast = m0 + kTile <= M && n0 + kTile <= N && (k_end - k_begin) % kMk == 0 && k_begin % 8 == 0 &&
+           la.aligned16() && lb.aligned16();
+  for (int k0 = k_begin; k0 < k_end; k0 += S * kMk) {
+    u16x8_t va[S], vb[S];
+    bool staged = false;
+    if constexpr (FAST) {
+      if (fast) {
+        // whole slices in range and aligned: every load of the step first,
+        // then the conversions
+        decltype(raw_at<true>(la, m0, k0)) ra[S];
+        decltype(raw_at<false>(lb, n0, k0)) rb[S];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      fa[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&sa[wm + 16 * i + (lane & 15)][8 * (lane >> 4)]));
-      fb[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&sb[wn + 16 * i + (lane & 15)][8 * (lane >> 4)]));
+        for (int s = 0; s < S; ++s)
+          if (k0 + s * kMk < k_end) {
+            ra[s] = raw_at<true>(la, m0, k0 + s * kMk);
+            rb[s] = raw_at<false>(lb, n0, k0 + s * kMk);
+          }
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          va[s] = k0 + s * kMk < k_end ? bits8(ra[s]) : u16x8_t{};
+          vb[s] = k0 + s * kMk < k_end ? bits8(rb[s]) : u16x8_t{};
+        }
+        staged = true;
+      }
+    }
+    if (!staged) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        va[s] = fetch_slice<true>(la, M, m0, k0 + s * kMk, k_end);
+        vb[s] = fetch_slice<false>(lb, N, n0, k0 + s * kMk, k_end);
+      }
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int s = 0; s < S; ++s) {
+      put_slice<LA>(va[s], sa[s]);
+      put_slice<LB>(vb[s], sb[s]);
+    }
+    __syncthreads();
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    for (int s = 0; s < S; ++s) {
+      if (k0 + s * kMk < k_end) {
+        bf16x8_t fa[2], fb[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          fa[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&sa[s][wm + 16 * i + (lane & 15)][8 * (lane >> 4)]));
+          fb[i] = __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u16x8_t*>(&sb[s][wn + 16 * i + (lane & 15)][8 * (lane >> 4)]));
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+      }
+    }
     __syncthreads();
   }
 }
@@ -260,11 +366,11 @@ template <typename LA, typename LB, int EPI>
 __global__ __launch_bounds__(256) void gemm_l(LA la, LB lb, float* __restrict__ C, int ldc, int M, int N, int K,
                                               const float* __restrict__ bias, float* __restrict__ Z,
                                               const float* __restrict__ Zprev) {
-  __shared__ __align__(16) uint16_t sa[kTile][kMRow];
-  __shared__ __align__(16) uint16_t sb[kTile][kMRow];
+  __shared__ __align__(16) uint16_t sa[kStageConv][kTile][kMRow];
+  __shared__ __align__(16) uint16_t sb[kStageConv][kTile][kMRow];
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
   f32x4_t acc[2][2];
-  mfma_tile(la, lb, M, N, 0, K, m0, n0, sa, sb, acc);
+  mfma_tile<kStageConv, false>(la, lb, M, N, 0, K, m0, n0, sa, sb, acc);
   mfma_store<EPI>(acc, C, ldc, M, N, m0, n0, bias, Z, Zprev);
 }
 
@@ -275,11 +381,11 @@ __global__ __launch_bounds__(256) void gemm(const TA* __restrict__ A, int lda, c
                                             int ldb, float* __restrict__ C, int ldc, int M, int N, int K,
                                             const float* __restrict__ bias, float* __restrict__ Z,
                                             const float* __restrict__ Zprev) {
-  __shared__ __align__(16) uint16_t sa[kTile][kMRow];
-  __shared__ __align__(16) uint16_t sb[kTile][kMRow];
+  __shared__ __align__(16) uint16_t sa[kStageDense][kTile][kMRow];
+  __shared__ __align__(16) uint16_t sb[kStageDense][kTile][kMRow];
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
   f32x4_t acc[2][2];
-  mfma_tile(StridedA<TA, TRA>{A, lda}, StridedB<TB, TRB>{B, ldb}, M, N, 0, K, m0, n0, sa, sb, acc);
+  mfma_tile<kStageDense, true>(StridedA<TA, TRA>{A, lda}, StridedB<TB, TRB>{B, ldb}, M, N, 0, K, m0, n0, sa, sb, acc);
   mfma_store<EPI>(acc, C, ldc, M, N, m0, n0, bias, Z, Zprev);
 }
 
@@ -324,13 +430,13 @@ __global__ __launch_bounds__(kColThreads) void col_partial(const float* __restri
 template <typename LX>
 __global__ __launch_bounds__(256) void wgrad_l(const float* __restrict__ Dm, int Ni, LX lx, int Nj, int M, int rows,
                                                float* __restrict__ part, size_t split_stride) {
-  __shared__ __align__(16) uint16_t sd[kTile][kMRow];
-  __shared__ __align__(16) uint16_t sx[kTile][kMRow];
+  __shared__ __align__(16) uint16_t sd[kStageWgrad][kTile][kMRow];
+  __shared__ __align__(16) uint16_t sx[kStageWgrad][kTile][kMRow];
   const int i0 = blockIdx.y * kTile, j0 = blockIdx.x * kTile;
   const int r_begin = blockIdx.z * rows, r_end = min(M, r_begin + rows);
   f32x4_t acc[2][2];
   // op(A)[i][r] = D[r][i]
-  mfma_tile(StridedA<float, true>{Dm, Ni}, lx, Ni, Nj, r_begin, r_end, i0, j0, sd, sx, acc);
+  mfma_tile<kStageWgrad, false>(StridedA<float, true>{Dm, Ni}, lx, Ni, Nj, r_begin, r_end, i0, j0, sd, sx, acc);
   float* out = part + blockIdx.z * split_stride;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
