@@ -39,7 +39,7 @@ constexpr int kG = 8, kP1 = kG * kG * kG;        // genome grid and positions
 constexpr int kG2 = kG / 2, kP2 = kG2 * kG2 * kG2;
 constexpr int kC0 = 5, kC1 = 32, kC2 = 32, kC3 = 64, kH = 256;
 constexpr int kK1 = 27 * kC0, kK2 = 27 * kC1, kK3 = 27 * kC2;
-constexpr int kSplit = 128;                       // split-K partials for weight gradients
+constexpr int kSplit = 64;                        // split-K partials for weight gradients
 
 __device__ __forceinline__ bool inside(int z, int y, int x, int D) {
   return z >= 0 && z < D && y >= 0 && y < D && x >= 0 && x < D;

@@ -137,7 +137,7 @@ def test_cnn_learns_spatial_target_that_summary_features_miss():
 
 @pytest.mark.gpu
 def test_cnn_train_step_is_adam_on_the_split_summed_gradient():
-    """mh_cnn_train leaves each gradient as 128 row-split partials and sums them
+    """mh_cnn_train leaves each gradient as 64 row-split partials and sums them
     inside the Adam kernel; one step must equal, bit for bit, a float32 Adam
     step on mh_cnn_loss_grad's gradient, and the reported loss must be that
     call's loss.  Every genome of the batch is the same, so the shuffle leaves
